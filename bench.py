@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X lattice-Boltzmann hot path.
+
+Workload (default): plain FP64 D3Q27 thermal BGK (f + g), single level,
+256^3 channel -- velocity inlet u = 0.05 at x-lo, zero-gradient outflow at
+x-hi, periodic y/z, EB sphere r = 24 at (64,128,128) with halfway
+bounce-back, tau = 0.55, 1000 steps.  Inputs are resident in HBM when the
+timed region starts.
+
+Prints ONE JSON line: MLUPS (cells x steps / s / 1e6), the HBM roofline of
+the step's kernels (algorithmic bytes = read + write of every population
+once), and the CPU oracle timed on this host's cores on a bounded sample of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "D3Q27 MLUPS & % HBM roofline, FP64 vs FP32-shifted at fixed error"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_info() -> tuple[int, str]:
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_baseline(cfg: dict, budget_s: float, threads: int) -> dict:
+    """The CPU oracle (restatement of the reference step, OpenMP over z-slabs
+    of each box) timed on this host: 1 warm-up step, then as many steps as fit
+    in the budget (at least 1)."""
+    from oracle import pyoracle
+
+    pyoracle.set_threads(threads)
+    t0 = time.perf_counter()
+    O = pyoracle.build_oracle_level(cfg)
+    t_init = time.perf_counter() - t0
+    O.evolve(1)  # warm-up
+    t0 = time.perf_counter()
+    O.evolve(1)
+    t1 = time.perf_counter() - t0
+    n_more = max(0, min(19, int(budget_s / max(t1, 1e-9)) - 1))
+    t0 = time.perf_counter()
+    if n_more:
+        O.evolve(n_more)
+    t_rest = time.perf_counter() - t0
+    steps = 1 + n_more
+    secs = t1 + t_rest
+    cells = cfg["ncell"][0] * cfg["ncell"][1] * cfg["ncell"][2]
+    O.close()
+    nproc, model = cpu_info()
+    return {
+        "value": cells * steps / secs / 1e6,
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{cfg['name']} ({len(cfg.get('boxes', [])) or 'max_box ' + str(cfg.get('max_box'))}), "
+                   f"1 warm-up + {steps} timed steps of the CPU oracle; init {t_init:.1f}s; "
+                   f"host {model} nproc={nproc}"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--storage", default="f64", choices=["f64", "f32s"])
+    ap.add_argument("--n", type=int, default=256, help="domain edge (256 = the named workload)")
+    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle timing")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is a design "
+              "note, not a measured path", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from marbles_amd import configs
+
+    if not torch.cuda.is_available():
+        log("bench.py: no GPU visible")
+        return 1
+    n = args.n
+    scale = n / 256.0
+    cfg = configs.channel_sphere((n, n, n), radius=24.0 * scale, center=(64.0 * scale, 128.0 * scale,
+                                                                      128.0 * scale), tau=0.55, max_box=64)
+    t0 = time.perf_counter()
+    lvl = configs.build_level(cfg, layout="compact", storage=args.storage)
+    log(f"level built in {time.perf_counter() - t0:.2f}s; kernel: {lvl.hot_kernel_name()}")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    lvl.advance(args.warmup, stream=sh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lvl.advance(args.steps, stream=sh)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    cells = n ** 3
+    mlups = cells * args.steps / secs / 1e6
+    log(f"timed {args.steps} steps: {secs:.4f}s -> {mlups:.1f} MLUPS")
+
+    # per-kernel durations from HIP events on the launch stream
+    prof = lvl.advance_timed(args.profile_steps, stream=sh)
+    per_kernel_ms = {k: v / args.profile_steps for k, v in prof.items()}
+    step_kernel_ms = sum(per_kernel_ms.values())
+    bpu = configs.bytes_per_update(args.storage, 2)
+    achieved = bpu * cells / (step_kernel_ms * 1e-3) / 1e9
+    log("per-kernel ms/step: " + ", ".join(f"{k}={v:.4f}" for k, v in per_kernel_ms.items()))
+    cons = lvl.conservation(stream=sh)
+    log(f"conservation (mass, mom x/y/z, 2rhoE): {cons.tolist()}")
+    lvl.close()
+
+    result = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32s": "f32"}[args.storage],
+        "data": "synthetic",
+        "config": {
+            "workload": f"{cfg['name']}: thermal D3Q27 f+g, velocity inlet u=0.05, outflow, periodic y/z, "
+                        f"EB sphere r={cfg['eb']['radius']:g} at {cfg['eb']['center']}, tau=0.55",
+            "cells": cells,
+            "storage": args.storage,
+            "kernel": "+".join(per_kernel_ms),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "+".join(per_kernel_ms),
+            "bytes_per_update": bpu,
+            "kernel_ms_per_step": {k: round(v, 5) for k, v in per_kernel_ms.items()},
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, min(args.cpu_threads, os.cpu_count() or 1))
+        except Exception as e:  # report, do not hide
+            log(f"cpu baseline failed: {e!r}")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,186 @@
+/*
+ * marbles_amd: C ABI of the MI355X (gfx950) lattice-Boltzmann engine.
+ *
+ * The engine replaces the per-level hot path of NREL/marbles: the D3Q27
+ * thermal collide (f and the energy populations g), the pull-stream with
+ * halfway bounce-back, the ghost-cell fill, and the coarse/fine coalescence.
+ * Each entry point names the reference interface it replaces (paths are in
+ * the NREL/marbles tree).
+ *
+ * Conventions
+ *  - Plain C: opaque handles, plain pointers and sizes.  Every function
+ *    returns 0 on success or a negative MRB_E* code; mrb_last_error()
+ *    returns the message of the last failure on the calling thread.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Calls are asynchronous on that stream unless stated otherwise and are
+ *    not thread-safe per handle.
+ *  - Host arrays use the reference's FAB convention: for each box (in the
+ *    order given at creation) a fab covering the valid box grown by `ng`
+ *    cells, component-major: data[((c*nz + k)*ny + j)*nx + i].
+ *  - No entry point allocates device memory except mrb_level_create (and
+ *    mrb_level_destroy, which frees it).
+ */
+#ifndef MARBLES_AMD_H
+#define MARBLES_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRB_OK 0
+#define MRB_E_ARG -1      /* invalid argument */
+#define MRB_E_HIP -2      /* HIP runtime error */
+#define MRB_E_UNSUPPORTED -3
+#define MRB_E_STATE -4    /* call not valid for this level's layout/state */
+
+/* Storage layouts of a level. */
+#define MRB_LAYOUT_FAB 0     /* reference layout: boxes with ng ghost cells */
+#define MRB_LAYOUT_COMPACT 1 /* one box == the domain, no ghosts; fused path */
+
+/* Population storage precision. */
+#define MRB_STORAGE_F64 0         /* FP64 */
+#define MRB_STORAGE_F32_SHIFTED 1 /* FP32 of f - f_eq(rho0, 0, T0) */
+#define MRB_STORAGE_F16_SCALED 2  /* FP16 of (f - shift) / scale (per box) */
+
+/* Boundary types, identical values to Source/BC.H:13-20. */
+#define MRB_BC_PERIODIC 0
+#define MRB_BC_NOSLIP 1
+#define MRB_BC_VELOCITY 2
+#define MRB_BC_PRESSURE 3 /* not supported */
+#define MRB_BC_OUTFLOW 5
+#define MRB_BC_SLIPX 6
+#define MRB_BC_SLIPY 7
+#define MRB_BC_SLIPZ 8
+
+/* Which array mrb_upload / mrb_download address. */
+#define MRB_ARR_F 0
+#define MRB_ARR_G 1
+#define MRB_ARR_ISFLUID 2 /* int32, 2 comps (is_fluid, eb_boundary) */
+
+typedef struct mrb_level mrb_level;
+
+/* Level geometry and storage (Source/LBM.cpp:1148-1199 defines the level
+ * data: f, g with 27 comps and 3 ghosts, is_fluid with 2 comps). */
+typedef struct {
+    int ncell[3];       /* domain size in cells; the domain is [0, ncell-1] */
+    int periodic[3];    /* geometry.is_periodic */
+    double prob_lo[3];  /* geometry.prob_lo */
+    double prob_hi[3];  /* geometry.prob_hi */
+    int nbox;           /* number of boxes */
+    const int *box_lo;  /* nbox*3 valid-box low corners */
+    const int *box_hi;  /* nbox*3 valid-box high corners (inclusive) */
+    int ng;             /* ghost layers (MRB_LAYOUT_FAB); 0 for COMPACT */
+    int layout;         /* MRB_LAYOUT_* */
+    int nlattice;       /* 2 = f and g (thermal model) */
+    int storage;        /* MRB_STORAGE_* */
+} mrb_level_desc;
+
+/* Physics of one level (Source/LBM.cpp:274-298, 639-644, 1047-1077). */
+typedef struct {
+    double nu;                 /* lbm.nu */
+    double alpha;              /* lbm.alpha */
+    double adiabatic_exponent; /* lbm.adiabatic_exponent */
+    double m_bar;              /* lbm.mean_molecular_mass */
+    double dt;                 /* level time step (1 on level 0) */
+    double mesh_speed;         /* lbm.dx_outer / lbm.dt_outer */
+} mrb_phys;
+
+/* Physical boundaries (Source/BC.H:22-63, 474-504; the "constant" velocity
+ * operator Source/VelocityBC.H:235-281 with VelocityBC.cpp:6-21). */
+typedef struct {
+    int bc_type[6];     /* m_bc_type: index idir + lohi*3 */
+    int velbc_kind;     /* 0 = "noop", 1 = "constant" */
+    int vel_dir;        /* velocity_bc_constant.dir */
+    double mach_ref;    /* velocity_bc_constant.Mach_ref */
+    double density;     /* velocity_bc_constant.initial_density */
+    double temperature; /* velocity_bc_constant.initial_temperature */
+    double adiabatic_exponent;
+    double m_bar;
+} mrb_bc;
+
+/* ---- lifetime ----------------------------------------------------------
+ * Replaces the MultiFab definitions of LBM::MakeNewLevelFromScratch
+ * (Source/LBM.cpp:1159-1178).  Allocates A/B population buffers, flags and
+ * scratch.  Synchronous. */
+int mrb_level_create(const mrb_level_desc *desc, mrb_level **out);
+int mrb_level_destroy(mrb_level *lvl);
+int mrb_set_phys(mrb_level *lvl, const mrb_phys *phys);
+int mrb_set_bc(mrb_level *lvl, const mrb_bc *bc);
+/* Reference shift state used by the reduced-precision storages:
+ * rho0, T0 (f_eq and g_eq at rest).  Defaults: 1, 1/3. */
+int mrb_set_shift_state(mrb_level *lvl, double rho0, double T0);
+/* Element count of one box's host fab for `arr`. */
+int64_t mrb_fab_size(const mrb_level *lvl, int arr, int box);
+
+/* ---- data movement (synchronous) ----------------------------------------
+ * Host arrays hold all boxes' fabs back to back (box order of creation). */
+int mrb_upload(mrb_level *lvl, int arr, const void *host);
+int mrb_download(const mrb_level *lvl, int arr, void *host);
+
+/* ---- flags (Source/LBM.cpp:1213-1262; EB sphere through Source/EB.cpp:5-38)
+ * After mrb_upload(MRB_ARR_ISFLUID) or mrb_compute_flags_sphere the level
+ * derives its stream masks.  Synchronous. */
+int mrb_compute_flags_sphere(
+    mrb_level *lvl, const double center[3], double radius, int has_fluid_inside);
+int mrb_finalize_flags(mrb_level *lvl);
+
+/* ---- initial state (Source/IC.H:460-525, IC.cpp:4-45; LBM.cpp:1278-1298)
+ * Host-evaluated (FP64 libm) and uploaded: f = product equilibrium,
+ * g = Grad equilibrium, then zero inside the EB.  Synchronous. */
+int mrb_init_taylorgreen(mrb_level *lvl, double rho0, double v0, const double omega[3]);
+int mrb_init_constant(
+    mrb_level *lvl, double density, const double mach[3], double temperature,
+    double adiabatic_exponent, double m_bar);
+
+/* ---- reference-layout stages (MRB_LAYOUT_FAB) ----------------------------
+ * FillPatchOps::fillpatch at level 0 (Source/FillPatchOps.H:75-132):
+ * FillBoundary across boxes and periodic images, then the physical
+ * boundary functor BCFill::operator() (Source/BC.H:344-471). */
+int mrb_fill_ghosts(mrb_level *lvl, int lattice, void *stream);
+/* FillPatchOps::physbc (Source/FillPatchOps.H:142-159) only. */
+int mrb_physbc(mrb_level *lvl, int lattice, void *stream);
+/* LBM::stream (Source/LBM.cpp:558-604): push-equivalent pull over every
+ * fab cell with halfway bounce-back and the -1 sentinel, then
+ * FillBoundary.  Bit-exact with the reference. */
+int mrb_stream(mrb_level *lvl, int lattice, void *stream);
+/* LBM::collide (Source/LBM.cpp:607-618): f_to_macrodata (810-906),
+ * compute_q_corrections (959-991), macrodata_to_equilibrium (621-762),
+ * relax_f_to_equilibrium (765-807) and its FillBoundary. */
+int mrb_collide(mrb_level *lvl, void *stream);
+
+/* ---- hot path (MRB_LAYOUT_COMPACT) ---------------------------------------
+ * One full single-level step (LBM::evolve body, Source/LBM.cpp:416-420 for
+ * one level): ghost fill, stream f and g, collide -- fused, with periodic
+ * wrap and physical boundaries resolved inside the kernel.  Swaps A/B. */
+int mrb_stream_collide(mrb_level *lvl, void *stream);
+/* nsteps fused steps. */
+int mrb_advance(mrb_level *lvl, int nsteps, void *stream);
+/* nsteps fused steps with HIP events recorded on `stream` around each kernel
+ * of the step; kernel_ms[k] receives the summed duration of kernel slot k
+ * (slots are named by mrb_hot_kernel_name, separated by '+'), *nslots the
+ * number of slots (at most 4).  Synchronous.  Compact layout only. */
+int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_ms, int *nslots);
+/* Stream only (debug/parity; LBM::stream for f and g on the compact
+ * layout, ghosts resolved inline).  Swaps A/B. */
+int mrb_stream_compact(mrb_level *lvl, void *stream);
+
+/* ---- reductions ------------------------------------------------------------
+ * FP64 sums over valid fluid cells: mass, x/y/z momentum, 2*rho*E.
+ * Synchronous (waits for `stream`). */
+int mrb_reduce_conservation(const mrb_level *lvl, double out[5], void *stream);
+
+/* ---- diagnostics ------------------------------------------------------------ */
+const char *mrb_last_error(void);
+/* Name of the kernel variant used by mrb_stream_collide. */
+const char *mrb_hot_kernel_name(const mrb_level *lvl);
+/* Device pointers of the current population buffers (for zero-copy use by
+ * the Python/torch wrappers).  lattice 0 = f, 1 = g. */
+int mrb_population_ptr(const mrb_level *lvl, int lattice, void **dev_ptr, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

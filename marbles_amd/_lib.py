@@ -1,0 +1,126 @@
+"""ctypes binding of libmarbles_amd.so (the C ABI declared in include/marbles_amd.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmarbles_amd.so")
+
+MRB_LAYOUT_FAB = 0
+MRB_LAYOUT_COMPACT = 1
+MRB_STORAGE_F64 = 0
+MRB_STORAGE_F32_SHIFTED = 1
+MRB_STORAGE_F16_SCALED = 2
+MRB_ARR_F = 0
+MRB_ARR_G = 1
+MRB_ARR_ISFLUID = 2
+
+BC_PERIODIC = 0
+BC_NOSLIP = 1
+BC_VELOCITY = 2
+BC_PRESSURE = 3
+BC_OUTFLOW = 5
+BC_SLIPX = 6
+BC_SLIPY = 7
+BC_SLIPZ = 8
+
+
+class LevelDesc(ctypes.Structure):
+    _fields_ = [
+        ("ncell", c_int * 3),
+        ("periodic", c_int * 3),
+        ("prob_lo", c_double * 3),
+        ("prob_hi", c_double * 3),
+        ("nbox", c_int),
+        ("box_lo", POINTER(c_int)),
+        ("box_hi", POINTER(c_int)),
+        ("ng", c_int),
+        ("layout", c_int),
+        ("nlattice", c_int),
+        ("storage", c_int),
+    ]
+
+
+class Phys(ctypes.Structure):
+    _fields_ = [
+        ("nu", c_double),
+        ("alpha", c_double),
+        ("adiabatic_exponent", c_double),
+        ("m_bar", c_double),
+        ("dt", c_double),
+        ("mesh_speed", c_double),
+    ]
+
+
+class BC(ctypes.Structure):
+    _fields_ = [
+        ("bc_type", c_int * 6),
+        ("velbc_kind", c_int),
+        ("vel_dir", c_int),
+        ("mach_ref", c_double),
+        ("density", c_double),
+        ("temperature", c_double),
+        ("adiabatic_exponent", c_double),
+        ("m_bar", c_double),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mrb_level_create": (c_int, [POINTER(LevelDesc), POINTER(c_void_p)]),
+    "mrb_level_destroy": (c_int, [c_void_p]),
+    "mrb_set_phys": (c_int, [c_void_p, POINTER(Phys)]),
+    "mrb_set_bc": (c_int, [c_void_p, POINTER(BC)]),
+    "mrb_set_shift_state": (c_int, [c_void_p, c_double, c_double]),
+    "mrb_fab_size": (c_int64, [c_void_p, c_int, c_int]),
+    "mrb_upload": (c_int, [c_void_p, c_int, c_void_p]),
+    "mrb_download": (c_int, [c_void_p, c_int, c_void_p]),
+    "mrb_compute_flags_sphere": (c_int, [c_void_p, POINTER(c_double), c_double, c_int]),
+    "mrb_finalize_flags": (c_int, [c_void_p]),
+    "mrb_init_taylorgreen": (c_int, [c_void_p, c_double, c_double, POINTER(c_double)]),
+    "mrb_init_constant": (
+        c_int, [c_void_p, c_double, POINTER(c_double), c_double, c_double, c_double]),
+    "mrb_fill_ghosts": (c_int, [c_void_p, c_int, c_void_p]),
+    "mrb_physbc": (c_int, [c_void_p, c_int, c_void_p]),
+    "mrb_stream": (c_int, [c_void_p, c_int, c_void_p]),
+    "mrb_collide": (c_int, [c_void_p, c_void_p]),
+    "mrb_stream_collide": (c_int, [c_void_p, c_void_p]),
+    "mrb_advance": (c_int, [c_void_p, c_int, c_void_p]),
+    "mrb_stream_compact": (c_int, [c_void_p, c_void_p]),
+    "mrb_advance_timed": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_double), POINTER(c_int)]),
+    "mrb_reduce_conservation": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
+    "mrb_last_error": (c_char_p, []),
+    "mrb_hot_kernel_name": (c_char_p, [c_void_p]),
+    "mrb_population_ptr": (c_int, [c_void_p, c_int, POINTER(c_void_p), POINTER(c_int64)]),
+}
+
+_lib = None
+
+
+class MrbError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Loads the HIP engine; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MrbError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().mrb_last_error().decode()
+        raise MrbError(f"marbles_amd error {rc}: {msg}")

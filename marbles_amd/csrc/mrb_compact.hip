@@ -1,0 +1,309 @@
+// Hot path on the compact layout (one box == the domain, no ghost cells):
+// pull-stream with halfway bounce-back, periodic wrap and physical-boundary
+// ghost values resolved inside the kernel, followed by the thermal collide.
+//
+// HBM layout: populations SoA, pop[q*N + cell], cell = (z*ny + y)*nx + x, two
+// buffers (A/B) per lattice; a 32-bit stream mask per cell (bit q = the pull
+// source x - c_q is fluid).
+//
+// The collide needs the Q-correction gradient of post-stream moments at the
+// six face neighbours (LBM.cpp:959-991), so one step is two passes:
+//   k_qpass:   pull 54 populations, moments -> Q_corr (3 doubles per cell)
+//   k_collide: pull 54 populations, moments, dQ from the Q array of the
+//              neighbours, equilibria, relax, write 54 populations to B.
+#include "mrb_internal.h"
+
+namespace mrb {
+namespace {
+
+struct CompactArgs {
+    int nx, ny, nz;
+    int per[3];
+    int64_t N;
+    int bc[6];
+};
+
+// Source coordinates of the pull for one direction component c in
+// {-1,0,1} along a dimension of extent n.  Returns false when the source
+// lies beyond a non-periodic face (a physical-boundary ghost).
+__device__ __forceinline__ void wrap3(int x, int n, int periodic, int s[3], bool ghost[3])
+{
+    // index c+1: source x - c
+    s[0] = x + 1;  // c = -1
+    s[1] = x;      // c = 0
+    s[2] = x - 1;  // c = +1
+    ghost[0] = ghost[1] = ghost[2] = false;
+    if (s[0] >= n) {
+        if (periodic)
+            s[0] -= n;
+        else
+            ghost[0] = true;
+    }
+    if (s[2] < 0) {
+        if (periodic)
+            s[2] += n;
+        else
+            ghost[2] = true;
+    }
+}
+
+// Index (into pop) and constant-value flag of the pulled population q.
+// For a physical-boundary ghost the reference's BCFill value is resolved:
+// the face applied is the last non-periodic face in BCFill's loop order
+// (BC.H:365-372); NOSLIP and SLIP read the cell's own pre-stream
+// populations, OUTFLOW the boundary cell next to the ghost, VELOCITY a
+// constant.
+template <int q>
+__device__ __forceinline__ int64_t pull_index(
+    const CompactArgs &A, int64_t cell, uint32_t mask, const int sx[3], const int sy[3],
+    const int sz[3], const bool gx[3], const bool gy[3], const bool gz[3], bool &use_const)
+{
+    constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+    constexpr int qb = kBounce[q];
+    const bool ghost = gx[cx + 1] || gy[cy + 1] || gz[cz + 1];
+    use_const = false;
+    if (!ghost) {
+        const int64_t src = (int64_t(sz[cz + 1]) * A.ny + sy[cy + 1]) * A.nx + sx[cx + 1];
+        return ((mask >> q) & 1u) ? int64_t(q) * A.N + src : int64_t(qb) * A.N + cell;
+    }
+    int face;
+    if (gz[cz + 1])
+        face = cz > 0 ? 2 : 5;
+    else if (gy[cy + 1])
+        face = cy > 0 ? 1 : 4;
+    else
+        face = cx > 0 ? 0 : 3;
+    const int t = A.bc[face];
+    if (t == MRB_BC_VELOCITY) {
+        use_const = true;
+        return cell;
+    }
+    if (t == MRB_BC_OUTFLOW) {
+        // data(s + n e_d, q): the boundary cell next to the ghost; other
+        // coordinates of the ghost are wrapped (periodic) or, at a corner of
+        // two non-periodic faces, clamped to the domain.
+        int tx = sx[cx + 1], ty = sy[cy + 1], tz = sz[cz + 1];
+        if (gx[cx + 1]) tx = cx > 0 ? 0 : A.nx - 1;
+        if (gy[cy + 1]) ty = cy > 0 ? 0 : A.ny - 1;
+        if (gz[cz + 1]) tz = cz > 0 ? 0 : A.nz - 1;
+        return int64_t(q) * A.N + (int64_t(tz) * A.ny + ty) * A.nx + tx;
+    }
+    if (t == MRB_BC_SLIPX) return int64_t(reflect(q, 0)) * A.N + cell;
+    if (t == MRB_BC_SLIPY) return int64_t(reflect(q, 1)) * A.N + cell;
+    if (t == MRB_BC_SLIPZ) return int64_t(reflect(q, 2)) * A.N + cell;
+    return int64_t(qb) * A.N + cell;  // NOSLIP
+}
+
+template <class T>
+__device__ __forceinline__ void pull_cell(
+    const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
+    int x, int y, int z, int64_t cell, uint32_t mask, double fv[NQ], double gv[NQ])
+{
+    int sx[3], sy[3], sz[3];
+    bool gx[3], gy[3], gz[3];
+    wrap3(x, A.nx, A.per[0], sx, gx);
+    wrap3(y, A.ny, A.per[1], sy, gy);
+    wrap3(z, A.nz, A.per[2], sz, gz);
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        bool use_const;
+        const int64_t idx = pull_index<q>(A, cell, mask, sx, sy, sz, gx, gy, gz, use_const);
+        const double fq = double(f[idx]);
+        const double gq = double(g[idx]);
+        fv[q] = use_const ? bc.fval[q] : fq;
+        gv[q] = use_const ? bc.gval[q] : gq;
+    });
+}
+
+__device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y, int &z, int64_t &cell)
+{
+    x = int(blockIdx.x * blockDim.x + threadIdx.x);
+    y = int(blockIdx.y * blockDim.y + threadIdx.y);
+    z = int(blockIdx.z);
+    if (x >= A.nx || y >= A.ny) return false;
+    cell = (int64_t(z) * A.ny + y) * A.nx + x;
+    return true;
+}
+
+// pass 1: Q_corr of the post-stream state
+__global__ void __launch_bounds__(256) k_qpass(
+    const double *__restrict__ f, const double *__restrict__ g, const uint32_t *__restrict__ mask,
+    double *__restrict__ Q, CompactArgs A, BCDev bc, CollideParams P)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    if (!(m & 1u)) return;
+    double fv[NQ], gv[NQ];
+    pull_cell(f, g, A, bc, x, y, z, cell, m, fv, gv);
+    double rho, jx, jy, jz, E;
+    accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
+    const Macro M = macro_from(rho, jx, jy, jz, E, P);
+    Q[cell] = M.Qx;
+    Q[A.N + cell] = M.Qy;
+    Q[2 * A.N + cell] = M.Qz;
+}
+
+// pass 2: stream + collide, writes the B buffers
+__global__ void __launch_bounds__(256) k_collide(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo,
+    double *__restrict__ go, const uint32_t *__restrict__ mask, const double *__restrict__ Q,
+    CompactArgs A, BCDev bc, CollideParams P)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    if (!(m & 1u)) {
+        // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            fo[int64_t(q) * A.N + cell] = -1.0;
+            go[int64_t(q) * A.N + cell] = -1.0;
+        });
+        return;
+    }
+    double fv[NQ], gv[NQ];
+    pull_cell(f, g, A, bc, x, y, z, cell, m, fv, gv);
+    Moments mo;
+    accumulate_moments(fv, gv, mo);
+    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+    // gradient(): neighbour inside the domain box and fluid
+    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
+    const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
+    const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
+    const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0,
+                                          P.inv_dx[0]);
+    const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
+                                          ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
+    const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
+                                          zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
+    collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
+        constexpr int q = decltype(qc)::value;
+        fo[int64_t(q) * A.N + cell] = fnew;
+        go[int64_t(q) * A.N + cell] = gnew;
+    });
+}
+
+// stream only (parity of the pull with the reference push)
+__global__ void __launch_bounds__(256) k_stream_only(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo,
+    double *__restrict__ go, const uint32_t *__restrict__ mask, CompactArgs A, BCDev bc)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    double fv[NQ], gv[NQ];
+    if (m & 1u) {
+        pull_cell(f, g, A, bc, x, y, z, cell, m, fv, gv);
+    } else {
+        for (int q = 0; q < NQ; ++q) fv[q] = gv[q] = -1.0;
+    }
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        fo[int64_t(q) * A.N + cell] = fv[q];
+        go[int64_t(q) * A.N + cell] = gv[q];
+    });
+}
+
+// stream mask: bit q = fluid(x - c_q); a physical-boundary ghost is fluid
+__global__ void k_build_mask(const int32_t *__restrict__ isf, uint32_t *__restrict__ mask, CompactArgs A)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    int sx[3], sy[3], sz[3];
+    bool gx[3], gy[3], gz[3];
+    wrap3(x, A.nx, A.per[0], sx, gx);
+    wrap3(y, A.ny, A.per[1], sy, gy);
+    wrap3(z, A.nz, A.per[2], sz, gz);
+    uint32_t m = 0;
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+        bool fl = true;
+        if (!(gx[cx + 1] || gy[cy + 1] || gz[cz + 1])) {
+            const int64_t src = (int64_t(sz[cz + 1]) * A.ny + sy[cy + 1]) * A.nx + sx[cx + 1];
+            fl = isf[src] != 0;
+        }
+        if (fl) m |= 1u << q;
+    });
+    // bit 0 is the cell's own is_fluid (c_0 = 0)
+    mask[cell] = m;
+}
+
+CompactArgs make_args(const mrb_level *L)
+{
+    CompactArgs A{};
+    A.nx = L->ncell[0];
+    A.ny = L->ncell[1];
+    A.nz = L->ncell[2];
+    for (int d = 0; d < 3; ++d) A.per[d] = L->periodic[d];
+    A.N = L->total_cells;
+    for (int n = 0; n < 6; ++n) A.bc[n] = L->bc.bc_type[n];
+    return A;
+}
+
+dim3 grid_of(const CompactArgs &A) { return dim3((A.nx + 63) / 64, (A.ny + 3) / 4, A.nz); }
+
+}  // namespace
+
+int compact_build_mask(mrb_level *L, hipStream_t s)
+{
+    const CompactArgs A = make_args(L);
+    k_build_mask<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(L->d_isf, L->d_mask, A);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
+{
+    if (L->storage != MRB_STORAGE_F64)
+        return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
+    const CompactArgs A = make_args(L);
+    const double *f = (const double *)L->pop[0][L->cur];
+    const double *g = (const double *)L->pop[1][L->cur];
+    double *fo = (double *)L->pop[0][1 - L->cur];
+    double *go = (double *)L->pop[1][1 - L->cur];
+    if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
+    k_qpass<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, L->cp);
+    MRB_HIP(hipGetLastError());
+    if (events) MRB_HIP(hipEventRecord(L->ev[1], s));
+    k_collide<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, L->cp);
+    MRB_HIP(hipGetLastError());
+    if (events) MRB_HIP(hipEventRecord(L->ev[2], s));
+    nslots = 2;
+    L->cur = 1 - L->cur;
+    return MRB_OK;
+}
+
+int compact_stream_collide(mrb_level *L, hipStream_t s)
+{
+    int n;
+    return step_impl(L, s, false, n);
+}
+
+int compact_stream_collide_events(mrb_level *L, hipStream_t s, int &nslots)
+{
+    return step_impl(L, s, true, nslots);
+}
+
+int compact_stream_only(mrb_level *L, hipStream_t s)
+{
+    if (L->storage != MRB_STORAGE_F64)
+        return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
+    const CompactArgs A = make_args(L);
+    k_stream_only<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(
+        (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur], (double *)L->pop[0][1 - L->cur],
+        (double *)L->pop[1][1 - L->cur], L->d_mask, A, L->bcdev);
+    MRB_HIP(hipGetLastError());
+    L->cur = 1 - L->cur;
+    return MRB_OK;
+}
+
+const char *compact_kernel_name(const mrb_level *) { return "k_qpass+k_collide"; }
+
+}  // namespace mrb

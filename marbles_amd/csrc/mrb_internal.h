@@ -1,0 +1,122 @@
+// Internal definitions of the marbles_amd engine (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lbm_math.h"
+#include "marbles_amd.h"
+
+namespace mrb {
+
+// One box of a level as the kernels see it.  A box's fab is the valid box
+// grown by ng; every per-box array stores its fab component-major starting
+// at base*ncomp (base = number of fab cells of all earlier boxes).
+struct BoxDev {
+    int lo[3];  // fab low corner (valid low corner - ng)
+    int n[3];   // fab extents
+    int vlo[3], vhi[3];
+    int64_t base;
+    int64_t ncell;
+};
+
+// FillBoundary copy: fab cell (dst_box, dst_local) <- (src_box, src_local).
+struct GhostPair {
+    int32_t dst_box, src_box;
+    int64_t dst_local, src_local;
+};
+
+// Velocity-inlet ghost values and boundary types.
+struct BCDev {
+    int type[6];
+    double fval[NQ];
+    double gval[NQ];
+};
+
+// Geometry of the compact (single box, no ghost) layout.
+struct CompactGeom {
+    int n[3];
+    int periodic[3];
+    int64_t N;  // cells
+};
+
+// Bits of the per-cell stream mask of the compact layout.
+//   bit q (0..26): the pull source x - c_q is fluid (a ghost beyond a
+//                  physical boundary counts as fluid); bit 0 = own fluid.
+//   kMaskGeneral:  some pull source needs a periodic wrap or is a ghost.
+constexpr uint32_t kMaskAllFluid = (1u << NQ) - 1u;
+constexpr uint32_t kMaskGeneral = 1u << 27;
+
+enum class Storage { F64 = 0, F32S = 1, F16S = 2 };
+
+}  // namespace mrb
+
+struct mrb_level {
+    // geometry
+    int ncell[3];
+    int periodic[3];
+    double prob_lo[3], prob_hi[3], dx[3], inv_dx[3];
+    int nbox = 0;
+    int ng = 0;
+    int layout = MRB_LAYOUT_FAB;
+    int nlattice = 2;
+    int storage = MRB_STORAGE_F64;
+    std::vector<mrb::BoxDev> boxes;  // host copy
+    int64_t total_cells = 0;         // fab cells over all boxes
+    // device memory
+    mrb::BoxDev *d_boxes = nullptr;
+    int64_t *d_box_base = nullptr;  // nbox+1 prefix sums (for lookups)
+    void *pop[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [lattice][A/B]
+    int cur = 0;                      // index of the current buffer
+    int32_t *d_isf = nullptr;         // is_fluid, 2 comps
+    uint32_t *d_mask = nullptr;       // compact: stream mask per cell
+    double *d_md = nullptr;           // FAB: macrodata, 19 comps
+    double *d_dq = nullptr;           // FAB: dQ, 3 comps
+    double *d_q = nullptr;            // compact: Q_corr, 3 comps
+    double *d_red = nullptr;          // reduction partials
+    double *d_small = nullptr;        // [fshift | gshift | init values] (4*NQ)
+    mrb::GhostPair *d_gmap = nullptr;   // FillBoundary map, ng layers
+    int64_t n_gmap = 0;
+    mrb::GhostPair *d_gmap1 = nullptr;  // FillBoundary map, 1 layer (macrodata)
+    int64_t n_gmap1 = 0;
+    int64_t *d_bcells = nullptr;        // fab cells outside non-periodic faces
+    int64_t n_bcells = 0;
+    // physics
+    mrb::CollideParams cp{};
+    mrb_phys phys{};
+    mrb_bc bc{};
+    mrb::BCDev bcdev{};
+    bool phys_set = false;
+    bool flags_ready = false;
+    double shift_rho0 = 1.0, shift_T0 = 1.0 / 3.0;
+    double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
+    // timing events (created with the level)
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace mrb {
+
+int set_error(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define MRB_HIP(call)                                       \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return ::mrb::hip_fail(e_, #call); \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// compact-layout kernels (mrb_compact.hip)
+int compact_stream_collide(mrb_level *L, hipStream_t s);
+// one step with an event recorded before kernel slot k (ev[k]) and after the
+// last slot; returns the number of slots
+int compact_stream_collide_events(mrb_level *L, hipStream_t s, int &nslots);
+int compact_stream_only(mrb_level *L, hipStream_t s);
+int compact_build_mask(mrb_level *L, hipStream_t s);
+const char *compact_kernel_name(const mrb_level *L);
+
+}  // namespace mrb

@@ -1,0 +1,1251 @@
+// Level object, data movement, flags, initial state, and the reference-layout
+// (MRB_LAYOUT_FAB) stages of the hot path: ghost fill, stream, collide.
+//
+// The FAB-layout kernels restate the reference stages one to one so that
+// every stage can be checked bit for bit against the reference CPU path;
+// the fused hot path lives in mrb_compact.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <thread>
+
+#include "mrb_internal.h"
+
+namespace mrb {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return set_error(MRB_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(int64_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+
+// global fab cell -> (box, local index) by binary search on the prefix sums
+__device__ __forceinline__ int find_box(const int64_t *__restrict__ base, int nbox, int64_t c)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= c)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void local_to_ijk(const BoxDev &B, int64_t loc, int &i, int &j, int &k)
+{
+    const int64_t nxy = int64_t(B.n[0]) * B.n[1];
+    k = int(loc / nxy);
+    const int64_t r = loc - int64_t(k) * nxy;
+    j = int(r / B.n[0]);
+    i = int(r - int64_t(j) * B.n[0]);
+    i += B.lo[0];
+    j += B.lo[1];
+    k += B.lo[2];
+}
+
+__device__ __forceinline__ bool in_fab(const BoxDev &B, int i, int j, int k)
+{
+    return i >= B.lo[0] && i < B.lo[0] + B.n[0] && j >= B.lo[1] && j < B.lo[1] + B.n[1] &&
+           k >= B.lo[2] && k < B.lo[2] + B.n[2];
+}
+
+__device__ __forceinline__ bool in_valid(const BoxDev &B, int i, int j, int k)
+{
+    return i >= B.vlo[0] && i <= B.vhi[0] && j >= B.vlo[1] && j <= B.vhi[1] && k >= B.vlo[2] &&
+           k <= B.vhi[2];
+}
+
+__device__ __forceinline__ int64_t loc_of(const BoxDev &B, int i, int j, int k)
+{
+    return (int64_t(k - B.lo[2]) * B.n[1] + (j - B.lo[1])) * B.n[0] + (i - B.lo[0]);
+}
+
+// ---------------------------------------------------------------------------
+// FillBoundary: ghost <- owner valid cell, all components
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_fill_map(
+    T *__restrict__ arr, int ncomp, const BoxDev *__restrict__ boxes,
+    const GhostPair *__restrict__ map, int64_t nmap)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nmap) return;
+    const GhostPair p = map[t];
+    const BoxDev &D = boxes[p.dst_box];
+    const BoxDev &S = boxes[p.src_box];
+    T *dst = arr + D.base * ncomp + p.dst_local;
+    const T *src = arr + S.base * ncomp + p.src_local;
+    for (int c = 0; c < ncomp; ++c) dst[c * D.ncell] = src[c * S.ncell];
+}
+
+// ---------------------------------------------------------------------------
+// LBM::stream (LBM.cpp:558-604) as a pull over every fab cell y:
+//   y fluid and y-c_p inside the fab:  fluid(y-c_p) ? f_p(y-c_p) : f_pbar(y)
+//   otherwise:                         -1
+// which is exactly the value the reference's push leaves in its -1
+// pre-filled temporary (each (y,p) has at most one writer).
+// ---------------------------------------------------------------------------
+__global__ void k_stream_fab(
+    const double *__restrict__ fin, double *__restrict__ fout, const int32_t *__restrict__ isf,
+    const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int i, j, k;
+    local_to_ijk(B, loc, i, j, k);
+    const double *f = fin + B.base * NQ;
+    double *fo = fout + B.base * NQ;
+    const int32_t *fl = isf + B.base * 2;
+    const bool fluid = fl[loc] == 1;
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int a = i - ev(q, 0), bb = j - ev(q, 1), e = k - ev(q, 2);
+        double v = -1.0;
+        if (fluid && in_fab(B, a, bb, e)) {
+            const int64_t sl = loc_of(B, a, bb, e);
+            v = fl[sl] != 0 ? f[q * B.ncell + sl] : f[kBounce[q] * B.ncell + loc];
+        }
+        fo[q * B.ncell + loc] = v;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Physical boundary functor BCFill::operator() (BC.H:344-471) for the listed
+// fab cells (outside the domain in a non-periodic direction).
+// ---------------------------------------------------------------------------
+__global__ void k_physbc_fab(
+    double *__restrict__ arr, const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base,
+    int nbox, const int64_t *__restrict__ cells, int64_t ncells, int dlo0, int dlo1, int dlo2,
+    int dhi0, int dhi1, int dhi2, BCDev bc, int is_energy)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    const int64_t c = cells[t];
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int iv[3];
+    local_to_ijk(B, loc, iv[0], iv[1], iv[2]);
+    double *data = arr + B.base * NQ;
+    const int domlo[3] = {dlo0, dlo1, dlo2};
+    const int domhi[3] = {dhi0, dhi1, dhi2};
+    const int normal_dirs[2] = {1, -1};
+    for (int idir = 0; idir < 3; ++idir) {
+        for (int lohi = 0; lohi < 2; ++lohi) {
+            const int normal_dir = normal_dirs[lohi];
+            const int bc_idx = idir + lohi * 3;
+            if (!((lohi == 0 && iv[idir] < domlo[idir]) || (lohi == 1 && iv[idir] > domhi[idir])))
+                continue;
+            const int t_bc = bc.type[bc_idx];
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const int n0 = iv[0] + ev(q, 0), n1 = iv[1] + ev(q, 1), n2 = iv[2] + ev(q, 2);
+                double *dq = data + q * B.ncell + loc;
+                if (in_valid(B, n0, n1, n2)) {
+                    if (t_bc == MRB_BC_NOSLIP) {
+                        *dq = data[kBounce[q] * B.ncell + loc_of(B, n0, n1, n2)];
+                    } else if (t_bc == MRB_BC_VELOCITY) {
+                        *dq = is_energy ? bc.gval[q] : bc.fval[q];
+                    } else if (t_bc == MRB_BC_OUTFLOW) {
+                        int s[3] = {iv[0], iv[1], iv[2]};
+                        s[idir] += normal_dir;
+                        *dq = data[q * B.ncell + loc_of(B, s[0], s[1], s[2])];
+                    } else if (t_bc == MRB_BC_SLIPX) {
+                        *dq = data[reflect(q, 0) * B.ncell + loc_of(B, n0, n1, n2)];
+                    } else if (t_bc == MRB_BC_SLIPY) {
+                        *dq = data[reflect(q, 1) * B.ncell + loc_of(B, n0, n1, n2)];
+                    } else if (t_bc == MRB_BC_SLIPZ) {
+                        *dq = data[reflect(q, 2) * B.ncell + loc_of(B, n0, n1, n2)];
+                    }
+                } else {
+                    *dq = -1.0;
+                }
+            });
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LBM::f_to_macrodata (LBM.cpp:810-906) over valid + 1 ghost
+// ---------------------------------------------------------------------------
+__global__ void k_moments_fab(
+    const double *__restrict__ fin, const double *__restrict__ gin,
+    const int32_t *__restrict__ isf, double *__restrict__ md, const BoxDev *__restrict__ boxes,
+    const int64_t *__restrict__ base, int nbox, int64_t total, CollideParams P)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int i, j, k;
+    local_to_ijk(B, loc, i, j, k);
+    if (i < B.vlo[0] - 1 || i > B.vhi[0] + 1 || j < B.vlo[1] - 1 || j > B.vhi[1] + 1 ||
+        k < B.vlo[2] - 1 || k > B.vhi[2] + 1)
+        return;
+    if (isf[B.base * 2 + loc] != 1) return;
+    double fv[NQ], gv[NQ];
+    const double *f = fin + B.base * NQ + loc;
+    const double *g = gin + B.base * NQ + loc;
+    for (int q = 0; q < NQ; ++q) {
+        fv[q] = f[q * B.ncell];
+        gv[q] = g[q * B.ncell];
+    }
+    Moments m;
+    accumulate_moments(fv, gv, m);
+    const Macro M = macro_from(m.rho, m.jx, m.jy, m.jz, m.E, P);
+    double *o = md + B.base * 19 + loc;
+    const int64_t s = B.ncell;
+    o[0 * s] = m.rho;
+    o[1 * s] = M.u;
+    o[2 * s] = M.v;
+    o[3 * s] = M.w;
+    o[4 * s] = sqrt(M.u * M.u + M.v * M.v + M.w * M.w);
+    o[5 * s] = m.E;
+    o[6 * s] = M.Qx;
+    o[7 * s] = M.Qy;
+    o[8 * s] = M.Qz;
+    o[9 * s] = m.pxx;
+    o[10 * s] = m.pyy;
+    o[11 * s] = m.pzz;
+    o[12 * s] = m.pxy;
+    o[13 * s] = m.pxz;
+    o[14 * s] = m.pyz;
+    o[15 * s] = m.qx;
+    o[16 * s] = m.qy;
+    o[17 * s] = m.qz;
+    o[18 * s] = M.T;
+}
+
+// ---------------------------------------------------------------------------
+// LBM::compute_q_corrections (LBM.cpp:959-991), valid cells
+// ---------------------------------------------------------------------------
+__global__ void k_dq_fab(
+    const double *__restrict__ md, const int32_t *__restrict__ isf, double *__restrict__ dq,
+    const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
+    int n0, int n1, int n2, CollideParams P)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int iv[3];
+    local_to_ijk(B, loc, iv[0], iv[1], iv[2]);
+    if (!in_valid(B, iv[0], iv[1], iv[2])) return;
+    const int32_t *fl = isf + B.base * 2;
+    if (fl[loc] != 1) return;
+    const double *m = md + B.base * 19;
+    const int nd[3] = {n0, n1, n2};
+    for (int d = 0; d < 3; ++d) {
+        int p[3] = {iv[0], iv[1], iv[2]}, q[3] = {iv[0], iv[1], iv[2]};
+        p[d] += 1;
+        q[d] -= 1;
+        const bool p_ok = p[d] >= 0 && p[d] < nd[d] && p[(d + 1) % 3] >= 0 &&
+                          p[(d + 1) % 3] < nd[(d + 1) % 3] && p[(d + 2) % 3] >= 0 &&
+                          p[(d + 2) % 3] < nd[(d + 2) % 3] &&
+                          fl[loc_of(B, p[0], p[1], p[2])] == 1;
+        const bool m_ok = q[d] >= 0 && q[d] < nd[d] && q[(d + 1) % 3] >= 0 &&
+                          q[(d + 1) % 3] < nd[(d + 1) % 3] && q[(d + 2) % 3] >= 0 &&
+                          q[(d + 2) % 3] < nd[(d + 2) % 3] &&
+                          fl[loc_of(B, q[0], q[1], q[2])] == 1;
+        const int comp = 6 + d;
+        const double qp = p_ok ? m[comp * B.ncell + loc_of(B, p[0], p[1], p[2])] : 0.0;
+        const double qm = m_ok ? m[comp * B.ncell + loc_of(B, q[0], q[1], q[2])] : 0.0;
+        const double qcv = m[comp * B.ncell + loc];
+        dq[B.base * 3 + d * B.ncell + loc] = one_sided_gradient(p_ok, m_ok, qp, qcv, qm, P.inv_dx[d]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// macrodata_to_equilibrium + relax_f_to_equilibrium (LBM.cpp:621-807),
+// valid fluid cells, in place
+// ---------------------------------------------------------------------------
+__global__ void k_eqrelax_fab(
+    double *__restrict__ f, double *__restrict__ g, const double *__restrict__ md,
+    const double *__restrict__ dq, const int32_t *__restrict__ isf,
+    const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
+    CollideParams P)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int i, j, k;
+    local_to_ijk(B, loc, i, j, k);
+    if (!in_valid(B, i, j, k)) return;
+    if (isf[B.base * 2 + loc] != 1) return;
+    const int64_t s = B.ncell;
+    const double *m = md + B.base * 19 + loc;
+    Moments mo;
+    mo.rho = m[0];
+    mo.E = m[5 * s];
+    mo.pxx = m[9 * s];
+    mo.pyy = m[10 * s];
+    mo.pzz = m[11 * s];
+    mo.pxy = m[12 * s];
+    mo.pxz = m[13 * s];
+    mo.pyz = m[14 * s];
+    mo.qx = m[15 * s];
+    mo.qy = m[16 * s];
+    mo.qz = m[17 * s];
+    Macro M;
+    M.u = m[1 * s];
+    M.v = m[2 * s];
+    M.w = m[3 * s];
+    M.T = m[18 * s];
+    const double *d = dq + B.base * 3 + loc;
+    double *fp = f + B.base * NQ + loc;
+    double *gp = g + B.base * NQ + loc;
+    double fv[NQ], gv[NQ];
+    for (int q = 0; q < NQ; ++q) {
+        fv[q] = fp[q * s];
+        gv[q] = gp[q * s];
+    }
+    collide_cell(fv, gv, mo, M, d[0], d[s], d[2 * s], P, [&](auto qc, double fo, double go) {
+        constexpr int q = decltype(qc)::value;
+        fp[q * s] = fo;
+        gp[q * s] = go;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// flags: EB sphere classification, eb_boundary ring (LBM.cpp:1213-1262)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool sphere_covered(
+    double x0, double y0, double z0, double dx, double dy, double dz, double cx, double cy,
+    double cz, double r, int fluid_inside)
+{
+    const double r2 = r * r;
+    for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 2; ++c)
+            for (int e = 0; e < 2; ++e) {
+                const double x = x0 + a * dx - cx;
+                const double y = y0 + c * dy - cy;
+                const double z = z0 + e * dz - cz;
+                const double d2 = x * x + y * y + z * z;
+                const bool in_body = fluid_inside ? (d2 >= r2) : (d2 <= r2);
+                if (!in_body) return false;
+            }
+    return true;
+}
+
+struct SphereArgs {
+    double plo[3], dx[3], c[3], r;
+    int fluid_inside;
+};
+
+__global__ void k_flags_sphere(
+    int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base,
+    int nbox, int64_t total, SphereArgs S)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int i, j, k;
+    local_to_ijk(B, loc, i, j, k);
+    const bool cov = sphere_covered(
+        S.plo[0] + i * S.dx[0], S.plo[1] + j * S.dx[1], S.plo[2] + k * S.dx[2], S.dx[0], S.dx[1],
+        S.dx[2], S.c[0], S.c[1], S.c[2], S.r, S.fluid_inside);
+    isf[B.base * 2 + loc] = cov ? 0 : 1;
+    isf[B.base * 2 + B.ncell + loc] = 0;
+}
+
+// comp1 over the fab shrunk by one cell (LBM.cpp:1237-1259)
+__global__ void k_flags_ring(
+    int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base,
+    int nbox, int64_t total)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int iv[3];
+    local_to_ijk(B, loc, iv[0], iv[1], iv[2]);
+    for (int d = 0; d < 3; ++d)
+        if (iv[d] <= B.lo[d] || iv[d] >= B.lo[d] + B.n[d] - 1) return;
+    int32_t *fl = isf + B.base * 2;
+    bool all_covered = true;
+    for (int d = 0; d < 3; ++d) {
+        int m[3] = {iv[0], iv[1], iv[2]}, p[3] = {iv[0], iv[1], iv[2]};
+        m[d] -= 1;
+        p[d] += 1;
+        all_covered &= (fl[loc_of(B, m[0], m[1], m[2])] == 0) && (fl[loc_of(B, p[0], p[1], p[2])] == 0);
+    }
+    fl[B.ncell + loc] = (all_covered || fl[loc] == 1) ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// conservation sums over valid fluid cells (FP64), two passes
+// ---------------------------------------------------------------------------
+template <class PopT>
+__global__ void k_conservation(
+    const PopT *__restrict__ f, const PopT *__restrict__ g, const int32_t *__restrict__ isf,
+    const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
+    const double *__restrict__ fshift, const double *__restrict__ gshift, double *__restrict__ part)
+{
+    __shared__ double sh[5][kBlock];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < total;
+         c += int64_t(gridDim.x) * blockDim.x) {
+        const int b = find_box(base, nbox, c);
+        const BoxDev B = boxes[b];
+        const int64_t loc = c - B.base;
+        int i, j, k;
+        local_to_ijk(B, loc, i, j, k);
+        if (!in_valid(B, i, j, k)) continue;
+        if (isf[B.base * 2 + loc] != 1) continue;
+        const PopT *fp = f + B.base * NQ + loc;
+        const PopT *gp = g + B.base * NQ + loc;
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const double fq = double(fp[q * B.ncell]) + fshift[q];
+            acc[0] += fq;
+            if constexpr (ev(q, 0) != 0) acc[1] += ev(q, 0) * fq;
+            if constexpr (ev(q, 1) != 0) acc[2] += ev(q, 1) * fq;
+            if constexpr (ev(q, 2) != 0) acc[3] += ev(q, 2) * fq;
+            acc[4] += double(gp[q * B.ncell]) + gshift[q];
+        });
+    }
+    for (int n = 0; n < 5; ++n) sh[n][threadIdx.x] = acc[n];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int n = 0; n < 5; ++n) sh[n][threadIdx.x] += sh[n][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int n = 0; n < 5; ++n) part[blockIdx.x * 5 + n] = sh[n][0];
+}
+
+__global__ void k_fill_const(double *__restrict__ a, int64_t n, double v)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// uniform initial state: comp q of every cell = val[q]; solid cells 0
+__global__ void k_init_uniform(
+    double *__restrict__ pop, const int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes,
+    const int64_t *__restrict__ base, int nbox, int64_t total, const double *__restrict__ val)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    const bool solid = isf[B.base * 2 + loc] == 0;
+    for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * B.ncell + loc] = solid ? 0.0 : val[q];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host-side helpers
+// ---------------------------------------------------------------------------
+
+// Fills `map` with every ghost cell of every box (within `depth` layers) that
+// AMReX FillBoundary would overwrite: its periodic image lies inside the
+// domain and in some box's valid region.
+static void build_ghost_map(const mrb_level *L, int depth, std::vector<GhostPair> &map)
+{
+    const int64_t nd0 = L->ncell[0], nd1 = L->ncell[1], nd2 = L->ncell[2];
+    std::vector<int32_t> owner(size_t(nd0 * nd1 * nd2), -1);
+    for (int b = 0; b < L->nbox; ++b) {
+        const BoxDev &B = L->boxes[b];
+        for (int k = B.vlo[2]; k <= B.vhi[2]; ++k)
+            for (int j = B.vlo[1]; j <= B.vhi[1]; ++j)
+                for (int i = B.vlo[0]; i <= B.vhi[0]; ++i) owner[size_t((k * nd1 + j) * nd0 + i)] = b;
+    }
+    map.clear();
+    for (int b = 0; b < L->nbox; ++b) {
+        const BoxDev &B = L->boxes[b];
+        for (int k = B.vlo[2] - depth; k <= B.vhi[2] + depth; ++k)
+            for (int j = B.vlo[1] - depth; j <= B.vhi[1] + depth; ++j)
+                for (int i = B.vlo[0] - depth; i <= B.vhi[0] + depth; ++i) {
+                    if (i >= B.vlo[0] && i <= B.vhi[0] && j >= B.vlo[1] && j <= B.vhi[1] &&
+                        k >= B.vlo[2] && k <= B.vhi[2])
+                        continue;
+                    int s[3] = {i, j, k};
+                    bool ok = true;
+                    for (int d = 0; d < 3; ++d) {
+                        if (s[d] < 0) {
+                            if (!L->periodic[d]) ok = false;
+                            s[d] += L->ncell[d];
+                        } else if (s[d] >= L->ncell[d]) {
+                            if (!L->periodic[d]) ok = false;
+                            s[d] -= L->ncell[d];
+                        }
+                    }
+                    if (!ok) continue;
+                    const int ob = owner[size_t((int64_t(s[2]) * nd1 + s[1]) * nd0 + s[0])];
+                    if (ob < 0) continue;
+                    const BoxDev &S = L->boxes[ob];
+                    GhostPair p;
+                    p.dst_box = b;
+                    p.src_box = ob;
+                    p.dst_local = (int64_t(k - B.lo[2]) * B.n[1] + (j - B.lo[1])) * B.n[0] + (i - B.lo[0]);
+                    p.src_local =
+                        (int64_t(s[2] - S.lo[2]) * S.n[1] + (s[1] - S.lo[1])) * S.n[0] + (s[0] - S.lo[0]);
+                    map.push_back(p);
+                }
+    }
+}
+
+static int fill_boundary(mrb_level *L, void *arr, int ncomp, bool is_int, bool one_layer, hipStream_t s)
+{
+    const GhostPair *map = one_layer ? L->d_gmap1 : L->d_gmap;
+    const int64_t n = one_layer ? L->n_gmap1 : L->n_gmap;
+    if (n == 0) return MRB_OK;
+    if (is_int)
+        k_fill_map<int32_t><<<nblocks(n), kBlock, 0, s>>>((int32_t *)arr, ncomp, L->d_boxes, map, n);
+    else
+        k_fill_map<double><<<nblocks(n), kBlock, 0, s>>>((double *)arr, ncomp, L->d_boxes, map, n);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+static void compute_bc_values(mrb_level *L)
+{
+    BCDev &d = L->bcdev;
+    for (int n = 0; n < 6; ++n) d.type[n] = L->bc.bc_type[n];
+    // BCFill::operator() (BC.H:395-419): R, T, gamma preset, then the
+    // velocity operator overrides them (VelocityBC.H:253-271).
+    double vel[3] = {0.0, 0.0, 0.0};
+    double R = 1.0, T = 1.0 / 3.0, gamma = 5.0 / 3.0, rho = 0.0;
+    if (L->bc.velbc_kind == 1) {
+        const double sos = std::sqrt(
+            L->bc.adiabatic_exponent * (kAirMolarMass / L->bc.m_bar) * L->bc.temperature);
+        const double u0 = L->bc.mach_ref * sos;
+        rho = L->bc.density;
+        vel[L->bc.vel_dir] = u0;
+        R = (kAirMolarMass / L->bc.m_bar);
+        T = L->bc.temperature;
+        gamma = L->bc.adiabatic_exponent;
+    }
+    for (int q = 0; q < NQ; ++q) {
+        d.fval[q] = set_equilibrium_value(rho, vel, R * T, q);
+        d.gval[q] = thermal_g_equilibrium(rho, vel, T, R, gamma, q);
+    }
+}
+
+static void compute_shift(mrb_level *L)
+{
+    // f_eq and g_eq of the rest state (rho0, u = 0, T0) with R and gamma of
+    // the level physics: the subtracted reference of the shifted storages.
+    const double R = kAirMolarMass / (L->phys_set ? L->phys.m_bar : kAirMolarMass);
+    const double gamma = L->phys_set ? L->phys.adiabatic_exponent : 5.0 / 3.0;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < NQ; ++q) {
+        L->fshift[q] = set_equilibrium_value(L->shift_rho0, zero, R * L->shift_T0, q);
+        L->gshift[q] = thermal_g_equilibrium(L->shift_rho0, zero, L->shift_T0, R, gamma, q);
+    }
+    if (L->d_small) {
+        double tab[2 * NQ];
+        for (int q = 0; q < NQ; ++q) {
+            tab[q] = L->storage == MRB_STORAGE_F64 ? 0.0 : L->fshift[q];
+            tab[NQ + q] = L->storage == MRB_STORAGE_F64 ? 0.0 : L->gshift[q];
+        }
+        (void)hipMemcpy(L->d_small, tab, sizeof(tab), hipMemcpyHostToDevice);
+    }
+}
+
+static size_t pop_elem_size(int storage)
+{
+    return storage == MRB_STORAGE_F64 ? 8 : storage == MRB_STORAGE_F32_SHIFTED ? 4 : 2;
+}
+
+}  // namespace mrb
+
+using namespace mrb;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *mrb_last_error(void) { return g_err.c_str(); }
+
+int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
+{
+    if (!desc || !out) return set_error(MRB_E_ARG, "null argument");
+    *out = nullptr;
+    if (desc->nbox < 1 || !desc->box_lo || !desc->box_hi)
+        return set_error(MRB_E_ARG, "level needs at least one box");
+    if (desc->nlattice != 2) return set_error(MRB_E_UNSUPPORTED, "nlattice must be 2 (f and g)");
+    if (desc->layout != MRB_LAYOUT_FAB && desc->layout != MRB_LAYOUT_COMPACT)
+        return set_error(MRB_E_ARG, "unknown layout");
+    if (desc->storage != MRB_STORAGE_F64 && desc->layout == MRB_LAYOUT_FAB)
+        return set_error(MRB_E_UNSUPPORTED, "reduced-precision storage needs the compact layout");
+    for (int d = 0; d < 3; ++d)
+        if (desc->ncell[d] < 1) return set_error(MRB_E_ARG, "ncell must be positive");
+    std::unique_ptr<mrb_level> L(new mrb_level());
+    for (int d = 0; d < 3; ++d) {
+        L->ncell[d] = desc->ncell[d];
+        L->periodic[d] = desc->periodic[d] ? 1 : 0;
+        L->prob_lo[d] = desc->prob_lo[d];
+        L->prob_hi[d] = desc->prob_hi[d];
+        L->dx[d] = (desc->prob_hi[d] - desc->prob_lo[d]) / desc->ncell[d];
+        L->inv_dx[d] = desc->ncell[d] / (desc->prob_hi[d] - desc->prob_lo[d]);
+    }
+    L->nbox = desc->nbox;
+    L->ng = desc->ng;
+    L->layout = desc->layout;
+    L->nlattice = desc->nlattice;
+    L->storage = desc->storage;
+    if (L->layout == MRB_LAYOUT_COMPACT) {
+        if (desc->nbox != 1 || desc->ng != 0)
+            return set_error(MRB_E_ARG, "compact layout is one box without ghost cells");
+        for (int d = 0; d < 3; ++d)
+            if (desc->box_lo[d] != 0 || desc->box_hi[d] != desc->ncell[d] - 1)
+                return set_error(MRB_E_ARG, "compact layout box must equal the domain");
+        if (int64_t(desc->ncell[0]) * desc->ncell[1] * desc->ncell[2] >= (int64_t(1) << 31))
+            return set_error(MRB_E_ARG, "compact layout limited to 2^31 cells");
+    } else if (desc->ng < 1) {
+        return set_error(MRB_E_ARG, "fab layout needs ng >= 1");
+    }
+    // boxes
+    int64_t base = 0;
+    std::vector<int64_t> bases;
+    for (int b = 0; b < desc->nbox; ++b) {
+        BoxDev B{};
+        B.ncell = 1;
+        for (int d = 0; d < 3; ++d) {
+            B.vlo[d] = desc->box_lo[3 * b + d];
+            B.vhi[d] = desc->box_hi[3 * b + d];
+            if (B.vhi[d] < B.vlo[d] || B.vlo[d] < 0 || B.vhi[d] >= desc->ncell[d])
+                return set_error(MRB_E_ARG, "box outside the domain");
+            B.lo[d] = B.vlo[d] - L->ng;
+            B.n[d] = B.vhi[d] - B.vlo[d] + 1 + 2 * L->ng;
+            B.ncell *= B.n[d];
+        }
+        B.base = base;
+        bases.push_back(base);
+        base += B.ncell;
+        L->boxes.push_back(B);
+    }
+    bases.push_back(base);
+    L->total_cells = base;
+    const int64_t T = base;
+    const size_t es = pop_elem_size(L->storage);
+    MRB_HIP(hipMalloc(&L->d_boxes, sizeof(BoxDev) * L->nbox));
+    MRB_HIP(hipMemcpy(L->d_boxes, L->boxes.data(), sizeof(BoxDev) * L->nbox, hipMemcpyHostToDevice));
+    MRB_HIP(hipMalloc(&L->d_box_base, sizeof(int64_t) * (L->nbox + 1)));
+    MRB_HIP(hipMemcpy(L->d_box_base, bases.data(), sizeof(int64_t) * (L->nbox + 1), hipMemcpyHostToDevice));
+    for (int l = 0; l < 2; ++l)
+        for (int ab = 0; ab < 2; ++ab) {
+            MRB_HIP(hipMalloc(&L->pop[l][ab], es * NQ * T));
+            MRB_HIP(hipMemset(L->pop[l][ab], 0, es * NQ * T));
+        }
+    MRB_HIP(hipMalloc(&L->d_isf, sizeof(int32_t) * 2 * T));
+    MRB_HIP(hipMemset(L->d_isf, 0, sizeof(int32_t) * 2 * T));
+    MRB_HIP(hipMalloc(&L->d_red, sizeof(double) * 5 * 4096));
+    MRB_HIP(hipMalloc(&L->d_small, sizeof(double) * 4 * NQ));
+    for (auto &e : L->ev) MRB_HIP(hipEventCreate(&e));
+    MRB_HIP(hipMemset(L->d_small, 0, sizeof(double) * 4 * NQ));
+    if (L->layout == MRB_LAYOUT_FAB) {
+        MRB_HIP(hipMalloc(&L->d_md, sizeof(double) * 19 * T));
+        MRB_HIP(hipMemset(L->d_md, 0, sizeof(double) * 19 * T));
+        MRB_HIP(hipMalloc(&L->d_dq, sizeof(double) * 3 * T));
+        MRB_HIP(hipMemset(L->d_dq, 0, sizeof(double) * 3 * T));
+        std::vector<GhostPair> map;
+        build_ghost_map(L.get(), L->ng, map);
+        L->n_gmap = int64_t(map.size());
+        if (!map.empty()) {
+            MRB_HIP(hipMalloc(&L->d_gmap, sizeof(GhostPair) * map.size()));
+            MRB_HIP(hipMemcpy(L->d_gmap, map.data(), sizeof(GhostPair) * map.size(), hipMemcpyHostToDevice));
+        }
+        build_ghost_map(L.get(), 1, map);
+        L->n_gmap1 = int64_t(map.size());
+        if (!map.empty()) {
+            MRB_HIP(hipMalloc(&L->d_gmap1, sizeof(GhostPair) * map.size()));
+            MRB_HIP(hipMemcpy(L->d_gmap1, map.data(), sizeof(GhostPair) * map.size(), hipMemcpyHostToDevice));
+        }
+        // fab cells outside a non-periodic domain face
+        std::vector<int64_t> cells;
+        for (int b = 0; b < L->nbox; ++b) {
+            const BoxDev &B = L->boxes[b];
+            for (int k = 0; k < B.n[2]; ++k)
+                for (int j = 0; j < B.n[1]; ++j)
+                    for (int i = 0; i < B.n[0]; ++i) {
+                        const int iv[3] = {B.lo[0] + i, B.lo[1] + j, B.lo[2] + k};
+                        bool outside = false;
+                        for (int d = 0; d < 3; ++d)
+                            if (!L->periodic[d] && (iv[d] < 0 || iv[d] >= L->ncell[d])) outside = true;
+                        if (outside) cells.push_back(B.base + (int64_t(k) * B.n[1] + j) * B.n[0] + i);
+                    }
+        }
+        L->n_bcells = int64_t(cells.size());
+        if (!cells.empty()) {
+            MRB_HIP(hipMalloc(&L->d_bcells, sizeof(int64_t) * cells.size()));
+            MRB_HIP(hipMemcpy(L->d_bcells, cells.data(), sizeof(int64_t) * cells.size(), hipMemcpyHostToDevice));
+        }
+    } else {
+        MRB_HIP(hipMalloc(&L->d_mask, sizeof(uint32_t) * T));
+        MRB_HIP(hipMemset(L->d_mask, 0, sizeof(uint32_t) * T));
+        MRB_HIP(hipMalloc(&L->d_q, sizeof(double) * 3 * T));
+        MRB_HIP(hipMemset(L->d_q, 0, sizeof(double) * 3 * T));
+    }
+    // defaults (LBM.H:272-318)
+    mrb_phys p{};
+    p.nu = 1.0;
+    p.alpha = 1.0;
+    p.adiabatic_exponent = 5.0 / 3.0;
+    p.m_bar = kAirMolarMass;
+    p.dt = 1.0;
+    p.mesh_speed = 1.0;
+    L->phys = p;
+    for (int n = 0; n < 6; ++n) L->bc.bc_type[n] = MRB_BC_PERIODIC;
+    L->bc.velbc_kind = 0;
+    L->bc.vel_dir = 1;
+    L->bc.mach_ref = 0.005;
+    L->bc.density = 1.0;
+    L->bc.temperature = 1.0 / 3.0;
+    L->bc.adiabatic_exponent = 5.0 / 3.0;
+    L->bc.m_bar = kAirMolarMass;
+    compute_bc_values(L.get());
+    mrb_set_phys(L.get(), &p);
+    L->phys_set = false;
+    compute_shift(L.get());
+    *out = L.release();
+    return MRB_OK;
+}
+
+int mrb_level_destroy(mrb_level *L)
+{
+    if (!L) return MRB_OK;
+    (void)hipDeviceSynchronize();
+    for (int l = 0; l < 2; ++l)
+        for (int ab = 0; ab < 2; ++ab) (void)hipFree(L->pop[l][ab]);
+    (void)hipFree(L->d_boxes);
+    (void)hipFree(L->d_box_base);
+    (void)hipFree(L->d_isf);
+    (void)hipFree(L->d_mask);
+    (void)hipFree(L->d_md);
+    (void)hipFree(L->d_dq);
+    (void)hipFree(L->d_q);
+    (void)hipFree(L->d_red);
+    (void)hipFree(L->d_small);
+    for (auto &e : L->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipFree(L->d_gmap);
+    (void)hipFree(L->d_gmap1);
+    (void)hipFree(L->d_bcells);
+    delete L;
+    return MRB_OK;
+}
+
+int mrb_set_phys(mrb_level *L, const mrb_phys *p)
+{
+    if (!L || !p) return set_error(MRB_E_ARG, "null argument");
+    if (!(p->dt > 0.0) || !(p->mesh_speed > 0.0) || !(p->adiabatic_exponent > 1.0) || !(p->m_bar > 0.0))
+        return set_error(MRB_E_ARG, "invalid physics parameters");
+    L->phys = *p;
+    L->phys_set = true;
+    CollideParams &c = L->cp;
+    c.nu = p->nu;
+    c.alpha = p->alpha;
+    c.R = kAirMolarMass / p->m_bar;          // LBM.cpp:639
+    c.cv = c.R / (p->adiabatic_exponent - 1.0);  // LBM.cpp:640
+    c.dt = p->dt;
+    c.ms = p->mesh_speed;
+    for (int d = 0; d < 3; ++d) c.inv_dx[d] = L->inv_dx[d];
+    compute_shift(L);
+    return MRB_OK;
+}
+
+int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
+{
+    if (!L || !bc) return set_error(MRB_E_ARG, "null argument");
+    for (int d = 0; d < 3; ++d) {
+        for (int lohi = 0; lohi < 2; ++lohi) {
+            const int t = bc->bc_type[d + 3 * lohi];
+            if (L->periodic[d] && t != MRB_BC_PERIODIC)
+                return set_error(MRB_E_ARG, "periodic direction needs bc 0 on both sides");
+            if (!L->periodic[d] && t == MRB_BC_PERIODIC)
+                return set_error(MRB_E_ARG, "bc 0 (interior) in a non-periodic direction");
+            if (t == MRB_BC_PRESSURE) return set_error(MRB_E_UNSUPPORTED, "pressure bc not supported");
+            if (t != MRB_BC_PERIODIC && t != MRB_BC_NOSLIP && t != MRB_BC_VELOCITY && t != MRB_BC_OUTFLOW &&
+                t != MRB_BC_SLIPX && t != MRB_BC_SLIPY && t != MRB_BC_SLIPZ)
+                return set_error(MRB_E_ARG, "unknown bc type");
+        }
+    }
+    if (bc->velbc_kind == 1 && (bc->vel_dir < 0 || bc->vel_dir > 2))
+        return set_error(MRB_E_ARG, "velocity bc dir must be 0, 1 or 2");
+    L->bc = *bc;
+    compute_bc_values(L);
+    return MRB_OK;
+}
+
+int mrb_set_shift_state(mrb_level *L, double rho0, double T0)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    L->shift_rho0 = rho0;
+    L->shift_T0 = T0;
+    compute_shift(L);
+    return MRB_OK;
+}
+
+int64_t mrb_fab_size(const mrb_level *L, int arr, int b)
+{
+    if (!L || b < 0 || b >= L->nbox) return -1;
+    const int64_t n = L->boxes[b].ncell;
+    return arr == MRB_ARR_ISFLUID ? 2 * n : NQ * n;
+}
+
+int mrb_upload(mrb_level *L, int arr, const void *host)
+{
+    if (!L || !host) return set_error(MRB_E_ARG, "null argument");
+    if (arr == MRB_ARR_ISFLUID) {
+        MRB_HIP(hipMemcpy(L->d_isf, host, sizeof(int32_t) * 2 * L->total_cells, hipMemcpyHostToDevice));
+        L->flags_ready = false;
+        return MRB_OK;
+    }
+    if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
+    const int64_t n = NQ * L->total_cells;
+    const double *h = static_cast<const double *>(host);
+    if (L->storage == MRB_STORAGE_F64) {
+        MRB_HIP(hipMemcpy(L->pop[arr][L->cur], host, sizeof(double) * n, hipMemcpyHostToDevice));
+        return MRB_OK;
+    }
+    // reduced precision: subtract the rest-state shift and round
+    const double *sh = arr == MRB_ARR_F ? L->fshift : L->gshift;
+    const int64_t N = L->total_cells;
+    if (L->storage == MRB_STORAGE_F32_SHIFTED) {
+        std::vector<float> buf(static_cast<size_t>(n));
+        for (int q = 0; q < NQ; ++q)
+            for (int64_t c = 0; c < N; ++c) buf[size_t(q * N + c)] = float(h[q * N + c] - sh[q]);
+        MRB_HIP(hipMemcpy(L->pop[arr][L->cur], buf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        return MRB_OK;
+    }
+    return set_error(MRB_E_UNSUPPORTED, "fp16 storage upload not available");
+}
+
+int mrb_download(const mrb_level *L, int arr, void *host)
+{
+    if (!L || !host) return set_error(MRB_E_ARG, "null argument");
+    MRB_HIP(hipDeviceSynchronize());
+    if (arr == MRB_ARR_ISFLUID) {
+        MRB_HIP(hipMemcpy(host, L->d_isf, sizeof(int32_t) * 2 * L->total_cells, hipMemcpyDeviceToHost));
+        return MRB_OK;
+    }
+    if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
+    const int64_t n = NQ * L->total_cells;
+    if (L->storage == MRB_STORAGE_F64) {
+        MRB_HIP(hipMemcpy(host, L->pop[arr][L->cur], sizeof(double) * n, hipMemcpyDeviceToHost));
+        return MRB_OK;
+    }
+    double *h = static_cast<double *>(host);
+    const double *sh = arr == MRB_ARR_F ? L->fshift : L->gshift;
+    const int64_t N = L->total_cells;
+    if (L->storage == MRB_STORAGE_F32_SHIFTED) {
+        std::vector<float> buf(static_cast<size_t>(n));
+        MRB_HIP(hipMemcpy(buf.data(), L->pop[arr][L->cur], sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (int q = 0; q < NQ; ++q)
+            for (int64_t c = 0; c < N; ++c) h[q * N + c] = double(buf[size_t(q * N + c)]) + sh[q];
+        return MRB_OK;
+    }
+    return set_error(MRB_E_UNSUPPORTED, "fp16 storage download not available");
+}
+
+int mrb_compute_flags_sphere(mrb_level *L, const double center[3], double radius, int has_fluid_inside)
+{
+    if (!L || !center) return set_error(MRB_E_ARG, "null argument");
+    SphereArgs S{};
+    for (int d = 0; d < 3; ++d) {
+        S.plo[d] = L->prob_lo[d];
+        S.dx[d] = L->dx[d];
+        S.c[d] = center[d];
+    }
+    S.r = radius;
+    S.fluid_inside = has_fluid_inside;
+    k_flags_sphere<<<nblocks(L->total_cells), kBlock>>>(L->d_isf, L->d_boxes, L->d_box_base, L->nbox,
+                                                        L->total_cells, S);
+    MRB_HIP(hipGetLastError());
+    if (L->layout == MRB_LAYOUT_FAB) {
+        int rc = fill_boundary(L, L->d_isf, 2, true, false, nullptr);
+        if (rc) return rc;
+        k_flags_ring<<<nblocks(L->total_cells), kBlock>>>(L->d_isf, L->d_boxes, L->d_box_base, L->nbox,
+                                                          L->total_cells);
+        MRB_HIP(hipGetLastError());
+        rc = fill_boundary(L, L->d_isf, 2, true, false, nullptr);
+        if (rc) return rc;
+    } else {
+        // ghost cells beyond non-periodic faces must be fluid for the
+        // compact layout's inline boundary handling
+        for (int d = 0; d < 3; ++d) {
+            if (L->periodic[d]) continue;
+            for (int side = 0; side < 2; ++side) {
+                const double node = side == 0 ? L->prob_lo[d] - L->dx[d] : L->prob_hi[d];
+                // the sphere covers a ghost cell only if it reaches beyond
+                // the face; a sufficient check is the distance of the centre
+                const double dist = side == 0 ? center[d] - L->prob_lo[d] : L->prob_hi[d] - center[d];
+                if (!has_fluid_inside && dist - radius <= 2.0 * L->dx[d])
+                    return set_error(MRB_E_UNSUPPORTED,
+                                     "compact layout: EB body reaches a non-periodic face");
+                (void)node;
+            }
+        }
+    }
+    MRB_HIP(hipDeviceSynchronize());
+    return mrb_finalize_flags(L);
+}
+
+int mrb_finalize_flags(mrb_level *L)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout == MRB_LAYOUT_COMPACT) {
+        // eb_boundary ring with periodic wrap (ghosts beyond physical faces fluid)
+        std::vector<int32_t> h(size_t(2 * L->total_cells));
+        MRB_HIP(hipMemcpy(h.data(), L->d_isf, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost));
+        const int nx = L->ncell[0], ny = L->ncell[1], nz = L->ncell[2];
+        const int64_t N = L->total_cells;
+        auto fluid = [&](int i, int j, int k) -> int {
+            int s[3] = {i, j, k};
+            const int n[3] = {nx, ny, nz};
+            for (int d = 0; d < 3; ++d) {
+                if (s[d] < 0 || s[d] >= n[d]) {
+                    if (!L->periodic[d]) return 1;
+                    s[d] = (s[d] + n[d]) % n[d];
+                }
+            }
+            return h[size_t((int64_t(s[2]) * ny + s[1]) * nx + s[0])];
+        };
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const int64_t c = (int64_t(k) * ny + j) * nx + i;
+                    bool all_covered = true;
+                    all_covered &= fluid(i - 1, j, k) == 0 && fluid(i + 1, j, k) == 0;
+                    all_covered &= fluid(i, j - 1, k) == 0 && fluid(i, j + 1, k) == 0;
+                    all_covered &= fluid(i, j, k - 1) == 0 && fluid(i, j, k + 1) == 0;
+                    h[size_t(N + c)] = (all_covered || h[size_t(c)] == 1) ? 0 : 1;
+                }
+        MRB_HIP(hipMemcpy(L->d_isf, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+        const int rc = compact_build_mask(L, nullptr);
+        if (rc) return rc;
+        MRB_HIP(hipDeviceSynchronize());
+    }
+    L->flags_ready = true;
+    return MRB_OK;
+}
+
+// ---- initial state -------------------------------------------------------
+
+static void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &fn)
+{
+    const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int64_t a = t * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back(fn, a, b);
+    }
+    for (auto &x : th) x.join();
+}
+
+// Evaluates f and g for every fab cell on the host from a per-cell state
+// function and uploads them; solid cells get 0 (LBM.cpp:1287-1295).
+static int init_from_state(
+    mrb_level *L,
+    const std::function<void(int, int, int, double &, double *, double &, double &, double &)> &state)
+{
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
+    const int64_t T = L->total_cells;
+    std::vector<int32_t> isf(size_t(2 * T));
+    MRB_HIP(hipMemcpy(isf.data(), L->d_isf, sizeof(int32_t) * isf.size(), hipMemcpyDeviceToHost));
+    std::vector<double> f(size_t(NQ * T)), g(size_t(NQ * T));
+    for (int b = 0; b < L->nbox; ++b) {
+        const BoxDev &B = L->boxes[b];
+        parallel_for(B.ncell, [&](int64_t a, int64_t e) {
+            for (int64_t loc = a; loc < e; ++loc) {
+                const int64_t nxy = int64_t(B.n[0]) * B.n[1];
+                const int k = int(loc / nxy) + B.lo[2];
+                const int j = int((loc % nxy) / B.n[0]) + B.lo[1];
+                const int i = int(loc % B.n[0]) + B.lo[0];
+                double rho = 1.0, vel[3] = {0.0, 0.0, 0.0}, T0 = 1.0 / 3.0, R = 1.0, gamma = 1.667;
+                state(i, j, k, rho, vel, T0, R, gamma);
+                const bool solid = isf[size_t(B.base * 2 + loc)] == 0;
+                for (int q = 0; q < NQ; ++q) {
+                    const size_t o = size_t(B.base * NQ + q * B.ncell + loc);
+                    f[o] = solid ? 0.0 : set_equilibrium_value(rho, vel, R * T0, q);
+                    g[o] = solid ? 0.0 : thermal_g_equilibrium(rho, vel, T0, R, gamma, q);
+                }
+            }
+        });
+    }
+    int rc = mrb_upload(L, MRB_ARR_F, f.data());
+    if (rc) return rc;
+    rc = mrb_upload(L, MRB_ARR_G, g.data());
+    if (rc) return rc;
+    if (L->layout == MRB_LAYOUT_FAB) {
+        rc = fill_boundary(L, L->pop[0][L->cur], NQ, false, false, nullptr);
+        if (rc) return rc;
+        rc = fill_boundary(L, L->pop[1][L->cur], NQ, false, false, nullptr);
+        if (rc) return rc;
+        MRB_HIP(hipDeviceSynchronize());
+    }
+    return MRB_OK;
+}
+
+int mrb_init_taylorgreen(mrb_level *L, double rho0, double v0, const double omega[3])
+{
+    if (!L || !omega) return set_error(MRB_E_ARG, "null argument");
+    const double Lw = 1.0 / M_PI;
+    const double o0 = omega[0], o1 = omega[1], o2 = omega[2];
+    return init_from_state(L, [&](int i, int j, int k, double &rho, double *vel, double &T, double &R,
+                                  double &gamma) {
+        // ic::TaylorGreen::DeviceOp::thermal_initialize (IC.H:119-153)
+        const double x = L->prob_lo[0] + (i + 0.5) * L->dx[0];
+        const double y = L->prob_lo[1] + (j + 0.5) * L->dx[1];
+        const double z = L->prob_lo[2] + (k + 0.5) * L->dx[2];
+        rho = rho0 + rho0 * v0 * v0 / 16.0 * (std::cos(2.0 * o0 * x / Lw) + std::cos(2.0 * o1 * y / Lw)) *
+                         (std::cos(2.0 * o2 * z / Lw) + 2.0);
+        vel[0] = v0 * std::sin(o0 * x / Lw) * std::cos(o1 * y / Lw) * std::cos(o2 * z / Lw);
+        vel[1] = -v0 * std::cos(o0 * x / Lw) * std::sin(o1 * y / Lw) * std::cos(o2 * z / Lw);
+        vel[2] = 0.0;
+        T = 1.0 / 3.0;
+        R = kAirMolarMass / kAirMolarMass;
+        gamma = 5.0 / 3.0;
+    });
+}
+
+int mrb_init_constant(
+    mrb_level *L, double density, const double mach[3], double temperature, double adiabatic_exponent,
+    double m_bar)
+{
+    if (!L || !mach) return set_error(MRB_E_ARG, "null argument");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
+    // ic::Constant (IC.cpp:4-32): velocity = mach * speed_of_sound_ref
+    const double sos = std::sqrt(adiabatic_exponent * (kAirMolarMass / m_bar) * temperature);
+    double vel[3];
+    for (int n = 0; n < 3; ++n) vel[n] = mach[n] * sos;
+    const double R = kAirMolarMass / m_bar;
+    double fv[NQ], gv[NQ];
+    for (int q = 0; q < NQ; ++q) {
+        fv[q] = set_equilibrium_value(density, vel, R * temperature, q);
+        gv[q] = thermal_g_equilibrium(density, vel, temperature, R, adiabatic_exponent, q);
+    }
+    if (L->storage != MRB_STORAGE_F64) {
+        // reduced-precision storage: build on the host through mrb_upload
+        return init_from_state(L, [&](int, int, int, double &rho, double *v, double &T, double &Rr,
+                                      double &gamma) {
+            rho = density;
+            v[0] = vel[0];
+            v[1] = vel[1];
+            v[2] = vel[2];
+            T = temperature;
+            Rr = R;
+            gamma = adiabatic_exponent;
+        });
+    }
+    double *dval = L->d_small + 2 * NQ;
+    for (int l = 0; l < 2; ++l) {
+        MRB_HIP(hipMemcpy(dval, l == 0 ? fv : gv, sizeof(double) * NQ, hipMemcpyHostToDevice));
+        k_init_uniform<<<nblocks(L->total_cells), kBlock>>>((double *)L->pop[l][L->cur], L->d_isf, L->d_boxes,
+                                                            L->d_box_base, L->nbox, L->total_cells, dval);
+        MRB_HIP(hipGetLastError());
+    }
+    MRB_HIP(hipDeviceSynchronize());
+    return MRB_OK;
+}
+
+// ---- reference-layout stages ---------------------------------------------
+
+int mrb_physbc(mrb_level *L, int lattice, void *stream)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_FAB) return set_error(MRB_E_STATE, "physbc needs the fab layout");
+    if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
+    if (L->n_bcells == 0) return MRB_OK;
+    hipStream_t s = as_stream(stream);
+    k_physbc_fab<<<nblocks(L->n_bcells), kBlock, 0, s>>>(
+        (double *)L->pop[lattice][L->cur], L->d_boxes, L->d_box_base, L->nbox, L->d_bcells, L->n_bcells, 0,
+        0, 0, L->ncell[0] - 1, L->ncell[1] - 1, L->ncell[2] - 1, L->bcdev, lattice);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+int mrb_fill_ghosts(mrb_level *L, int lattice, void *stream)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_FAB) return set_error(MRB_E_STATE, "fill_ghosts needs the fab layout");
+    if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
+    // The bounce prefill of FillPatchOps.H:92-110 is overwritten in full by
+    // FillBoundary and the boundary functor, so it is not executed.
+    int rc = fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, as_stream(stream));
+    if (rc) return rc;
+    return mrb_physbc(L, lattice, stream);
+}
+
+int mrb_stream(mrb_level *L, int lattice, void *stream)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_FAB) return set_error(MRB_E_STATE, "mrb_stream needs the fab layout");
+    if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    hipStream_t s = as_stream(stream);
+    // the other lattice keeps its buffer index: stream lattice by lattice
+    // through the spare buffer and copy back so that cur stays shared
+    double *fin = (double *)L->pop[lattice][L->cur];
+    double *fout = (double *)L->pop[lattice][1 - L->cur];
+    k_stream_fab<<<nblocks(L->total_cells), kBlock, 0, s>>>(fin, fout, L->d_isf, L->d_boxes, L->d_box_base,
+                                                            L->nbox, L->total_cells);
+    MRB_HIP(hipGetLastError());
+    std::swap(L->pop[lattice][0], L->pop[lattice][1]);
+    return fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, s);
+}
+
+int mrb_collide(mrb_level *L, void *stream)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_FAB) return set_error(MRB_E_STATE, "mrb_collide needs the fab layout");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    hipStream_t s = as_stream(stream);
+    double *f = (double *)L->pop[0][L->cur];
+    double *g = (double *)L->pop[1][L->cur];
+    const unsigned nb = nblocks(L->total_cells);
+    k_moments_fab<<<nb, kBlock, 0, s>>>(f, g, L->d_isf, L->d_md, L->d_boxes, L->d_box_base, L->nbox,
+                                        L->total_cells, L->cp);
+    MRB_HIP(hipGetLastError());
+    int rc = fill_boundary(L, L->d_md, 19, false, true, s);
+    if (rc) return rc;
+    k_dq_fab<<<nb, kBlock, 0, s>>>(L->d_md, L->d_isf, L->d_dq, L->d_boxes, L->d_box_base, L->nbox,
+                                   L->total_cells, L->ncell[0], L->ncell[1], L->ncell[2], L->cp);
+    MRB_HIP(hipGetLastError());
+    k_eqrelax_fab<<<nb, kBlock, 0, s>>>(f, g, L->d_md, L->d_dq, L->d_isf, L->d_boxes, L->d_box_base,
+                                        L->nbox, L->total_cells, L->cp);
+    MRB_HIP(hipGetLastError());
+    rc = fill_boundary(L, f, NQ, false, false, s);
+    if (rc) return rc;
+    return fill_boundary(L, g, NQ, false, false, s);
+}
+
+// ---- hot path --------------------------------------------------------------
+
+int mrb_stream_collide(mrb_level *L, void *stream)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    if (L->layout == MRB_LAYOUT_COMPACT) return compact_stream_collide(L, as_stream(stream));
+    // reference layout: the literal stage sequence
+    int rc;
+    for (int l = 0; l < 2; ++l)
+        if ((rc = mrb_fill_ghosts(L, l, stream))) return rc;
+    for (int l = 0; l < 2; ++l)
+        if ((rc = mrb_stream(L, l, stream))) return rc;
+    return mrb_collide(L, stream);
+}
+
+int mrb_advance(mrb_level *L, int nsteps, void *stream)
+{
+    for (int n = 0; n < nsteps; ++n) {
+        const int rc = mrb_stream_collide(L, stream);
+        if (rc) return rc;
+    }
+    return MRB_OK;
+}
+
+int mrb_advance_timed(mrb_level *L, int nsteps, void *stream, double *kernel_ms, int *nslots)
+{
+    if (!L || !kernel_ms || !nslots) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    hipStream_t s = as_stream(stream);
+    for (int k = 0; k < 4; ++k) kernel_ms[k] = 0.0;
+    int ns = 0;
+    for (int n = 0; n < nsteps; ++n) {
+        const int rc = compact_stream_collide_events(L, s, ns);
+        if (rc) return rc;
+        MRB_HIP(hipEventSynchronize(L->ev[ns]));
+        for (int k = 0; k < ns; ++k) {
+            float ms = 0.f;
+            MRB_HIP(hipEventElapsedTime(&ms, L->ev[k], L->ev[k + 1]));
+            kernel_ms[k] += ms;
+        }
+    }
+    *nslots = ns;
+    return MRB_OK;
+}
+
+int mrb_stream_compact(mrb_level *L, void *stream)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    return compact_stream_only(L, as_stream(stream));
+}
+
+int mrb_reduce_conservation(const mrb_level *L, double out[5], void *stream)
+{
+    if (!L || !out) return set_error(MRB_E_ARG, "null argument");
+    hipStream_t s = as_stream(stream);
+    const unsigned nb = std::min<unsigned>(4096, nblocks(L->total_cells));
+    // shift tables live at d_small[0..2*NQ) (zero for FP64 storage)
+    const double *shifts = L->d_small;
+    if (L->storage == MRB_STORAGE_F64) {
+        k_conservation<double><<<nb, kBlock, 0, s>>>(
+            (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red);
+    } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
+        k_conservation<float><<<nb, kBlock, 0, s>>>(
+            (const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red);
+    } else {
+        return set_error(MRB_E_UNSUPPORTED, "conservation sums for this storage");
+    }
+    MRB_HIP(hipGetLastError());
+    std::vector<double> part(size_t(5 * nb));
+    MRB_HIP(hipMemcpyAsync(part.data(), L->d_red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+    MRB_HIP(hipStreamSynchronize(s));
+    for (int n = 0; n < 5; ++n) out[n] = 0.0;
+    for (unsigned b = 0; b < nb; ++b)
+        for (int n = 0; n < 5; ++n) out[n] += part[size_t(5 * b + n)];
+    return MRB_OK;
+}
+
+const char *mrb_hot_kernel_name(const mrb_level *L)
+{
+    if (!L) return "";
+    return L->layout == MRB_LAYOUT_COMPACT ? compact_kernel_name(L) : "fab-stages";
+}
+
+int mrb_population_ptr(const mrb_level *L, int lattice, void **dev_ptr, int64_t *bytes)
+{
+    if (!L || !dev_ptr || lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "bad argument");
+    *dev_ptr = L->pop[lattice][L->cur];
+    if (bytes) *bytes = int64_t(pop_elem_size(L->storage)) * NQ * L->total_cells;
+    return MRB_OK;
+}
+
+}  // extern "C"

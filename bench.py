@@ -79,6 +79,27 @@ def cpu_baseline(cfg: dict, budget_s: float, threads: int) -> dict:
     }
 
 
+def copy_ceiling(torch, nbytes: int = 2 << 30, iters: int = 20) -> float:
+    """Device-to-device copy bandwidth (read + write bytes / s) of torch's copy
+    kernel on `nbytes` buffers: the measured HBM ceiling of this box, and the
+    known-size kernel that calibrates the rocprof byte counters."""
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(iters):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    secs = s.elapsed_time(e) * 1e-3
+    del a, b
+    return 2.0 * nbytes * iters / secs / 1e9
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +155,8 @@ def main() -> int:
     cons = lvl.conservation(stream=sh)
     log(f"conservation (mass, mom x/y/z, 2rhoE): {cons.tolist()}")
     lvl.close()
+    copy_gbs = copy_ceiling(torch)
+    log(f"copy-kernel ceiling: {copy_gbs:.1f} GB/s")
 
     result = {
         "metric": METRIC,
@@ -165,6 +188,8 @@ def main() -> int:
             "kernel": "+".join(per_kernel_ms),
             "bytes_per_update": bpu,
             "kernel_ms_per_step": {k: round(v, 5) for k, v in per_kernel_ms.items()},
+            "copy_ceiling_gbs": round(copy_gbs, 1),
+            "frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
         },
         "cpu_baseline": None,
     }

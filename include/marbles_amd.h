@@ -163,6 +163,18 @@ int mrb_advance(mrb_level *lvl, int nsteps, void *stream);
  * (slots are named by mrb_hot_kernel_name, separated by '+'), *nslots the
  * number of slots (at most 4).  Synchronous.  Compact layout only. */
 int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_ms, int *nslots);
+/* Kernel variants of the compact-layout step: index 0 is the two-pass step
+ * (Q-correction pass + collide pass), indices 1.. the single-pass z-sweep
+ * with different tile shapes.  All variants produce identical results. */
+int mrb_hot_kernel_count(void);
+const char *mrb_hot_kernel_variant_name(int index);
+int mrb_set_hot_kernel(mrb_level *lvl, int index);
+/* Autotuner: times every variant (`reps` launches each) on the level's
+ * current state without advancing it -- each trial reads the current
+ * buffers and writes the spare ones -- and selects the fastest.
+ * ms_out[i] (may be NULL) receives variant i's mean step time and *best its
+ * index.  Synchronous. */
+int mrb_autotune(mrb_level *lvl, int reps, void *stream, double *ms_out, int *best);
 /* Stream only (debug/parity; LBM::stream for f and g on the compact
  * layout, ghosts resolved inline).  Swaps A/B. */
 int mrb_stream_compact(mrb_level *lvl, void *stream);

@@ -23,6 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = [
     "mrb_level.hip",
     "mrb_compact.hip",
+    "mrb_fused.hip",
 ]
 
 COMMON_FLAGS = [

@@ -90,6 +90,10 @@ SIGNATURES = {
     "mrb_advance": (c_int, [c_void_p, c_int, c_void_p]),
     "mrb_stream_compact": (c_int, [c_void_p, c_void_p]),
     "mrb_advance_timed": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_double), POINTER(c_int)]),
+    "mrb_hot_kernel_count": (c_int, []),
+    "mrb_hot_kernel_variant_name": (c_char_p, [c_int]),
+    "mrb_set_hot_kernel": (c_int, [c_void_p, c_int]),
+    "mrb_autotune": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_double), POINTER(c_int)]),
     "mrb_reduce_conservation": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
     "mrb_last_error": (c_char_p, []),
     "mrb_hot_kernel_name": (c_char_p, [c_void_p]),

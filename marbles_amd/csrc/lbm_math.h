@@ -203,14 +203,20 @@ __host__ __device__ __forceinline__ double phi_of(double u, double P)
     return c * 0.5 * u + iabs(c) * (1.50 * P - 1.0) - P + 1.0;
 }
 
-// Equilibria of LBM::macrodata_to_equilibrium (LBM.cpp:659-758) hoisted per
-// cell, followed by the BGK relaxation of LBM::relax_f_to_equilibrium
-// (LBM.cpp:793-801).  fo/go may alias f/g.
-template <class TIn, class FOut>
-__host__ __device__ __forceinline__ void collide_cell(
-    const TIn* __restrict__ f, const TIn* __restrict__ g, const Moments& m, const Macro& M,
-    double dqx, double dqy, double dqz, const CollideParams& P, FOut&& out)
+// Per-cell coefficients of the f and g equilibria (LBM.cpp:659-758) and the
+// relaxation rate (LBM.cpp:795-797).
+struct CollideCoeffs {
+    double omega;
+    double rphx[3], phy[3], phz[3];  // rho*phi_x, phi_y, phi_z at c+1
+    double E, a1x, a1y, a1z;
+    double tx[2], ty[2], tz[2];      // (c*c - theta0) * a2_aa for c*c = 0, 1
+    double dxy, dxz, dyz;            // 2 * a2_ab
+};
+
+__host__ __device__ __forceinline__ CollideCoeffs collide_coeffs(
+    const Moments& m, const Macro& M, double dqx, double dqy, double dqz, const CollideParams& P)
 {
+    CollideCoeffs C;
     const double R = P.R, dt = P.dt;
     const double rho = m.rho;
     const double T = M.T;
@@ -219,20 +225,21 @@ __host__ __device__ __forceinline__ void collide_cell(
     const double omega_one = 1.0 / (P.alpha / (R * T * dt) + 0.5);
     const double omega_one_by_omega = omega_one / omega;
     const double omega_corr = (2.0 - omega) / (2.0 * omega * rho);
+    C.omega = omega;
 
     const double pxx_ext = u * u + R * T + dt * (omega_corr)*dqx;
     const double pyy_ext = v * v + R * T + dt * (omega_corr)*dqy;
     const double pzz_ext = w * w + R * T + dt * (omega_corr)*dqz;
+    C.rphx[0] = rho * phi_of<-1>(u, pxx_ext);
+    C.rphx[1] = rho * phi_of<0>(u, pxx_ext);
+    C.rphx[2] = rho * phi_of<1>(u, pxx_ext);
+    C.phy[0] = phi_of<-1>(v, pyy_ext);
+    C.phy[1] = phi_of<0>(v, pyy_ext);
+    C.phy[2] = phi_of<1>(v, pyy_ext);
+    C.phz[0] = phi_of<-1>(w, pzz_ext);
+    C.phz[1] = phi_of<0>(w, pzz_ext);
+    C.phz[2] = phi_of<1>(w, pzz_ext);
 
-    // phi[c+1] for c = -1, 0, 1
-    const double phx[3] = {phi_of<-1>(u, pxx_ext), phi_of<0>(u, pxx_ext), phi_of<1>(u, pxx_ext)};
-    const double phy[3] = {phi_of<-1>(v, pyy_ext), phi_of<0>(v, pyy_ext), phi_of<1>(v, pyy_ext)};
-    const double phz[3] = {phi_of<-1>(w, pzz_ext), phi_of<0>(w, pzz_ext), phi_of<1>(w, pzz_ext)};
-    const double rphx[3] = {rho * phx[0], rho * phx[1], rho * phx[2]};
-
-    // get_equilibrium_moments, RealVect overload (Utilities.H:245-277); its
-    // temperature comes from the vector get_temperature (Utilities.H:198-207),
-    // whose AMREX_D_TERM expansion subtracts u^2 and adds v^2 and w^2.
     const double E = m.E;
     const double energy = E / (2.0 * rho);
     const double Tv = (0.50 / P.cv) * ((E / rho) - u * u + v * v + w * w);
@@ -248,8 +255,6 @@ __host__ __device__ __forceinline__ void collide_cell(
     const double rxy = 2.0 * rho * u * v * hp + 0;
     const double rxz = 2.0 * rho * u * w * hp + 0;
     const double ryz = 2.0 * rho * v * w * hp + 0;
-
-    // Heat-flux blend (LBM.cpp:727-748)
     qx_eq *= omega_one_by_omega;
     qy_eq *= omega_one_by_omega;
     qz_eq *= omega_one_by_omega;
@@ -259,51 +264,76 @@ __host__ __device__ __forceinline__ void collide_cell(
              (m.qy - 2.0 * u * m.pxy - 2.0 * v * m.pyy - 2.0 * w * m.pyz - v * dt * dqy);
     qz_eq += (1.0 - omega_one_by_omega) *
              (m.qz - 2.0 * u * m.pxz - 2.0 * v * m.pyz - 2.0 * w * m.pzz - w * dt * dqz);
-
-    // Grad expansion (Utilities.H:65-163) with rho := 2 rho E, frame velocity
-    // 0 and s = 1: every "- rho*u" and "- u*(...)" term is an exact zero.
     const double one_bystheta0 = 1.0 / kTheta0;
-    const double a1x = qx_eq * one_bystheta0;
-    const double a1y = qy_eq * one_bystheta0;
-    const double a1z = qz_eq * one_bystheta0;
+    C.E = E;
+    C.a1x = qx_eq * one_bystheta0;
+    C.a1y = qy_eq * one_bystheta0;
+    C.a1z = qz_eq * one_bystheta0;
     const double a2xx = (rxx - E * 1.0 * kTheta0) * one_bystheta0 * one_bystheta0;
     const double a2yy = (ryy - E * 1.0 * kTheta0) * one_bystheta0 * one_bystheta0;
     const double a2zz = (rzz - E * 1.0 * kTheta0) * one_bystheta0 * one_bystheta0;
     const double a2xy = rxy * one_bystheta0 * one_bystheta0;
     const double a2xz = rxz * one_bystheta0 * one_bystheta0;
     const double a2yz = ryz * one_bystheta0 * one_bystheta0;
-    // (c*c - theta0) * a2 for c*c = 0 and 1
-    const double tx[2] = {(0 - kTheta0) * a2xx, (1 - kTheta0) * a2xx};
-    const double ty[2] = {(0 - kTheta0) * a2yy, (1 - kTheta0) * a2yy};
-    const double tz[2] = {(0 - kTheta0) * a2zz, (1 - kTheta0) * a2zz};
-    const double dxy = 2.0 * a2xy, dxz = 2.0 * a2xz, dyz = 2.0 * a2yz;
+    C.tx[0] = (0 - kTheta0) * a2xx;
+    C.tx[1] = (1 - kTheta0) * a2xx;
+    C.ty[0] = (0 - kTheta0) * a2yy;
+    C.ty[1] = (1 - kTheta0) * a2yy;
+    C.tz[0] = (0 - kTheta0) * a2zz;
+    C.tz[1] = (1 - kTheta0) * a2zz;
+    C.dxy = 2.0 * a2xy;
+    C.dxz = 2.0 * a2xz;
+    C.dyz = 2.0 * a2yz;
+    return C;
+}
 
+// f equilibrium of direction q (Utilities.H:42-63, product order kept)
+template <int q>
+__host__ __device__ __forceinline__ double feq_of(const CollideCoeffs& C)
+{
+    constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+    return ((C.rphx[cx + 1]) * C.phy[cy + 1]) * C.phz[cz + 1];
+}
+
+// g equilibrium of direction q (Utilities.H:147-160, summation order kept)
+template <int q>
+__host__ __device__ __forceinline__ double geq_of(const CollideCoeffs& C)
+{
+    constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+    double ge = C.E;
+    if constexpr (cx == 1) ge = ge + C.a1x;
+    if constexpr (cx == -1) ge = ge + -C.a1x;
+    if constexpr (cy == 1) ge = ge + C.a1y;
+    if constexpr (cy == -1) ge = ge + -C.a1y;
+    if constexpr (cz == 1) ge = ge + C.a1z;
+    if constexpr (cz == -1) ge = ge + -C.a1z;
+    double S = C.tx[cx * cx] + C.ty[cy * cy];
+    if constexpr (cx * cy == 1) S = S + C.dxy;
+    if constexpr (cx * cy == -1) S = S + -C.dxy;
+    S = S + C.tz[cz * cz];
+    if constexpr (cx * cz == 1) S = S + C.dxz;
+    if constexpr (cx * cz == -1) S = S + -C.dxz;
+    if constexpr (cy * cz == 1) S = S + C.dyz;
+    if constexpr (cy * cz == -1) S = S + -C.dyz;
+    ge = ge + 0.5 * S;
+    return ge * weight(q);
+}
+
+// Equilibria of LBM::macrodata_to_equilibrium (LBM.cpp:659-758) hoisted per
+// cell, followed by the BGK relaxation of LBM::relax_f_to_equilibrium
+// (LBM.cpp:793-801).  fo/go may alias f/g.
+template <class TIn, class FOut>
+__host__ __device__ __forceinline__ void collide_cell(
+    const TIn* __restrict__ f, const TIn* __restrict__ g, const Moments& m, const Macro& M,
+    double dqx, double dqy, double dqz, const CollideParams& P, FOut&& out)
+{
+    const CollideCoeffs C = collide_coeffs(m, M, dqx, dqy, dqz, P);
+    const double omega = C.omega;
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-        const double feq = ((rphx[cx + 1]) * phy[cy + 1]) * phz[cz + 1];
-
-        double ge = E;
-        if constexpr (cx == 1) ge = ge + a1x;
-        if constexpr (cx == -1) ge = ge + -a1x;
-        if constexpr (cy == 1) ge = ge + a1y;
-        if constexpr (cy == -1) ge = ge + -a1y;
-        if constexpr (cz == 1) ge = ge + a1z;
-        if constexpr (cz == -1) ge = ge + -a1z;
-        double S = tx[cx * cx] + ty[cy * cy];
-        if constexpr (cx * cy == 1) S = S + dxy;
-        if constexpr (cx * cy == -1) S = S + -dxy;
-        S = S + tz[cz * cz];
-        if constexpr (cx * cz == 1) S = S + dxz;
-        if constexpr (cx * cz == -1) S = S + -dxz;
-        if constexpr (cy * cz == 1) S = S + dyz;
-        if constexpr (cy * cz == -1) S = S + -dyz;
-        ge = ge + 0.5 * S;
-        ge = ge * weight(q);
-
         const double fq = double(f[q]);
         const double gq = double(g[q]);
-        out(qc, fq + omega * (feq - fq), gq + omega * (ge - gq));
+        out(qc, fq + omega * (feq_of<q>(C) - fq), gq + omega * (geq_of<q>(C) - gq));
     });
 }
 

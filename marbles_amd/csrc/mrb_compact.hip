@@ -2,9 +2,10 @@
 // pull-stream with halfway bounce-back, periodic wrap and physical-boundary
 // ghost values resolved inside the kernel, followed by the thermal collide.
 //
-// HBM layout: populations SoA, pop[q*N + cell], cell = (z*ny + y)*nx + x, two
-// buffers (A/B) per lattice; a 32-bit stream mask per cell (bit q = the pull
-// source x - c_q is fluid).
+// HBM layout: populations SoA with f and g interleaved by component,
+// pop[(2q + lattice)*N + cell], cell = (z*ny + y)*nx + x, so f_q = f[q*cs + cell]
+// and g_q = g[q*cs + cell] with g = f + N and cs = 2N; two buffers (A/B); a
+// 32-bit stream mask per cell (bit q = the pull source x - c_q is fluid).
 //
 // The collide needs the Q-correction gradient of post-stream moments at the
 // six face neighbours (LBM.cpp:959-991), so one step is two passes:
@@ -19,7 +20,8 @@ namespace {
 struct CompactArgs {
     int nx, ny, nz;
     int per[3];
-    int64_t N;
+    int64_t N;   // cells
+    int64_t cs;  // component stride of the population arrays (2N)
     int bc[6];
 };
 
@@ -64,7 +66,7 @@ __device__ __forceinline__ int64_t pull_index(
     use_const = false;
     if (!ghost) {
         const int64_t src = (int64_t(sz[cz + 1]) * A.ny + sy[cy + 1]) * A.nx + sx[cx + 1];
-        return ((mask >> q) & 1u) ? int64_t(q) * A.N + src : int64_t(qb) * A.N + cell;
+        return ((mask >> q) & 1u) ? int64_t(q) * A.cs + src : int64_t(qb) * A.cs + cell;
     }
     int face;
     if (gz[cz + 1])
@@ -86,12 +88,12 @@ __device__ __forceinline__ int64_t pull_index(
         if (gx[cx + 1]) tx = cx > 0 ? 0 : A.nx - 1;
         if (gy[cy + 1]) ty = cy > 0 ? 0 : A.ny - 1;
         if (gz[cz + 1]) tz = cz > 0 ? 0 : A.nz - 1;
-        return int64_t(q) * A.N + (int64_t(tz) * A.ny + ty) * A.nx + tx;
+        return int64_t(q) * A.cs + (int64_t(tz) * A.ny + ty) * A.nx + tx;
     }
-    if (t == MRB_BC_SLIPX) return int64_t(reflect(q, 0)) * A.N + cell;
-    if (t == MRB_BC_SLIPY) return int64_t(reflect(q, 1)) * A.N + cell;
-    if (t == MRB_BC_SLIPZ) return int64_t(reflect(q, 2)) * A.N + cell;
-    return int64_t(qb) * A.N + cell;  // NOSLIP
+    if (t == MRB_BC_SLIPX) return int64_t(reflect(q, 0)) * A.cs + cell;
+    if (t == MRB_BC_SLIPY) return int64_t(reflect(q, 1)) * A.cs + cell;
+    if (t == MRB_BC_SLIPZ) return int64_t(reflect(q, 2)) * A.cs + cell;
+    return int64_t(qb) * A.cs + cell;  // NOSLIP
 }
 
 template <class T>
@@ -159,8 +161,8 @@ __global__ void __launch_bounds__(256) k_collide(
         // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            fo[int64_t(q) * A.N + cell] = -1.0;
-            go[int64_t(q) * A.N + cell] = -1.0;
+            fo[int64_t(q) * A.cs + cell] = -1.0;
+            go[int64_t(q) * A.cs + cell] = -1.0;
         });
         return;
     }
@@ -182,8 +184,8 @@ __global__ void __launch_bounds__(256) k_collide(
                                           zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
     collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.N + cell] = fnew;
-        go[int64_t(q) * A.N + cell] = gnew;
+        fo[int64_t(q) * A.cs + cell] = fnew;
+        go[int64_t(q) * A.cs + cell] = gnew;
     });
 }
 
@@ -204,8 +206,8 @@ __global__ void __launch_bounds__(256) k_stream_only(
     }
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.N + cell] = fv[q];
-        go[int64_t(q) * A.N + cell] = gv[q];
+        fo[int64_t(q) * A.cs + cell] = fv[q];
+        go[int64_t(q) * A.cs + cell] = gv[q];
     });
 }
 
@@ -243,6 +245,7 @@ CompactArgs make_args(const mrb_level *L)
     A.nz = L->ncell[2];
     for (int d = 0; d < 3; ++d) A.per[d] = L->periodic[d];
     A.N = L->total_cells;
+    A.cs = 2 * A.N;
     for (int n = 0; n < 6; ++n) A.bc[n] = L->bc.bc_type[n];
     return A;
 }
@@ -259,7 +262,9 @@ int compact_build_mask(mrb_level *L, hipStream_t s)
     return MRB_OK;
 }
 
-static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
+// Two-pass step; reads the current buffers, writes the spare ones, does not
+// flip.  ev (if not null) receives events before, between and after.
+static int two_pass(mrb_level *L, hipStream_t s, hipEvent_t *ev)
 {
     if (L->storage != MRB_STORAGE_F64)
         return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
@@ -268,14 +273,31 @@ static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
     const double *g = (const double *)L->pop[1][L->cur];
     double *fo = (double *)L->pop[0][1 - L->cur];
     double *go = (double *)L->pop[1][1 - L->cur];
-    if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
+    if (ev) MRB_HIP(hipEventRecord(ev[0], s));
     k_qpass<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, L->cp);
     MRB_HIP(hipGetLastError());
-    if (events) MRB_HIP(hipEventRecord(L->ev[1], s));
+    if (ev) MRB_HIP(hipEventRecord(ev[1], s));
     k_collide<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, L->cp);
     MRB_HIP(hipGetLastError());
-    if (events) MRB_HIP(hipEventRecord(L->ev[2], s));
-    nslots = 2;
+    if (ev) MRB_HIP(hipEventRecord(ev[2], s));
+    return MRB_OK;
+}
+
+int compact_two_pass_launch(mrb_level *L, hipStream_t s) { return two_pass(L, s, nullptr); }
+
+static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
+{
+    int rc;
+    if (L->hot == 0) {
+        rc = two_pass(L, s, events ? L->ev : nullptr);
+        nslots = 2;
+    } else {
+        if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
+        rc = fused_launch(L, L->hot - 1, s);
+        if (events && !rc) MRB_HIP(hipEventRecord(L->ev[1], s));
+        nslots = 1;
+    }
+    if (rc) return rc;
     L->cur = 1 - L->cur;
     return MRB_OK;
 }
@@ -304,6 +326,9 @@ int compact_stream_only(mrb_level *L, hipStream_t s)
     return MRB_OK;
 }
 
-const char *compact_kernel_name(const mrb_level *) { return "k_qpass+k_collide"; }
+const char *compact_kernel_name(const mrb_level *L)
+{
+    return L->hot == 0 ? "k_qpass+k_collide" : kFusedVariants[L->hot - 1].name;
+}
 
 }  // namespace mrb

@@ -52,6 +52,16 @@ constexpr uint32_t kMaskGeneral = 1u << 27;
 
 enum class Storage { F64 = 0, F32S = 1, F16S = 2 };
 
+// Single-pass z-sweep kernel shapes: rows per tile and z planes per chunk.
+struct FusedVariant {
+    const char *name;
+    int ty;
+    int zc;
+    int hold;  // 1: keep the look-ahead plane's populations in registers
+};
+extern const FusedVariant kFusedVariants[];
+extern const int kNumFusedVariants;
+
 }  // namespace mrb
 
 struct mrb_level {
@@ -77,7 +87,7 @@ struct mrb_level {
     double *d_dq = nullptr;           // FAB: dQ, 3 comps
     double *d_q = nullptr;            // compact: Q_corr, 3 comps
     double *d_red = nullptr;          // reduction partials
-    double *d_small = nullptr;        // [fshift | gshift | init values] (4*NQ)
+    double *d_small = nullptr;        // [fshift | gshift | init values | inlet f | inlet g] (6*NQ)
     mrb::GhostPair *d_gmap = nullptr;   // FillBoundary map, ng layers
     int64_t n_gmap = 0;
     mrb::GhostPair *d_gmap1 = nullptr;  // FillBoundary map, 1 layer (macrodata)
@@ -95,6 +105,8 @@ struct mrb_level {
     double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
     // timing events (created with the level)
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // hot-path kernel of the compact layout: 0 = two-pass, 1 + i = fused variant i
+    int hot = 0;
 };
 
 namespace mrb {
@@ -118,5 +130,11 @@ int compact_stream_collide_events(mrb_level *L, hipStream_t s, int &nslots);
 int compact_stream_only(mrb_level *L, hipStream_t s);
 int compact_build_mask(mrb_level *L, hipStream_t s);
 const char *compact_kernel_name(const mrb_level *L);
+// single-pass kernels (mrb_fused.hip): launch reads the current buffer and
+// writes the spare one; step also flips the buffers
+int fused_launch(mrb_level *L, int variant, hipStream_t s);
+int fused_step(mrb_level *L, int variant, hipStream_t s);
+// two-pass step without flipping (mrb_compact.hip)
+int compact_two_pass_launch(mrb_level *L, hipStream_t s);
 
 }  // namespace mrb

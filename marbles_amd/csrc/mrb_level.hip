@@ -411,7 +411,8 @@ template <class PopT>
 __global__ void k_conservation(
     const PopT *__restrict__ f, const PopT *__restrict__ g, const int32_t *__restrict__ isf,
     const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
-    const double *__restrict__ fshift, const double *__restrict__ gshift, double *__restrict__ part)
+    const double *__restrict__ fshift, const double *__restrict__ gshift, double *__restrict__ part,
+    int64_t cs)
 {
     __shared__ double sh[5][kBlock];
     double acc[5] = {0, 0, 0, 0, 0};
@@ -426,14 +427,15 @@ __global__ void k_conservation(
         if (isf[B.base * 2 + loc] != 1) continue;
         const PopT *fp = f + B.base * NQ + loc;
         const PopT *gp = g + B.base * NQ + loc;
+        const int64_t stride = cs ? cs : B.ncell;
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const double fq = double(fp[q * B.ncell]) + fshift[q];
+            const double fq = double(fp[q * stride]) + fshift[q];
             acc[0] += fq;
             if constexpr (ev(q, 0) != 0) acc[1] += ev(q, 0) * fq;
             if constexpr (ev(q, 1) != 0) acc[2] += ev(q, 1) * fq;
             if constexpr (ev(q, 2) != 0) acc[3] += ev(q, 2) * fq;
-            acc[4] += double(gp[q * B.ncell]) + gshift[q];
+            acc[4] += double(gp[q * stride]) + gshift[q];
         });
     }
     for (int n = 0; n < 5; ++n) sh[n][threadIdx.x] = acc[n];
@@ -453,10 +455,11 @@ __global__ void k_fill_const(double *__restrict__ a, int64_t n, double v)
     if (t < n) a[t] = v;
 }
 
-// uniform initial state: comp q of every cell = val[q]; solid cells 0
+// uniform initial state: comp q of every cell = val[q]; solid cells 0.
+// cs = component stride (0: the box's own fab size, the reference layout)
 __global__ void k_init_uniform(
     double *__restrict__ pop, const int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes,
-    const int64_t *__restrict__ base, int nbox, int64_t total, const double *__restrict__ val)
+    const int64_t *__restrict__ base, int nbox, int64_t total, const double *__restrict__ val, int64_t cs)
 {
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= total) return;
@@ -464,7 +467,8 @@ __global__ void k_init_uniform(
     const BoxDev B = boxes[b];
     const int64_t loc = c - B.base;
     const bool solid = isf[B.base * 2 + loc] == 0;
-    for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * B.ncell + loc] = solid ? 0.0 : val[q];
+    const int64_t stride = cs ? cs : B.ncell;
+    for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * stride + loc] = solid ? 0.0 : val[q];
 }
 
 }  // namespace
@@ -472,6 +476,22 @@ __global__ void k_init_uniform(
 // ---------------------------------------------------------------------------
 // host-side helpers
 // ---------------------------------------------------------------------------
+
+// Component stride of the population arrays: the compact layout interleaves
+// f and g components (stride 2N); the reference layout uses each box's fab
+// size (0 = per box).
+static int64_t comp_stride(const mrb_level *L)
+{
+    return L->layout == MRB_LAYOUT_COMPACT ? 2 * L->total_cells : 0;
+}
+
+// Host <-> device copy of one lattice's 27 components (host: contiguous
+// components of n elements; device: component stride dst/src_stride).
+static hipError_t copy_components(
+    void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, hipMemcpyKind kind)
+{
+    return hipMemcpy2D(dst, dpitch, src, spitch, width, NQ, kind);
+}
 
 // Fills `map` with every ghost cell of every box (within `depth` layers) that
 // AMReX FillBoundary would overwrite: its periodic image lies inside the
@@ -555,6 +575,10 @@ static void compute_bc_values(mrb_level *L)
     for (int q = 0; q < NQ; ++q) {
         d.fval[q] = set_equilibrium_value(rho, vel, R * T, q);
         d.gval[q] = thermal_g_equilibrium(rho, vel, T, R, gamma, q);
+    }
+    if (L->d_small) {
+        (void)hipMemcpy(L->d_small + 4 * NQ, d.fval, sizeof(double) * NQ, hipMemcpyHostToDevice);
+        (void)hipMemcpy(L->d_small + 5 * NQ, d.gval, sizeof(double) * NQ, hipMemcpyHostToDevice);
     }
 }
 
@@ -661,17 +685,27 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
     MRB_HIP(hipMemcpy(L->d_boxes, L->boxes.data(), sizeof(BoxDev) * L->nbox, hipMemcpyHostToDevice));
     MRB_HIP(hipMalloc(&L->d_box_base, sizeof(int64_t) * (L->nbox + 1)));
     MRB_HIP(hipMemcpy(L->d_box_base, bases.data(), sizeof(int64_t) * (L->nbox + 1), hipMemcpyHostToDevice));
-    for (int l = 0; l < 2; ++l)
+    if (L->layout == MRB_LAYOUT_COMPACT) {
+        // f and g interleaved by component in one allocation per buffer:
+        // f_q at component 2q, g_q at 2q+1 (component stride 2N, g = f + N)
         for (int ab = 0; ab < 2; ++ab) {
-            MRB_HIP(hipMalloc(&L->pop[l][ab], es * NQ * T));
-            MRB_HIP(hipMemset(L->pop[l][ab], 0, es * NQ * T));
+            MRB_HIP(hipMalloc(&L->pop[0][ab], es * 2 * NQ * T));
+            MRB_HIP(hipMemset(L->pop[0][ab], 0, es * 2 * NQ * T));
+            L->pop[1][ab] = static_cast<char *>(L->pop[0][ab]) + es * T;
         }
+    } else {
+        for (int l = 0; l < 2; ++l)
+            for (int ab = 0; ab < 2; ++ab) {
+                MRB_HIP(hipMalloc(&L->pop[l][ab], es * NQ * T));
+                MRB_HIP(hipMemset(L->pop[l][ab], 0, es * NQ * T));
+            }
+    }
     MRB_HIP(hipMalloc(&L->d_isf, sizeof(int32_t) * 2 * T));
     MRB_HIP(hipMemset(L->d_isf, 0, sizeof(int32_t) * 2 * T));
     MRB_HIP(hipMalloc(&L->d_red, sizeof(double) * 5 * 4096));
-    MRB_HIP(hipMalloc(&L->d_small, sizeof(double) * 4 * NQ));
+    MRB_HIP(hipMalloc(&L->d_small, sizeof(double) * 6 * NQ));
     for (auto &e : L->ev) MRB_HIP(hipEventCreate(&e));
-    MRB_HIP(hipMemset(L->d_small, 0, sizeof(double) * 4 * NQ));
+    MRB_HIP(hipMemset(L->d_small, 0, sizeof(double) * 6 * NQ));
     if (L->layout == MRB_LAYOUT_FAB) {
         MRB_HIP(hipMalloc(&L->d_md, sizeof(double) * 19 * T));
         MRB_HIP(hipMemset(L->d_md, 0, sizeof(double) * 19 * T));
@@ -744,7 +778,7 @@ int mrb_level_destroy(mrb_level *L)
 {
     if (!L) return MRB_OK;
     (void)hipDeviceSynchronize();
-    for (int l = 0; l < 2; ++l)
+    for (int l = 0; l < (L->layout == MRB_LAYOUT_COMPACT ? 1 : 2); ++l)
         for (int ab = 0; ab < 2; ++ab) (void)hipFree(L->pop[l][ab]);
     (void)hipFree(L->d_boxes);
     (void)hipFree(L->d_box_base);
@@ -822,6 +856,25 @@ int64_t mrb_fab_size(const mrb_level *L, int arr, int b)
     return arr == MRB_ARR_ISFLUID ? 2 * n : NQ * n;
 }
 
+// Copies one lattice between the host layout (components contiguous) and
+// the device buffer of the current state (component stride of the layout).
+static int copy_lattice(const mrb_level *L, int arr, void *host, bool to_device)
+{
+    const size_t es = pop_elem_size(L->storage);
+    const int64_t N = L->total_cells;
+    void *dev = L->pop[arr][L->cur];
+    if (L->layout == MRB_LAYOUT_COMPACT) {
+        const size_t w = es * size_t(N), dp = es * size_t(2 * N);
+        MRB_HIP(to_device ? copy_components(dev, dp, host, w, w, hipMemcpyHostToDevice)
+                          : copy_components(host, w, dev, dp, w, hipMemcpyDeviceToHost));
+    } else {
+        const size_t bytes = es * NQ * size_t(N);
+        MRB_HIP(to_device ? hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice)
+                          : hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+    }
+    return MRB_OK;
+}
+
 int mrb_upload(mrb_level *L, int arr, const void *host)
 {
     if (!L || !host) return set_error(MRB_E_ARG, "null argument");
@@ -833,10 +886,7 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
     if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
     const int64_t n = NQ * L->total_cells;
     const double *h = static_cast<const double *>(host);
-    if (L->storage == MRB_STORAGE_F64) {
-        MRB_HIP(hipMemcpy(L->pop[arr][L->cur], host, sizeof(double) * n, hipMemcpyHostToDevice));
-        return MRB_OK;
-    }
+    if (L->storage == MRB_STORAGE_F64) return copy_lattice(L, arr, const_cast<void *>(host), true);
     // reduced precision: subtract the rest-state shift and round
     const double *sh = arr == MRB_ARR_F ? L->fshift : L->gshift;
     const int64_t N = L->total_cells;
@@ -844,8 +894,7 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
         std::vector<float> buf(static_cast<size_t>(n));
         for (int q = 0; q < NQ; ++q)
             for (int64_t c = 0; c < N; ++c) buf[size_t(q * N + c)] = float(h[q * N + c] - sh[q]);
-        MRB_HIP(hipMemcpy(L->pop[arr][L->cur], buf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-        return MRB_OK;
+        return copy_lattice(L, arr, buf.data(), true);
     }
     return set_error(MRB_E_UNSUPPORTED, "fp16 storage upload not available");
 }
@@ -860,16 +909,14 @@ int mrb_download(const mrb_level *L, int arr, void *host)
     }
     if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
     const int64_t n = NQ * L->total_cells;
-    if (L->storage == MRB_STORAGE_F64) {
-        MRB_HIP(hipMemcpy(host, L->pop[arr][L->cur], sizeof(double) * n, hipMemcpyDeviceToHost));
-        return MRB_OK;
-    }
+    if (L->storage == MRB_STORAGE_F64) return copy_lattice(L, arr, host, false);
     double *h = static_cast<double *>(host);
     const double *sh = arr == MRB_ARR_F ? L->fshift : L->gshift;
     const int64_t N = L->total_cells;
     if (L->storage == MRB_STORAGE_F32_SHIFTED) {
         std::vector<float> buf(static_cast<size_t>(n));
-        MRB_HIP(hipMemcpy(buf.data(), L->pop[arr][L->cur], sizeof(float) * n, hipMemcpyDeviceToHost));
+        const int rc = copy_lattice(L, arr, buf.data(), false);
+        if (rc) return rc;
         for (int q = 0; q < NQ; ++q)
             for (int64_t c = 0; c < N; ++c) h[q * N + c] = double(buf[size_t(q * N + c)]) + sh[q];
         return MRB_OK;
@@ -1073,7 +1120,8 @@ int mrb_init_constant(
     for (int l = 0; l < 2; ++l) {
         MRB_HIP(hipMemcpy(dval, l == 0 ? fv : gv, sizeof(double) * NQ, hipMemcpyHostToDevice));
         k_init_uniform<<<nblocks(L->total_cells), kBlock>>>((double *)L->pop[l][L->cur], L->d_isf, L->d_boxes,
-                                                            L->d_box_base, L->nbox, L->total_cells, dval);
+                                                            L->d_box_base, L->nbox, L->total_cells, dval,
+                                                            comp_stride(L));
         MRB_HIP(hipGetLastError());
     }
     MRB_HIP(hipDeviceSynchronize());
@@ -1198,6 +1246,57 @@ int mrb_advance_timed(mrb_level *L, int nsteps, void *stream, double *kernel_ms,
     return MRB_OK;
 }
 
+int mrb_hot_kernel_count(void) { return 1 + kNumFusedVariants; }
+
+const char *mrb_hot_kernel_variant_name(int index)
+{
+    if (index == 0) return "k_qpass+k_collide";
+    if (index < 0 || index > kNumFusedVariants) return "";
+    return kFusedVariants[index - 1].name;
+}
+
+int mrb_set_hot_kernel(mrb_level *L, int index)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (index < 0 || index > kNumFusedVariants) return set_error(MRB_E_ARG, "no such kernel variant");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    L->hot = index;
+    return MRB_OK;
+}
+
+int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best)
+{
+    if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    if (reps < 1) return set_error(MRB_E_ARG, "reps must be positive");
+    hipStream_t s = as_stream(stream);
+    const int n = mrb_hot_kernel_count();
+    int arg = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < n; ++v) {
+        auto launch = [&]() { return v == 0 ? compact_two_pass_launch(L, s) : fused_launch(L, v - 1, s); };
+        int rc = launch();  // warm-up
+        if (rc) return rc;
+        MRB_HIP(hipEventRecord(L->ev[3], s));
+        for (int r = 0; r < reps; ++r)
+            if ((rc = launch())) return rc;
+        MRB_HIP(hipEventRecord(L->ev[4], s));
+        MRB_HIP(hipEventSynchronize(L->ev[4]));
+        float ms = 0.f;
+        MRB_HIP(hipEventElapsedTime(&ms, L->ev[3], L->ev[4]));
+        const double mean = double(ms) / reps;
+        if (ms_out) ms_out[v] = mean;
+        if (mean < best_ms) {
+            best_ms = mean;
+            arg = v;
+        }
+    }
+    L->hot = arg;
+    if (best) *best = arg;
+    return MRB_OK;
+}
+
 int mrb_stream_compact(mrb_level *L, void *stream)
 {
     if (!L) return set_error(MRB_E_ARG, "null argument");
@@ -1216,11 +1315,11 @@ int mrb_reduce_conservation(const mrb_level *L, double out[5], void *stream)
     if (L->storage == MRB_STORAGE_F64) {
         k_conservation<double><<<nb, kBlock, 0, s>>>(
             (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red);
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L));
     } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
         k_conservation<float><<<nb, kBlock, 0, s>>>(
             (const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red);
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L));
     } else {
         return set_error(MRB_E_UNSUPPORTED, "conservation sums for this storage");
     }

@@ -204,6 +204,24 @@ class Level:
         names = self.hot_kernel_name().split("+")
         return {names[k] if k < len(names) else f"k{k}": ms[k] for k in range(ns.value)}
 
+    @staticmethod
+    def hot_kernel_variants() -> list[str]:
+        lib = _lib.load()
+        return [lib.mrb_hot_kernel_variant_name(i).decode() for i in range(lib.mrb_hot_kernel_count())]
+
+    def set_hot_kernel(self, name_or_index) -> None:
+        idx = name_or_index if isinstance(name_or_index, int) else self.hot_kernel_variants().index(name_or_index)
+        check(self.lib.mrb_set_hot_kernel(self.h, idx))
+
+    def autotune(self, reps: int = 5, stream=None) -> dict:
+        """Times every kernel variant without advancing the state; selects the fastest.
+        Returns {variant name: mean ms per step}."""
+        names = self.hot_kernel_variants()
+        ms = (c_double * len(names))()
+        best = c_int()
+        check(self.lib.mrb_autotune(self.h, int(reps), stream, ms, ctypes.byref(best)))
+        return {n: ms[i] for i, n in enumerate(names)}
+
     def stream_compact(self, stream=None) -> None:
         check(self.lib.mrb_stream_compact(self.h, stream))
 

@@ -152,3 +152,44 @@ def test_conservation_reduction(oracle, engine):
     E.advance(5)
     a, b = E.conservation(), O.conservation()
     assert np.allclose(a, b, rtol=1e-12, atol=1e-10), (a, b)
+
+
+@pytest.mark.parametrize("case", ["tg", "channel", "odd"])
+def test_every_hot_kernel_variant_matches_oracle(oracle, engine, case):
+    """Every compact-layout kernel variant (two-pass and the single-pass
+    z-sweep tile shapes) gives the oracle's populations."""
+    from oracle.pyoracle import build_oracle_level
+
+    if case == "tg":
+        cfg, n = configs.taylor_green(32), 30
+    elif case == "channel":
+        cfg, n = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 40
+    else:  # sizes that leave partial tiles in x, y and z
+        cfg, n = configs.channel_sphere((37, 29, 23), radius=4.0, center=(12.0, 14.0, 11.0)), 20
+    O = build_oracle_level(cfg)
+    O.evolve(n)
+    ref = {name: O.valid(name) for name in ("f", "g")}
+    for v, vname in enumerate(engine.Level.hot_kernel_variants()):
+        O0 = build_oracle_level(cfg)
+        E = _compact_from(O0, engine, cfg)
+        E.set_hot_kernel(v)
+        E.advance(n)
+        for name in ("f", "g"):
+            err = _rel_err(E.valid(name), ref[name])
+            assert err <= REL_TOL, (case, vname, name, err)
+        E.close()
+
+
+def test_autotune_does_not_advance_the_state(oracle, engine):
+    from oracle.pyoracle import build_oracle_level
+
+    cfg = configs.channel_sphere((40, 24, 24), radius=5.0, center=(14.0, 12.0, 12.0))
+    O = build_oracle_level(cfg)
+    E = _compact_from(O, engine, cfg)
+    f0 = E.valid("f")
+    times = E.autotune(reps=2)
+    assert len(times) == len(engine.Level.hot_kernel_variants())
+    assert np.array_equal(E.valid("f"), f0)
+    E.advance(5)
+    O.evolve(5)
+    assert _rel_err(E.valid("f"), O.valid("f")) <= REL_TOL

@@ -128,3 +128,150 @@ def bytes_per_update(storage: str = "f64", lattices: int = 2) -> int:
 def taylor_green_decay(t: float, nu: float, k: float) -> float:
     """Velocity amplitude factor of a 2D Taylor-Green mode with wavenumber k per axis."""
     return math.exp(-2.0 * nu * k * k * t)
+
+
+# ---------------------------------------------------------------------------
+# Two-level workload
+# ---------------------------------------------------------------------------
+
+def random_boxes(ncell, edges=(8, 16, 32, 64), nbox: int | None = None, seed: int = 0):
+    """Domain chopped into boxes whose edges are powers of two in `edges`:
+    start from max(edges)-sized blocks and split a random box in half along a
+    random splittable axis until there are `nbox` boxes (default: as many as
+    boxes of edge 16 would give).  Deterministic for a seed."""
+    import numpy as np
+
+    emax, emin = max(edges), min(edges)
+    for n in ncell:
+        if n % emax:
+            raise ValueError("domain extents must be multiples of the largest box edge")
+    boxes = [((x, y, z), (x + emax - 1, y + emax - 1, z + emax - 1))
+             for z in range(0, ncell[2], emax) for y in range(0, ncell[1], emax)
+             for x in range(0, ncell[0], emax)]
+    if nbox is None:
+        nbox = int(np.prod(ncell)) // (16 ** 3)
+    rng = np.random.default_rng(seed)
+    while len(boxes) < nbox:
+        splittable = [i for i, (lo, hi) in enumerate(boxes) if any(h - l + 1 > emin for l, h in zip(lo, hi))]
+        if not splittable:
+            break
+        i = splittable[int(rng.integers(len(splittable)))]
+        lo, hi = boxes[i]
+        axes = [d for d in range(3) if hi[d] - lo[d] + 1 > emin]
+        d = axes[int(rng.integers(len(axes)))]
+        half = (hi[d] - lo[d] + 1) // 2
+        hi1 = list(hi)
+        hi1[d] = lo[d] + half - 1
+        lo2 = list(lo)
+        lo2[d] = lo[d] + half
+        boxes[i] = (lo, tuple(hi1))
+        boxes.append((tuple(lo2), hi))
+    return boxes
+
+
+def refine_boxes(coarse_boxes, fraction: float, tol: float = 0.005, max_box: int = 64, seed: int = 0):
+    """Fine-level boxes (refinement ratio 2) over a seeded random subset of
+    the coarse boxes whose volume is fraction +- tol of the union of all of
+    them; refined boxes are chopped to at most max_box per side."""
+    import numpy as np
+
+    vols = [int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in coarse_boxes]
+    total = sum(vols)
+    rng = np.random.default_rng(seed)
+    chosen, acc = [], 0
+    for i in rng.permutation(len(coarse_boxes)):
+        if acc >= (fraction - tol) * total:
+            break
+        if acc + vols[i] <= (fraction + tol) * total:
+            chosen.append(int(i))
+            acc += vols[i]
+    fine = []
+    for i in sorted(chosen):
+        lo, hi = coarse_boxes[i]
+        flo = [2 * a for a in lo]
+        fhi = [2 * h + 1 for h in hi]
+        for z in range(flo[2], fhi[2] + 1, max_box):
+            for y in range(flo[1], fhi[1] + 1, max_box):
+                for x in range(flo[0], fhi[0] + 1, max_box):
+                    blo = (x, y, z)
+                    bhi = tuple(min(a + max_box - 1, h) for a, h in zip(blo, fhi))
+                    fine.append((blo, bhi))
+    return fine, acc / total
+
+
+def amr_two_level(ncell=(256, 256, 256), edges=(8, 16, 32, 64), nbox: int | None = None,
+                  fine_fraction: float = 0.2, tau: float = 0.6, prandtl: float = 0.71,
+                  temperature: float = 0.2, noise: float = 0.05, seed: int = 3) -> dict:
+    """Two levels, ratio 2, periodic: seeded uneven coarse boxes with edges in
+    `edges`, a fine level over a seeded 20% of them, thermal f+g with
+    nu = (tau - 1/2) R T0 and alpha = nu / Pr, at rest with temperature
+    T0 (1 + noise * n(x)), n a seeded smooth field with max |n| <= 1.
+
+    T0 = 0.2: the reference thermal model is linearly unstable at T0 = 0.3
+    with tau = 0.6 (a grid-scale energy mode grows), stable at T0 = 0.2 on
+    both levels (coarse tau = 0.6, fine tau = 2 (tau - 1/2) + 1/2 = 0.7)."""
+    coarse = random_boxes(ncell, edges, nbox, seed)
+    fine, frac = refine_boxes(coarse, fine_fraction, seed=seed + 1)
+    nu = (tau - 0.5) * temperature
+    return {
+        "name": "amr2_{}x{}x{}".format(*ncell),
+        "ncell": tuple(ncell),
+        "periodic": (1, 1, 1),
+        "prob_lo": (0.0, 0.0, 0.0),
+        "prob_hi": tuple(float(n) for n in ncell),
+        "bc_lo": (0, 0, 0),
+        "bc_hi": (0, 0, 0),
+        "velocity_bc": None,
+        "nu": nu,
+        "alpha": nu / prandtl,
+        "tau": tau,
+        "T0": temperature,
+        "ic": {"type": "thermal_noise", "density": 1.0, "temperature": temperature, "noise": noise,
+               "seed": seed + 2, "adiabatic_exponent": 5.0 / 3.0},
+        "eb": {"type": "all_regular"},
+        "coarse_boxes": coarse,
+        "fine_boxes": fine,
+        "fine_fraction": frac,
+        "ratio": 2,
+    }
+
+
+def smooth_noise(ic: dict, prob_lo, prob_hi, x, y, z):
+    """Seeded smooth field with max |n| <= 1 at physical points (x, y, z):
+    a sum of 24 random Fourier modes with integer wavevectors |k_d| <= 3 over
+    the domain, normalised by the sum of the amplitudes."""
+    import numpy as np
+
+    rng = np.random.default_rng(ic["seed"])
+    nm = 24
+    k = rng.integers(-3, 4, size=(nm, 3))
+    amp = rng.uniform(0.5, 1.0, size=nm)
+    phase = rng.uniform(0.0, 2.0 * math.pi, size=nm)
+    L = [h - l for l, h in zip(prob_lo, prob_hi)]
+    out = np.zeros(np.broadcast(x, y, z).shape)
+    for m in range(nm):
+        arg = (2.0 * math.pi * (k[m, 0] * (x - prob_lo[0]) / L[0] + k[m, 1] * (y - prob_lo[1]) / L[1]
+                                + k[m, 2] * (z - prob_lo[2]) / L[2]) + phase[m])
+        out += amp[m] * np.cos(arg)
+    return out / amp.sum()
+
+
+def level_fields(cfg: dict, ncell, boxes, ng: int):
+    """(rho, u, v, w, T) of the thermal-noise initial condition at the cell
+    centres of every fab cell (ghosts included) of `boxes` on a level with
+    `ncell` cells over the workload's physical domain; box order, x fastest."""
+    import numpy as np
+
+    ic = cfg["ic"]
+    plo, phi = cfg["prob_lo"], cfg["prob_hi"]
+    dx = [(h - l) / n for l, h, n in zip(plo, phi, ncell)]
+    parts = []
+    for lo, hi in boxes:
+        ax = [plo[d] + (np.arange(lo[d] - ng, hi[d] + ng + 1) + 0.5) * dx[d] for d in range(3)]
+        zz, yy, xx = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        parts.append(smooth_noise(ic, plo, phi, xx, yy, zz).ravel())
+    n = np.concatenate(parts)
+    T = ic["temperature"] * (1.0 + ic["noise"] * n)
+    rho = np.full_like(T, ic["density"])
+    zero = np.zeros_like(T)
+    return rho, zero, zero.copy(), zero.copy(), T

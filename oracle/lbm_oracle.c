@@ -697,9 +697,10 @@ void orc_physbc(orc_level *L, int lattice)
 
 /* FillPatchOps::fillpatch for level 0 (FillPatchOps.H:75-132): the bounce
  * prefill (92-110), then FillPatchSingleLevel = FillBoundary + physbc. */
-void orc_fillpatch(orc_level *L, int lattice)
+/* FillPatchOps.H:92-108: domain-outside ghosts take the opposite population
+ * of their in-domain neighbour (overwritten afterwards by physbc). */
+static void bounce_prefill(orc_level *L, double **arr)
 {
-    double **arr = lattice == 0 ? L->f : L->g;
     for (int b = 0; b < L->nbox; b++) {
         const fabdim *fd = &L->fdg[b];
         double *f = arr[b];
@@ -715,6 +716,11 @@ void orc_fillpatch(orc_level *L, int lattice)
                     }
                 }
     }
+}
+
+void orc_fillpatch(orc_level *L, int lattice)
+{
+    bounce_prefill(L, lattice == 0 ? L->f : L->g);
     orc_fill_boundary(L, lattice == 0 ? ORC_FAB_F : ORC_FAB_G);
     orc_physbc(L, lattice);
 }
@@ -1293,4 +1299,228 @@ void orc_init_finish(orc_level *L)
     orc_f_to_macrodata(L);
     orc_macrodata_to_equilibrium(L);
     orc_compute_q_corrections(L);
+}
+
+/* ic::Initializer::initialize (IC.H:460-525) from per-cell fields. */
+void orc_ic_fields(
+    orc_level *L, const double *rho, const double *u, const double *v, const double *w,
+    const double *T, double adiabatic_exponent, double m_bar)
+{
+    const double R = AIR_MOLAR_MASS / m_bar;
+    int64_t base = 0;
+    for (int b = 0; b < L->nbox; b++) {
+        const fabdim *fd = &L->fdg[b];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int k = fd->lo[2]; k <= fd->hi[2]; k++)
+            for (int j = fd->lo[1]; j <= fd->hi[1]; j++)
+                for (int i = fd->lo[0]; i <= fd->hi[0]; i++) {
+                    const int64_t c = base + fidx(fd, 0, i, j, k);
+                    const double vel[3] = {u[c], v[c], w[c]};
+                    init_cell(L, b, i, j, k, rho[c], vel, T[c], R, adiabatic_exponent);
+                }
+        base += fd->ncell;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Two levels, refinement ratio 2.                                           */
+/* ------------------------------------------------------------------------ */
+
+static void amr_check(const orc_level *crse, const orc_level *fine)
+{
+    for (int d = 0; d < 3; d++) {
+        if (fine->nd[d] != 2 * crse->nd[d] || fine->periodic[d] != crse->periodic[d]) {
+            fprintf(stderr, "oracle: fine level is not the coarse domain refined by 2\n");
+            abort();
+        }
+    }
+    if (fine->ng < 2) {
+        fprintf(stderr, "oracle: the fine level needs at least 2 ghost layers\n");
+        abort();
+    }
+    for (int b = 0; b < fine->nbox; b++)
+        for (int d = 0; d < 3; d++)
+            if ((fine->vlo[b][d] & 1) || !(fine->vhi[b][d] & 1)) {
+                fprintf(stderr, "oracle: fine box %d is not coarse-aligned\n", b);
+                abort();
+            }
+}
+
+/* average_down_with_ghosts (Utilities.cpp:5-28), crse_ng = 1, ratio 2.
+ * Step 1 (cfine.ParallelCopy(crse)): every cfine cell starts as the coarse
+ * value of its periodic image (the reference copies without periodicity
+ * and leaves out-of-domain cfine ghosts uninitialised; they only matter
+ * where all 8 children are masked).  Step 2: masked_avgdown in place.
+ * Step 3 (crse.ParallelCopy(cfine, periodic)): cfine boxes are copied back
+ * in box order, ghosts included, so the last box wins on overlaps. */
+void orc_amr_coalesce(const orc_level *fine, orc_level *crse, int lattice)
+{
+    amr_check(crse, fine);
+    double **fa = lattice == 0 ? fine->f : fine->g;
+    double **ca = lattice == 0 ? crse->f : crse->g;
+    fabdim *cf = calloc((size_t)fine->nbox, sizeof(fabdim));
+    double **cv = calloc((size_t)fine->nbox, sizeof(double *));
+    for (int b = 0; b < fine->nbox; b++) {
+        int clo[3], chi[3];
+        for (int d = 0; d < 3; d++) {
+            clo[d] = fine->vlo[b][d] / 2;
+            chi[d] = fine->vhi[b][d] / 2;
+        }
+        fabdim_set(&cf[b], clo, chi, 1);
+        cv[b] = dalloc(ORC_NQ * cf[b].ncell, 0.0);
+        const fabdim *cd = &cf[b];
+        const fabdim *fd = &fine->fdg[b];
+        const double *fv = fa[b];
+        double *out = cv[b];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int k = cd->lo[2]; k <= cd->hi[2]; k++)
+            for (int j = cd->lo[1]; j <= cd->hi[1]; j++)
+                for (int i = cd->lo[0]; i <= cd->hi[0]; i++) {
+                    const int iv[3] = {i, j, k};
+                    int s[3];
+                    const int has_image = periodic_image(crse, iv, s);
+                    const int ob = has_image ? owner_of(crse, s) : -1;
+                    for (int n = 0; n < ORC_NQ; n++) {
+                        /* cfine.ParallelCopy(crse, ..., IntVect(0), ng) */
+                        out[fidx(cd, n, i, j, k)] =
+                            ob >= 0 ? ca[ob][fidx(&crse->fdg[ob], n, s[0], s[1], s[2])] : 0.0;
+                        /* masked_avgdown (Utilities.H:315-350) */
+                        double c = 0;
+                        double vol = 0;
+                        for (int kref = 0; kref < 2; ++kref)
+                            for (int jref = 0; jref < 2; ++jref)
+                                for (int iref = 0; iref < 2; ++iref) {
+                                    const double fval = fv[fidx(
+                                        fd, n, 2 * i + iref, 2 * j + jref, 2 * k + kref)];
+                                    if (fabs(fval - (-1.0)) > SMALL_NUM) {
+                                        c += fval;
+                                        vol += 1.0;
+                                    }
+                                }
+                        if (vol > 0.0) out[fidx(cd, n, i, j, k)] = c / vol;
+                    }
+                }
+    }
+    /* crse.ParallelCopy(cfine, ..., ng, IntVect(0), periodicity) */
+    for (int b = 0; b < fine->nbox; b++) {
+        const fabdim *cd = &cf[b];
+        for (int k = cd->lo[2]; k <= cd->hi[2]; k++)
+            for (int j = cd->lo[1]; j <= cd->hi[1]; j++)
+                for (int i = cd->lo[0]; i <= cd->hi[0]; i++) {
+                    const int iv[3] = {i, j, k};
+                    int s[3];
+                    if (!periodic_image(crse, iv, s)) continue;
+                    const int ob = owner_of(crse, s);
+                    if (ob < 0) continue;
+                    for (int n = 0; n < ORC_NQ; n++)
+                        ca[ob][fidx(&crse->fdg[ob], n, s[0], s[1], s[2])] =
+                            cv[b][fidx(cd, n, i, j, k)];
+                }
+        free(cv[b]);
+    }
+    free(cv);
+    free(cf);
+}
+
+/* Conservative linear interpolation with monotonised-central limited
+ * slopes: child = c + s_x o_x + s_y o_y + s_z o_z, o = -1/4 (low child) or
+ * +1/4 (high child) in coarse-cell units, so the mean of the 8 children is c
+ * and linear coarse data is reproduced exactly.  Slope per direction:
+ * s = sign(dc) min(|dc|, 2|dl|, 2|dr|) with dc = (c+ - c-)/2, dl = c - c-,
+ * dr = c+ - c, and 0 where dl and dr differ in sign (local extremum) or a
+ * neighbour is outside the domain. */
+double orc_amr_interp(
+    double c, const double cm[3], const double cp[3], const int has_m[3],
+    const int has_p[3], const int hi_child[3])
+{
+    double s[3];
+    for (int d = 0; d < 3; d++) {
+        s[d] = 0.0;
+        if (!has_m[d] || !has_p[d]) continue;
+        const double dc = 0.5 * (cp[d] - cm[d]);
+        const double dl = c - cm[d];
+        const double dr = cp[d] - c;
+        if (dl * dr > 0.0)
+            s[d] = copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc);
+    }
+    const double ox = hi_child[0] ? 0.25 : -0.25;
+    const double oy = hi_child[1] ? 0.25 : -0.25;
+    const double oz = hi_child[2] ? 0.25 : -0.25;
+    return ((c + s[0] * ox) + s[1] * oy) + s[2] * oz;
+}
+
+void orc_amr_explode(const orc_level *crse, orc_level *fine, int lattice)
+{
+    amr_check(crse, fine);
+    double **fa = lattice == 0 ? fine->f : fine->g;
+    double **ca = lattice == 0 ? crse->f : crse->g;
+    bounce_prefill(fine, fa);
+    orc_fill_boundary(fine, lattice == 0 ? ORC_FAB_F : ORC_FAB_G);
+    for (int b = 0; b < fine->nbox; b++) {
+        const fabdim *fd = &fine->fdg[b];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int k = fd->lo[2]; k <= fd->hi[2]; k++)
+            for (int j = fd->lo[1]; j <= fd->hi[1]; j++)
+                for (int i = fd->lo[0]; i <= fd->hi[0]; i++) {
+                    if (valid_contains(fine, b, i, j, k)) continue;
+                    const int iv[3] = {i, j, k};
+                    int s[3];
+                    if (!periodic_image(fine, iv, s)) continue; /* physbc */
+                    if (owner_of(fine, s) >= 0) continue;       /* FillBoundary */
+                    const int P[3] = {s[0] / 2, s[1] / 2, s[2] / 2};
+                    const int ob = owner_of(crse, P);
+                    if (ob < 0) {
+                        fprintf(stderr, "oracle: coarse level does not cover its domain\n");
+                        abort();
+                    }
+                    const fabdim *cd = &crse->fdg[ob];
+                    int has_m[3], has_p[3], hi_child[3];
+                    for (int d = 0; d < 3; d++) {
+                        has_m[d] = crse->periodic[d] || P[d] - 1 >= crse->dom_lo[d];
+                        has_p[d] = crse->periodic[d] || P[d] + 1 <= crse->dom_hi[d];
+                        hi_child[d] = s[d] & 1;
+                    }
+                    for (int q = 0; q < ORC_NQ; q++) {
+                        const double *cq = ca[ob];
+                        const double c = cq[fidx(cd, q, P[0], P[1], P[2])];
+                        double cm[3], cp[3];
+                        for (int d = 0; d < 3; d++) {
+                            int m[3] = {P[0], P[1], P[2]}, p[3] = {P[0], P[1], P[2]};
+                            m[d] -= 1;
+                            p[d] += 1;
+                            cm[d] = has_m[d] ? cq[fidx(cd, q, m[0], m[1], m[2])] : 0.0;
+                            cp[d] = has_p[d] ? cq[fidx(cd, q, p[0], p[1], p[2])] : 0.0;
+                        }
+                        fa[b][fidx(fd, q, i, j, k)] =
+                            orc_amr_interp(c, cm, cp, has_m, has_p, hi_child);
+                    }
+                }
+    }
+    orc_physbc(fine, lattice);
+}
+
+/* LBM::evolve -> time_step(0) -> [fillpatch(1); 2 x (physbc(1); time_step(1)
+ * -> advance(1))] -> advance(0) with average_down_to (LBM.cpp:416-419,
+ * 498-512, 523-544). */
+void orc_amr_evolve(orc_level *crse, orc_level *fine, int nsteps)
+{
+    amr_check(crse, fine);
+    for (int s = 0; s < nsteps; s++) {
+        orc_fillpatch(crse, 0);
+        orc_fillpatch(crse, 1);
+        orc_amr_explode(crse, fine, 0);
+        orc_amr_explode(crse, fine, 1);
+        for (int sub = 0; sub < 2; sub++) {
+            orc_physbc(fine, 0);
+            orc_physbc(fine, 1);
+            orc_stream(fine, 0);
+            orc_stream(fine, 1);
+            orc_collide(fine);
+        }
+        orc_stream(crse, 0);
+        orc_stream(crse, 1);
+        orc_amr_coalesce(fine, crse, 0);
+        orc_amr_coalesce(fine, crse, 1);
+        orc_collide(crse);
+    }
 }

@@ -95,6 +95,12 @@ void orc_ic_taylorgreen(orc_level *L, double rho0, double v0, const double omega
 void orc_ic_constant(
     orc_level *L, double density, const double mach[3], double temperature,
     double adiabatic_exponent, double m_bar);
+/* ic::Initializer::initialize (IC.H:460-525) from per-cell macroscopic
+ * fields given for every fab cell of every box (box order, x fastest):
+ * f = f_eq^prod(rho, u, R T), g = Grad equilibrium. */
+void orc_ic_fields(
+    orc_level *L, const double *rho, const double *u, const double *v, const double *w,
+    const double *T, double adiabatic_exponent, double m_bar);
 /* fill_f_inside_eb, FillBoundary, f_to_macrodata, equilibria, dQ */
 void orc_init_finish(orc_level *L);
 
@@ -112,6 +118,34 @@ void orc_collide(orc_level *L);                     /* LBM::collide */
 void orc_evolve(orc_level *L, int nsteps);
 /* conservation sums over valid fluid cells: mass, x/y/z momentum, 2rhoE */
 void orc_conservation(const orc_level *L, double out[5]);
+
+/* ---- two levels, refinement ratio 2 -------------------------------------
+ * The coarse level must cover its domain; the fine level's domain is the
+ * coarse one refined by 2, its boxes are coarse-aligned (even lo, odd hi)
+ * and it has ng >= 2 ghost layers. */
+/* average_down_with_ghosts (Utilities.cpp:5-28) with ng = 1, ratio 2:
+ * coarse cells under the fine boxes and one coarse ring around each become
+ * the masked mean of their 8 fine children (masked_avgdown, Utilities.H:
+ * 315-350).  Where the rings of several fine boxes overlap, the box latest
+ * in the box list wins (AMReX leaves the order of overlapping ParallelCopy
+ * sources unspecified). */
+void orc_amr_coalesce(const orc_level *fine, orc_level *crse, int lattice);
+/* FillPatchOps::fillpatch, two-level branch (FillPatchOps.H:118-129):
+ * fine ghost cells covered by fine boxes are copied (FillBoundary), fine
+ * ghost cells over uncovered domain are interpolated from the coarse level
+ * (own conservative, MC-limited linear scheme in place of AMReX
+ * cell_cons_interp), ghost cells beyond non-periodic faces get physbc. */
+void orc_amr_explode(const orc_level *crse, orc_level *fine, int lattice);
+/* the interpolation formula alone for one fine child: coarse value c, its
+ * -/+ neighbours per direction (has_m/has_p = 0 drops that direction's
+ * slope), child offset sign per direction (0 = low child, 1 = high) */
+double orc_amr_interp(
+    double c, const double cm[3], const double cp[3], const int has_m[3],
+    const int has_p[3], const int hi_child[3]);
+/* nsteps coarse steps of LBM::evolve/time_step/advance (LBM.cpp:398-544)
+ * for a two-level hierarchy; the fine level advances two substeps per
+ * coarse step with its own phys (dt halved). */
+void orc_amr_evolve(orc_level *crse, orc_level *fine, int nsteps);
 
 /* ---- per-cell math, exposed so tests can pin it directly ---------------- */
 void orc_stencil(int *evs /*27*3*/, int *bounce /*27*/, double *w /*27*/);

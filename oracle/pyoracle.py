@@ -88,6 +88,12 @@ def load() -> ctypes.CDLL:
         "orc_get_temperature_vec": (c_double, [c_double, c_double, D3, c_double]),
         "orc_masked_avgdown8": (c_double, [D3]),
         "orc_sphere_covered": (c_int, [D3, D3, D3, c_double, c_int]),
+        "orc_ic_fields": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_double, c_double]),
+        "orc_amr_coalesce": (None, [c_void_p, c_void_p, c_int]),
+        "orc_amr_explode": (None, [c_void_p, c_void_p, c_int]),
+        "orc_amr_interp": (c_double, [c_double, D3, D3, I3, I3, I3]),
+        "orc_amr_evolve": (None, [c_void_p, c_void_p, c_int]),
         "orc_set_threads": (None, [c_int]),
         "orc_get_threads": (c_int, []),
     }
@@ -233,6 +239,15 @@ class OracleLevel:
                     m_bar=AIR_MOLAR_MASS):
         self.lib.orc_ic_constant(self.h, density, d3(mach), temperature, adiabatic_exponent, m_bar)
 
+    def ic_fields(self, rho, u, v, w, T, adiabatic_exponent=5.0 / 3.0, m_bar=AIR_MOLAR_MASS):
+        """Per-cell macroscopic fields over every fab cell (box order, x fastest)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w, T)]
+        n = sum(int(np.prod(sh)) for sh in self.fab_shapes)
+        for a in arrs:
+            assert a.size == n, (a.size, n)
+        self.lib.orc_ic_fields(self.h, *[a.ctypes.data_as(c_void_p) for a in arrs],
+                               float(adiabatic_exponent), float(m_bar))
+
     def init_finish(self):
         self.lib.orc_init_finish(self.h)
 
@@ -270,6 +285,22 @@ class OracleLevel:
         out = (c_double * 5)()
         self.lib.orc_conservation(self.h, out)
         return np.array(out[:])
+
+
+def amr_coalesce(fine: OracleLevel, crse: OracleLevel, lattice: int) -> None:
+    load().orc_amr_coalesce(fine.h, crse.h, int(lattice))
+
+
+def amr_explode(crse: OracleLevel, fine: OracleLevel, lattice: int) -> None:
+    load().orc_amr_explode(crse.h, fine.h, int(lattice))
+
+
+def amr_evolve(crse: OracleLevel, fine: OracleLevel, nsteps: int) -> None:
+    load().orc_amr_evolve(crse.h, fine.h, int(nsteps))
+
+
+def amr_interp(c, cm, cp, has_m, has_p, hi_child) -> float:
+    return load().orc_amr_interp(float(c), d3(cm), d3(cp), i3(has_m), i3(has_p), i3(hi_child))
 
 
 def set_threads(n: int) -> None:

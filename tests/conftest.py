@@ -26,6 +26,7 @@ def oracle():
     from oracle import pyoracle
 
     pyoracle.load()
+    pyoracle.set_threads(min(8, os.cpu_count() or 1))
     return pyoracle
 
 

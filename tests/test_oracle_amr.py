@@ -1,0 +1,214 @@
+"""Pins of the oracle's two-level pieces: coalescence (masked average-down
+with ghosts), explosion (fine ghost fill from the coarse level) and the
+coarse-step schedule.
+
+Explosion uses this project's own interpolation (conservative linear with
+monotonised-central limited slopes) in place of AMReX's cell_cons_interp,
+which is not available here; it is pinned by conservation, linear
+exactness and hand-computed limiter cases.  Coalescence follows
+Utilities.cpp:5-28 / Utilities.H:315-350 and is pinned by a vectorised
+restatement of its definition on random data with sentinels; the overlap
+order of fine-box rings (last box wins) is this project's choice.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from amr_cases import fine_ncell, masked_mean, oracle_hierarchy, small_amr
+from marbles_amd import configs
+
+
+def _children_mean(oracle, c, cm, cp, hm, hp):
+    vals = [oracle.amr_interp(c, cm, cp, hm, hp, hc) for hc in itertools.product((0, 1), repeat=3)]
+    return np.mean(vals), vals
+
+
+def test_interp_conserves_the_coarse_value(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        c = rng.uniform(0.1, 2.0)
+        cm, cp = rng.uniform(0.1, 2.0, 3), rng.uniform(0.1, 2.0, 3)
+        hm, hp = rng.integers(0, 2, 3), rng.integers(0, 2, 3)
+        mean, _ = _children_mean(oracle, c, cm, cp, hm, hp)
+        assert abs(mean - c) <= 4e-16 * c, (c, mean)
+
+
+def test_interp_reproduces_linear_data(oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(100):
+        a, b = rng.uniform(1, 2), rng.uniform(-0.1, 0.1, 3)
+        cm = a - b
+        cp = a + b
+        for hc in itertools.product((0, 1), repeat=3):
+            v = oracle.amr_interp(a, cm, cp, (1, 1, 1), (1, 1, 1), hc)
+            expect = a + sum(b[d] * (0.25 if hc[d] else -0.25) for d in range(3))
+            assert abs(v - expect) <= 1e-15, (v, expect)
+
+
+def test_interp_limiter_known_answers(oracle):
+    # steep right side: dc = 5, 2|dl| = 2, 2|dr| = 18 -> slope 2 -> children 1 -+ 0.5
+    flat = (1.0, 1.0)
+    v = [oracle.amr_interp(1.0, (0.0, 1.0, 1.0), (10.0, 1.0, 1.0), (1, 1, 1), (1, 1, 1), (h, 0, 0))
+         for h in (0, 1)]
+    assert v == [0.5, 1.5]
+    # local extremum -> zero slope
+    v = [oracle.amr_interp(1.0, (2.0, 1.0, 1.0), (3.0, 1.0, 1.0), (1, 1, 1), (1, 1, 1), (h, 0, 0))
+         for h in (0, 1)]
+    assert v == [1.0, 1.0]
+    # neighbour outside the domain -> zero slope in that direction only
+    v = oracle.amr_interp(1.0, (0.0, 0.5, 1.0), (2.0, 1.5, 1.0), (0, 1, 1), (1, 1, 1), (1, 1, 0))
+    assert v == 1.0 + 0.5 * 0.25
+    del flat
+
+
+def _linear(q, x, y, z):
+    return 1.0 + 0.01 * q + 0.003 * x - 0.002 * y + 0.001 * z
+
+
+def test_explode_copies_fine_neighbours_and_interpolates_linear_data(oracle):
+    cfg = small_amr()
+    C, F = oracle_hierarchy(cfg)
+    # coarse fabs (ghosts included) := linear function of the cell centre
+    for b, (lo, hi) in enumerate(C.boxes):
+        fab = C.fab("f", b)
+        nz, ny, nx = fab.shape[1:]
+        z, y, x = np.meshgrid(*[np.arange(lo[d] - C.ng, lo[d] - C.ng + n) + 0.5
+                                for d, n in zip((2, 1, 0), (nz, ny, nx))], indexing="ij")
+        for q in range(27):
+            fab[q] = _linear(q, x, y, z)
+        C.set_fab("f", b, fab)
+    oracle.amr_explode(C, F, 0)
+    nfine = fine_ncell(cfg)
+    owner = -np.ones(tuple(reversed(nfine)), dtype=np.int64)
+    for b, (lo, hi) in enumerate(F.boxes):
+        owner[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = b
+    fvalid = F.valid("f")
+    ncoarse = cfg["ncell"]
+    checked_interp = checked_copy = 0
+    for b, (lo, hi) in enumerate(F.boxes):
+        fab = F.fab("f", b)
+        for kk, jj, ii in itertools.product(*[range(fab.shape[1 + a]) for a in range(3)]):
+            i, j, k = lo[0] - F.ng + ii, lo[1] - F.ng + jj, lo[2] - F.ng + kk
+            if lo[0] <= i <= hi[0] and lo[1] <= j <= hi[1] and lo[2] <= k <= hi[2]:
+                continue
+            s = (i % nfine[0], j % nfine[1], k % nfine[2])
+            if owner[s[2], s[1], s[0]] >= 0:
+                assert np.array_equal(fab[:, kk, jj, ii], fvalid[:, s[2], s[1], s[0]])
+                checked_copy += 1
+                continue
+            P = [v // 2 for v in s]
+            if all(1 <= P[d] <= ncoarse[d] - 2 for d in range(3)):
+                xc = [(s[d] + 0.5) / 2.0 for d in range(3)]
+                expect = np.array([_linear(q, *xc) for q in range(27)])
+                assert np.allclose(fab[:, kk, jj, ii], expect, rtol=0, atol=2e-15)
+                checked_interp += 1
+    assert checked_interp > 100 and checked_copy > 100, (checked_interp, checked_copy)
+
+
+def _coarsen_owner(cfg, F, C):
+    """coarse cell -> (last) fine box whose coarsened box grown by 1 holds it
+    (periodic images), -1 elsewhere."""
+    nc = cfg["ncell"]
+    win = -np.ones(tuple(reversed(nc)), dtype=np.int64)
+    where = {}
+    for b, (lo, hi) in enumerate(F.boxes):
+        for k, j, i in itertools.product(*[range(lo[d] // 2 - 1, hi[d] // 2 + 2) for d in (2, 1, 0)]):
+            s = (i % nc[0], j % nc[1], k % nc[2])
+            win[s[2], s[1], s[0]] = b
+            where[(s, b)] = (i, j, k)
+    return win, where
+
+
+def test_coalesce_matches_masked_average_definition(oracle):
+    cfg = small_amr(seed=7)
+    C, F = oracle_hierarchy(cfg)
+    rng = np.random.default_rng(3)
+    for b in range(len(F.boxes)):
+        fab = rng.uniform(0.5, 1.5, F.fab("f", b).shape)
+        fab[rng.random(fab.shape) < 0.3] = -1.0
+        fab[:, ::3, ::2, :] = -1.0   # some cells with all 8 children masked
+        F.set_fab("f", b, fab)
+    before = C.valid("f")
+    oracle.amr_coalesce(F, C, 0)
+    after = C.valid("f")
+    win, where = _coarsen_owner(cfg, F, C)
+    nc = cfg["ncell"]
+    expect = before.copy()
+    for k, j, i in itertools.product(*[range(n) for n in reversed(nc)]):
+        b = win[k, j, i]
+        if b < 0:
+            continue
+        ci, cj, ck = where[((i, j, k), b)]
+        lo = F.boxes[b][0]
+        fab = F.fab("f", b)
+        ch = []
+        for kr, jr, ir in itertools.product((0, 1), repeat=3):
+            fi, fj, fk = 2 * ci + ir, 2 * cj + jr, 2 * ck + kr
+            ch.append(fab[:, fk - lo[2] + F.ng, fj - lo[1] + F.ng, fi - lo[0] + F.ng])
+        m = masked_mean(np.array(ch))
+        expect[:, k, j, i] = np.where(np.isnan(m), before[:, k, j, i], m)
+    np.testing.assert_allclose(after, expect, rtol=2e-16, atol=0)
+    assert (win >= 0).any() and (win < 0).any()
+
+
+def test_full_cover_fine_level_equals_single_level_fine_run(oracle):
+    """With the fine level covering the whole domain, its evolution is the
+    single-level run at the fine resolution (dt = 1/2, 2 substeps per
+    coarse step): pins the substep schedule bit for bit."""
+    from oracle.pyoracle import OracleLevel
+
+    cfg = small_amr(n=8, nbox=4)
+    full = configs.refine_boxes(cfg["coarse_boxes"], 1.0, tol=0.0)[0]
+    C, F = oracle_hierarchy(cfg, fine_boxes=full)
+    S = OracleLevel(fine_ncell(cfg), cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=full, ng=3)
+    S.set_phys(cfg["nu"], cfg["alpha"], dt=0.5)
+    S.flags_all_regular()
+    S.ic_fields(*configs.level_fields(cfg, S.ncell, S.boxes, 3))
+    S.init_finish()
+    oracle.amr_evolve(C, F, 3)
+    S.evolve(6)
+    for name in ("f", "g"):
+        assert np.array_equal(F.valid(name), S.valid(name)), name
+    # the coarse level is then the average of the fine one, collided once
+    assert np.isfinite(C.valid("f")).all()
+
+
+def test_rest_state_is_a_fixed_point_of_the_hierarchy(oracle):
+    cfg = small_amr(noise=0.0)
+    C, F = oracle_hierarchy(cfg)
+    c0, f0 = C.valid("f"), F.valid("f")
+    g0 = C.valid("g")
+    oracle.amr_evolve(C, F, 3)
+    # rounding of the collision only: a few ulps of the largest population per step
+    np.testing.assert_allclose(C.valid("f"), c0, rtol=0, atol=2e-14 * c0.max())
+    np.testing.assert_allclose(F.valid("f"), f0, rtol=0, atol=2e-14 * f0.max())
+    np.testing.assert_allclose(C.valid("g"), g0, rtol=0, atol=2e-14 * g0.max())
+
+
+def test_hierarchy_tracks_the_coarse_single_level_run(oracle):
+    """A smooth temperature perturbation evolved on the hierarchy stays
+    within a few percent of the coarse single-level run (same physics,
+    refined in 30 % of the volume; the difference, 3-4 % of the initial
+    perturbation, is mostly the cell average vs point value of the initial
+    field).  Coalescing before the coarse stream instead of after it gives
+    13 %, so the bound catches ordering mistakes of the schedule."""
+    from oracle.pyoracle import OracleLevel
+
+    cfg = small_amr(n=32, nbox=64, seed=11)
+    C, F = oracle_hierarchy(cfg)
+    S = OracleLevel(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=cfg["coarse_boxes"], ng=3)
+    S.set_phys(cfg["nu"], cfg["alpha"])
+    S.flags_all_regular()
+    S.ic_fields(*configs.level_fields(cfg, S.ncell, S.boxes, 3))
+    S.init_finish()
+    T0 = cfg["T0"]
+    dT0 = np.abs(S.valid("md")[18] - T0).max()
+    n = 10
+    oracle.amr_evolve(C, F, n)
+    S.evolve(n)
+    C.f_to_macrodata()
+    S.f_to_macrodata()
+    a, b = C.valid("md")[18] - T0, S.valid("md")[18] - T0
+    rel = np.abs(a - b).max() / dT0
+    assert np.isfinite(a).all() and rel < 0.06, rel

@@ -134,6 +134,15 @@ int mrb_init_taylorgreen(mrb_level *lvl, double rho0, double v0, const double om
 int mrb_init_constant(
     mrb_level *lvl, double density, const double mach[3], double temperature,
     double adiabatic_exponent, double m_bar);
+/* Initial state from per-cell macroscopic fields, the data a user
+ * ic::InitializerBase plugin (Source/IC.H:427-531) computes per cell: rho,
+ * velocity (u, v, w) and temperature for every fab cell of every box (box
+ * order, x fastest; mrb_fab_size(lvl, MRB_ARR_ISFLUID, b) / 2 cells per
+ * box), then f = f_eq(rho, u, R T), g = Grad equilibrium, zero inside the EB
+ * (IC.H:460-525, LBM.cpp:1278-1298).  Synchronous. */
+int mrb_init_fields(
+    mrb_level *lvl, const double *rho, const double *u, const double *v, const double *w,
+    const double *T, double adiabatic_exponent, double m_bar);
 
 /* ---- reference-layout stages (MRB_LAYOUT_FAB) ----------------------------
  * FillPatchOps::fillpatch at level 0 (Source/FillPatchOps.H:75-132):
@@ -183,6 +192,35 @@ int mrb_stream_compact(mrb_level *lvl, void *stream);
  * FP64 sums over valid fluid cells: mass, x/y/z momentum, 2*rho*E.
  * Synchronous (waits for `stream`). */
 int mrb_reduce_conservation(const mrb_level *lvl, double out[5], void *stream);
+
+/* ---- two levels, refinement ratio 2 (MRB_LAYOUT_FAB, FP64) ---------------
+ * The coarse level covers its domain; the fine level's domain is the coarse
+ * one refined by 2 (same physical extent and periodicity), its boxes are
+ * coarse-aligned (even lo, odd hi) and it has ng >= 2.  mrb_amr_link builds
+ * the transfer maps once (host work proportional to the fab cells) and keeps
+ * them with the fine level, which must not outlive the coarse one.
+ * Relinking replaces the maps (after a regrid). */
+int mrb_amr_link(mrb_level *crse, mrb_level *fine);
+/* LBM::average_down_to (Source/LBM.cpp:1571-1584) ->
+ * average_down_with_ghosts (Source/Utilities.cpp:5-28, Utilities.H:353-358)
+ * with crse_ng = 1: coarse cells under the fine boxes and one coarse ring
+ * around each become the masked mean of their 8 fine children
+ * (masked_avgdown, Utilities.H:315-350; unchanged when all 8 are -1).
+ * Where rings of several fine boxes overlap, the box latest in the fine box
+ * list wins.  Both lattices; asynchronous on `stream`. */
+int mrb_coalesce(const mrb_level *fine, mrb_level *crse, void *stream);
+/* FillPatchOps::fillpatch on a fine level (Source/FillPatchOps.H:66-130,
+ * FillPatchTwoLevels branch): FillBoundary among fine boxes, fine ghost
+ * cells over uncovered domain interpolated from the coarse level
+ * (conservative linear, monotonised-central limited slopes; replaces AMReX
+ * cell_cons_interp), then the physical boundary functor.  Asynchronous. */
+int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *stream);
+/* nsteps coarse steps of LBM::evolve -> time_step -> advance
+ * (Source/LBM.cpp:398-544) on a linked pair: coarse ghost fill; explode
+ * f, g; twice (fine physbc, fine stream f, g, fine collide); coarse
+ * stream f, g; coalesce; coarse collide.  The fine level's dt must be half
+ * the coarse one (compute_dt, LBM.cpp:1047-1077).  Asynchronous. */
+int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream);
 
 /* ---- diagnostics ------------------------------------------------------------ */
 const char *mrb_last_error(void);

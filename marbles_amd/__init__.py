@@ -11,3 +11,4 @@ from .level import Level  # noqa: F401
 from . import configs  # noqa: F401
 
 __all__ = ["Level", "MrbError", "load", "configs"]
+from .amr import Hierarchy  # noqa: F401,E402

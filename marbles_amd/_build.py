@@ -24,6 +24,7 @@ SOURCES = [
     "mrb_level.hip",
     "mrb_compact.hip",
     "mrb_fused.hip",
+    "mrb_amr.hip",
 ]
 
 COMMON_FLAGS = [

@@ -98,6 +98,12 @@ SIGNATURES = {
     "mrb_last_error": (c_char_p, []),
     "mrb_hot_kernel_name": (c_char_p, [c_void_p]),
     "mrb_population_ptr": (c_int, [c_void_p, c_int, POINTER(c_void_p), POINTER(c_int64)]),
+    "mrb_init_fields": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                c_double]),
+    "mrb_amr_link": (c_int, [c_void_p, c_void_p]),
+    "mrb_coalesce": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "mrb_explode": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "mrb_amr_advance": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
 }
 
 _lib = None

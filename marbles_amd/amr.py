@@ -1,0 +1,60 @@
+"""Two-level hierarchy over the C ABI.
+
+Mirrors the level loop of the reference solver (NREL/marbles
+Source/LBM.cpp:398-544, evolve -> time_step -> advance) for refinement
+ratio 2: the fine level runs two substeps with dt/2 per coarse step, fine
+ghost cells are filled from the coarse level once per coarse step
+(FillPatchOps.H:118-129) and the coarse level takes the masked average of
+the fine one after its stream (LBM.cpp:1571-1584, Utilities.cpp:5-28).
+"""
+from __future__ import annotations
+
+from . import _lib
+from ._lib import check
+from .level import Level
+
+
+class Hierarchy:
+    """Coarse level covering the domain plus one fine level (ratio 2)."""
+
+    def __init__(self, cfg: dict, ng: int = 3):
+        from . import configs
+
+        self.cfg = cfg
+        nc = tuple(cfg["ncell"])
+        nf = tuple(2 * n for n in nc)
+        self.crse = Level(nc, cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=cfg["coarse_boxes"],
+                          ng=ng, layout="fab")
+        self.fine = Level(nf, cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=cfg["fine_boxes"],
+                          ng=ng, layout="fab")
+        gamma = cfg["ic"].get("adiabatic_exponent", 5.0 / 3.0)
+        for lvl, dt in ((self.crse, 1.0), (self.fine, 0.5)):
+            lvl.set_phys(cfg["nu"], cfg.get("alpha"), dt=dt)
+            lvl.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+            lvl.flags_all_regular()
+            lvl.init_fields(*configs.level_fields(cfg, lvl.ncell, lvl.boxes, ng), adiabatic_exponent=gamma)
+        check(_lib.load().mrb_amr_link(self.crse.h, self.fine.h))
+
+    @property
+    def levels(self):
+        return (self.crse, self.fine)
+
+    def updates_per_step(self) -> int:
+        """Lattice updates per coarse step: N_coarse + 2 N_fine."""
+        def cells(lvl):
+            return sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in lvl.boxes)
+        return cells(self.crse) + 2 * cells(self.fine)
+
+    def explode(self, lattice: int, stream=None) -> None:
+        check(_lib.load().mrb_explode(self.crse.h, self.fine.h, int(lattice), stream))
+
+    def coalesce(self, stream=None) -> None:
+        check(_lib.load().mrb_coalesce(self.fine.h, self.crse.h, stream))
+
+    def advance(self, nsteps: int, stream=None) -> None:
+        check(_lib.load().mrb_amr_advance(self.crse.h, self.fine.h, int(nsteps), stream))
+
+    def close(self) -> None:
+        # the fine level holds maps into the coarse one: release it first
+        self.fine.close()
+        self.crse.close()

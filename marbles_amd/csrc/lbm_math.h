@@ -58,6 +58,8 @@ constexpr double weight(int q)
 }
 constexpr double kTheta0 = 1.0 / 3.0;          // Stencil.H:42
 constexpr double kAirMolarMass = 28.96;         // Constants.H:65
+// Constants.H:57-58: SMALL_NUM = epsilon * 1e10, the sentinel tolerance of
+// masked_avgdown
 constexpr double kSmallNum = 2.220446049250313080847263336181640625e-16 * 1e10;
 
 // Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1.

@@ -1,0 +1,346 @@
+// Two levels with refinement ratio 2: coalescence (coarse <- masked mean of
+// the fine children), explosion (fine ghost cells <- fine neighbours or the
+// coarse level) and the coarse-step schedule.
+//
+// Both transfers run from gather maps built once on the host when the
+// levels are linked: one entry per written cell, so every launch is a
+// flat, race-free pass over its entries with no box search.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mrb_internal.h"
+
+namespace mrb {
+
+// coarse cell <- masked mean of the 2x2x2 fine children
+struct CoalesceEntry {
+    int64_t crse_local;  // coarse fab cell (local index in crse_box's fab)
+    int64_t fine_local;  // child (0,0,0) (local index in fine_box's fab)
+    int32_t crse_box;
+    int32_t fine_box;
+};
+
+// fine ghost cell <- interpolation from its coarse parent
+struct ExplodeEntry {
+    int64_t fine_local;
+    int64_t crse_local;  // the parent
+    int32_t fine_box;
+    int32_t crse_box;
+    int32_t bits;  // bit d: high child in direction d; bit 3+d: both neighbours in d inside the domain
+    int32_t pad;
+};
+
+namespace {
+
+constexpr int kAmrBlock = 256;
+
+inline unsigned blocks_for(int64_t n) { return unsigned((n + kAmrBlock - 1) / kAmrBlock); }
+
+// masked_avgdown (Utilities.H:315-350), ratio 2, (kref, jref, iref) order
+__global__ void __launch_bounds__(kAmrBlock) k_coalesce(
+    const double *__restrict__ fine, double *__restrict__ crse, const BoxDev *__restrict__ fboxes,
+    const BoxDev *__restrict__ cboxes, const CoalesceEntry *__restrict__ map, int64_t n)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const CoalesceEntry e = map[t];
+    const BoxDev &FB = fboxes[e.fine_box];
+    const BoxDev &CB = cboxes[e.crse_box];
+    const double *f = fine + FB.base * NQ + e.fine_local;
+    double *c = crse + CB.base * NQ + e.crse_local;
+    const int64_t sy = FB.n[0], sz = int64_t(FB.n[0]) * FB.n[1];
+    for (int q = 0; q < NQ; ++q) {
+        const double *fq = f + q * FB.ncell;
+        double sum = 0;
+        double vol = 0;
+        for (int kref = 0; kref < 2; ++kref)
+            for (int jref = 0; jref < 2; ++jref)
+                for (int iref = 0; iref < 2; ++iref) {
+                    const double fval = fq[kref * sz + jref * sy + iref];
+                    if (fabs(fval - (-1.0)) > kSmallNum) {
+                        sum += fval;
+                        vol += 1.0;
+                    }
+                }
+        if (vol > 0.0) c[q * CB.ncell] = sum / vol;
+    }
+}
+
+// Conservative linear interpolation with monotonised-central limited
+// slopes: child = ((c + s_x o_x) + s_y o_y) + s_z o_z, o = -+1/4.
+__global__ void __launch_bounds__(kAmrBlock) k_explode(
+    const double *__restrict__ crse, double *__restrict__ fine, const BoxDev *__restrict__ fboxes,
+    const BoxDev *__restrict__ cboxes, const ExplodeEntry *__restrict__ map, int64_t n)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const ExplodeEntry e = map[t];
+    const BoxDev &FB = fboxes[e.fine_box];
+    const BoxDev &CB = cboxes[e.crse_box];
+    const double *c0 = crse + CB.base * NQ + e.crse_local;
+    double *fo = fine + FB.base * NQ + e.fine_local;
+    const int64_t stride[3] = {1, CB.n[0], int64_t(CB.n[0]) * CB.n[1]};
+    double o[3];
+    for (int d = 0; d < 3; ++d) o[d] = ((e.bits >> d) & 1) ? 0.25 : -0.25;
+    for (int q = 0; q < NQ; ++q) {
+        const double *cq = c0 + q * CB.ncell;
+        const double c = cq[0];
+        double s[3];
+        for (int d = 0; d < 3; ++d) {
+            s[d] = 0.0;
+            if (!((e.bits >> (3 + d)) & 1)) continue;
+            const double cm = cq[-stride[d]];
+            const double cp = cq[stride[d]];
+            const double dc = 0.5 * (cp - cm);
+            const double dl = c - cm;
+            const double dr = cp - c;
+            if (dl * dr > 0.0) s[d] = copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc);
+        }
+        fo[q * FB.ncell] = ((c + s[0] * o[0]) + s[1] * o[1]) + s[2] * o[2];
+    }
+}
+
+int64_t local_of(const BoxDev &B, int i, int j, int k)
+{
+    return (int64_t(k - B.lo[2]) * B.n[1] + (j - B.lo[1])) * B.n[0] + (i - B.lo[0]);
+}
+
+bool periodic_image(const mrb_level *L, const int iv[3], int s[3])
+{
+    for (int d = 0; d < 3; ++d) {
+        s[d] = iv[d];
+        if (s[d] < 0) {
+            if (!L->periodic[d]) return false;
+            s[d] += L->ncell[d];
+        } else if (s[d] >= L->ncell[d]) {
+            if (!L->periodic[d]) return false;
+            s[d] -= L->ncell[d];
+        }
+    }
+    return true;
+}
+
+// domain cell -> owning box (-1 if uncovered)
+std::vector<int32_t> owner_map(const mrb_level *L)
+{
+    const int64_t n0 = L->ncell[0], n1 = L->ncell[1];
+    std::vector<int32_t> own(size_t(n0 * n1 * L->ncell[2]), -1);
+    for (int b = 0; b < L->nbox; ++b) {
+        const BoxDev &B = L->boxes[b];
+        for (int k = B.vlo[2]; k <= B.vhi[2]; ++k)
+            for (int j = B.vlo[1]; j <= B.vhi[1]; ++j)
+                for (int i = B.vlo[0]; i <= B.vhi[0]; ++i) own[size_t((k * n1 + j) * n0 + i)] = b;
+    }
+    return own;
+}
+
+int64_t dom_index(const mrb_level *L, const int s[3])
+{
+    return (int64_t(s[2]) * L->ncell[1] + s[1]) * L->ncell[0] + s[0];
+}
+
+int check_pair(const mrb_level *crse, const mrb_level *fine)
+{
+    if (!crse || !fine) return set_error(MRB_E_ARG, "null level");
+    if (crse->layout != MRB_LAYOUT_FAB || fine->layout != MRB_LAYOUT_FAB)
+        return set_error(MRB_E_UNSUPPORTED, "two-level transfers need the fab layout");
+    if (crse->storage != MRB_STORAGE_F64 || fine->storage != MRB_STORAGE_F64)
+        return set_error(MRB_E_UNSUPPORTED, "two-level transfers need FP64 storage");
+    for (int d = 0; d < 3; ++d) {
+        if (fine->ncell[d] != 2 * crse->ncell[d] || fine->periodic[d] != crse->periodic[d])
+            return set_error(MRB_E_ARG, "fine level is not the coarse domain refined by 2");
+    }
+    if (fine->ng < 2) return set_error(MRB_E_ARG, "fine level needs at least 2 ghost layers");
+    if (crse->ng < 1) return set_error(MRB_E_ARG, "coarse level needs at least 1 ghost layer");
+    for (const BoxDev &B : fine->boxes)
+        for (int d = 0; d < 3; ++d)
+            if ((B.vlo[d] & 1) || !(B.vhi[d] & 1))
+                return set_error(MRB_E_ARG, "fine boxes must be coarse-aligned (even lo, odd hi)");
+    return MRB_OK;
+}
+
+}  // namespace
+
+void amr_release(mrb_level *L)
+{
+    if (L->d_coal) (void)hipFree(L->d_coal);
+    if (L->d_expl) (void)hipFree(L->d_expl);
+    L->d_coal = L->d_expl = nullptr;
+    L->n_coal = L->n_expl = 0;
+    L->amr_crse = nullptr;
+}
+
+}  // namespace mrb
+
+using namespace mrb;
+
+extern "C" {
+
+int mrb_amr_link(mrb_level *crse, mrb_level *fine)
+{
+    int rc = check_pair(crse, fine);
+    if (rc) return rc;
+    const std::vector<int32_t> cown = owner_map(crse);
+    for (int32_t o : cown)
+        if (o < 0) return set_error(MRB_E_ARG, "the coarse level must cover its domain");
+    const std::vector<int32_t> fown = owner_map(fine);
+
+    // coalescence: coarsened fine boxes grown by one, box order, the last
+    // box wins where rings overlap (average_down_with_ghosts,
+    // Utilities.cpp:5-28, ParallelCopy back with periodicity)
+    std::vector<CoalesceEntry> coal;
+    std::vector<int64_t> slot(cown.size(), -1);
+    for (int b = 0; b < fine->nbox; ++b) {
+        const BoxDev &FB = fine->boxes[b];
+        for (int k = FB.vlo[2] / 2 - 1; k <= FB.vhi[2] / 2 + 1; ++k)
+            for (int j = FB.vlo[1] / 2 - 1; j <= FB.vhi[1] / 2 + 1; ++j)
+                for (int i = FB.vlo[0] / 2 - 1; i <= FB.vhi[0] / 2 + 1; ++i) {
+                    const int iv[3] = {i, j, k};
+                    int s[3];
+                    if (!periodic_image(crse, iv, s)) continue;
+                    const int64_t di = dom_index(crse, s);
+                    const int ob = cown[size_t(di)];
+                    CoalesceEntry e;
+                    e.crse_box = ob;
+                    e.crse_local = local_of(crse->boxes[ob], s[0], s[1], s[2]);
+                    e.fine_box = b;
+                    e.fine_local = local_of(FB, 2 * i, 2 * j, 2 * k);
+                    if (slot[size_t(di)] >= 0) {
+                        coal[size_t(slot[size_t(di)])] = e;
+                    } else {
+                        slot[size_t(di)] = int64_t(coal.size());
+                        coal.push_back(e);
+                    }
+                }
+    }
+    // explosion: fine ghost cells whose periodic image is inside the domain
+    // and not covered by a fine box
+    std::vector<ExplodeEntry> expl;
+    for (int b = 0; b < fine->nbox; ++b) {
+        const BoxDev &FB = fine->boxes[b];
+        for (int k = FB.lo[2]; k < FB.lo[2] + FB.n[2]; ++k)
+            for (int j = FB.lo[1]; j < FB.lo[1] + FB.n[1]; ++j)
+                for (int i = FB.lo[0]; i < FB.lo[0] + FB.n[0]; ++i) {
+                    if (i >= FB.vlo[0] && i <= FB.vhi[0] && j >= FB.vlo[1] && j <= FB.vhi[1] &&
+                        k >= FB.vlo[2] && k <= FB.vhi[2])
+                        continue;
+                    const int iv[3] = {i, j, k};
+                    int s[3];
+                    if (!periodic_image(fine, iv, s)) continue;
+                    if (fown[size_t(dom_index(fine, s))] >= 0) continue;
+                    const int P[3] = {s[0] / 2, s[1] / 2, s[2] / 2};
+                    const int ob = cown[size_t(dom_index(crse, P))];
+                    ExplodeEntry e{};
+                    e.fine_box = b;
+                    e.fine_local = local_of(FB, i, j, k);
+                    e.crse_box = ob;
+                    e.crse_local = local_of(crse->boxes[ob], P[0], P[1], P[2]);
+                    for (int d = 0; d < 3; ++d) {
+                        const bool hm = crse->periodic[d] || P[d] - 1 >= 0;
+                        const bool hp = crse->periodic[d] || P[d] + 1 < crse->ncell[d];
+                        e.bits |= (s[d] & 1) << d;
+                        e.bits |= int(hm && hp) << (3 + d);
+                    }
+                    expl.push_back(e);
+                }
+    }
+    // every address the kernels form must lie inside its fab
+    for (const CoalesceEntry &e : coal) {
+        const BoxDev &FB = fine->boxes[e.fine_box];
+        const BoxDev &CB = crse->boxes[e.crse_box];
+        const int64_t last = e.fine_local + int64_t(FB.n[0]) * FB.n[1] + FB.n[0] + 1;
+        if (e.fine_local < 0 || last >= FB.ncell || e.crse_local < 0 || e.crse_local >= CB.ncell)
+            return set_error(MRB_E_ARG, "coalescence map leaves a fab (fine ghost layers too few)");
+    }
+    for (const ExplodeEntry &e : expl) {
+        const BoxDev &FB = fine->boxes[e.fine_box];
+        const BoxDev &CB = crse->boxes[e.crse_box];
+        const int64_t sz = int64_t(CB.n[0]) * CB.n[1];
+        const int64_t lo = (e.bits & (7 << 3)) ? e.crse_local - sz : e.crse_local;
+        const int64_t hi = (e.bits & (7 << 3)) ? e.crse_local + sz : e.crse_local;
+        if (e.fine_local < 0 || e.fine_local >= FB.ncell || lo < 0 || hi >= CB.ncell)
+            return set_error(MRB_E_ARG, "explosion map leaves a fab");
+    }
+    amr_release(fine);
+    if (!coal.empty()) {
+        MRB_HIP(hipMalloc(&fine->d_coal, sizeof(CoalesceEntry) * coal.size()));
+        MRB_HIP(hipMemcpy(fine->d_coal, coal.data(), sizeof(CoalesceEntry) * coal.size(), hipMemcpyHostToDevice));
+    }
+    if (!expl.empty()) {
+        MRB_HIP(hipMalloc(&fine->d_expl, sizeof(ExplodeEntry) * expl.size()));
+        MRB_HIP(hipMemcpy(fine->d_expl, expl.data(), sizeof(ExplodeEntry) * expl.size(), hipMemcpyHostToDevice));
+    }
+    fine->n_coal = int64_t(coal.size());
+    fine->n_expl = int64_t(expl.size());
+    fine->amr_crse = crse;
+    return MRB_OK;
+}
+
+int mrb_coalesce(const mrb_level *fine, mrb_level *crse, void *stream)
+{
+    if (!fine || !crse) return set_error(MRB_E_ARG, "null level");
+    if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
+    if (fine->n_coal == 0) return MRB_OK;
+    hipStream_t s = as_stream(stream);
+    for (int l = 0; l < 2; ++l) {
+        k_coalesce<<<blocks_for(fine->n_coal), kAmrBlock, 0, s>>>(
+            (const double *)fine->pop[l][fine->cur], (double *)crse->pop[l][crse->cur], fine->d_boxes,
+            crse->d_boxes, (const CoalesceEntry *)fine->d_coal, fine->n_coal);
+        MRB_HIP(hipGetLastError());
+    }
+    return MRB_OK;
+}
+
+int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *stream)
+{
+    if (!fine || !crse) return set_error(MRB_E_ARG, "null level");
+    if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
+    if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
+    hipStream_t s = as_stream(stream);
+    // fine-fine ghosts first, then the uncovered ones, then the domain
+    // boundary (the three sets of cells are disjoint; physbc reads valid
+    // and in-domain ghost cells only)
+    int rc = fab_fill_boundary(fine, lattice, s);
+    if (rc) return rc;
+    if (fine->n_expl > 0) {
+        k_explode<<<blocks_for(fine->n_expl), kAmrBlock, 0, s>>>(
+            (const double *)crse->pop[lattice][crse->cur], (double *)fine->pop[lattice][fine->cur],
+            fine->d_boxes, crse->d_boxes, (const ExplodeEntry *)fine->d_expl, fine->n_expl);
+        MRB_HIP(hipGetLastError());
+    }
+    return mrb_physbc(fine, lattice, stream);
+}
+
+int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream)
+{
+    if (!fine || !crse) return set_error(MRB_E_ARG, "null level");
+    if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
+    if (!crse->flags_ready || !fine->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    if (fine->phys.dt != 0.5 * crse->phys.dt)
+        return set_error(MRB_E_STATE, "fine level dt must be half the coarse dt");
+    int rc;
+    for (int n = 0; n < nsteps; ++n) {
+        // LBM::evolve (LBM.cpp:416-418) and time_step(0) (498-512)
+        for (int l = 0; l < 2; ++l)
+            if ((rc = mrb_fill_ghosts(crse, l, stream))) return rc;
+        for (int l = 0; l < 2; ++l)
+            if ((rc = mrb_explode(crse, fine, l, stream))) return rc;
+        for (int sub = 0; sub < 2; ++sub) {
+            for (int l = 0; l < 2; ++l)
+                if ((rc = mrb_physbc(fine, l, stream))) return rc;
+            // time_step(1) -> advance(1) (LBM.cpp:523-544)
+            for (int l = 0; l < 2; ++l)
+                if ((rc = mrb_stream(fine, l, stream))) return rc;
+            if ((rc = mrb_collide(fine, stream))) return rc;
+        }
+        // advance(0): stream, average_down_to, collide
+        for (int l = 0; l < 2; ++l)
+            if ((rc = mrb_stream(crse, l, stream))) return rc;
+        if ((rc = mrb_coalesce(fine, crse, stream))) return rc;
+        if ((rc = mrb_collide(crse, stream))) return rc;
+    }
+    return MRB_OK;
+}
+
+}  // extern "C"

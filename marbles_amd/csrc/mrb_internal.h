@@ -107,6 +107,12 @@ struct mrb_level {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // hot-path kernel of the compact layout: 0 = two-pass, 1 + i = fused variant i
     int hot = 0;
+    // two-level link, held by the fine level (mrb_amr_link)
+    mrb_level *amr_crse = nullptr;
+    void *d_coal = nullptr;  // mrb::CoalesceEntry[n_coal]
+    int64_t n_coal = 0;
+    void *d_expl = nullptr;  // mrb::ExplodeEntry[n_expl]
+    int64_t n_expl = 0;
 };
 
 namespace mrb {
@@ -136,5 +142,9 @@ int fused_launch(mrb_level *L, int variant, hipStream_t s);
 int fused_step(mrb_level *L, int variant, hipStream_t s);
 // two-pass step without flipping (mrb_compact.hip)
 int compact_two_pass_launch(mrb_level *L, hipStream_t s);
+// FillBoundary of one lattice's current fab buffer (mrb_level.hip)
+int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s);
+// frees the two-level maps held by a fine level (mrb_amr.hip)
+void amr_release(mrb_level *L);
 
 }  // namespace mrb

@@ -554,6 +554,11 @@ static int fill_boundary(mrb_level *L, void *arr, int ncomp, bool is_int, bool o
     return MRB_OK;
 }
 
+int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s)
+{
+    return fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, s);
+}
+
 static void compute_bc_values(mrb_level *L)
 {
     BCDev &d = L->bcdev;
@@ -794,6 +799,7 @@ int mrb_level_destroy(mrb_level *L)
     (void)hipFree(L->d_gmap);
     (void)hipFree(L->d_gmap1);
     (void)hipFree(L->d_bcells);
+    amr_release(L);
     delete L;
     return MRB_OK;
 }
@@ -1025,7 +1031,7 @@ static void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> 
 // function and uploads them; solid cells get 0 (LBM.cpp:1287-1295).
 static int init_from_state(
     mrb_level *L,
-    const std::function<void(int, int, int, double &, double *, double &, double &, double &)> &state)
+    const std::function<void(int64_t, int, int, int, double &, double *, double &, double &, double &)> &state)
 {
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
     const int64_t T = L->total_cells;
@@ -1041,7 +1047,7 @@ static int init_from_state(
                 const int j = int((loc % nxy) / B.n[0]) + B.lo[1];
                 const int i = int(loc % B.n[0]) + B.lo[0];
                 double rho = 1.0, vel[3] = {0.0, 0.0, 0.0}, T0 = 1.0 / 3.0, R = 1.0, gamma = 1.667;
-                state(i, j, k, rho, vel, T0, R, gamma);
+                state(B.base + loc, i, j, k, rho, vel, T0, R, gamma);
                 const bool solid = isf[size_t(B.base * 2 + loc)] == 0;
                 for (int q = 0; q < NQ; ++q) {
                     const size_t o = size_t(B.base * NQ + q * B.ncell + loc);
@@ -1070,7 +1076,7 @@ int mrb_init_taylorgreen(mrb_level *L, double rho0, double v0, const double omeg
     if (!L || !omega) return set_error(MRB_E_ARG, "null argument");
     const double Lw = 1.0 / M_PI;
     const double o0 = omega[0], o1 = omega[1], o2 = omega[2];
-    return init_from_state(L, [&](int i, int j, int k, double &rho, double *vel, double &T, double &R,
+    return init_from_state(L, [&](int64_t, int i, int j, int k, double &rho, double *vel, double &T, double &R,
                                   double &gamma) {
         // ic::TaylorGreen::DeviceOp::thermal_initialize (IC.H:119-153)
         const double x = L->prob_lo[0] + (i + 0.5) * L->dx[0];
@@ -1105,7 +1111,7 @@ int mrb_init_constant(
     }
     if (L->storage != MRB_STORAGE_F64) {
         // reduced-precision storage: build on the host through mrb_upload
-        return init_from_state(L, [&](int, int, int, double &rho, double *v, double &T, double &Rr,
+        return init_from_state(L, [&](int64_t, int, int, int, double &rho, double *v, double &T, double &Rr,
                                       double &gamma) {
             rho = density;
             v[0] = vel[0];
@@ -1126,6 +1132,26 @@ int mrb_init_constant(
     }
     MRB_HIP(hipDeviceSynchronize());
     return MRB_OK;
+}
+
+int mrb_init_fields(
+    mrb_level *L, const double *rho, const double *u, const double *v, const double *w, const double *T,
+    double adiabatic_exponent, double m_bar)
+{
+    if (!L || !rho || !u || !v || !w || !T) return set_error(MRB_E_ARG, "null argument");
+    // ic::Initializer::initialize (IC.H:460-525) with the per-cell state
+    // given for every fab cell (box order, local index x fastest)
+    const double R = kAirMolarMass / m_bar;
+    return init_from_state(L, [&](int64_t c, int, int, int, double &rho_c, double *vel, double &Tc,
+                                  double &Rr, double &gamma) {
+        rho_c = rho[c];
+        vel[0] = u[c];
+        vel[1] = v[c];
+        vel[2] = w[c];
+        Tc = T[c];
+        Rr = R;
+        gamma = adiabatic_exponent;
+    });
 }
 
 // ---- reference-layout stages ---------------------------------------------

@@ -130,6 +130,17 @@ class Level:
         self.upload("is_fluid", np.ascontiguousarray(isf, dtype=np.int32))
         check(self.lib.mrb_finalize_flags(self.h))
 
+    def init_fields(self, rho, u, v, w, T, adiabatic_exponent=5.0 / 3.0, m_bar=AIR_MOLAR_MASS) -> None:
+        """Initial state from per-cell fields over every fab cell (box order,
+        x fastest), like an ic::InitializerBase plugin (IC.H:427-531)."""
+        n = sum(int(np.prod(shp)) for shp in self.fab_shapes)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (rho, u, v, w, T)]
+        for a in arrs:
+            if a.size != n:
+                raise ValueError(f"field has {a.size} cells, the level's fabs hold {n}")
+        check(self.lib.mrb_init_fields(self.h, *[a.ctypes.data_as(c_void_p) for a in arrs],
+                                       float(adiabatic_exponent), float(m_bar)))
+
     def init_taylorgreen(self, rho0=1.0, v0=1.0, omega=(1.0, 1.0, 1.0)) -> None:
         check(self.lib.mrb_init_taylorgreen(self.h, rho0, v0, _d3(omega)))
 

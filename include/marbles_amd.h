@@ -180,10 +180,15 @@ const char *mrb_hot_kernel_variant_name(int index);
 int mrb_set_hot_kernel(mrb_level *lvl, int index);
 /* Autotuner: times every variant (`reps` launches each) on the level's
  * current state without advancing it -- each trial reads the current
- * buffers and writes the spare ones -- and selects the fastest.
- * ms_out[i] (may be NULL) receives variant i's mean step time and *best its
- * index.  Synchronous. */
+ * buffers and writes the spare ones -- and selects the fastest.  Compact
+ * layout: the kernel variants above (a variant that does not support the
+ * level reports -1).  Fab layout: the persistent-block count of the batched
+ * two-pass step over the box work-list.  ms_out[i] (may be NULL, else
+ * mrb_autotune_count(lvl) entries) receives variant i's mean time and *best
+ * its index.  Synchronous. */
 int mrb_autotune(mrb_level *lvl, int reps, void *stream, double *ms_out, int *best);
+int mrb_autotune_count(const mrb_level *lvl);
+const char *mrb_autotune_name(const mrb_level *lvl, int index);
 /* Stream only (debug/parity; LBM::stream for f and g on the compact
  * layout, ghosts resolved inline).  Swaps A/B. */
 int mrb_stream_compact(mrb_level *lvl, void *stream);
@@ -221,6 +226,11 @@ int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *strea
  * stream f, g; coalesce; coarse collide.  The fine level's dt must be half
  * the coarse one (compute_dt, LBM.cpp:1047-1077).  Asynchronous. */
 int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream);
+/* Same as mrb_amr_advance, with each coarse step replayed from a hipGraph
+ * captured on first use (one graph per coarse buffer parity; recaptured
+ * after mrb_set_phys, mrb_set_bc, a stage-level mrb_stream or a relink).
+ * Results are identical to mrb_amr_advance.  Asynchronous. */
+int mrb_amr_advance_graph(mrb_level *crse, mrb_level *fine, int nsteps, void *stream);
 
 /* ---- diagnostics ------------------------------------------------------------ */
 const char *mrb_last_error(void);

@@ -25,6 +25,7 @@ SOURCES = [
     "mrb_compact.hip",
     "mrb_fused.hip",
     "mrb_amr.hip",
+    "mrb_batched.hip",
 ]
 
 COMMON_FLAGS = [

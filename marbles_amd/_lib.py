@@ -104,6 +104,9 @@ SIGNATURES = {
     "mrb_coalesce": (c_int, [c_void_p, c_void_p, c_void_p]),
     "mrb_explode": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "mrb_amr_advance": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "mrb_amr_advance_graph": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "mrb_autotune_count": (c_int, [c_void_p]),
+    "mrb_autotune_name": (ctypes.c_char_p, [c_void_p, c_int]),
 }
 
 _lib = None

@@ -51,8 +51,10 @@ class Hierarchy:
     def coalesce(self, stream=None) -> None:
         check(_lib.load().mrb_coalesce(self.fine.h, self.crse.h, stream))
 
-    def advance(self, nsteps: int, stream=None) -> None:
-        check(_lib.load().mrb_amr_advance(self.crse.h, self.fine.h, int(nsteps), stream))
+    def advance(self, nsteps: int, stream=None, graph: bool = False) -> None:
+        """nsteps coarse steps; graph=True replays a captured hipGraph per step."""
+        fn = _lib.load().mrb_amr_advance_graph if graph else _lib.load().mrb_amr_advance
+        check(fn(self.crse.h, self.fine.h, int(nsteps), stream))
 
     def close(self) -> None:
         # the fine level holds maps into the coarse one: release it first

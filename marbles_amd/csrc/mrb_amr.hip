@@ -13,24 +13,6 @@
 
 namespace mrb {
 
-// coarse cell <- masked mean of the 2x2x2 fine children
-struct CoalesceEntry {
-    int64_t crse_local;  // coarse fab cell (local index in crse_box's fab)
-    int64_t fine_local;  // child (0,0,0) (local index in fine_box's fab)
-    int32_t crse_box;
-    int32_t fine_box;
-};
-
-// fine ghost cell <- interpolation from its coarse parent
-struct ExplodeEntry {
-    int64_t fine_local;
-    int64_t crse_local;  // the parent
-    int32_t fine_box;
-    int32_t crse_box;
-    int32_t bits;  // bit d: high child in direction d; bit 3+d: both neighbours in d inside the domain
-    int32_t pad;
-};
-
 namespace {
 
 constexpr int kAmrBlock = 256;
@@ -162,12 +144,26 @@ int check_pair(const mrb_level *crse, const mrb_level *fine)
 
 }  // namespace
 
+static void drop_graphs(mrb_level *L)
+{
+    for (auto &row : L->amr_graph)
+        for (auto &G : row) {
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            G = mrb_level::AmrGraph{};
+        }
+}
+
 void amr_release(mrb_level *L)
 {
+    drop_graphs(L);
+    if (L->cap_stream) (void)hipStreamDestroy(L->cap_stream);
+    L->cap_stream = nullptr;
     if (L->d_coal) (void)hipFree(L->d_coal);
     if (L->d_expl) (void)hipFree(L->d_expl);
+    if (L->d_coal_cell) (void)hipFree(L->d_coal_cell);
     L->d_coal = L->d_expl = nullptr;
-    L->n_coal = L->n_expl = 0;
+    L->d_coal_cell = nullptr;
+    L->n_coal = L->n_expl = L->n_qring = 0;
     L->amr_crse = nullptr;
 }
 
@@ -245,6 +241,42 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
                     expl.push_back(e);
                 }
     }
+    // ring-1 ghost cells first: the batched step computes their Q from
+    // their own pulls (they have no owning box to copy it from)
+    std::stable_partition(expl.begin(), expl.end(), [&](const ExplodeEntry &e) {
+        const BoxDev &FB = fine->boxes[e.fine_box];
+        const int64_t nxy = int64_t(FB.n[0]) * FB.n[1];
+        const int k = int(e.fine_local / nxy) + FB.lo[2];
+        const int j = int((e.fine_local % nxy) / FB.n[0]) + FB.lo[1];
+        const int i = int(e.fine_local % FB.n[0]) + FB.lo[0];
+        return i >= FB.vlo[0] - 1 && i <= FB.vhi[0] + 1 && j >= FB.vlo[1] - 1 && j <= FB.vhi[1] + 1 &&
+               k >= FB.vlo[2] - 1 && k <= FB.vhi[2] + 1;
+    });
+    int64_t n_qring = 0;
+    for (const ExplodeEntry &e : expl) {
+        const BoxDev &FB = fine->boxes[e.fine_box];
+        const int64_t nxy = int64_t(FB.n[0]) * FB.n[1];
+        const int k = int(e.fine_local / nxy) + FB.lo[2];
+        const int j = int((e.fine_local % nxy) / FB.n[0]) + FB.lo[1];
+        const int i = int(e.fine_local % FB.n[0]) + FB.lo[0];
+        if (!(i >= FB.vlo[0] - 1 && i <= FB.vhi[0] + 1 && j >= FB.vlo[1] - 1 && j <= FB.vhi[1] + 1 &&
+              k >= FB.vlo[2] - 1 && k <= FB.vhi[2] + 1))
+            break;
+        ++n_qring;
+    }
+    // per coarse fab cell: the coalescence entry of valid cells under the
+    // fine footprint or its ring
+    std::vector<int32_t> coal_cell(size_t(crse->total_cells), -1);
+    for (int b = 0; b < crse->nbox; ++b) {
+        const BoxDev &CB = crse->boxes[b];
+        for (int k = CB.vlo[2]; k <= CB.vhi[2]; ++k)
+            for (int j = CB.vlo[1]; j <= CB.vhi[1]; ++j)
+                for (int i = CB.vlo[0]; i <= CB.vhi[0]; ++i) {
+                    const int s[3] = {i, j, k};
+                    const int64_t sl = slot[size_t(dom_index(crse, s))];
+                    if (sl >= 0) coal_cell[size_t(CB.base + local_of(CB, i, j, k))] = int32_t(sl);
+                }
+    }
     // every address the kernels form must lie inside its fab
     for (const CoalesceEntry &e : coal) {
         const BoxDev &FB = fine->boxes[e.fine_box];
@@ -271,9 +303,14 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
         MRB_HIP(hipMalloc(&fine->d_expl, sizeof(ExplodeEntry) * expl.size()));
         MRB_HIP(hipMemcpy(fine->d_expl, expl.data(), sizeof(ExplodeEntry) * expl.size(), hipMemcpyHostToDevice));
     }
+    MRB_HIP(hipMalloc(&fine->d_coal_cell, sizeof(int32_t) * coal_cell.size()));
+    MRB_HIP(hipMemcpy(fine->d_coal_cell, coal_cell.data(), sizeof(int32_t) * coal_cell.size(),
+                      hipMemcpyHostToDevice));
     fine->n_coal = int64_t(coal.size());
     fine->n_expl = int64_t(expl.size());
+    fine->n_qring = n_qring;
     fine->amr_crse = crse;
+    ++fine->gen;
     return MRB_OK;
 }
 
@@ -283,6 +320,7 @@ int mrb_coalesce(const mrb_level *fine, mrb_level *crse, void *stream)
     if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
     if (fine->n_coal == 0) return MRB_OK;
     hipStream_t s = as_stream(stream);
+    crse->ghosts_fresh = false;
     for (int l = 0; l < 2; ++l) {
         k_coalesce<<<blocks_for(fine->n_coal), kAmrBlock, 0, s>>>(
             (const double *)fine->pop[l][fine->cur], (double *)crse->pop[l][crse->cur], fine->d_boxes,
@@ -312,33 +350,91 @@ int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *strea
     return mrb_physbc(fine, lattice, stream);
 }
 
-int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream)
+static int check_advance(mrb_level *crse, mrb_level *fine)
 {
     if (!fine || !crse) return set_error(MRB_E_ARG, "null level");
     if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
     if (!crse->flags_ready || !fine->flags_ready) return set_error(MRB_E_STATE, "flags not set");
     if (fine->phys.dt != 0.5 * crse->phys.dt)
         return set_error(MRB_E_STATE, "fine level dt must be half the coarse dt");
+    return MRB_OK;
+}
+
+// one coarse step: LBM::evolve (LBM.cpp:416-418) and time_step(0) (498-512)
+static int amr_step(mrb_level *crse, mrb_level *fine, void *stream)
+{
     int rc;
+    for (int l = 0; l < 2; ++l)
+        if ((rc = mrb_fill_ghosts(crse, l, stream))) return rc;
+    for (int l = 0; l < 2; ++l)
+        if ((rc = mrb_explode(crse, fine, l, stream))) return rc;
+    for (int sub = 0; sub < 2; ++sub) {
+        for (int l = 0; l < 2; ++l)
+            if ((rc = mrb_physbc(fine, l, stream))) return rc;
+        // time_step(1) -> advance(1) (LBM.cpp:523-544): stream f, g and
+        // collide as one batched step
+        if ((rc = fab_batched_step(fine, nullptr, as_stream(stream)))) return rc;
+    }
+    // advance(0): stream, average_down_to, collide, with the coalescence
+    // fused into the coarse passes
+    return fab_batched_step(crse, fine, as_stream(stream));
+}
+
+int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream)
+{
+    int rc = check_advance(crse, fine);
+    if (rc) return rc;
+    for (int n = 0; n < nsteps; ++n)
+        if ((rc = amr_step(crse, fine, stream))) return rc;
+    return MRB_OK;
+}
+
+int mrb_amr_advance_graph(mrb_level *crse, mrb_level *fine, int nsteps, void *stream)
+{
+    int rc = check_advance(crse, fine);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
     for (int n = 0; n < nsteps; ++n) {
-        // LBM::evolve (LBM.cpp:416-418) and time_step(0) (498-512)
-        for (int l = 0; l < 2; ++l)
-            if ((rc = mrb_fill_ghosts(crse, l, stream))) return rc;
-        for (int l = 0; l < 2; ++l)
-            if ((rc = mrb_explode(crse, fine, l, stream))) return rc;
-        for (int sub = 0; sub < 2; ++sub) {
-            for (int l = 0; l < 2; ++l)
-                if ((rc = mrb_physbc(fine, l, stream))) return rc;
-            // time_step(1) -> advance(1) (LBM.cpp:523-544)
-            for (int l = 0; l < 2; ++l)
-                if ((rc = mrb_stream(fine, l, stream))) return rc;
-            if ((rc = mrb_collide(fine, stream))) return rc;
+        mrb_level::AmrGraph &G = fine->amr_graph[crse->cur][crse->ghosts_fresh ? 1 : 0];
+        if (!G.exec || G.gen_c != crse->gen || G.gen_f != fine->gen || G.fine_cur != fine->cur) {
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            G = mrb_level::AmrGraph{};
+            if (!fine->cap_stream) MRB_HIP(hipStreamCreateWithFlags(&fine->cap_stream, hipStreamNonBlocking));
+            // capture one step: the host-side state advances during the
+            // capture as if the step had run; it is put back afterwards and
+            // set to the recorded post-step values after each launch
+            const int cur_c = crse->cur, cur_f = fine->cur;
+            const bool fresh_c = crse->ghosts_fresh, fresh_f = fine->ghosts_fresh;
+            G.gen_c = crse->gen;
+            G.gen_f = fine->gen;
+            G.fine_cur = cur_f;
+            MRB_HIP(hipStreamBeginCapture(fine->cap_stream, hipStreamCaptureModeThreadLocal));
+            rc = amr_step(crse, fine, fine->cap_stream);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(fine->cap_stream, &graph);
+            G.post_cur_c = crse->cur;
+            G.post_cur_f = fine->cur;
+            crse->cur = cur_c;
+            fine->cur = cur_f;
+            crse->ghosts_fresh = fresh_c;
+            fine->ghosts_fresh = fresh_f;
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
+            const hipError_t ei = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                G = mrb_level::AmrGraph{};
+                return hip_fail(ei, "hipGraphInstantiate");
+            }
         }
-        // advance(0): stream, average_down_to, collide
-        for (int l = 0; l < 2; ++l)
-            if ((rc = mrb_stream(crse, l, stream))) return rc;
-        if ((rc = mrb_coalesce(fine, crse, stream))) return rc;
-        if ((rc = mrb_collide(crse, stream))) return rc;
+        MRB_HIP(hipGraphLaunch(G.exec, s));
+        crse->cur = G.post_cur_c;
+        fine->cur = G.post_cur_f;
+        crse->ghosts_fresh = true;
+        fine->ghosts_fresh = true;
     }
     return MRB_OK;
 }

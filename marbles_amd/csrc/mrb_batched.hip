@@ -1,0 +1,432 @@
+// Batched step on the reference (fab) layout: every box of a level in one
+// pass, driven by a flat tile work-list built when the level is created.
+//
+//   pass 1 (k_fab_qpass):   valid cells of every box: pull 54 populations,
+//                           moments -> Q_corr; FillBoundary of Q (one layer)
+//   pass 2 (k_fab_collide): valid cells: pull 54 populations, moments, dQ
+//                           from the Q of the face neighbours, collide,
+//                           write the spare buffer
+//
+// followed by FillBoundary of the spare buffer and a buffer flip.  This is
+// LBM::stream + LBM::collide (LBM.cpp:558-807) without materialising the
+// streamed state.  The Q of a ring-1 ghost cell comes from its owning box
+// by FillBoundary, as the reference's macrodata does (LBM.cpp:905); on a
+// fine level the ring-1 ghost cells over uncovered domain have no owner
+// and compute Q from their own pulled populations (k_fab_qpass_listed).
+// Those uncovered ghost cells also keep the streamed values that the
+// reference keeps (they feed the second substep and the coalescence), so
+// they are streamed from the explosion map; all other ghost values are
+// rewritten by FillBoundary or the boundary functor before anything reads
+// them.
+//
+// On a coarse level linked to a fine one, coalescence (average_down_with_
+// ghosts, Utilities.cpp:5-28) is fused into both passes: a cell under the
+// fine footprint (or its one-cell ring) takes the masked mean of its fine
+// children instead of its pulled value, population by population.
+//
+// A tile is kFabTile consecutive cells (x fastest) of one box region.
+// Blocks are grouped by XCD (block b runs on XCD b % 8): each XCD walks a
+// contiguous chunk of the tile list so that neighbouring tiles, which
+// share the cache lines of their halo pulls, meet in the same L2.
+#include <algorithm>
+#include <vector>
+
+#include "mrb_internal.h"
+
+namespace mrb {
+
+namespace {
+
+struct FabArgs {
+    const BoxDev *boxes;
+    const uint32_t *mask;  // per fab cell: bit q = is_fluid(x - c_q) != 0, bit 0 own fluid
+    int dom[3];
+    int grow;
+    int ntiles;
+};
+
+// coalescence source for the fused coarse passes
+struct CoalArgs {
+    const double *fine_f;
+    const double *fine_g;
+    const BoxDev *fine_boxes;
+    const CoalesceEntry *map;
+    const int32_t *cell;  // per coarse fab cell: entry index or -1
+};
+
+__device__ __forceinline__ bool tile_cell(
+    const FabArgs &A, const FabTile &t, int r, const BoxDev &B, int &i, int &j, int &k, int64_t &loc)
+{
+    const int g = A.grow;
+    const int rx = B.vhi[0] - B.vlo[0] + 1 + 2 * g;
+    const int ry = B.vhi[1] - B.vlo[1] + 1 + 2 * g;
+    const int rz = B.vhi[2] - B.vlo[2] + 1 + 2 * g;
+    const int c = t.first + r;
+    if (c >= rx * ry * rz) return false;
+    const int kk = c / (rx * ry);
+    const int rem = c - kk * rx * ry;
+    const int jj = rem / rx;
+    const int ii = rem - jj * rx;
+    i = B.vlo[0] - g + ii;
+    j = B.vlo[1] - g + jj;
+    k = B.vlo[2] - g + kk;
+    loc = (int64_t(k - B.lo[2]) * B.n[1] + (j - B.lo[1])) * B.n[0] + (i - B.lo[0]);
+    return true;
+}
+
+// pull of cell loc (fluid) in box B: f*_q = fluid(x - c_q) ? f_q(x - c_q) : f_qbar(x)
+__device__ __forceinline__ void pull_fab(
+    const double *__restrict__ f, const double *__restrict__ g, const BoxDev &B, int64_t loc,
+    uint32_t m, double fv[NQ], double gv[NQ])
+{
+    const double *fb = f + B.base * NQ;
+    const double *gb = g + B.base * NQ;
+    const int64_t sy = B.n[0], sz = int64_t(B.n[0]) * B.n[1];
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+        constexpr int qb = kBounce[q];
+        const int64_t src = loc - (cx + cy * sy + cz * sz);
+        const int64_t e = ((m >> q) & 1u) ? int64_t(q) * B.ncell + src : int64_t(qb) * B.ncell + loc;
+        fv[q] = fb[e];
+        gv[q] = gb[e];
+    });
+}
+
+// masked_avgdown (Utilities.H:315-350) of one population; false if all 8
+// children are the -1 sentinel
+__device__ __forceinline__ bool masked_mean(const double *__restrict__ c0, int64_t sy, int64_t sz, double &out)
+{
+    double sum = 0;
+    double vol = 0;
+    for (int kref = 0; kref < 2; ++kref)
+        for (int jref = 0; jref < 2; ++jref)
+            for (int iref = 0; iref < 2; ++iref) {
+                const double fval = c0[kref * sz + jref * sy + iref];
+                if (fabs(fval - (-1.0)) > kSmallNum) {
+                    sum += fval;
+                    vol += 1.0;
+                }
+            }
+    if (vol > 0.0) {
+        out = sum / vol;
+        return true;
+    }
+    return false;
+}
+
+template <bool COAL>
+__device__ __forceinline__ void gather_cell(
+    const double *__restrict__ f, const double *__restrict__ g, const BoxDev &B, int64_t loc, uint32_t m,
+    const CoalArgs &C, double fv[NQ], double gv[NQ])
+{
+    if constexpr (COAL) {
+        const int32_t ci = C.cell[B.base + loc];
+        if (ci >= 0) {
+            const CoalesceEntry e = C.map[ci];
+            const BoxDev &FB = C.fine_boxes[e.fine_box];
+            const double *ff = C.fine_f + FB.base * NQ + e.fine_local;
+            const double *fg = C.fine_g + FB.base * NQ + e.fine_local;
+            const int64_t sy = FB.n[0], sz = int64_t(FB.n[0]) * FB.n[1];
+            const double *fb = f + B.base * NQ;
+            const double *gb = g + B.base * NQ;
+            const int64_t csy = B.n[0], csz = int64_t(B.n[0]) * B.n[1];
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+                const int64_t src = loc - (cx + cy * csy + cz * csz);
+                const int64_t pe = ((m >> q) & 1u) ? int64_t(q) * B.ncell + src
+                                                   : int64_t(kBounce[q]) * B.ncell + loc;
+                double v;
+                fv[q] = masked_mean(ff + q * FB.ncell, sy, sz, v) ? v : fb[pe];
+                gv[q] = masked_mean(fg + q * FB.ncell, sy, sz, v) ? v : gb[pe];
+            });
+            return;
+        }
+    }
+    pull_fab(f, g, B, loc, m, fv, gv);
+}
+
+__device__ __forceinline__ int xcd_tile_begin(int ntiles, int &step, int &end)
+{
+    // gridDim.x is a multiple of 8; XCD x = blockIdx.x % 8 walks tiles
+    // [x*ntiles/8, (x+1)*ntiles/8)
+    const int x = int(blockIdx.x & 7u);
+    const int per = int(gridDim.x >> 3);
+    const int lo = int(int64_t(ntiles) * x / 8);
+    end = int(int64_t(ntiles) * (x + 1) / 8);
+    step = per;
+    return lo + int(blockIdx.x >> 3);
+}
+
+template <bool COAL>
+__global__ void __launch_bounds__(kFabTile) k_fab_qpass(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ Q,
+    const FabTile *__restrict__ tiles, FabArgs A, CoalArgs C, CollideParams P)
+{
+    int step, end;
+    for (int t = xcd_tile_begin(A.ntiles, step, end); t < end; t += step) {
+        const FabTile tile = tiles[t];
+        const BoxDev &B = A.boxes[tile.box];
+        int i, j, k;
+        int64_t loc;
+        if (!tile_cell(A, tile, int(threadIdx.x), B, i, j, k, loc)) continue;
+        const uint32_t m = A.mask[B.base + loc];
+        if (!(m & 1u)) continue;
+        double fv[NQ], gv[NQ];
+        gather_cell<COAL>(f, g, B, loc, m, C, fv, gv);
+        double rho, jx, jy, jz, E;
+        accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
+        const Macro M = macro_from(rho, jx, jy, jz, E, P);
+        double *q = Q + B.base * 3 + loc;
+        q[0] = M.Qx;
+        q[B.ncell] = M.Qy;
+        q[2 * B.ncell] = M.Qz;
+    }
+}
+
+// pass 1 at the listed ghost cells (the first n entries of the explosion
+// map, which are the ring-1 cells)
+__global__ void __launch_bounds__(256) k_fab_qpass_listed(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ Q,
+    const BoxDev *__restrict__ boxes, const uint32_t *__restrict__ mask, const ExplodeEntry *__restrict__ list,
+    int64_t n, CollideParams P)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const ExplodeEntry e = list[t];
+    const BoxDev &B = boxes[e.fine_box];
+    const int64_t loc = e.fine_local;
+    const uint32_t m = mask[B.base + loc];
+    if (!(m & 1u)) return;
+    double fv[NQ], gv[NQ];
+    pull_fab(f, g, B, loc, m, fv, gv);
+    double rho, jx, jy, jz, E;
+    accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
+    const Macro M = macro_from(rho, jx, jy, jz, E, P);
+    double *q = Q + B.base * 3 + loc;
+    q[0] = M.Qx;
+    q[B.ncell] = M.Qy;
+    q[2 * B.ncell] = M.Qz;
+}
+
+template <bool COAL>
+__global__ void __launch_bounds__(kFabTile) k_fab_collide(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo,
+    double *__restrict__ go, const double *__restrict__ Q, const FabTile *__restrict__ tiles, FabArgs A,
+    CoalArgs C, CollideParams P)
+{
+    int step, end;
+    for (int t = xcd_tile_begin(A.ntiles, step, end); t < end; t += step) {
+        const FabTile tile = tiles[t];
+        const BoxDev &B = A.boxes[tile.box];
+        int i, j, k;
+        int64_t loc;
+        if (!tile_cell(A, tile, int(threadIdx.x), B, i, j, k, loc)) continue;
+        const uint32_t m = A.mask[B.base + loc];
+        double *fob = fo + B.base * NQ + loc;
+        double *gob = go + B.base * NQ + loc;
+        const int64_t s = B.ncell;
+        if (!(m & 1u)) {
+            // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
+            for (int q = 0; q < NQ; ++q) {
+                fob[q * s] = -1.0;
+                gob[q * s] = -1.0;
+            }
+            continue;
+        }
+        double fv[NQ], gv[NQ];
+        gather_cell<COAL>(f, g, B, loc, m, C, fv, gv);
+        Moments mo;
+        accumulate_moments(fv, gv, mo);
+        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        // gradient(): neighbour inside the domain box and fluid (bit 2 of
+        // the mask is the source x + e_x, bit 1 the source x - e_x, ...)
+        const bool xp = i + 1 < A.dom[0] && ((m >> 2) & 1u), xm = i - 1 >= 0 && ((m >> 1) & 1u);
+        const bool yp = j + 1 < A.dom[1] && ((m >> 4) & 1u), ym = j - 1 >= 0 && ((m >> 3) & 1u);
+        const bool zp = k + 1 < A.dom[2] && ((m >> 6) & 1u), zm = k - 1 >= 0 && ((m >> 5) & 1u);
+        const double *q = Q + B.base * 3 + loc;
+        const int64_t sy = B.n[0], sz = int64_t(B.n[0]) * B.n[1];
+        const double dqx = one_sided_gradient(xp, xm, xp ? q[1] : 0.0, M.Qx, xm ? q[-1] : 0.0, P.inv_dx[0]);
+        const double dqy = one_sided_gradient(yp, ym, yp ? q[s + sy] : 0.0, M.Qy, ym ? q[s - sy] : 0.0,
+                                              P.inv_dx[1]);
+        const double dqz = one_sided_gradient(zp, zm, zp ? q[2 * s + sz] : 0.0, M.Qz,
+                                              zm ? q[2 * s - sz] : 0.0, P.inv_dx[2]);
+        collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
+            constexpr int qq = decltype(qc)::value;
+            __builtin_nontemporal_store(fnew, fob + qq * s);
+            __builtin_nontemporal_store(gnew, gob + qq * s);
+        });
+    }
+}
+
+// LBM::stream (LBM.cpp:558-604) at the listed ghost cells (fine cells over
+// uncovered domain, from the explosion map): pull with the in-fab test
+__global__ void __launch_bounds__(256) k_stream_listed(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo,
+    double *__restrict__ go, const int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes,
+    const ExplodeEntry *__restrict__ list, int64_t n)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const ExplodeEntry e = list[t];
+    const BoxDev &B = boxes[e.fine_box];
+    const int64_t loc = e.fine_local;
+    const int64_t nxy = int64_t(B.n[0]) * B.n[1];
+    const int kk = int(loc / nxy);
+    const int jj = int((loc - kk * nxy) / B.n[0]);
+    const int ii = int(loc - kk * nxy - int64_t(jj) * B.n[0]);
+    const int32_t *fl = isf + B.base * 2;
+    const bool fluid = fl[loc] == 1;
+    const double *fb = f + B.base * NQ;
+    const double *gb = g + B.base * NQ;
+    double *fob = fo + B.base * NQ;
+    double *gob = go + B.base * NQ;
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+        const int a = ii - cx, b = jj - cy, c = kk - cz;
+        double vf = -1.0, vg = -1.0;
+        if (fluid && a >= 0 && a < B.n[0] && b >= 0 && b < B.n[1] && c >= 0 && c < B.n[2]) {
+            const int64_t sl = (int64_t(c) * B.n[1] + b) * B.n[0] + a;
+            const int64_t idx = fl[sl] != 0 ? int64_t(q) * B.ncell + sl : int64_t(kBounce[q]) * B.ncell + loc;
+            vf = fb[idx];
+            vg = gb[idx];
+        }
+        fob[q * B.ncell + loc] = vf;
+        gob[q * B.ncell + loc] = vg;
+    });
+}
+
+// per fab cell: bit q = is_fluid(x - c_q) != 0 (sources inside the fab),
+// bit 0 = own is_fluid == 1
+__global__ void k_fab_mask(
+    const int32_t *__restrict__ isf, uint32_t *__restrict__ mask, const BoxDev *__restrict__ boxes, int nbox,
+    const int64_t *__restrict__ base, int64_t total)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= c)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BoxDev B = boxes[lo];
+    const int64_t loc = c - B.base;
+    const int64_t nxy = int64_t(B.n[0]) * B.n[1];
+    const int kk = int(loc / nxy);
+    const int jj = int((loc - kk * nxy) / B.n[0]);
+    const int ii = int(loc - kk * nxy - int64_t(jj) * B.n[0]);
+    const int32_t *fl = isf + B.base * 2;
+    uint32_t m = fl[loc] == 1 ? 1u : 0u;
+    for (int q = 1; q < NQ; ++q) {
+        const int a = ii - kEV[q][0], b = jj - kEV[q][1], e = kk - kEV[q][2];
+        if (a >= 0 && a < B.n[0] && b >= 0 && b < B.n[1] && e >= 0 && e < B.n[2] &&
+            fl[(int64_t(e) * B.n[1] + b) * B.n[0] + a] != 0)
+            m |= 1u << q;
+    }
+    mask[c] = m;
+}
+
+unsigned grid_for(int ntiles, int blocks)
+{
+    // a multiple of 8 (one group per XCD), at most one block per tile per XCD
+    const int full = (ntiles + 7) / 8 * 8;
+    int g = blocks > 0 ? std::min(blocks, full) : full;
+    g = std::max(8, (g + 7) / 8 * 8);
+    return unsigned(g);
+}
+
+}  // namespace
+
+void build_fab_tiles(const mrb_level *L, int grow, std::vector<FabTile> &tiles)
+{
+    tiles.clear();
+    for (int b = 0; b < L->nbox; ++b) {
+        const BoxDev &B = L->boxes[b];
+        int64_t n = 1;
+        for (int d = 0; d < 3; ++d) n *= B.vhi[d] - B.vlo[d] + 1 + 2 * grow;
+        for (int64_t c = 0; c < n; c += kFabTile) tiles.push_back(FabTile{b, int32_t(c)});
+    }
+}
+
+int fab_build_mask(mrb_level *L, hipStream_t s)
+{
+    const int64_t T = L->total_cells;
+    k_fab_mask<<<unsigned((T + 255) / 256), 256, 0, s>>>(L->d_isf, L->d_mask, L->d_boxes, L->nbox,
+                                                          L->d_box_base, T);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+const int kFabBlockChoices[] = {0, 1024, 2048, 4096, 8192};
+const int kNumFabBlockChoices = int(sizeof(kFabBlockChoices) / sizeof(kFabBlockChoices[0]));
+
+int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s);
+
+int fab_batched_step(mrb_level *L, const mrb_level *coal_from, hipStream_t s)
+{
+    int rc = fab_batched_passes(L, coal_from, s);
+    if (rc) return rc;
+    L->cur = 1 - L->cur;
+    rc = fab_fill_boundary(L, 0, s);
+    if (rc) return rc;
+    rc = fab_fill_boundary(L, 1, s);
+    if (rc) return rc;
+    L->ghosts_fresh = true;
+    return MRB_OK;
+}
+
+// passes 1 and 2 (and the listed ghost stream) into the spare buffer; the
+// current buffer is not modified
+int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s)
+{
+    const int cur = L->cur, nxt = 1 - L->cur;
+    const double *f = (const double *)L->pop[0][cur];
+    const double *g = (const double *)L->pop[1][cur];
+    double *fo = (double *)L->pop[0][nxt];
+    double *go = (double *)L->pop[1][nxt];
+    FabArgs A{};
+    A.boxes = L->d_boxes;
+    A.mask = L->d_mask;
+    for (int d = 0; d < 3; ++d) A.dom[d] = L->ncell[d];
+    CoalArgs C{};
+    if (coal_from) {
+        C.fine_f = (const double *)coal_from->pop[0][coal_from->cur];
+        C.fine_g = (const double *)coal_from->pop[1][coal_from->cur];
+        C.fine_boxes = coal_from->d_boxes;
+        C.map = (const CoalesceEntry *)coal_from->d_coal;
+        C.cell = coal_from->d_coal_cell;
+    }
+    A.grow = 0;
+    A.ntiles = L->n_tiles_c;
+    const unsigned gc = grid_for(A.ntiles, L->fab_blocks);
+    if (coal_from)
+        k_fab_qpass<true><<<gc, kFabTile, 0, s>>>(f, g, L->d_dq, L->d_tiles_c, A, C, L->cp);
+    else
+        k_fab_qpass<false><<<gc, kFabTile, 0, s>>>(f, g, L->d_dq, L->d_tiles_c, A, C, L->cp);
+    MRB_HIP(hipGetLastError());
+    if (L->n_qring > 0) {
+        k_fab_qpass_listed<<<unsigned((L->n_qring + 255) / 256), 256, 0, s>>>(
+            f, g, L->d_dq, L->d_boxes, L->d_mask, (const ExplodeEntry *)L->d_expl, L->n_qring, L->cp);
+        MRB_HIP(hipGetLastError());
+    }
+    int rc = fab_fill_boundary_q(L, s);
+    if (rc) return rc;
+    if (coal_from)
+        k_fab_collide<true><<<gc, kFabTile, 0, s>>>(f, g, fo, go, L->d_dq, L->d_tiles_c, A, C, L->cp);
+    else
+        k_fab_collide<false><<<gc, kFabTile, 0, s>>>(f, g, fo, go, L->d_dq, L->d_tiles_c, A, C, L->cp);
+    MRB_HIP(hipGetLastError());
+    if (L->n_expl > 0) {
+        k_stream_listed<<<unsigned((L->n_expl + 255) / 256), 256, 0, s>>>(
+            f, g, fo, go, L->d_isf, L->d_boxes, (const ExplodeEntry *)L->d_expl, L->n_expl);
+        MRB_HIP(hipGetLastError());
+    }
+    return MRB_OK;
+}
+
+}  // namespace mrb

@@ -12,6 +12,8 @@
 //   k_qpass:   pull 54 populations, moments -> Q_corr (3 doubles per cell)
 //   k_collide: pull 54 populations, moments, dQ from the Q array of the
 //              neighbours, equilibria, relax, write 54 populations to B.
+#include <type_traits>
+
 #include "mrb_internal.h"
 
 namespace mrb {
@@ -96,10 +98,39 @@ __device__ __forceinline__ int64_t pull_index(
     return int64_t(qb) * A.cs + cell;  // NOSLIP
 }
 
+// Rest-state populations subtracted by the shifted storages (stored value =
+// population - shift); all zero for FP64 storage.
+struct ShiftTab {
+    double f[NQ];
+    double g[NQ];
+};
+
+// storage <-> FP64 working value
+template <class T>
+__device__ __forceinline__ double widen(T v, double shift)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        (void)shift;
+        return v;
+    } else {
+        return double(v) + shift;
+    }
+}
+template <class T>
+__device__ __forceinline__ T narrow(double v, double shift)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        (void)shift;
+        return v;
+    } else {
+        return T(v - shift);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
-    int x, int y, int z, int64_t cell, uint32_t mask, double fv[NQ], double gv[NQ])
+    const ShiftTab &sh, int x, int y, int z, int64_t cell, uint32_t mask, double fv[NQ], double gv[NQ])
 {
     int sx[3], sy[3], sz[3];
     bool gx[3], gy[3], gz[3];
@@ -110,8 +141,8 @@ __device__ __forceinline__ void pull_cell(
         constexpr int q = decltype(qc)::value;
         bool use_const;
         const int64_t idx = pull_index<q>(A, cell, mask, sx, sy, sz, gx, gy, gz, use_const);
-        const double fq = double(f[idx]);
-        const double gq = double(g[idx]);
+        const double fq = widen(f[idx], sh.f[q]);
+        const double gq = widen(g[idx], sh.g[q]);
         fv[q] = use_const ? bc.fval[q] : fq;
         gv[q] = use_const ? bc.gval[q] : gq;
     });
@@ -128,9 +159,10 @@ __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y
 }
 
 // pass 1: Q_corr of the post-stream state
+template <class T>
 __global__ void __launch_bounds__(256) k_qpass(
-    const double *__restrict__ f, const double *__restrict__ g, const uint32_t *__restrict__ mask,
-    double *__restrict__ Q, CompactArgs A, BCDev bc, CollideParams P)
+    const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask,
+    double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, CollideParams P)
 {
     int x, y, z;
     int64_t cell;
@@ -138,7 +170,7 @@ __global__ void __launch_bounds__(256) k_qpass(
     const uint32_t m = mask[cell];
     if (!(m & 1u)) return;
     double fv[NQ], gv[NQ];
-    pull_cell(f, g, A, bc, x, y, z, cell, m, fv, gv);
+    pull_cell(f, g, A, bc, sh, x, y, z, cell, m, fv, gv);
     double rho, jx, jy, jz, E;
     accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
     const Macro M = macro_from(rho, jx, jy, jz, E, P);
@@ -148,10 +180,11 @@ __global__ void __launch_bounds__(256) k_qpass(
 }
 
 // pass 2: stream + collide, writes the B buffers
+template <class T>
 __global__ void __launch_bounds__(256) k_collide(
-    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo,
-    double *__restrict__ go, const uint32_t *__restrict__ mask, const double *__restrict__ Q,
-    CompactArgs A, BCDev bc, CollideParams P)
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    CollideParams P)
 {
     int x, y, z;
     int64_t cell;
@@ -161,13 +194,13 @@ __global__ void __launch_bounds__(256) k_collide(
         // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            fo[int64_t(q) * A.cs + cell] = -1.0;
-            go[int64_t(q) * A.cs + cell] = -1.0;
+            fo[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.f[q]);
+            go[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.g[q]);
         });
         return;
     }
     double fv[NQ], gv[NQ];
-    pull_cell(f, g, A, bc, x, y, z, cell, m, fv, gv);
+    pull_cell(f, g, A, bc, sh, x, y, z, cell, m, fv, gv);
     Moments mo;
     accumulate_moments(fv, gv, mo);
     const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
@@ -184,15 +217,16 @@ __global__ void __launch_bounds__(256) k_collide(
                                           zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
     collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.cs + cell] = fnew;
-        go[int64_t(q) * A.cs + cell] = gnew;
+        fo[int64_t(q) * A.cs + cell] = narrow<T>(fnew, sh.f[q]);
+        go[int64_t(q) * A.cs + cell] = narrow<T>(gnew, sh.g[q]);
     });
 }
 
 // stream only (parity of the pull with the reference push)
+template <class T>
 __global__ void __launch_bounds__(256) k_stream_only(
-    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo,
-    double *__restrict__ go, const uint32_t *__restrict__ mask, CompactArgs A, BCDev bc)
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, CompactArgs A, BCDev bc, ShiftTab sh)
 {
     int x, y, z;
     int64_t cell;
@@ -200,14 +234,14 @@ __global__ void __launch_bounds__(256) k_stream_only(
     const uint32_t m = mask[cell];
     double fv[NQ], gv[NQ];
     if (m & 1u) {
-        pull_cell(f, g, A, bc, x, y, z, cell, m, fv, gv);
+        pull_cell(f, g, A, bc, sh, x, y, z, cell, m, fv, gv);
     } else {
         for (int q = 0; q < NQ; ++q) fv[q] = gv[q] = -1.0;
     }
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.cs + cell] = fv[q];
-        go[int64_t(q) * A.cs + cell] = gv[q];
+        fo[int64_t(q) * A.cs + cell] = narrow<T>(fv[q], sh.f[q]);
+        go[int64_t(q) * A.cs + cell] = narrow<T>(gv[q], sh.g[q]);
     });
 }
 
@@ -264,23 +298,41 @@ int compact_build_mask(mrb_level *L, hipStream_t s)
 
 // Two-pass step; reads the current buffers, writes the spare ones, does not
 // flip.  ev (if not null) receives events before, between and after.
-static int two_pass(mrb_level *L, hipStream_t s, hipEvent_t *ev)
+static ShiftTab shift_tab(const mrb_level *L)
 {
+    ShiftTab t{};
     if (L->storage != MRB_STORAGE_F64)
-        return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
+        for (int q = 0; q < NQ; ++q) {
+            t.f[q] = L->fshift[q];
+            t.g[q] = L->gshift[q];
+        }
+    return t;
+}
+
+template <class T>
+static int two_pass_t(mrb_level *L, hipStream_t s, hipEvent_t *ev)
+{
     const CompactArgs A = make_args(L);
-    const double *f = (const double *)L->pop[0][L->cur];
-    const double *g = (const double *)L->pop[1][L->cur];
-    double *fo = (double *)L->pop[0][1 - L->cur];
-    double *go = (double *)L->pop[1][1 - L->cur];
+    const ShiftTab sh = shift_tab(L);
+    const T *f = (const T *)L->pop[0][L->cur];
+    const T *g = (const T *)L->pop[1][L->cur];
+    T *fo = (T *)L->pop[0][1 - L->cur];
+    T *go = (T *)L->pop[1][1 - L->cur];
     if (ev) MRB_HIP(hipEventRecord(ev[0], s));
-    k_qpass<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, L->cp);
+    k_qpass<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[1], s));
-    k_collide<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, L->cp);
+    k_collide<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[2], s));
     return MRB_OK;
+}
+
+static int two_pass(mrb_level *L, hipStream_t s, hipEvent_t *ev)
+{
+    if (L->storage == MRB_STORAGE_F64) return two_pass_t<double>(L, s, ev);
+    if (L->storage == MRB_STORAGE_F32_SHIFTED) return two_pass_t<float>(L, s, ev);
+    return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
 }
 
 int compact_two_pass_launch(mrb_level *L, hipStream_t s) { return two_pass(L, s, nullptr); }
@@ -315,12 +367,19 @@ int compact_stream_collide_events(mrb_level *L, hipStream_t s, int &nslots)
 
 int compact_stream_only(mrb_level *L, hipStream_t s)
 {
-    if (L->storage != MRB_STORAGE_F64)
-        return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
     const CompactArgs A = make_args(L);
-    k_stream_only<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(
-        (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur], (double *)L->pop[0][1 - L->cur],
-        (double *)L->pop[1][1 - L->cur], L->d_mask, A, L->bcdev);
+    const ShiftTab sh = shift_tab(L);
+    if (L->storage == MRB_STORAGE_F64) {
+        k_stream_only<double><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(
+            (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur],
+            (double *)L->pop[0][1 - L->cur], (double *)L->pop[1][1 - L->cur], L->d_mask, A, L->bcdev, sh);
+    } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
+        k_stream_only<float><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(
+            (const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur], (float *)L->pop[0][1 - L->cur],
+            (float *)L->pop[1][1 - L->cur], L->d_mask, A, L->bcdev, sh);
+    } else {
+        return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
+    }
     MRB_HIP(hipGetLastError());
     L->cur = 1 - L->cur;
     return MRB_OK;

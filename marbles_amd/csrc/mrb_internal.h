@@ -52,6 +52,32 @@ constexpr uint32_t kMaskGeneral = 1u << 27;
 
 enum class Storage { F64 = 0, F32S = 1, F16S = 2 };
 
+// Batched fab-layout passes: a tile is kFabTile consecutive cells (x
+// fastest) of one box's valid region, grown by 0 or 1 cells.
+constexpr int kFabTile = 256;
+struct FabTile {
+    int32_t box;
+    int32_t first;
+};
+
+// Two-level transfer maps (built by mrb_amr_link, held by the fine level).
+// coarse cell <- masked mean of its 2x2x2 fine children
+struct CoalesceEntry {
+    int64_t crse_local;  // coarse fab cell (local index in crse_box's fab)
+    int64_t fine_local;  // child (0,0,0) (local index in fine_box's fab)
+    int32_t crse_box;
+    int32_t fine_box;
+};
+// fine ghost cell over uncovered domain <- interpolation from its parent
+struct ExplodeEntry {
+    int64_t fine_local;
+    int64_t crse_local;  // the parent
+    int32_t fine_box;
+    int32_t crse_box;
+    int32_t bits;  // bit d: high child in direction d; bit 3+d: both neighbours in d inside the domain
+    int32_t pad;
+};
+
 // Single-pass z-sweep kernel shapes: rows per tile and z planes per chunk.
 struct FusedVariant {
     const char *name;
@@ -113,6 +139,30 @@ struct mrb_level {
     int64_t n_coal = 0;
     void *d_expl = nullptr;  // mrb::ExplodeEntry[n_expl]
     int64_t n_expl = 0;
+    int32_t *d_coal_cell = nullptr;  // per coarse fab cell: coalescence entry or -1
+    // the first n_qring entries of the explosion map are ring-1 ghost cells
+    int64_t n_qring = 0;
+    // batched fab-layout step: tile list of the valid cells and the number
+    // of persistent blocks (0 = one block per tile)
+    mrb::FabTile *d_tiles_c = nullptr;
+    int n_tiles_c = 0;
+    int fab_blocks = 0;
+    // fab layout: every ghost cell FillBoundary writes holds its owner's
+    // current value (set after the post-collide FillBoundary, cleared by
+    // anything that changes valid data otherwise)
+    bool ghosts_fresh = false;
+    // bumped whenever captured kernel arguments would go stale (physics,
+    // boundary values, buffer pointers, launch shapes, links)
+    uint64_t gen = 1;
+    // hipGraph of one coarse step, held by the fine level of a linked pair:
+    // [coarse buffer index][coarse ghosts fresh]
+    struct AmrGraph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t gen_c = 0, gen_f = 0;
+        int fine_cur = 0;
+        int post_cur_c = 0, post_cur_f = 0;
+    } amr_graph[2][2];
+    hipStream_t cap_stream = nullptr;
 };
 
 namespace mrb {
@@ -144,7 +194,19 @@ int fused_step(mrb_level *L, int variant, hipStream_t s);
 int compact_two_pass_launch(mrb_level *L, hipStream_t s);
 // FillBoundary of one lattice's current fab buffer (mrb_level.hip)
 int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s);
+// FillBoundary of the batched step's Q array (3 comps in d_dq), one layer
+int fab_fill_boundary_q(mrb_level *L, hipStream_t s);
 // frees the two-level maps held by a fine level (mrb_amr.hip)
 void amr_release(mrb_level *L);
+// batched fab-layout step (mrb_batched.hip): pass 1 + pass 2 into the spare
+// buffer (coalescing from `coal_from` when given), ghost streaming over the
+// explosion map of a fine level, FillBoundary, flip
+int fab_batched_step(mrb_level *L, const mrb_level *coal_from, hipStream_t s);
+int fab_build_mask(mrb_level *L, hipStream_t s);
+int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s);
+// persistent-block counts the fab-layout autotuner tries (0 = one block per tile)
+extern const int kFabBlockChoices[];
+extern const int kNumFabBlockChoices;
+void build_fab_tiles(const mrb_level *L, int grow, std::vector<FabTile> &tiles);
 
 }  // namespace mrb

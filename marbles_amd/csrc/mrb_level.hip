@@ -12,6 +12,8 @@
 #include <memory>
 #include <thread>
 
+#include <string>
+
 #include "mrb_internal.h"
 
 namespace mrb {
@@ -559,6 +561,8 @@ int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s)
     return fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, s);
 }
 
+int fab_fill_boundary_q(mrb_level *L, hipStream_t s) { return fill_boundary(L, L->d_dq, 3, false, true, s); }
+
 static void compute_bc_values(mrb_level *L)
 {
     BCDev &d = L->bcdev;
@@ -729,6 +733,13 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
             MRB_HIP(hipMalloc(&L->d_gmap1, sizeof(GhostPair) * map.size()));
             MRB_HIP(hipMemcpy(L->d_gmap1, map.data(), sizeof(GhostPair) * map.size(), hipMemcpyHostToDevice));
         }
+        MRB_HIP(hipMalloc(&L->d_mask, sizeof(uint32_t) * T));
+        MRB_HIP(hipMemset(L->d_mask, 0, sizeof(uint32_t) * T));
+        std::vector<FabTile> tiles;
+        build_fab_tiles(L.get(), 0, tiles);
+        L->n_tiles_c = int(tiles.size());
+        MRB_HIP(hipMalloc(&L->d_tiles_c, sizeof(FabTile) * tiles.size()));
+        MRB_HIP(hipMemcpy(L->d_tiles_c, tiles.data(), sizeof(FabTile) * tiles.size(), hipMemcpyHostToDevice));
         // fab cells outside a non-periodic domain face
         std::vector<int64_t> cells;
         for (int b = 0; b < L->nbox; ++b) {
@@ -799,6 +810,7 @@ int mrb_level_destroy(mrb_level *L)
     (void)hipFree(L->d_gmap);
     (void)hipFree(L->d_gmap1);
     (void)hipFree(L->d_bcells);
+    (void)hipFree(L->d_tiles_c);
     amr_release(L);
     delete L;
     return MRB_OK;
@@ -811,6 +823,7 @@ int mrb_set_phys(mrb_level *L, const mrb_phys *p)
         return set_error(MRB_E_ARG, "invalid physics parameters");
     L->phys = *p;
     L->phys_set = true;
+    ++L->gen;
     CollideParams &c = L->cp;
     c.nu = p->nu;
     c.alpha = p->alpha;
@@ -826,6 +839,7 @@ int mrb_set_phys(mrb_level *L, const mrb_phys *p)
 int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
 {
     if (!L || !bc) return set_error(MRB_E_ARG, "null argument");
+    ++L->gen;
     for (int d = 0; d < 3; ++d) {
         for (int lohi = 0; lohi < 2; ++lohi) {
             const int t = bc->bc_type[d + 3 * lohi];
@@ -890,6 +904,7 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
         return MRB_OK;
     }
     if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
+    L->ghosts_fresh = false;
     const int64_t n = NQ * L->total_cells;
     const double *h = static_cast<const double *>(host);
     if (L->storage == MRB_STORAGE_F64) return copy_lattice(L, arr, const_cast<void *>(host), true);
@@ -1005,6 +1020,10 @@ int mrb_finalize_flags(mrb_level *L)
                 }
         MRB_HIP(hipMemcpy(L->d_isf, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
         const int rc = compact_build_mask(L, nullptr);
+        if (rc) return rc;
+        MRB_HIP(hipDeviceSynchronize());
+    } else {
+        const int rc = fab_build_mask(L, nullptr);
         if (rc) return rc;
         MRB_HIP(hipDeviceSynchronize());
     }
@@ -1177,8 +1196,12 @@ int mrb_fill_ghosts(mrb_level *L, int lattice, void *stream)
     if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
     // The bounce prefill of FillPatchOps.H:92-110 is overwritten in full by
     // FillBoundary and the boundary functor, so it is not executed.
-    int rc = fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, as_stream(stream));
-    if (rc) return rc;
+    // FillBoundary is skipped when the ghosts already hold the owners'
+    // current values (after a collide); the boundary functor always runs.
+    if (!L->ghosts_fresh) {
+        const int rc = fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, as_stream(stream));
+        if (rc) return rc;
+    }
     return mrb_physbc(L, lattice, stream);
 }
 
@@ -1197,6 +1220,8 @@ int mrb_stream(mrb_level *L, int lattice, void *stream)
                                                             L->nbox, L->total_cells);
     MRB_HIP(hipGetLastError());
     std::swap(L->pop[lattice][0], L->pop[lattice][1]);
+    L->ghosts_fresh = false;
+    ++L->gen;
     return fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, s);
 }
 
@@ -1222,7 +1247,10 @@ int mrb_collide(mrb_level *L, void *stream)
     MRB_HIP(hipGetLastError());
     rc = fill_boundary(L, f, NQ, false, false, s);
     if (rc) return rc;
-    return fill_boundary(L, g, NQ, false, false, s);
+    rc = fill_boundary(L, g, NQ, false, false, s);
+    if (rc) return rc;
+    L->ghosts_fresh = true;
+    return MRB_OK;
 }
 
 // ---- hot path --------------------------------------------------------------
@@ -1232,13 +1260,19 @@ int mrb_stream_collide(mrb_level *L, void *stream)
     if (!L) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
     if (L->layout == MRB_LAYOUT_COMPACT) return compact_stream_collide(L, as_stream(stream));
-    // reference layout: the literal stage sequence
+    // reference layout: ghost fill (FillPatchOps::fillpatch), then stream
+    // and collide as one batched step over every box
     int rc;
     for (int l = 0; l < 2; ++l)
         if ((rc = mrb_fill_ghosts(L, l, stream))) return rc;
-    for (int l = 0; l < 2; ++l)
-        if ((rc = mrb_stream(L, l, stream))) return rc;
-    return mrb_collide(L, stream);
+    if (L->ng < 2) {
+        // the batched pass 1 pulls ring-1 ghosts from ring 2: with a single
+        // ghost layer run the stages one by one
+        for (int l = 0; l < 2; ++l)
+            if ((rc = mrb_stream(L, l, stream))) return rc;
+        return mrb_collide(L, stream);
+    }
+    return fab_batched_step(L, nullptr, as_stream(stream));
 }
 
 int mrb_advance(mrb_level *L, int nsteps, void *stream)
@@ -1290,10 +1324,59 @@ int mrb_set_hot_kernel(mrb_level *L, int index)
     return MRB_OK;
 }
 
+int mrb_autotune_count(const mrb_level *L)
+{
+    if (!L) return 0;
+    return L->layout == MRB_LAYOUT_COMPACT ? mrb_hot_kernel_count() : kNumFabBlockChoices;
+}
+
+const char *mrb_autotune_name(const mrb_level *L, int index)
+{
+    static thread_local std::string name;
+    if (!L || index < 0 || index >= mrb_autotune_count(L)) return "";
+    if (L->layout == MRB_LAYOUT_COMPACT) return mrb_hot_kernel_variant_name(index);
+    const int b = kFabBlockChoices[index];
+    name = b == 0 ? std::string("fab_2pass_block_per_tile") : "fab_2pass_persistent_" + std::to_string(b);
+    return name.c_str();
+}
+
+static int autotune_fab(mrb_level *L, int reps, hipStream_t s, double *ms_out, int *best)
+{
+    if (L->ng < 2) return set_error(MRB_E_UNSUPPORTED, "the batched fab step needs ng >= 2");
+    int arg = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < kNumFabBlockChoices; ++v) {
+        L->fab_blocks = kFabBlockChoices[v];
+        int rc = fab_batched_passes(L, nullptr, s);  // warm-up
+        if (rc) return rc;
+        MRB_HIP(hipEventRecord(L->ev[3], s));
+        for (int r = 0; r < reps; ++r)
+            if ((rc = fab_batched_passes(L, nullptr, s))) return rc;
+        MRB_HIP(hipEventRecord(L->ev[4], s));
+        MRB_HIP(hipEventSynchronize(L->ev[4]));
+        float ms = 0.f;
+        MRB_HIP(hipEventElapsedTime(&ms, L->ev[3], L->ev[4]));
+        const double mean = double(ms) / reps;
+        if (ms_out) ms_out[v] = mean;
+        if (mean < best_ms) {
+            best_ms = mean;
+            arg = v;
+        }
+    }
+    L->fab_blocks = kFabBlockChoices[arg];
+    ++L->gen;
+    if (best) *best = arg;
+    return MRB_OK;
+}
+
 int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best)
 {
     if (!L) return set_error(MRB_E_ARG, "null argument");
-    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    if (reps < 1) return set_error(MRB_E_ARG, "reps must be positive");
+    if (L->layout == MRB_LAYOUT_FAB) {
+        if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+        return autotune_fab(L, reps, as_stream(stream), ms_out, best);
+    }
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
     if (reps < 1) return set_error(MRB_E_ARG, "reps must be positive");
     hipStream_t s = as_stream(stream);
@@ -1303,6 +1386,11 @@ int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best
     for (int v = 0; v < n; ++v) {
         auto launch = [&]() { return v == 0 ? compact_two_pass_launch(L, s) : fused_launch(L, v - 1, s); };
         int rc = launch();  // warm-up
+        if (rc == MRB_E_UNSUPPORTED) {
+            // variant not available for this level (storage, boundary types)
+            if (ms_out) ms_out[v] = -1.0;
+            continue;
+        }
         if (rc) return rc;
         MRB_HIP(hipEventRecord(L->ev[3], s));
         for (int r = 0; r < reps; ++r)

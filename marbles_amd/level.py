@@ -227,7 +227,8 @@ class Level:
     def autotune(self, reps: int = 5, stream=None) -> dict:
         """Times every kernel variant without advancing the state; selects the fastest.
         Returns {variant name: mean ms per step}."""
-        names = self.hot_kernel_variants()
+        names = [self.lib.mrb_autotune_name(self.h, i).decode()
+                 for i in range(self.lib.mrb_autotune_count(self.h))]
         ms = (c_double * len(names))()
         best = c_int()
         check(self.lib.mrb_autotune(self.h, int(reps), stream, ms, ctypes.byref(best)))

@@ -94,3 +94,67 @@ def test_amr_advance_matches_oracle(oracle, engine, case):
             assert err <= 1e-12, (case, name, err)
     print(f"{case}: {n} coarse steps, bit-exact={exact}")
     H.close()
+
+
+@pytest.mark.parametrize("case", ["outflow_x", "uneven_32"])
+def test_batched_fab_step_on_uneven_boxes_matches_oracle(oracle, engine, case):
+    """Single level over the coarse level's uneven boxes (edges 4 and 8):
+    the batched fab step (one launch per pass over every box) against the
+    oracle's literal stages, including ring-1 ghosts next to outflow faces."""
+    from oracle.pyoracle import OracleLevel
+    from marbles_amd import configs
+
+    cfg = CASES[case]()
+    S = OracleLevel(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=cfg["coarse_boxes"], ng=3)
+    S.set_phys(cfg["nu"], cfg["alpha"])
+    S.set_bc(cfg["bc_lo"], cfg["bc_hi"])
+    S.flags_all_regular()
+    S.ic_fields(*configs.level_fields(cfg, S.ncell, S.boxes, 3))
+    S.init_finish()
+    E = engine.Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=cfg["coarse_boxes"],
+                     ng=3, layout="fab")
+    E.set_phys(cfg["nu"], cfg["alpha"])
+    E.set_bc(cfg["bc_lo"], cfg["bc_hi"])
+    E.flags_all_regular()
+    E.init_fields(*configs.level_fields(cfg, E.ncell, E.boxes, 3))
+    S.evolve(5)
+    E.advance(5)
+    for name in ("f", "g"):
+        a, b = E.valid(name), S.valid(name)
+        assert np.array_equal(a, b), (case, name, float(np.abs(a - b).max()))
+
+
+def test_graph_replay_matches_eager_steps(oracle, engine):
+    """hipGraph replay of the coarse step (two graphs, one per coarse buffer
+    parity) against eager launches, replay after replay, with eager work and
+    synchronisations between the replays."""
+    cfg = CASES["uneven_32"]()
+    He = engine.Hierarchy(cfg)
+    Hg = engine.Hierarchy(cfg)
+    for rep in range(5):
+        He.advance(1)
+        Hg.advance(1, graph=True)
+        for lvl_e, lvl_g in ((He.crse, Hg.crse), (He.fine, Hg.fine)):
+            for name in ("f", "g"):
+                assert np.array_equal(lvl_e.valid(name), lvl_g.valid(name)), (rep, name)
+        # eager work on the graph's levels between replays
+        assert np.isfinite(Hg.crse.conservation()).all()
+    He.close()
+    Hg.close()
+
+
+def test_fab_autotune_keeps_the_state_and_results(oracle, engine):
+    cfg = CASES["uneven_32"]()
+    C, F = oracle_hierarchy(cfg)
+    H = engine.Hierarchy(cfg)
+    before = H.fine.download("f")
+    times = H.fine.autotune(reps=2)
+    times.update({"crse:" + k: v for k, v in H.crse.autotune(reps=2).items()})
+    assert all(v > 0 for v in times.values()), times
+    assert np.array_equal(H.fine.download("f"), before)
+    oracle.amr_evolve(C, F, 3)
+    H.advance(3)
+    for lvl, O in ((H.crse, C), (H.fine, F)):
+        for name in ("f", "g"):
+            assert np.array_equal(lvl.valid(name), O.valid(name)), name
+    H.close()

@@ -193,3 +193,32 @@ def test_autotune_does_not_advance_the_state(oracle, engine):
     E.advance(5)
     O.evolve(5)
     assert _rel_err(E.valid("f"), O.valid("f")) <= REL_TOL
+
+
+def _velocity(E):
+    ev, _, _ = __import__("oracle.pyoracle", fromlist=["stencil"]).stencil()
+    f = E.valid("f")
+    rho = f.sum(axis=0)
+    return np.stack([np.tensordot(ev[:, d].astype(float), f, axes=(0, 0)) / rho for d in range(3)]), rho
+
+
+@pytest.mark.parametrize("case", ["tg_tau1.0", "channel_T0.2"])
+def test_f32_shifted_storage_tracks_f64(oracle, engine, case):
+    """FP32 storage of f - f_eq(rho0, 0, T0) with FP64 arithmetic: velocity
+    and density within 1e-6 (relative to max |u|, rho) of the FP64 run."""
+    if case == "tg_tau1.0":
+        cfg, n = configs.taylor_green(32, tau=1.0), 100
+    else:
+        cfg, n = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 100
+    runs = {}
+    for storage in ("f64", "f32s"):
+        E = configs.build_level(cfg, layout="compact", storage=storage)
+        E.advance(n)
+        runs[storage] = _velocity(E)
+        E.close()
+    (u64, r64), (u32, r32) = runs["f64"], runs["f32s"]
+    fluid = r64 > 0
+    du = np.abs(u32 - u64)[:, fluid].max() / np.abs(u64[:, fluid]).max()
+    dr = np.abs(r32 - r64)[fluid].max() / r64[fluid].max()
+    print(f"{case}: {n} steps, max rel velocity error {du:.2e}, density {dr:.2e}")
+    assert du <= 1e-6 and dr <= 1e-6, (du, dr)

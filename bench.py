@@ -100,6 +100,24 @@ def copy_ceiling(torch, nbytes: int = 2 << 30, iters: int = 20) -> float:
     return 2.0 * nbytes * iters / secs / 1e9
 
 
+def velocity_density(lvl, cfg) -> tuple:
+    """(u, v, w, rho) of the level's current state, FP64 on the host."""
+    from marbles_amd.stencil import velocity_density as vd
+
+    return vd(lvl.valid("f").reshape(27, -1))
+
+
+def time_level(lvl, steps: int, warmup: int, profile_steps: int, sh, torch) -> dict:
+    lvl.advance(warmup, stream=sh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lvl.advance(steps, stream=sh)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    prof = lvl.advance_timed(profile_steps, stream=sh)
+    return {"secs": secs, "per_kernel_ms": {k: v / profile_steps for k, v in prof.items()}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +129,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle timing")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-f32", action="store_true", help="skip the shifted-FP32 comparison run")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -135,26 +154,54 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
 
-    lvl.advance(args.warmup, stream=sh)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    lvl.advance(args.steps, stream=sh)
-    torch.cuda.synchronize()
-    secs = time.perf_counter() - t0
     cells = n ** 3
+    run = time_level(lvl, args.steps, args.warmup, args.profile_steps, sh, torch)
+    secs = run["secs"]
     mlups = cells * args.steps / secs / 1e6
     log(f"timed {args.steps} steps: {secs:.4f}s -> {mlups:.1f} MLUPS")
-
     # per-kernel durations from HIP events on the launch stream
-    prof = lvl.advance_timed(args.profile_steps, stream=sh)
-    per_kernel_ms = {k: v / args.profile_steps for k, v in prof.items()}
+    per_kernel_ms = run["per_kernel_ms"]
     step_kernel_ms = sum(per_kernel_ms.values())
     bpu = configs.bytes_per_update(args.storage, 2)
     achieved = bpu * cells / (step_kernel_ms * 1e-3) / 1e9
     log("per-kernel ms/step: " + ", ".join(f"{k}={v:.4f}" for k, v in per_kernel_ms.items()))
     cons = lvl.conservation(stream=sh)
     log(f"conservation (mass, mom x/y/z, 2rhoE): {cons.tolist()}")
-    lvl.close()
+
+    # the same run with shifted FP32 populations, compared at equal step count
+    f32 = None
+    if args.storage == "f64" and not args.no_f32:
+        (u64, rho64) = velocity_density(lvl, cfg)
+        lvl.close()
+        l32 = configs.build_level(cfg, layout="compact", storage="f32s")
+        r32 = time_level(l32, args.steps, args.warmup, args.profile_steps, sh, torch)
+        (u32, rho32) = velocity_density(l32, cfg)
+        l32.close()
+        import numpy as np
+
+        fluid = rho64 > 0
+        umax = max(float(np.abs(c[fluid]).max()) for c in u64)
+        du = max(float(np.abs(a[fluid] - b[fluid]).max()) for a, b in zip(u32, u64)) / umax
+        dr = float(np.abs(rho32[fluid] - rho64[fluid]).max() / np.abs(rho64[fluid]).max())
+        k32 = sum(r32["per_kernel_ms"].values())
+        m32 = cells * args.steps / r32["secs"] / 1e6
+        b32 = configs.bytes_per_update("f32s", 2)
+        f32 = {
+            "value": round(m32, 2),
+            "unit": "MLUPS",
+            "speedup_vs_f64": round(m32 / mlups, 3),
+            "max_rel_velocity_error": du,
+            "max_rel_density_error": dr,
+            "compared_after_steps": args.warmup + args.steps + args.profile_steps,
+            "arithmetic": "f64",
+            "bytes_per_update": b32,
+            "achieved_gbs": round(b32 * cells / (k32 * 1e-3) / 1e9, 1),
+            "kernel_ms_per_step": {k: round(v, 5) for k, v in r32["per_kernel_ms"].items()},
+        }
+        log(f"f32-shifted: {m32:.1f} MLUPS ({m32 / mlups:.2f}x), max rel velocity error {du:.2e}, "
+            f"density {dr:.2e}")
+    else:
+        lvl.close()
     copy_gbs = copy_ceiling(torch)
     log(f"copy-kernel ceiling: {copy_gbs:.1f} GB/s")
 
@@ -192,6 +239,7 @@ def main() -> int:
             "frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
         },
         "cpu_baseline": None,
+        "f32_shifted": f32,
     }
     if not args.no_cpu_baseline:
         try:

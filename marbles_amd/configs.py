@@ -109,6 +109,10 @@ def build_level(cfg: dict, layout: str = "compact", storage: str = "f64", ng: in
     ic = cfg["ic"]
     if ic["type"] == "taylorgreen":
         lvl.init_taylorgreen(ic["rho0"], ic["v0"], ic["omega"])
+    elif ic["type"] == "spectral":
+        if layout != "compact":
+            raise ValueError("spectral initial condition: compact layout only")
+        lvl.init_fields(*compact_fields(cfg), adiabatic_exponent=ic["adiabatic_exponent"])
     else:
         lvl.init_constant(ic["density"], ic["mach"], ic["temperature"], ic["adiabatic_exponent"])
     return lvl
@@ -275,3 +279,77 @@ def level_fields(cfg: dict, ncell, boxes, ng: int):
     rho = np.full_like(T, ic["density"])
     zero = np.zeros_like(T)
     return rho, zero, zero.copy(), zero.copy(), T
+
+
+# ---------------------------------------------------------------------------
+# Precision workload
+# ---------------------------------------------------------------------------
+
+def spectral_turbulence(n: int = 512, kmax: int = 32, amplitude: float = 0.04, tau: float = 0.52,
+                        temperature: float = 0.2, seed: int = 4) -> dict:
+    """Periodic box with a seeded random-phase divergence-free velocity field,
+    energy spectrum E(k) ~ k^(-5/3) for 1 <= |k| <= kmax (zero elsewhere),
+    max |u| = amplitude, rho = 1, uniform T0.
+
+    T0 = 0.2 rather than the reference default 1/3: at T0 = 1/3 the
+    reference thermal model amplifies a grid-scale energy mode for tau < 1
+    (tau = 0.52 diverges); at T0 = 0.2 it is stable.  nu = (tau - 1/2) R T0."""
+    return {
+        "name": f"spectral_{n}",
+        "ncell": (n, n, n),
+        "periodic": (1, 1, 1),
+        "prob_lo": (0.0, 0.0, 0.0),
+        "prob_hi": (float(n),) * 3,
+        "bc_lo": (0, 0, 0),
+        "bc_hi": (0, 0, 0),
+        "velocity_bc": None,
+        "nu": (tau - 0.5) * temperature,
+        "tau": tau,
+        "T0": temperature,
+        "ic": {"type": "spectral", "kmax": kmax, "amplitude": amplitude, "seed": seed, "density": 1.0,
+               "temperature": temperature, "adiabatic_exponent": 5.0 / 3.0},
+        "eb": {"type": "all_regular"},
+        "max_box": None,
+    }
+
+
+def spectral_velocity(n: int, kmax: int, amplitude: float, seed: int):
+    """(u, v, w) on an n^3 periodic grid (arrays indexed [z, y, x]): random
+    phases, |u_hat(k)|^2 ~ E(|k|) / (4 pi |k|^2) with E ~ k^(-5/3) on
+    1 <= |k| <= kmax, projected onto k . u_hat = 0, real by construction,
+    scaled to max |u| = amplitude."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kz, ky, kx = np.meshgrid(k1, k1, k1[: n // 2 + 1], indexing="ij")
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    shell = (kk >= 1.0) & (kk <= kmax)
+    amp = np.zeros_like(kk)
+    amp[shell] = np.sqrt(kk[shell] ** (-5.0 / 3.0) / (4.0 * np.pi * kk[shell] ** 2))
+    uh = []
+    for _ in range(3):
+        ph = rng.uniform(0.0, 2.0 * np.pi, kk.shape)
+        uh.append(amp * np.exp(1j * ph))
+    # solenoidal projection u_hat -= k (k . u_hat) / |k|^2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        kdotu = (kx * uh[0] + ky * uh[1] + kz * uh[2]) / np.where(kk > 0, kk * kk, 1.0)
+    uh = [uh[0] - kx * kdotu, uh[1] - ky * kdotu, uh[2] - kz * kdotu]
+    u = [np.fft.irfftn(c, s=(n, n, n), axes=(0, 1, 2)) for c in uh]
+    scale = amplitude / max(float(np.abs(c).max()) for c in u)
+    return [c * scale for c in u]
+
+
+def compact_fields(cfg: dict):
+    """(rho, u, v, w, T) over the compact layout's cells (x fastest) for a
+    field-defined initial condition."""
+    import numpy as np
+
+    ic = cfg["ic"]
+    n = cfg["ncell"][0]
+    if ic["type"] != "spectral":
+        raise ValueError("compact_fields: spectral initial conditions only")
+    u, v, w = spectral_velocity(n, ic["kmax"], ic["amplitude"], ic["seed"])
+    rho = np.full(n ** 3, ic["density"])
+    T = np.full(n ** 3, ic["temperature"])
+    return rho, u.ravel(), v.ravel(), w.ravel(), T

@@ -473,6 +473,33 @@ __global__ void k_init_uniform(
     for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * stride + loc] = solid ? 0.0 : val[q];
 }
 
+// initial state from per-cell fields (device copies of the host arrays):
+// f = f_eq(rho, u, R T), g = Grad equilibrium, 0 in solid cells, stored with
+// the level's storage (shifted FP32: value - shift)
+template <class PopT>
+__global__ void k_init_fields(
+    PopT *__restrict__ f, PopT *__restrict__ g, const int32_t *__restrict__ isf,
+    const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
+    const double *__restrict__ fields, double R, double gamma, const double *__restrict__ shift, int64_t cs)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    const bool solid = isf[B.base * 2 + loc] == 0;
+    const double rho = fields[c];
+    const double vel[3] = {fields[total + c], fields[2 * total + c], fields[3 * total + c]};
+    const double T = fields[4 * total + c];
+    const int64_t stride = cs ? cs : B.ncell;
+    for (int q = 0; q < NQ; ++q) {
+        const double fv = solid ? 0.0 : set_equilibrium_value(rho, vel, R * T, q);
+        const double gv = solid ? 0.0 : thermal_g_equilibrium(rho, vel, T, R, gamma, q);
+        f[B.base * NQ + q * stride + loc] = PopT(fv - shift[q]);
+        g[B.base * NQ + q * stride + loc] = PopT(gv - shift[NQ + q]);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1158,19 +1185,44 @@ int mrb_init_fields(
     double adiabatic_exponent, double m_bar)
 {
     if (!L || !rho || !u || !v || !w || !T) return set_error(MRB_E_ARG, "null argument");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
+    if (!(m_bar > 0.0) || !(adiabatic_exponent > 1.0)) return set_error(MRB_E_ARG, "invalid gas parameters");
+    if (L->storage == MRB_STORAGE_F16_SCALED) return set_error(MRB_E_UNSUPPORTED, "fp16 storage initial state");
     // ic::Initializer::initialize (IC.H:460-525) with the per-cell state
-    // given for every fab cell (box order, local index x fastest)
-    const double R = kAirMolarMass / m_bar;
-    return init_from_state(L, [&](int64_t c, int, int, int, double &rho_c, double *vel, double &Tc,
-                                  double &Rr, double &gamma) {
-        rho_c = rho[c];
-        vel[0] = u[c];
-        vel[1] = v[c];
-        vel[2] = w[c];
-        Tc = T[c];
-        Rr = R;
-        gamma = adiabatic_exponent;
-    });
+    // given for every fab cell (box order, local index x fastest); the
+    // equilibria are evaluated on the device with the same expressions
+    const int64_t n = L->total_cells;
+    double *d = nullptr;
+    MRB_HIP(hipMalloc(&d, sizeof(double) * 5 * size_t(n)));
+    const double *src[5] = {rho, u, v, w, T};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 5 && e == hipSuccess; ++k)
+        e = hipMemcpy(d + k * n, src[k], sizeof(double) * size_t(n), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        // d_small[0 .. 2NQ) holds the storage shift (zero for FP64)
+        const double R = kAirMolarMass / m_bar;
+        if (L->storage == MRB_STORAGE_F64)
+            k_init_fields<double><<<nblocks(n), kBlock>>>((double *)L->pop[0][L->cur], (double *)L->pop[1][L->cur],
+                                                          L->d_isf, L->d_boxes, L->d_box_base, L->nbox, n, d, R,
+                                                          adiabatic_exponent, L->d_small, comp_stride(L));
+        else
+            k_init_fields<float><<<nblocks(n), kBlock>>>((float *)L->pop[0][L->cur], (float *)L->pop[1][L->cur],
+                                                         L->d_isf, L->d_boxes, L->d_box_base, L->nbox, n, d, R,
+                                                         adiabatic_exponent, L->d_small, comp_stride(L));
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "mrb_init_fields");
+    L->ghosts_fresh = false;
+    if (L->layout == MRB_LAYOUT_FAB) {
+        int rc = fill_boundary(L, L->pop[0][L->cur], NQ, false, false, nullptr);
+        if (rc) return rc;
+        rc = fill_boundary(L, L->pop[1][L->cur], NQ, false, false, nullptr);
+        if (rc) return rc;
+        MRB_HIP(hipDeviceSynchronize());
+    }
+    return MRB_OK;
 }
 
 // ---- reference-layout stages ---------------------------------------------

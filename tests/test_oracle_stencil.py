@@ -61,3 +61,9 @@ def test_weights(oracle):
     # fourth-order isotropy used by the Grad expansion's second moment
     assert sum(W[q] * ev[q][0] ** 4 for q in range(27)) == 3 * th * th
     assert sum(W[q] * ev[q][0] ** 2 * ev[q][1] ** 2 for q in range(27)) == th * th
+
+
+def test_engine_host_stencil_matches_reference_table():
+    from marbles_amd.stencil import EV
+
+    assert EV.tolist() == [list(c) for c in REF_EVS]

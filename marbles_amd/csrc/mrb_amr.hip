@@ -161,8 +161,12 @@ void amr_release(mrb_level *L)
     if (L->d_coal) (void)hipFree(L->d_coal);
     if (L->d_expl) (void)hipFree(L->d_expl);
     if (L->d_coal_cell) (void)hipFree(L->d_coal_cell);
+    if (L->d_coal_vals) (void)hipFree(L->d_coal_vals);
+    if (L->d_coal_has) (void)hipFree(L->d_coal_has);
     L->d_coal = L->d_expl = nullptr;
     L->d_coal_cell = nullptr;
+    L->d_coal_vals = nullptr;
+    L->d_coal_has = nullptr;
     L->n_coal = L->n_expl = L->n_qring = 0;
     L->amr_crse = nullptr;
 }
@@ -185,6 +189,20 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
     // coalescence: coarsened fine boxes grown by one, box order, the last
     // box wins where rings overlap (average_down_with_ghosts,
     // Utilities.cpp:5-28, ParallelCopy back with periodicity)
+    // A coarse cell under a fine box takes its children from that box's
+    // valid cells: any other box's copies of them (FillBoundary ghosts) are
+    // equal, so the last-box rule only decides the ring cells over
+    // uncovered domain, whose children differ between boxes.
+    std::vector<int32_t> under(cown.size(), -1);
+    for (int b = 0; b < fine->nbox; ++b) {
+        const BoxDev &FB = fine->boxes[b];
+        for (int k = FB.vlo[2] / 2; k <= FB.vhi[2] / 2; ++k)
+            for (int j = FB.vlo[1] / 2; j <= FB.vhi[1] / 2; ++j)
+                for (int i = FB.vlo[0] / 2; i <= FB.vhi[0] / 2; ++i) {
+                    const int s[3] = {i, j, k};
+                    under[size_t(dom_index(crse, s))] = b;
+                }
+    }
     std::vector<CoalesceEntry> coal;
     std::vector<int64_t> slot(cown.size(), -1);
     for (int b = 0; b < fine->nbox; ++b) {
@@ -196,6 +214,11 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
                     int s[3];
                     if (!periodic_image(crse, iv, s)) continue;
                     const int64_t di = dom_index(crse, s);
+                    // under a footprint: only the owner's own valid children
+                    // (not a periodic image, whose children are ghosts)
+                    if (under[size_t(di)] >= 0 &&
+                        (under[size_t(di)] != b || s[0] != i || s[1] != j || s[2] != k))
+                        continue;
                     const int ob = cown[size_t(di)];
                     CoalesceEntry e;
                     e.crse_box = ob;
@@ -303,6 +326,10 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
         MRB_HIP(hipMalloc(&fine->d_expl, sizeof(ExplodeEntry) * expl.size()));
         MRB_HIP(hipMemcpy(fine->d_expl, expl.data(), sizeof(ExplodeEntry) * expl.size(), hipMemcpyHostToDevice));
     }
+    if (!coal.empty()) {
+        MRB_HIP(hipMalloc(&fine->d_coal_vals, sizeof(double) * 2 * NQ * coal.size()));
+        MRB_HIP(hipMalloc(&fine->d_coal_has, sizeof(uint64_t) * coal.size()));
+    }
     MRB_HIP(hipMalloc(&fine->d_coal_cell, sizeof(int32_t) * coal_cell.size()));
     MRB_HIP(hipMemcpy(fine->d_coal_cell, coal_cell.data(), sizeof(int32_t) * coal_cell.size(),
                       hipMemcpyHostToDevice));
@@ -320,7 +347,7 @@ int mrb_coalesce(const mrb_level *fine, mrb_level *crse, void *stream)
     if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
     if (fine->n_coal == 0) return MRB_OK;
     hipStream_t s = as_stream(stream);
-    crse->ghosts_fresh = false;
+    crse->fresh_depth = 0;
     for (int l = 0; l < 2; ++l) {
         k_coalesce<<<blocks_for(fine->n_coal), kAmrBlock, 0, s>>>(
             (const double *)fine->pop[l][fine->cur], (double *)crse->pop[l][crse->cur], fine->d_boxes,
@@ -341,6 +368,7 @@ int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *strea
     // and in-domain ghost cells only)
     int rc = fab_fill_boundary(fine, lattice, s);
     if (rc) return rc;
+    if (lattice == 1) fine->fresh_depth = fine->ng;
     if (fine->n_expl > 0) {
         k_explode<<<blocks_for(fine->n_expl), kAmrBlock, 0, s>>>(
             (const double *)crse->pop[lattice][crse->cur], (double *)fine->pop[lattice][fine->cur],
@@ -360,12 +388,20 @@ static int check_advance(mrb_level *crse, mrb_level *fine)
     return MRB_OK;
 }
 
-// one coarse step: LBM::evolve (LBM.cpp:416-418) and time_step(0) (498-512)
+// one coarse step: LBM::evolve (LBM.cpp:416-418) and time_step(0) (498-512).
+// FillBoundary depths follow the readers: the coarse level's pulls and the
+// explosion read one ghost layer of the coarse level; the fine level's
+// uncovered ghosts are streamed from up to ng layers during the first
+// substep; after the second substep nothing reads fine ghosts covered by
+// fine boxes (the coalescence takes those cells' children from the owning
+// box) before the next explosion refills them.
 static int amr_step(mrb_level *crse, mrb_level *fine, void *stream)
 {
     int rc;
+    hipStream_t s = as_stream(stream);
+    if (crse->fresh_depth < 1 && (rc = fab_fill_boundary_depth(crse, 1, s))) return rc;
     for (int l = 0; l < 2; ++l)
-        if ((rc = mrb_fill_ghosts(crse, l, stream))) return rc;
+        if ((rc = mrb_physbc(crse, l, stream))) return rc;
     for (int l = 0; l < 2; ++l)
         if ((rc = mrb_explode(crse, fine, l, stream))) return rc;
     for (int sub = 0; sub < 2; ++sub) {
@@ -373,11 +409,11 @@ static int amr_step(mrb_level *crse, mrb_level *fine, void *stream)
             if ((rc = mrb_physbc(fine, l, stream))) return rc;
         // time_step(1) -> advance(1) (LBM.cpp:523-544): stream f, g and
         // collide as one batched step
-        if ((rc = fab_batched_step(fine, nullptr, as_stream(stream)))) return rc;
+        if ((rc = fab_batched_step(fine, nullptr, sub == 0 ? fine->ng : 0, s))) return rc;
     }
     // advance(0): stream, average_down_to, collide, with the coalescence
     // fused into the coarse passes
-    return fab_batched_step(crse, fine, as_stream(stream));
+    return fab_batched_step(crse, fine, 1, s);
 }
 
 int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream)
@@ -395,7 +431,7 @@ int mrb_amr_advance_graph(mrb_level *crse, mrb_level *fine, int nsteps, void *st
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     for (int n = 0; n < nsteps; ++n) {
-        mrb_level::AmrGraph &G = fine->amr_graph[crse->cur][crse->ghosts_fresh ? 1 : 0];
+        mrb_level::AmrGraph &G = fine->amr_graph[crse->cur][crse->fresh_depth >= 1 ? 1 : 0];
         if (!G.exec || G.gen_c != crse->gen || G.gen_f != fine->gen || G.fine_cur != fine->cur) {
             if (G.exec) (void)hipGraphExecDestroy(G.exec);
             G = mrb_level::AmrGraph{};
@@ -404,7 +440,7 @@ int mrb_amr_advance_graph(mrb_level *crse, mrb_level *fine, int nsteps, void *st
             // capture as if the step had run; it is put back afterwards and
             // set to the recorded post-step values after each launch
             const int cur_c = crse->cur, cur_f = fine->cur;
-            const bool fresh_c = crse->ghosts_fresh, fresh_f = fine->ghosts_fresh;
+            const int fresh_c = crse->fresh_depth, fresh_f = fine->fresh_depth;
             G.gen_c = crse->gen;
             G.gen_f = fine->gen;
             G.fine_cur = cur_f;
@@ -416,8 +452,10 @@ int mrb_amr_advance_graph(mrb_level *crse, mrb_level *fine, int nsteps, void *st
             G.post_cur_f = fine->cur;
             crse->cur = cur_c;
             fine->cur = cur_f;
-            crse->ghosts_fresh = fresh_c;
-            fine->ghosts_fresh = fresh_f;
+            G.post_fresh_c = crse->fresh_depth;
+            G.post_fresh_f = fine->fresh_depth;
+            crse->fresh_depth = fresh_c;
+            fine->fresh_depth = fresh_f;
             if (rc) {
                 if (graph) (void)hipGraphDestroy(graph);
                 return rc;
@@ -433,8 +471,8 @@ int mrb_amr_advance_graph(mrb_level *crse, mrb_level *fine, int nsteps, void *st
         MRB_HIP(hipGraphLaunch(G.exec, s));
         crse->cur = G.post_cur_c;
         fine->cur = G.post_cur_f;
-        crse->ghosts_fresh = true;
-        fine->ghosts_fresh = true;
+        crse->fresh_depth = G.post_fresh_c;
+        fine->fresh_depth = G.post_fresh_f;
     }
     return MRB_OK;
 }

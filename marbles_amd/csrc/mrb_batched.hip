@@ -45,13 +45,13 @@ struct FabArgs {
     int ntiles;
 };
 
-// coalescence source for the fused coarse passes
+// coalescence source for the fused coarse passes: the masked means of each
+// entry's children, computed once per coarse step by k_coalesce_vals
 struct CoalArgs {
-    const double *fine_f;
-    const double *fine_g;
-    const BoxDev *fine_boxes;
-    const CoalesceEntry *map;
-    const int32_t *cell;  // per coarse fab cell: entry index or -1
+    const double *vals;    // [2q + lattice][entry]
+    const uint64_t *has;   // per entry: bit 2q + lattice = some child not -1
+    const int32_t *cell;   // per coarse fab cell: entry index or -1
+    int64_t n;             // entries
 };
 
 __device__ __forceinline__ bool tile_cell(
@@ -123,11 +123,10 @@ __device__ __forceinline__ void gather_cell(
     if constexpr (COAL) {
         const int32_t ci = C.cell[B.base + loc];
         if (ci >= 0) {
-            const CoalesceEntry e = C.map[ci];
-            const BoxDev &FB = C.fine_boxes[e.fine_box];
-            const double *ff = C.fine_f + FB.base * NQ + e.fine_local;
-            const double *fg = C.fine_g + FB.base * NQ + e.fine_local;
-            const int64_t sy = FB.n[0], sz = int64_t(FB.n[0]) * FB.n[1];
+            // coalesced cell: the children's masked mean, or the pulled
+            // value where all 8 children are -1 (masked_avgdown leaves the
+            // coarse value unchanged)
+            const uint64_t has = C.has[ci];
             const double *fb = f + B.base * NQ;
             const double *gb = g + B.base * NQ;
             const int64_t csy = B.n[0], csz = int64_t(B.n[0]) * B.n[1];
@@ -137,14 +136,36 @@ __device__ __forceinline__ void gather_cell(
                 const int64_t src = loc - (cx + cy * csy + cz * csz);
                 const int64_t pe = ((m >> q) & 1u) ? int64_t(q) * B.ncell + src
                                                    : int64_t(kBounce[q]) * B.ncell + loc;
-                double v;
-                fv[q] = masked_mean(ff + q * FB.ncell, sy, sz, v) ? v : fb[pe];
-                gv[q] = masked_mean(fg + q * FB.ncell, sy, sz, v) ? v : gb[pe];
+                fv[q] = ((has >> (2 * q)) & 1u) ? C.vals[(2 * q) * C.n + ci] : fb[pe];
+                gv[q] = ((has >> (2 * q + 1)) & 1u) ? C.vals[(2 * q + 1) * C.n + ci] : gb[pe];
             });
             return;
         }
     }
     pull_fab(f, g, B, loc, m, fv, gv);
+}
+
+// masked means of every coalescence entry's children (both lattices)
+__global__ void __launch_bounds__(256) k_coalesce_vals(
+    const double *__restrict__ ff, const double *__restrict__ fg, const BoxDev *__restrict__ fboxes,
+    const CoalesceEntry *__restrict__ map, int64_t n, double *__restrict__ vals, uint64_t *__restrict__ has)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const CoalesceEntry e = map[t];
+    const BoxDev &FB = fboxes[e.fine_box];
+    const int64_t sy = FB.n[0], sz = int64_t(FB.n[0]) * FB.n[1];
+    const double *src[2] = {ff + FB.base * NQ + e.fine_local, fg + FB.base * NQ + e.fine_local};
+    uint64_t bits = 0;
+    for (int q = 0; q < NQ; ++q)
+        for (int l = 0; l < 2; ++l) {
+            double v;
+            if (masked_mean(src[l] + q * FB.ncell, sy, sz, v)) {
+                vals[(2 * q + l) * n + t] = v;
+                bits |= uint64_t(1) << (2 * q + l);
+            }
+        }
+    has[t] = bits;
 }
 
 __device__ __forceinline__ int xcd_tile_begin(int ntiles, int &step, int &end)
@@ -367,16 +388,13 @@ const int kNumFabBlockChoices = int(sizeof(kFabBlockChoices) / sizeof(kFabBlockC
 
 int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s);
 
-int fab_batched_step(mrb_level *L, const mrb_level *coal_from, hipStream_t s)
+int fab_batched_step(mrb_level *L, const mrb_level *coal_from, int out_depth, hipStream_t s)
 {
     int rc = fab_batched_passes(L, coal_from, s);
     if (rc) return rc;
     L->cur = 1 - L->cur;
-    rc = fab_fill_boundary(L, 0, s);
-    if (rc) return rc;
-    rc = fab_fill_boundary(L, 1, s);
-    if (rc) return rc;
-    L->ghosts_fresh = true;
+    L->fresh_depth = 0;
+    if (out_depth > 0) return fab_fill_boundary_depth(L, out_depth, s);
     return MRB_OK;
 }
 
@@ -395,11 +413,17 @@ int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s)
     for (int d = 0; d < 3; ++d) A.dom[d] = L->ncell[d];
     CoalArgs C{};
     if (coal_from) {
-        C.fine_f = (const double *)coal_from->pop[0][coal_from->cur];
-        C.fine_g = (const double *)coal_from->pop[1][coal_from->cur];
-        C.fine_boxes = coal_from->d_boxes;
-        C.map = (const CoalesceEntry *)coal_from->d_coal;
+        C.vals = coal_from->d_coal_vals;
+        C.has = coal_from->d_coal_has;
         C.cell = coal_from->d_coal_cell;
+        C.n = coal_from->n_coal;
+        if (C.n > 0) {
+            k_coalesce_vals<<<unsigned((C.n + 255) / 256), 256, 0, s>>>(
+                (const double *)coal_from->pop[0][coal_from->cur], (const double *)coal_from->pop[1][coal_from->cur],
+                coal_from->d_boxes, (const CoalesceEntry *)coal_from->d_coal, C.n, coal_from->d_coal_vals,
+                coal_from->d_coal_has);
+            MRB_HIP(hipGetLastError());
+        }
     }
     A.grow = 0;
     A.ntiles = L->n_tiles_c;

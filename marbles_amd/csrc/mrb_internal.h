@@ -140,6 +140,8 @@ struct mrb_level {
     void *d_expl = nullptr;  // mrb::ExplodeEntry[n_expl]
     int64_t n_expl = 0;
     int32_t *d_coal_cell = nullptr;  // per coarse fab cell: coalescence entry or -1
+    double *d_coal_vals = nullptr;   // [2q + lattice][entry]: masked means of the children
+    uint64_t *d_coal_has = nullptr;  // per entry: bit 2q + lattice set if some child is not -1
     // the first n_qring entries of the explosion map are ring-1 ghost cells
     int64_t n_qring = 0;
     // batched fab-layout step: tile list of the valid cells and the number
@@ -147,10 +149,10 @@ struct mrb_level {
     mrb::FabTile *d_tiles_c = nullptr;
     int n_tiles_c = 0;
     int fab_blocks = 0;
-    // fab layout: every ghost cell FillBoundary writes holds its owner's
-    // current value (set after the post-collide FillBoundary, cleared by
-    // anything that changes valid data otherwise)
-    bool ghosts_fresh = false;
+    // fab layout: ghost layers 1..fresh_depth that FillBoundary writes hold
+    // their owners' current values (0: stale; set by FillBoundary, cleared
+    // by anything that changes valid data otherwise)
+    int fresh_depth = 0;
     // bumped whenever captured kernel arguments would go stale (physics,
     // boundary values, buffer pointers, launch shapes, links)
     uint64_t gen = 1;
@@ -161,6 +163,7 @@ struct mrb_level {
         uint64_t gen_c = 0, gen_f = 0;
         int fine_cur = 0;
         int post_cur_c = 0, post_cur_f = 0;
+        int post_fresh_c = 0, post_fresh_f = 0;
     } amr_graph[2][2];
     hipStream_t cap_stream = nullptr;
 };
@@ -194,14 +197,17 @@ int fused_step(mrb_level *L, int variant, hipStream_t s);
 int compact_two_pass_launch(mrb_level *L, hipStream_t s);
 // FillBoundary of one lattice's current fab buffer (mrb_level.hip)
 int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s);
+// FillBoundary of both lattices, `depth` ghost layers (1 or ng); sets fresh_depth
+int fab_fill_boundary_depth(mrb_level *L, int depth, hipStream_t s);
 // FillBoundary of the batched step's Q array (3 comps in d_dq), one layer
 int fab_fill_boundary_q(mrb_level *L, hipStream_t s);
 // frees the two-level maps held by a fine level (mrb_amr.hip)
 void amr_release(mrb_level *L);
 // batched fab-layout step (mrb_batched.hip): pass 1 + pass 2 into the spare
 // buffer (coalescing from `coal_from` when given), ghost streaming over the
-// explosion map of a fine level, FillBoundary, flip
-int fab_batched_step(mrb_level *L, const mrb_level *coal_from, hipStream_t s);
+// explosion map of a fine level, flip, FillBoundary of out_depth layers
+// (0: none; 1: what the next step's pulls read; ng: all)
+int fab_batched_step(mrb_level *L, const mrb_level *coal_from, int out_depth, hipStream_t s);
 int fab_build_mask(mrb_level *L, hipStream_t s);
 int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s);
 // persistent-block counts the fab-layout autotuner tries (0 = one block per tile)

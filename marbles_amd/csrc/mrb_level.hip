@@ -590,6 +590,17 @@ int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s)
 
 int fab_fill_boundary_q(mrb_level *L, hipStream_t s) { return fill_boundary(L, L->d_dq, 3, false, true, s); }
 
+int fab_fill_boundary_depth(mrb_level *L, int depth, hipStream_t s)
+{
+    const bool one = depth <= 1;
+    for (int l = 0; l < 2; ++l) {
+        const int rc = fill_boundary(L, L->pop[l][L->cur], NQ, false, one, s);
+        if (rc) return rc;
+    }
+    L->fresh_depth = one ? 1 : L->ng;
+    return MRB_OK;
+}
+
 static void compute_bc_values(mrb_level *L)
 {
     BCDev &d = L->bcdev;
@@ -931,7 +942,7 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
         return MRB_OK;
     }
     if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
-    L->ghosts_fresh = false;
+    L->fresh_depth = 0;
     const int64_t n = NQ * L->total_cells;
     const double *h = static_cast<const double *>(host);
     if (L->storage == MRB_STORAGE_F64) return copy_lattice(L, arr, const_cast<void *>(host), true);
@@ -1214,11 +1225,9 @@ int mrb_init_fields(
     }
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "mrb_init_fields");
-    L->ghosts_fresh = false;
+    L->fresh_depth = 0;
     if (L->layout == MRB_LAYOUT_FAB) {
-        int rc = fill_boundary(L, L->pop[0][L->cur], NQ, false, false, nullptr);
-        if (rc) return rc;
-        rc = fill_boundary(L, L->pop[1][L->cur], NQ, false, false, nullptr);
+        const int rc = fab_fill_boundary_depth(L, L->ng, nullptr);
         if (rc) return rc;
         MRB_HIP(hipDeviceSynchronize());
     }
@@ -1248,11 +1257,13 @@ int mrb_fill_ghosts(mrb_level *L, int lattice, void *stream)
     if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
     // The bounce prefill of FillPatchOps.H:92-110 is overwritten in full by
     // FillBoundary and the boundary functor, so it is not executed.
-    // FillBoundary is skipped when the ghosts already hold the owners'
-    // current values (after a collide); the boundary functor always runs.
-    if (!L->ghosts_fresh) {
+    // FillBoundary is skipped when all ghost layers already hold the
+    // owners' current values (after a collide); the boundary functor
+    // always runs.
+    if (L->fresh_depth < L->ng) {
         const int rc = fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, as_stream(stream));
         if (rc) return rc;
+        if (lattice == 1) L->fresh_depth = L->ng;
     }
     return mrb_physbc(L, lattice, stream);
 }
@@ -1272,7 +1283,7 @@ int mrb_stream(mrb_level *L, int lattice, void *stream)
                                                             L->nbox, L->total_cells);
     MRB_HIP(hipGetLastError());
     std::swap(L->pop[lattice][0], L->pop[lattice][1]);
-    L->ghosts_fresh = false;
+    L->fresh_depth = 0;
     ++L->gen;
     return fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, s);
 }
@@ -1301,7 +1312,7 @@ int mrb_collide(mrb_level *L, void *stream)
     if (rc) return rc;
     rc = fill_boundary(L, g, NQ, false, false, s);
     if (rc) return rc;
-    L->ghosts_fresh = true;
+    L->fresh_depth = L->ng;
     return MRB_OK;
 }
 
@@ -1315,16 +1326,20 @@ int mrb_stream_collide(mrb_level *L, void *stream)
     // reference layout: ghost fill (FillPatchOps::fillpatch), then stream
     // and collide as one batched step over every box
     int rc;
-    for (int l = 0; l < 2; ++l)
-        if ((rc = mrb_fill_ghosts(L, l, stream))) return rc;
-    if (L->ng < 2) {
-        // the batched pass 1 pulls ring-1 ghosts from ring 2: with a single
-        // ghost layer run the stages one by one
+    if (L->ng < 2 || L->amr_crse) {
+        // one ghost layer, or a linked fine level (its uncovered ghosts
+        // belong to the two-level schedule): the stages one by one
+        for (int l = 0; l < 2; ++l)
+            if ((rc = mrb_fill_ghosts(L, l, stream))) return rc;
         for (int l = 0; l < 2; ++l)
             if ((rc = mrb_stream(L, l, stream))) return rc;
         return mrb_collide(L, stream);
     }
-    return fab_batched_step(L, nullptr, as_stream(stream));
+    // the batched step pulls ring 1 only: FillBoundary one layer deep
+    if (L->fresh_depth < 1 && (rc = fab_fill_boundary_depth(L, 1, as_stream(stream)))) return rc;
+    for (int l = 0; l < 2; ++l)
+        if ((rc = mrb_physbc(L, l, stream))) return rc;
+    return fab_batched_step(L, nullptr, 1, as_stream(stream));
 }
 
 int mrb_advance(mrb_level *L, int nsteps, void *stream)

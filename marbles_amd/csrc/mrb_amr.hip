@@ -162,6 +162,9 @@ void amr_release(mrb_level *L)
     if (L->d_expl) (void)hipFree(L->d_expl);
     if (L->d_coal_cell) (void)hipFree(L->d_coal_cell);
     if (L->d_coal_vals) (void)hipFree(L->d_coal_vals);
+    if (L->d_qring) (void)hipFree(L->d_qring);
+    L->d_qring = nullptr;
+    L->n_expl_le2 = 0;
     if (L->d_coal_has) (void)hipFree(L->d_coal_has);
     L->d_coal = L->d_expl = nullptr;
     L->d_coal_cell = nullptr;
@@ -264,29 +267,40 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
                     expl.push_back(e);
                 }
     }
-    // ring-1 ghost cells first: the batched step computes their Q from
-    // their own pulls (they have no owning box to copy it from)
-    std::stable_partition(expl.begin(), expl.end(), [&](const ExplodeEntry &e) {
+    // Order the uncovered ghost cells as (ring <= 2 first, then ring 3),
+    // each group by (box, fab index): consecutive threads of the listed
+    // kernels then touch consecutive cells of a row.  The second substep
+    // streams rings 1-2 only (ring-3 values streamed then are overwritten
+    // by the next explosion before anything reads them).  Ring-1 cells are
+    // also listed apart: the batched step computes their Q from their own
+    // pulls (they have no owning box to copy it from).
+    auto ring_of = [&](const ExplodeEntry &e) {
         const BoxDev &FB = fine->boxes[e.fine_box];
         const int64_t nxy = int64_t(FB.n[0]) * FB.n[1];
-        const int k = int(e.fine_local / nxy) + FB.lo[2];
-        const int j = int((e.fine_local % nxy) / FB.n[0]) + FB.lo[1];
-        const int i = int(e.fine_local % FB.n[0]) + FB.lo[0];
-        return i >= FB.vlo[0] - 1 && i <= FB.vhi[0] + 1 && j >= FB.vlo[1] - 1 && j <= FB.vhi[1] + 1 &&
-               k >= FB.vlo[2] - 1 && k <= FB.vhi[2] + 1;
-    });
-    int64_t n_qring = 0;
-    for (const ExplodeEntry &e : expl) {
-        const BoxDev &FB = fine->boxes[e.fine_box];
-        const int64_t nxy = int64_t(FB.n[0]) * FB.n[1];
-        const int k = int(e.fine_local / nxy) + FB.lo[2];
-        const int j = int((e.fine_local % nxy) / FB.n[0]) + FB.lo[1];
-        const int i = int(e.fine_local % FB.n[0]) + FB.lo[0];
-        if (!(i >= FB.vlo[0] - 1 && i <= FB.vhi[0] + 1 && j >= FB.vlo[1] - 1 && j <= FB.vhi[1] + 1 &&
-              k >= FB.vlo[2] - 1 && k <= FB.vhi[2] + 1))
-            break;
-        ++n_qring;
+        const int c[3] = {int(e.fine_local % FB.n[0]) + FB.lo[0], int((e.fine_local % nxy) / FB.n[0]) + FB.lo[1],
+                          int(e.fine_local / nxy) + FB.lo[2]};
+        int r = 0;
+        for (int d = 0; d < 3; ++d) r = std::max(r, std::max(FB.vlo[d] - c[d], c[d] - FB.vhi[d]));
+        return r;
+    };
+    std::vector<uint8_t> ring(expl.size());
+    std::vector<std::pair<uint64_t, uint32_t>> key(expl.size());
+    for (size_t n = 0; n < expl.size(); ++n) {
+        ring[n] = uint8_t(ring_of(expl[n]));
+        key[n] = {(uint64_t(ring[n] >= 3) << 63) | (uint64_t(expl[n].fine_box) << 40) | uint64_t(expl[n].fine_local),
+                  uint32_t(n)};
     }
+    std::sort(key.begin(), key.end());
+    std::vector<ExplodeEntry> sorted(expl.size());
+    int64_t n_le2 = 0;
+    std::vector<ExplodeEntry> qring;
+    for (size_t n = 0; n < key.size(); ++n) {
+        const uint32_t o = key[n].second;
+        sorted[n] = expl[o];
+        if (ring[o] <= 2) ++n_le2;
+        if (ring[o] == 1) qring.push_back(expl[o]);
+    }
+    expl.swap(sorted);
     // per coarse fab cell: the coalescence entry of valid cells under the
     // fine footprint or its ring
     std::vector<int32_t> coal_cell(size_t(crse->total_cells), -1);
@@ -326,6 +340,10 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
         MRB_HIP(hipMalloc(&fine->d_expl, sizeof(ExplodeEntry) * expl.size()));
         MRB_HIP(hipMemcpy(fine->d_expl, expl.data(), sizeof(ExplodeEntry) * expl.size(), hipMemcpyHostToDevice));
     }
+    if (!qring.empty()) {
+        MRB_HIP(hipMalloc(&fine->d_qring, sizeof(ExplodeEntry) * qring.size()));
+        MRB_HIP(hipMemcpy(fine->d_qring, qring.data(), sizeof(ExplodeEntry) * qring.size(), hipMemcpyHostToDevice));
+    }
     if (!coal.empty()) {
         MRB_HIP(hipMalloc(&fine->d_coal_vals, sizeof(double) * 2 * NQ * coal.size()));
         MRB_HIP(hipMalloc(&fine->d_coal_has, sizeof(uint64_t) * coal.size()));
@@ -335,7 +353,8 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
                       hipMemcpyHostToDevice));
     fine->n_coal = int64_t(coal.size());
     fine->n_expl = int64_t(expl.size());
-    fine->n_qring = n_qring;
+    fine->n_qring = int64_t(qring.size());
+    fine->n_expl_le2 = n_le2;
     fine->amr_crse = crse;
     ++fine->gen;
     return MRB_OK;
@@ -409,7 +428,10 @@ static int amr_step(mrb_level *crse, mrb_level *fine, void *stream)
             if ((rc = mrb_physbc(fine, l, stream))) return rc;
         // time_step(1) -> advance(1) (LBM.cpp:523-544): stream f, g and
         // collide as one batched step
-        if ((rc = fab_batched_step(fine, nullptr, sub == 0 ? fine->ng : 0, s))) return rc;
+        fine->stream_rings_le2 = sub == 1;
+        rc = fab_batched_step(fine, nullptr, sub == 0 ? fine->ng : 0, s);
+        fine->stream_rings_le2 = false;
+        if (rc) return rc;
     }
     // advance(0): stream, average_down_to, collide, with the coalescence
     // fused into the coarse passes

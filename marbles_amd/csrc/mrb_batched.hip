@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(kFabTile) k_fab_collide(
 // uncovered domain, from the explosion map): pull with the in-fab test
 __global__ void __launch_bounds__(256) k_stream_listed(
     const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo,
-    double *__restrict__ go, const int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes,
+    double *__restrict__ go, const uint32_t *__restrict__ mask, const BoxDev *__restrict__ boxes,
     const ExplodeEntry *__restrict__ list, int64_t n)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -297,20 +297,23 @@ __global__ void __launch_bounds__(256) k_stream_listed(
     const int kk = int(loc / nxy);
     const int jj = int((loc - kk * nxy) / B.n[0]);
     const int ii = int(loc - kk * nxy - int64_t(jj) * B.n[0]);
-    const int32_t *fl = isf + B.base * 2;
-    const bool fluid = fl[loc] == 1;
+    // bit q of the mask: the source x - c_q is inside the fab and fluid
+    const uint32_t m = mask[B.base + loc];
+    const bool fluid = m & 1u;
     const double *fb = f + B.base * NQ;
     const double *gb = g + B.base * NQ;
     double *fob = fo + B.base * NQ;
     double *gob = go + B.base * NQ;
+    const int64_t sy = B.n[0], sz = nxy;
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
         const int a = ii - cx, b = jj - cy, c = kk - cz;
+        const bool in_fab = a >= 0 && a < B.n[0] && b >= 0 && b < B.n[1] && c >= 0 && c < B.n[2];
         double vf = -1.0, vg = -1.0;
-        if (fluid && a >= 0 && a < B.n[0] && b >= 0 && b < B.n[1] && c >= 0 && c < B.n[2]) {
-            const int64_t sl = (int64_t(c) * B.n[1] + b) * B.n[0] + a;
-            const int64_t idx = fl[sl] != 0 ? int64_t(q) * B.ncell + sl : int64_t(kBounce[q]) * B.ncell + loc;
+        if (fluid && in_fab) {
+            const int64_t idx = ((m >> q) & 1u) ? int64_t(q) * B.ncell + loc - (cx + cy * sy + cz * sz)
+                                                : int64_t(kBounce[q]) * B.ncell + loc;
             vf = fb[idx];
             vg = gb[idx];
         }
@@ -435,7 +438,7 @@ int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s)
     MRB_HIP(hipGetLastError());
     if (L->n_qring > 0) {
         k_fab_qpass_listed<<<unsigned((L->n_qring + 255) / 256), 256, 0, s>>>(
-            f, g, L->d_dq, L->d_boxes, L->d_mask, (const ExplodeEntry *)L->d_expl, L->n_qring, L->cp);
+            f, g, L->d_dq, L->d_boxes, L->d_mask, (const ExplodeEntry *)L->d_qring, L->n_qring, L->cp);
         MRB_HIP(hipGetLastError());
     }
     int rc = fab_fill_boundary_q(L, s);
@@ -445,9 +448,10 @@ int fab_batched_passes(mrb_level *L, const mrb_level *coal_from, hipStream_t s)
     else
         k_fab_collide<false><<<gc, kFabTile, 0, s>>>(f, g, fo, go, L->d_dq, L->d_tiles_c, A, C, L->cp);
     MRB_HIP(hipGetLastError());
-    if (L->n_expl > 0) {
-        k_stream_listed<<<unsigned((L->n_expl + 255) / 256), 256, 0, s>>>(
-            f, g, fo, go, L->d_isf, L->d_boxes, (const ExplodeEntry *)L->d_expl, L->n_expl);
+    const int64_t nst = L->stream_rings_le2 ? L->n_expl_le2 : L->n_expl;
+    if (nst > 0) {
+        k_stream_listed<<<unsigned((nst + 255) / 256), 256, 0, s>>>(
+            f, g, fo, go, L->d_mask, L->d_boxes, (const ExplodeEntry *)L->d_expl, nst);
         MRB_HIP(hipGetLastError());
     }
     return MRB_OK;

@@ -142,8 +142,12 @@ struct mrb_level {
     int32_t *d_coal_cell = nullptr;  // per coarse fab cell: coalescence entry or -1
     double *d_coal_vals = nullptr;   // [2q + lattice][entry]: masked means of the children
     uint64_t *d_coal_has = nullptr;  // per entry: bit 2q + lattice set if some child is not -1
-    // the first n_qring entries of the explosion map are ring-1 ghost cells
+    // explosion entries are ordered ring <= 2 first (n_expl_le2 of them);
+    // d_qring lists the ring-1 entries again
+    int64_t n_expl_le2 = 0;
+    void *d_qring = nullptr;  // mrb::ExplodeEntry[n_qring]
     int64_t n_qring = 0;
+    bool stream_rings_le2 = false;  // the listed ghost stream covers rings 1-2 only
     // batched fab-layout step: tile list of the valid cells and the number
     // of persistent blocks (0 = one block per tile)
     mrb::FabTile *d_tiles_c = nullptr;

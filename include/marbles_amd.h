@@ -172,9 +172,11 @@ int mrb_advance(mrb_level *lvl, int nsteps, void *stream);
  * (slots are named by mrb_hot_kernel_name, separated by '+'), *nslots the
  * number of slots (at most 4).  Synchronous.  Compact layout only. */
 int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_ms, int *nslots);
-/* Kernel variants of the compact-layout step: index 0 is the two-pass step
- * (Q-correction pass + collide pass), indices 1.. the single-pass z-sweep
- * with different tile shapes.  All variants produce identical results. */
+/* Kernel variants of the compact-layout step: indices 0 and 1 are the
+ * two-pass step (Q-correction pass + collide pass) with the compiler's
+ * register budget and with a 3-waves-per-SIMD budget; the following ones the
+ * single-pass z-sweep with different tile shapes.  All variants produce
+ * identical results. */
 int mrb_hot_kernel_count(void);
 const char *mrb_hot_kernel_variant_name(int index);
 int mrb_set_hot_kernel(mrb_level *lvl, int index);

@@ -99,10 +99,13 @@ __device__ __forceinline__ int64_t pull_index(
 }
 
 // Rest-state populations subtracted by the shifted storages (stored value =
-// population - shift); all zero for FP64 storage.
+// population - shift), read through a pointer to [f shifts | g shifts] so
+// that they are scalar loads at their use (54 doubles of kernel arguments
+// would stay live in vector registers); unused for FP64 storage.
 struct ShiftTab {
-    double f[NQ];
-    double g[NQ];
+    const double *__restrict__ v;
+    __device__ __forceinline__ double f(int q) const { return v[q]; }
+    __device__ __forceinline__ double g(int q) const { return v[NQ + q]; }
 };
 
 // storage <-> FP64 working value
@@ -141,8 +144,8 @@ __device__ __forceinline__ void pull_cell(
         constexpr int q = decltype(qc)::value;
         bool use_const;
         const int64_t idx = pull_index<q>(A, cell, mask, sx, sy, sz, gx, gy, gz, use_const);
-        const double fq = widen(f[idx], sh.f[q]);
-        const double gq = widen(g[idx], sh.g[q]);
+        const double fq = widen(f[idx], sh.f(q));
+        const double gq = widen(g[idx], sh.g(q));
         fv[q] = use_const ? bc.fval[q] : fq;
         gv[q] = use_const ? bc.gval[q] : gq;
     });
@@ -181,10 +184,10 @@ __global__ void __launch_bounds__(256) k_qpass(
 
 // pass 2: stream + collide, writes the B buffers
 template <class T>
-__global__ void __launch_bounds__(256) k_collide(
+__device__ __forceinline__ void collide_body(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
-    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
-    CollideParams P)
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
+    const ShiftTab &sh, const CollideParams &P)
 {
     int x, y, z;
     int64_t cell;
@@ -194,8 +197,8 @@ __global__ void __launch_bounds__(256) k_collide(
         // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            fo[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.f[q]);
-            go[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.g[q]);
+            fo[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.f(q));
+            go[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.g(q));
         });
         return;
     }
@@ -217,9 +220,29 @@ __global__ void __launch_bounds__(256) k_collide(
                                           zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
     collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.cs + cell] = narrow<T>(fnew, sh.f[q]);
-        go[int64_t(q) * A.cs + cell] = narrow<T>(gnew, sh.g[q]);
+        fo[int64_t(q) * A.cs + cell] = narrow<T>(fnew, sh.f(q));
+        go[int64_t(q) * A.cs + cell] = narrow<T>(gnew, sh.g(q));
     });
+}
+
+// register budget as the compiler chooses (2 waves per SIMD)
+template <class T>
+__global__ void __launch_bounds__(256) k_collide(
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    CollideParams P)
+{
+    collide_body(f, g, fo, go, mask, Q, A, bc, sh, P);
+}
+
+// at most 168 registers (3 waves per SIMD), some values spilled to scratch
+template <class T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_collide_w3(
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    CollideParams P)
+{
+    collide_body(f, g, fo, go, mask, Q, A, bc, sh, P);
 }
 
 // stream only (parity of the pull with the reference push)
@@ -240,8 +263,8 @@ __global__ void __launch_bounds__(256) k_stream_only(
     }
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.cs + cell] = narrow<T>(fv[q], sh.f[q]);
-        go[int64_t(q) * A.cs + cell] = narrow<T>(gv[q], sh.g[q]);
+        fo[int64_t(q) * A.cs + cell] = narrow<T>(fv[q], sh.f(q));
+        go[int64_t(q) * A.cs + cell] = narrow<T>(gv[q], sh.g(q));
     });
 }
 
@@ -300,17 +323,14 @@ int compact_build_mask(mrb_level *L, hipStream_t s)
 // flip.  ev (if not null) receives events before, between and after.
 static ShiftTab shift_tab(const mrb_level *L)
 {
+    // d_small[0 .. 2NQ): the storage shifts (zero for FP64 storage)
     ShiftTab t{};
-    if (L->storage != MRB_STORAGE_F64)
-        for (int q = 0; q < NQ; ++q) {
-            t.f[q] = L->fshift[q];
-            t.g[q] = L->gshift[q];
-        }
+    t.v = L->d_small;
     return t;
 }
 
 template <class T>
-static int two_pass_t(mrb_level *L, hipStream_t s, hipEvent_t *ev)
+static int two_pass_t(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
     const CompactArgs A = make_args(L);
     const ShiftTab sh = shift_tab(L);
@@ -322,30 +342,48 @@ static int two_pass_t(mrb_level *L, hipStream_t s, hipEvent_t *ev)
     k_qpass<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[1], s));
-    k_collide<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, L->cp);
+    if (variant == 1)
+        k_collide_w3<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh,
+                                                              L->cp);
+    else
+        k_collide<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[2], s));
     return MRB_OK;
 }
 
-static int two_pass(mrb_level *L, hipStream_t s, hipEvent_t *ev)
+static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
-    if (L->storage == MRB_STORAGE_F64) return two_pass_t<double>(L, s, ev);
-    if (L->storage == MRB_STORAGE_F32_SHIFTED) return two_pass_t<float>(L, s, ev);
+    if (L->storage == MRB_STORAGE_F64) return two_pass_t<double>(L, variant, s, ev);
+    if (L->storage == MRB_STORAGE_F32_SHIFTED) return two_pass_t<float>(L, variant, s, ev);
     return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
 }
 
-int compact_two_pass_launch(mrb_level *L, hipStream_t s) { return two_pass(L, s, nullptr); }
+const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_qpass+k_collide_w3"};
+
+int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
+
+int compact_variant_launch(mrb_level *L, int index, hipStream_t s)
+{
+    return index < kNumTwoPassVariants ? compact_two_pass_launch(L, index, s)
+                                       : fused_launch(L, index - kNumTwoPassVariants, s);
+}
+
+const char *compact_variant_name(int index)
+{
+    if (index < 0 || index >= kNumTwoPassVariants + kNumFusedVariants) return "";
+    return index < kNumTwoPassVariants ? kTwoPassNames[index] : kFusedVariants[index - kNumTwoPassVariants].name;
+}
 
 static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
 {
     int rc;
-    if (L->hot == 0) {
-        rc = two_pass(L, s, events ? L->ev : nullptr);
+    if (L->hot < kNumTwoPassVariants) {
+        rc = two_pass(L, L->hot, s, events ? L->ev : nullptr);
         nslots = 2;
     } else {
         if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
-        rc = fused_launch(L, L->hot - 1, s);
+        rc = fused_launch(L, L->hot - kNumTwoPassVariants, s);
         if (events && !rc) MRB_HIP(hipEventRecord(L->ev[1], s));
         nslots = 1;
     }
@@ -387,7 +425,7 @@ int compact_stream_only(mrb_level *L, hipStream_t s)
 
 const char *compact_kernel_name(const mrb_level *L)
 {
-    return L->hot == 0 ? "k_qpass+k_collide" : kFusedVariants[L->hot - 1].name;
+    return compact_variant_name(L->hot);
 }
 
 }  // namespace mrb

@@ -131,7 +131,7 @@ struct mrb_level {
     double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
     // timing events (created with the level)
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    // hot-path kernel of the compact layout: 0 = two-pass, 1 + i = fused variant i
+    // hot-path kernel variant of the compact layout (compact_variant_name)
     int hot = 0;
     // two-level link, held by the fine level (mrb_amr_link)
     mrb_level *amr_crse = nullptr;
@@ -197,8 +197,14 @@ const char *compact_kernel_name(const mrb_level *L);
 // writes the spare one; step also flips the buffers
 int fused_launch(mrb_level *L, int variant, hipStream_t s);
 int fused_step(mrb_level *L, int variant, hipStream_t s);
-// two-pass step without flipping (mrb_compact.hip)
-int compact_two_pass_launch(mrb_level *L, hipStream_t s);
+// compact-layout step variants (mrb_compact.hip): index < kNumTwoPassVariants
+// is a two-pass step (register budgets differ), the rest the fused
+// variants; launch reads the current buffers, writes the spare ones, does
+// not flip
+constexpr int kNumTwoPassVariants = 2;
+int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s);
+int compact_variant_launch(mrb_level *L, int index, hipStream_t s);
+const char *compact_variant_name(int index);
 // FillBoundary of one lattice's current fab buffer (mrb_level.hip)
 int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s);
 // FillBoundary of both lattices, `depth` ghost layers (1 or ng); sets fresh_depth

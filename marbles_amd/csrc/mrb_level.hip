@@ -1373,19 +1373,14 @@ int mrb_advance_timed(mrb_level *L, int nsteps, void *stream, double *kernel_ms,
     return MRB_OK;
 }
 
-int mrb_hot_kernel_count(void) { return 1 + kNumFusedVariants; }
+int mrb_hot_kernel_count(void) { return kNumTwoPassVariants + kNumFusedVariants; }
 
-const char *mrb_hot_kernel_variant_name(int index)
-{
-    if (index == 0) return "k_qpass+k_collide";
-    if (index < 0 || index > kNumFusedVariants) return "";
-    return kFusedVariants[index - 1].name;
-}
+const char *mrb_hot_kernel_variant_name(int index) { return compact_variant_name(index); }
 
 int mrb_set_hot_kernel(mrb_level *L, int index)
 {
     if (!L) return set_error(MRB_E_ARG, "null argument");
-    if (index < 0 || index > kNumFusedVariants) return set_error(MRB_E_ARG, "no such kernel variant");
+    if (index < 0 || index >= mrb_hot_kernel_count()) return set_error(MRB_E_ARG, "no such kernel variant");
     if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
     L->hot = index;
     return MRB_OK;
@@ -1451,7 +1446,7 @@ int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best
     int arg = 0;
     double best_ms = 1e300;
     for (int v = 0; v < n; ++v) {
-        auto launch = [&]() { return v == 0 ? compact_two_pass_launch(L, s) : fused_launch(L, v - 1, s); };
+        auto launch = [&]() { return compact_variant_launch(L, v, s); };
         int rc = launch();  // warm-up
         if (rc == MRB_E_UNSUPPORTED) {
             // variant not available for this level (storage, boundary types)

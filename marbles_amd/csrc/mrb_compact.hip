@@ -25,6 +25,7 @@ struct CompactArgs {
     int64_t N;   // cells
     int64_t cs;  // component stride of the population arrays (2N)
     int bc[6];
+    uint32_t bcpack;  // bc[face] in bits 4*face .. 4*face+3
 };
 
 // Source coordinates of the pull for one direction component c in
@@ -49,53 +50,6 @@ __device__ __forceinline__ void wrap3(int x, int n, int periodic, int s[3], bool
         else
             ghost[2] = true;
     }
-}
-
-// Index (into pop) and constant-value flag of the pulled population q.
-// For a physical-boundary ghost the reference's BCFill value is resolved:
-// the face applied is the last non-periodic face in BCFill's loop order
-// (BC.H:365-372); NOSLIP and SLIP read the cell's own pre-stream
-// populations, OUTFLOW the boundary cell next to the ghost, VELOCITY a
-// constant.
-template <int q>
-__device__ __forceinline__ int64_t pull_index(
-    const CompactArgs &A, int64_t cell, uint32_t mask, const int sx[3], const int sy[3],
-    const int sz[3], const bool gx[3], const bool gy[3], const bool gz[3], bool &use_const)
-{
-    constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-    constexpr int qb = kBounce[q];
-    const bool ghost = gx[cx + 1] || gy[cy + 1] || gz[cz + 1];
-    use_const = false;
-    if (!ghost) {
-        const int64_t src = (int64_t(sz[cz + 1]) * A.ny + sy[cy + 1]) * A.nx + sx[cx + 1];
-        return ((mask >> q) & 1u) ? int64_t(q) * A.cs + src : int64_t(qb) * A.cs + cell;
-    }
-    int face;
-    if (gz[cz + 1])
-        face = cz > 0 ? 2 : 5;
-    else if (gy[cy + 1])
-        face = cy > 0 ? 1 : 4;
-    else
-        face = cx > 0 ? 0 : 3;
-    const int t = A.bc[face];
-    if (t == MRB_BC_VELOCITY) {
-        use_const = true;
-        return cell;
-    }
-    if (t == MRB_BC_OUTFLOW) {
-        // data(s + n e_d, q): the boundary cell next to the ghost; other
-        // coordinates of the ghost are wrapped (periodic) or, at a corner of
-        // two non-periodic faces, clamped to the domain.
-        int tx = sx[cx + 1], ty = sy[cy + 1], tz = sz[cz + 1];
-        if (gx[cx + 1]) tx = cx > 0 ? 0 : A.nx - 1;
-        if (gy[cy + 1]) ty = cy > 0 ? 0 : A.ny - 1;
-        if (gz[cz + 1]) tz = cz > 0 ? 0 : A.nz - 1;
-        return int64_t(q) * A.cs + (int64_t(tz) * A.ny + ty) * A.nx + tx;
-    }
-    if (t == MRB_BC_SLIPX) return int64_t(reflect(q, 0)) * A.cs + cell;
-    if (t == MRB_BC_SLIPY) return int64_t(reflect(q, 1)) * A.cs + cell;
-    if (t == MRB_BC_SLIPZ) return int64_t(reflect(q, 2)) * A.cs + cell;
-    return int64_t(qb) * A.cs + cell;  // NOSLIP
 }
 
 // Rest-state populations subtracted by the shifted storages (stored value =
@@ -130,24 +84,69 @@ __device__ __forceinline__ T narrow(double v, double shift)
     }
 }
 
+// Per dimension, for the three lattice-velocity components c = -1, 0, +1
+// (index c + 1): t = the coordinate the pull reads, x - c wrapped on a
+// periodic axis and, beyond a non-periodic face, the boundary cell next to
+// the ghost (the cell an outflow ghost copies); g = the source is a ghost.
+__device__ __forceinline__ void targets3(int x, int n, int periodic, int t[3], bool g[3])
+{
+    const int p = x + 1, m = x - 1;
+    const bool hi = p >= n, lo = m < 0;
+    g[0] = hi && !periodic;
+    g[1] = false;
+    g[2] = lo && !periodic;
+    t[0] = hi ? (periodic ? p - n : n - 1) : p;
+    t[1] = x;
+    t[2] = lo ? (periodic ? m + n : 0) : m;
+}
+
+// Pull of one cell, without branches so that all 54 loads are in flight
+// together.  For a pull source beyond a physical face the reference's
+// BCFill value is resolved: the face applied is the last non-periodic face
+// in BCFill's loop order (BC.H:365-372, z over y over x); NOSLIP and SLIP
+// read the cell's own pre-stream populations, OUTFLOW the boundary cell
+// next to the ghost (at a corner of two non-periodic faces, clamped in
+// both), VELOCITY a constant.
 template <class T>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
     const ShiftTab &sh, int x, int y, int z, int64_t cell, uint32_t mask, double fv[NQ], double gv[NQ])
 {
-    int sx[3], sy[3], sz[3];
+    int tx[3], ty[3], tz[3];
     bool gx[3], gy[3], gz[3];
-    wrap3(x, A.nx, A.per[0], sx, gx);
-    wrap3(y, A.ny, A.per[1], sy, gy);
-    wrap3(z, A.nz, A.per[2], sz, gz);
+    targets3(x, A.nx, A.per[0], tx, gx);
+    targets3(y, A.ny, A.per[1], ty, gy);
+    targets3(z, A.nz, A.per[2], tz, gz);
+    const uint32_t c32 = uint32_t(cell);
+    T rf[NQ], rg[NQ];
+    uint32_t cmask = 0;
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        bool use_const;
-        const int64_t idx = pull_index<q>(A, cell, mask, sx, sy, sz, gx, gy, gz, use_const);
-        const double fq = widen(f[idx], sh.f(q));
-        const double gq = widen(g[idx], sh.g(q));
-        fv[q] = use_const ? bc.fval[q] : fq;
-        gv[q] = use_const ? bc.gval[q] : gq;
+        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+        constexpr int qb = kBounce[q];
+        const bool hx = gx[cx + 1], hy = gy[cy + 1], hz = gz[cz + 1];
+        const bool ghost = hx || hy || hz;
+        const uint32_t src = (uint32_t(tz[cz + 1]) * uint32_t(A.ny) + uint32_t(ty[cy + 1])) * uint32_t(A.nx) +
+                             uint32_t(tx[cx + 1]);
+        int face = cx > 0 ? 0 : 3;
+        face = hy ? (cy > 0 ? 1 : 4) : face;
+        face = hz ? (cz > 0 ? 2 : 5) : face;
+        const uint32_t t = (A.bcpack >> (4 * face)) & 15u;
+        const bool direct = ghost ? t == MRB_BC_OUTFLOW : ((mask >> q) & 1u) != 0;
+        int comp = direct ? q : qb;
+        comp = (ghost && t == MRB_BC_SLIPX) ? reflect(q, 0) : comp;
+        comp = (ghost && t == MRB_BC_SLIPY) ? reflect(q, 1) : comp;
+        comp = (ghost && t == MRB_BC_SLIPZ) ? reflect(q, 2) : comp;
+        const int64_t e = int64_t(comp) * A.cs + (direct ? src : c32);
+        rf[q] = f[e];
+        rg[q] = g[e];
+        cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
+    });
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const bool use_const = (cmask >> q) & 1u;
+        fv[q] = use_const ? bc.fval[q] : widen(rf[q], sh.f(q));
+        gv[q] = use_const ? bc.gval[q] : widen(rg[q], sh.g(q));
     });
 }
 
@@ -304,6 +303,8 @@ CompactArgs make_args(const mrb_level *L)
     A.N = L->total_cells;
     A.cs = 2 * A.N;
     for (int n = 0; n < 6; ++n) A.bc[n] = L->bc.bc_type[n];
+    A.bcpack = 0;
+    for (int n = 0; n < 6; ++n) A.bcpack |= uint32_t(A.bc[n] & 15) << (4 * n);
     return A;
 }
 

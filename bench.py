@@ -100,6 +100,29 @@ def copy_ceiling(torch, nbytes: int = 2 << 30, iters: int = 20) -> float:
     return 2.0 * nbytes * iters / secs / 1e9
 
 
+def measured_traffic(storage: str, kernels) -> float | None:
+    """HBM bytes per step of the given kernels from the committed rocprofv3
+    summary (calibrated FETCH_SIZE + WRITE_SIZE per dispatch), or None when
+    the summary does not cover them."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "k_qpass_k_collide_channel256.md")
+    try:
+        text = open(path).read()
+        block = re.search(r"```json\n(.*?)\n```", text, re.S).group(1)
+        data = json.loads(block)[storage]["kernels"]
+    except (OSError, AttributeError, KeyError, ValueError):
+        return None
+    elem = {"f64": "double", "f32s": "float"}[storage]
+    total = 0.0
+    for k in kernels:
+        rec = data.get(f"{k}<{elem}>")
+        if rec is None:
+            return None
+        total += rec["read_bytes"] + rec["write_bytes"]
+    return total
+
+
 def velocity_density(lvl, cfg) -> tuple:
     """(u, v, w, rho) of the level's current state, FP64 on the host."""
     from marbles_amd.stencil import velocity_density as vd
@@ -118,6 +141,42 @@ def time_level(lvl, steps: int, warmup: int, profile_steps: int, sh, torch) -> d
     return {"secs": secs, "per_kernel_ms": {k: v / profile_steps for k, v in prof.items()}}
 
 
+def run_amr(args, torch) -> int:
+    """Two-level workload: seeded uneven boxes (edges 8-64), fine level over
+    20 % of the volume, thermal f+g FP64; MLUPS counts N_coarse + 2 N_fine
+    updates per coarse step."""
+    from marbles_amd import Hierarchy, configs
+
+    n = args.n
+    cfg = configs.amr_two_level(ncell=(n, n, n), nbox=(n ** 3) // 4096)
+    t0 = time.perf_counter()
+    H = Hierarchy(cfg)
+    log(f"hierarchy built in {time.perf_counter() - t0:.1f}s: {len(cfg['coarse_boxes'])} coarse boxes, "
+        f"{len(cfg['fine_boxes'])} fine boxes, fine fraction {cfg['fine_fraction']:.4f}")
+    sh = torch.cuda.current_stream().cuda_stream
+    ups = H.updates_per_step()
+    H.advance(args.warmup, stream=sh, graph=args.graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H.advance(args.steps, stream=sh, graph=args.graph)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    mlups = ups * args.steps / secs / 1e6
+    log(f"timed {args.steps} coarse steps: {secs:.4f}s -> {mlups:.1f} MLUPS")
+    H.close()
+    print(json.dumps({
+        "metric": "two-level D3Q27 MLUPS (N_coarse + 2 N_fine updates per coarse step)",
+        "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{cfg['name']}: {len(cfg['coarse_boxes'])} coarse boxes (edges 8-64), "
+                               f"{len(cfg['fine_boxes'])} fine boxes over {cfg['fine_fraction']:.4f} of the "
+                               f"volume, thermal f+g, T0={cfg['T0']}, tau={cfg['tau']}, Pr=0.71",
+                   "updates_per_step": ups, "graph": bool(args.graph)},
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +189,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the shifted-FP32 comparison run")
+    ap.add_argument("--workload", default="channel", choices=["channel", "amr"],
+                    help="channel: the single-level 256^3 channel (default); amr: the two-level workload")
+    ap.add_argument("--graph", action="store_true", help="amr: replay captured hipGraphs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -144,6 +206,12 @@ def main() -> int:
     if not torch.cuda.is_available():
         log("bench.py: no GPU visible")
         return 1
+    if args.workload == "amr":
+        if "--steps" not in sys.argv:
+            args.steps = 10
+        if "--warmup" not in sys.argv:
+            args.warmup = 2
+        return run_amr(args, torch)
     n = args.n
     scale = n / 256.0
     cfg = configs.channel_sphere((n, n, n), radius=24.0 * scale, center=(64.0 * scale, 128.0 * scale,
@@ -196,6 +264,7 @@ def main() -> int:
             "arithmetic": "f64",
             "bytes_per_update": b32,
             "achieved_gbs": round(b32 * cells / (k32 * 1e-3) / 1e9, 1),
+            "traffic": measured_traffic("f32s", r32["per_kernel_ms"]) if n == 256 else None,
             "kernel_ms_per_step": {k: round(v, 5) for k, v in r32["per_kernel_ms"].items()},
         }
         log(f"f32-shifted: {m32:.1f} MLUPS ({m32 / mlups:.2f}x), max rel velocity error {du:.2e}, "
@@ -231,7 +300,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.storage, per_kernel_ms) if n == 256 else None,
             "kernel": "+".join(per_kernel_ms),
             "bytes_per_update": bpu,
             "kernel_ms_per_step": {k: round(v, 5) for k, v in per_kernel_ms.items()},

@@ -2,10 +2,12 @@
 // pull-stream with halfway bounce-back, periodic wrap and physical-boundary
 // ghost values resolved inside the kernel, followed by the thermal collide.
 //
-// HBM layout: populations SoA with f and g interleaved by component,
-// pop[(2q + lattice)*N + cell], cell = (z*ny + y)*nx + x, so f_q = f[q*cs + cell]
-// and g_q = g[q*cs + cell] with g = f + N and cs = 2N; two buffers (A/B); a
-// 32-bit stream mask per cell (bit q = the pull source x - c_q is fluid).
+// HBM layout: populations SoA, cell = (z*ny + y)*nx + x.  FP64: f and g
+// interleaved by component, pop[(2q + lattice)*N + cell], so f_q =
+// f[q*cs + cell], g_q = g[q*cs + cell] with g = f + N and cs = 2N.  Shifted
+// FP32: pop[(q*N + cell)*2 + lattice], f_q = f[q*cs + 2*cell] with g = f + 1.
+// Two buffers (A/B); a 32-bit stream mask per cell (bit q = the pull
+// source x - c_q is fluid).
 //
 // The collide needs the Q-correction gradient of post-stream moments at the
 // six face neighbours (LBM.cpp:959-991), so one step is two passes:
@@ -53,14 +55,33 @@ __device__ __forceinline__ void wrap3(int x, int n, int periodic, int s[3], bool
 }
 
 // Rest-state populations subtracted by the shifted storages (stored value =
-// population - shift), read through a pointer to [f shifts | g shifts] so
-// that they are scalar loads at their use (54 doubles of kernel arguments
-// would stay live in vector registers); unused for FP64 storage.
+// population - shift); one value per speed class |c_q|^2 = 0..3, so the
+// eight values sit in scalar registers; unused for FP64 storage.
 struct ShiftTab {
-    const double *__restrict__ v;
-    __device__ __forceinline__ double f(int q) const { return v[q]; }
-    __device__ __forceinline__ double g(int q) const { return v[NQ + q]; }
+    double fk[4], gk[4];
+    __device__ __forceinline__ double f(int q) const { return fk[csq(q)]; }
+    __device__ __forceinline__ double g(int q) const { return gk[csq(q)]; }
 };
+
+// Element stride between neighbouring cells of one component: FP64
+// storage keeps the components of a lattice contiguous
+// (pop[(2q + lattice)*N + cell]); shifted FP32 storage keeps f_q and g_q
+// of a cell side by side (pop[(q*N + cell)*2 + lattice]) so that one
+// 8-byte load or store moves both.
+template <class T>
+constexpr int kCellStride = std::is_same<T, float>::value ? 2 : 1;
+
+// store of a cell's f_q and g_q
+template <class T>
+__device__ __forceinline__ void store_pair(T *__restrict__ fo, T *__restrict__ go, int64_t e, T fv, T gv)
+{
+    if constexpr (kCellStride<T> == 2) {
+        *reinterpret_cast<float2 *>(fo + e) = make_float2(fv, gv);
+    } else {
+        fo[e] = fv;
+        go[e] = gv;
+    }
+}
 
 // storage <-> FP64 working value
 template <class T>
@@ -137,9 +158,16 @@ __device__ __forceinline__ void pull_cell(
         comp = (ghost && t == MRB_BC_SLIPX) ? reflect(q, 0) : comp;
         comp = (ghost && t == MRB_BC_SLIPY) ? reflect(q, 1) : comp;
         comp = (ghost && t == MRB_BC_SLIPZ) ? reflect(q, 2) : comp;
-        const int64_t e = int64_t(comp) * A.cs + (direct ? src : c32);
-        rf[q] = f[e];
-        rg[q] = g[e];
+        const int64_t e = int64_t(comp) * A.cs + int64_t(direct ? src : c32) * kCellStride<T>;
+        if constexpr (kCellStride<T> == 2) {
+            // f_q and g_q side by side: one 8-byte load (g == f + 1)
+            const float2 v = *reinterpret_cast<const float2 *>(f + e);
+            rf[q] = v.x;
+            rg[q] = v.y;
+        } else {
+            rf[q] = f[e];
+            rg[q] = g[e];
+        }
         cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
     });
     static_for<NQ>([&](auto qc) {
@@ -196,8 +224,8 @@ __device__ __forceinline__ void collide_body(
         // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            fo[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.f(q));
-            go[int64_t(q) * A.cs + cell] = narrow<T>(-1.0, sh.g(q));
+            store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(-1.0, sh.f(q)),
+                       narrow<T>(-1.0, sh.g(q)));
         });
         return;
     }
@@ -219,8 +247,8 @@ __device__ __forceinline__ void collide_body(
                                           zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
     collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.cs + cell] = narrow<T>(fnew, sh.f(q));
-        go[int64_t(q) * A.cs + cell] = narrow<T>(gnew, sh.g(q));
+        store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(fnew, sh.f(q)),
+                   narrow<T>(gnew, sh.g(q)));
     });
 }
 
@@ -262,8 +290,8 @@ __global__ void __launch_bounds__(256) k_stream_only(
     }
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        fo[int64_t(q) * A.cs + cell] = narrow<T>(fv[q], sh.f(q));
-        go[int64_t(q) * A.cs + cell] = narrow<T>(gv[q], sh.g(q));
+        store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(fv[q], sh.f(q)),
+                   narrow<T>(gv[q], sh.g(q)));
     });
 }
 
@@ -324,9 +352,14 @@ int compact_build_mask(mrb_level *L, hipStream_t s)
 // flip.  ev (if not null) receives events before, between and after.
 static ShiftTab shift_tab(const mrb_level *L)
 {
-    // d_small[0 .. 2NQ): the storage shifts (zero for FP64 storage)
     ShiftTab t{};
-    t.v = L->d_small;
+    if (L->storage != MRB_STORAGE_F64) {
+        const int rep[4] = {0, 1, 7, 19};
+        for (int k = 0; k < 4; ++k) {
+            t.fk[k] = L->fshift[rep[k]];
+            t.gk[k] = L->gshift[rep[k]];
+        }
+    }
     return t;
 }
 

@@ -414,7 +414,7 @@ __global__ void k_conservation(
     const PopT *__restrict__ f, const PopT *__restrict__ g, const int32_t *__restrict__ isf,
     const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
     const double *__restrict__ fshift, const double *__restrict__ gshift, double *__restrict__ part,
-    int64_t cs)
+    int64_t cs, int cst)
 {
     __shared__ double sh[5][kBlock];
     double acc[5] = {0, 0, 0, 0, 0};
@@ -427,8 +427,8 @@ __global__ void k_conservation(
         local_to_ijk(B, loc, i, j, k);
         if (!in_valid(B, i, j, k)) continue;
         if (isf[B.base * 2 + loc] != 1) continue;
-        const PopT *fp = f + B.base * NQ + loc;
-        const PopT *gp = g + B.base * NQ + loc;
+        const PopT *fp = f + B.base * NQ + loc * cst;
+        const PopT *gp = g + B.base * NQ + loc * cst;
         const int64_t stride = cs ? cs : B.ncell;
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -473,6 +473,14 @@ __global__ void k_init_uniform(
     for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * stride + loc] = solid ? 0.0 : val[q];
 }
 
+// dst[i * ds] = src[i * ss]
+template <class T>
+__global__ void k_strided_copy(const T *__restrict__ src, int ss, T *__restrict__ dst, int ds, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i * ds] = src[i * ss];
+}
+
 // initial state from per-cell fields (device copies of the host arrays):
 // f = f_eq(rho, u, R T), g = Grad equilibrium, 0 in solid cells, stored with
 // the level's storage (shifted FP32: value - shift)
@@ -480,7 +488,8 @@ template <class PopT>
 __global__ void k_init_fields(
     PopT *__restrict__ f, PopT *__restrict__ g, const int32_t *__restrict__ isf,
     const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
-    const double *__restrict__ fields, double R, double gamma, const double *__restrict__ shift, int64_t cs)
+    const double *__restrict__ fields, double R, double gamma, const double *__restrict__ shift, int64_t cs,
+    int cst)
 {
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= total) return;
@@ -495,8 +504,8 @@ __global__ void k_init_fields(
     for (int q = 0; q < NQ; ++q) {
         const double fv = solid ? 0.0 : set_equilibrium_value(rho, vel, R * T, q);
         const double gv = solid ? 0.0 : thermal_g_equilibrium(rho, vel, T, R, gamma, q);
-        f[B.base * NQ + q * stride + loc] = PopT(fv - shift[q]);
-        g[B.base * NQ + q * stride + loc] = PopT(gv - shift[NQ + q]);
+        f[B.base * NQ + q * stride + loc * cst] = PopT(fv - shift[q]);
+        g[B.base * NQ + q * stride + loc * cst] = PopT(gv - shift[NQ + q]);
     }
 }
 
@@ -512,6 +521,14 @@ __global__ void k_init_fields(
 static int64_t comp_stride(const mrb_level *L)
 {
     return L->layout == MRB_LAYOUT_COMPACT ? 2 * L->total_cells : 0;
+}
+
+// Element stride between neighbouring cells: shifted FP32 storage on the
+// compact layout keeps f_q and g_q of a cell side by side
+// (pop[(q*N + cell)*2 + lattice]) so that one 8-byte load fetches both.
+static int cell_stride(const mrb_level *L)
+{
+    return (L->layout == MRB_LAYOUT_COMPACT && L->storage == MRB_STORAGE_F32_SHIFTED) ? 2 : 1;
 }
 
 // Host <-> device copy of one lattice's 27 components (host: contiguous
@@ -636,9 +653,14 @@ static void compute_shift(mrb_level *L)
     const double R = kAirMolarMass / (L->phys_set ? L->phys.m_bar : kAirMolarMass);
     const double gamma = L->phys_set ? L->phys.adiabatic_exponent : 5.0 / 3.0;
     const double zero[3] = {0.0, 0.0, 0.0};
+    // The rest state depends on |c_q|^2 only; one value per speed class
+    // (taken from its first direction) so that kernels can hold the four
+    // values of each lattice in scalar registers.
+    const int rep[4] = {0, 1, 7, 19};
     for (int q = 0; q < NQ; ++q) {
-        L->fshift[q] = set_equilibrium_value(L->shift_rho0, zero, R * L->shift_T0, q);
-        L->gshift[q] = thermal_g_equilibrium(L->shift_rho0, zero, L->shift_T0, R, gamma, q);
+        const int r = rep[csq(q)];
+        L->fshift[q] = set_equilibrium_value(L->shift_rho0, zero, R * L->shift_T0, r);
+        L->gshift[q] = thermal_g_equilibrium(L->shift_rho0, zero, L->shift_T0, R, gamma, r);
     }
     if (L->d_small) {
         double tab[2 * NQ];
@@ -738,7 +760,7 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
         for (int ab = 0; ab < 2; ++ab) {
             MRB_HIP(hipMalloc(&L->pop[0][ab], es * 2 * NQ * T));
             MRB_HIP(hipMemset(L->pop[0][ab], 0, es * 2 * NQ * T));
-            L->pop[1][ab] = static_cast<char *>(L->pop[0][ab]) + es * T;
+            L->pop[1][ab] = static_cast<char *>(L->pop[0][ab]) + es * (cell_stride(L.get()) == 2 ? 1 : T);
         }
     } else {
         for (int l = 0; l < 2; ++l)
@@ -921,6 +943,28 @@ static int copy_lattice(const mrb_level *L, int arr, void *host, bool to_device)
     const size_t es = pop_elem_size(L->storage);
     const int64_t N = L->total_cells;
     void *dev = L->pop[arr][L->cur];
+    if (L->layout == MRB_LAYOUT_COMPACT && cell_stride(L) == 2) {
+        // pair layout: element (q, cell) of the lattice at dev[(q*N + cell)*2]
+        void *tmp = nullptr;
+        MRB_HIP(hipMalloc(&tmp, es * NQ * size_t(N)));
+        hipError_t e = hipSuccess;
+        const int64_t n = NQ * N;
+        if (to_device) {
+            e = hipMemcpy(tmp, host, es * size_t(n), hipMemcpyHostToDevice);
+            if (e == hipSuccess) {
+                k_strided_copy<float><<<nblocks(n), kBlock>>>((const float *)tmp, 1, (float *)dev, 2, n);
+                e = hipGetLastError();
+            }
+        } else {
+            k_strided_copy<float><<<nblocks(n), kBlock>>>((const float *)dev, 2, (float *)tmp, 1, n);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpy(host, tmp, es * size_t(n), hipMemcpyDeviceToHost);
+        }
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return hip_fail(e, "copy of the pair layout");
+        return MRB_OK;
+    }
     if (L->layout == MRB_LAYOUT_COMPACT) {
         const size_t w = es * size_t(N), dp = es * size_t(2 * N);
         MRB_HIP(to_device ? copy_components(dev, dp, host, w, w, hipMemcpyHostToDevice)
@@ -1215,11 +1259,11 @@ int mrb_init_fields(
         if (L->storage == MRB_STORAGE_F64)
             k_init_fields<double><<<nblocks(n), kBlock>>>((double *)L->pop[0][L->cur], (double *)L->pop[1][L->cur],
                                                           L->d_isf, L->d_boxes, L->d_box_base, L->nbox, n, d, R,
-                                                          adiabatic_exponent, L->d_small, comp_stride(L));
+                                                          adiabatic_exponent, L->d_small, comp_stride(L), cell_stride(L));
         else
             k_init_fields<float><<<nblocks(n), kBlock>>>((float *)L->pop[0][L->cur], (float *)L->pop[1][L->cur],
                                                          L->d_isf, L->d_boxes, L->d_box_base, L->nbox, n, d, R,
-                                                         adiabatic_exponent, L->d_small, comp_stride(L));
+                                                         adiabatic_exponent, L->d_small, comp_stride(L), cell_stride(L));
         e = hipGetLastError();
         if (e == hipSuccess) e = hipDeviceSynchronize();
     }
@@ -1491,11 +1535,11 @@ int mrb_reduce_conservation(const mrb_level *L, double out[5], void *stream)
     if (L->storage == MRB_STORAGE_F64) {
         k_conservation<double><<<nb, kBlock, 0, s>>>(
             (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L));
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L));
     } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
         k_conservation<float><<<nb, kBlock, 0, s>>>(
             (const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L));
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L));
     } else {
         return set_error(MRB_E_UNSUPPORTED, "conservation sums for this storage");
     }

@@ -199,6 +199,13 @@ int mrb_stream_compact(mrb_level *lvl, void *stream);
  * FP64 sums over valid fluid cells: mass, x/y/z momentum, 2*rho*E.
  * Synchronous (waits for `stream`). */
 int mrb_reduce_conservation(const mrb_level *lvl, double out[5], void *stream);
+/* LBM::compute_eb_forces (Source/LBM.cpp:994-1044) on one level (no finer
+ * level masking): momentum-exchange force on the embedded boundary,
+ * F = sum over solid cells with a fluid face neighbour and over q of
+ * 2 c_q f_q(x - c_q) is_fluid(x - c_q), from the current populations.  FP64
+ * sums; synchronous.  Compact layout: sources beyond a non-periodic face
+ * contribute nothing. */
+int mrb_eb_force(const mrb_level *lvl, double out[3], void *stream);
 
 /* ---- two levels, refinement ratio 2 (MRB_LAYOUT_FAB, FP64) ---------------
  * The coarse level covers its domain; the fine level's domain is the coarse

@@ -106,6 +106,7 @@ SIGNATURES = {
     "mrb_amr_advance": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "mrb_amr_advance_graph": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "mrb_autotune_count": (c_int, [c_void_p]),
+    "mrb_eb_force": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
     "mrb_autotune_name": (ctypes.c_char_p, [c_void_p, c_int]),
 }
 

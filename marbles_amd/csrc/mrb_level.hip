@@ -473,6 +473,73 @@ __global__ void k_init_uniform(
     for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * stride + loc] = solid ? 0.0 : val[q];
 }
 
+// LBM::compute_eb_forces (LBM.cpp:994-1044) on one level: over valid cells
+// with is_fluid comp1 == 1, F += 2 c_q f_q(x - c_q) is_fluid(x - c_q, 0);
+// per-block partial sums.  On the compact layout x - c_q wraps on periodic
+// axes; a source beyond a non-periodic face contributes nothing (the
+// reference reads a physical-boundary ghost there).
+template <class PopT>
+__global__ void k_eb_force(
+    const PopT *__restrict__ f, const int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes,
+    const int64_t *__restrict__ base, int nbox, int64_t total, const double *__restrict__ fshift,
+    double *__restrict__ part, int64_t cs, int cst, int compact, int n0, int n1, int n2, int p0, int p1,
+    int p2)
+{
+    __shared__ double sh[3][kBlock];
+    double acc[3] = {0, 0, 0};
+    for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < total;
+         c += int64_t(gridDim.x) * blockDim.x) {
+        const int b = find_box(base, nbox, c);
+        const BoxDev B = boxes[b];
+        const int64_t loc = c - B.base;
+        int i, j, k;
+        local_to_ijk(B, loc, i, j, k);
+        if (!in_valid(B, i, j, k)) continue;
+        const int32_t *fl = isf + B.base * 2;
+        if (fl[B.ncell + loc] != 1) continue;
+        const int64_t stride = cs ? cs : B.ncell;
+        double fs[3] = {0.0, 0.0, 0.0};
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            int a = i - ev(q, 0), bb = j - ev(q, 1), e = k - ev(q, 2);
+            bool ok = true;
+            if (compact) {
+                const int n[3] = {n0, n1, n2}, per[3] = {p0, p1, p2};
+                int s[3] = {a, bb, e};
+                for (int d = 0; d < 3; ++d) {
+                    if (s[d] < 0 || s[d] >= n[d]) {
+                        if (!per[d]) ok = false;
+                        s[d] = s[d] < 0 ? s[d] + n[d] : s[d] - n[d];
+                    }
+                }
+                a = s[0];
+                bb = s[1];
+                e = s[2];
+            }
+            if (ok) {
+                const int64_t sl = loc_of(B, a, bb, e);
+                const double fq = double(f[B.base * NQ + q * stride + sl * cst]) + fshift[q];
+                const int32_t w = fl[sl];
+                fs[0] += 2.0 * ev(q, 0) * fq * w;
+                fs[1] += 2.0 * ev(q, 1) * fq * w;
+                fs[2] += 2.0 * ev(q, 2) * fq * w;
+            }
+        });
+        acc[0] += fs[0];
+        acc[1] += fs[1];
+        acc[2] += fs[2];
+    }
+    for (int n = 0; n < 3; ++n) sh[n][threadIdx.x] = acc[n];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int n = 0; n < 3; ++n) sh[n][threadIdx.x] += sh[n][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int n = 0; n < 3; ++n) part[blockIdx.x * 5 + n] = sh[n][0];
+}
+
 // dst[i * ds] = src[i * ss]
 template <class T>
 __global__ void k_strided_copy(const T *__restrict__ src, int ss, T *__restrict__ dst, int ds, int64_t n)
@@ -1550,6 +1617,37 @@ int mrb_reduce_conservation(const mrb_level *L, double out[5], void *stream)
     for (int n = 0; n < 5; ++n) out[n] = 0.0;
     for (unsigned b = 0; b < nb; ++b)
         for (int n = 0; n < 5; ++n) out[n] += part[size_t(5 * b + n)];
+    return MRB_OK;
+}
+
+int mrb_eb_force(const mrb_level *L, double out[3], void *stream)
+{
+    if (!L || !out) return set_error(MRB_E_ARG, "null argument");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    hipStream_t s = as_stream(stream);
+    const unsigned nb = std::min<unsigned>(4096u, nblocks(L->total_cells));
+    const double *shifts = L->d_small;  // zero for FP64 storage
+    const int compact = L->layout == MRB_LAYOUT_COMPACT;
+    if (L->storage == MRB_STORAGE_F64) {
+        k_eb_force<double><<<nb, kBlock, 0, s>>>(
+            (const double *)L->pop[0][L->cur], L->d_isf, L->d_boxes, L->d_box_base, L->nbox, L->total_cells,
+            shifts, L->d_red, comp_stride(L), cell_stride(L), compact, L->ncell[0], L->ncell[1], L->ncell[2],
+            L->periodic[0], L->periodic[1], L->periodic[2]);
+    } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
+        k_eb_force<float><<<nb, kBlock, 0, s>>>(
+            (const float *)L->pop[0][L->cur], L->d_isf, L->d_boxes, L->d_box_base, L->nbox, L->total_cells,
+            shifts, L->d_red, comp_stride(L), cell_stride(L), compact, L->ncell[0], L->ncell[1], L->ncell[2],
+            L->periodic[0], L->periodic[1], L->periodic[2]);
+    } else {
+        return set_error(MRB_E_UNSUPPORTED, "EB force for this storage");
+    }
+    MRB_HIP(hipGetLastError());
+    std::vector<double> part(size_t(5 * nb));
+    MRB_HIP(hipMemcpyAsync(part.data(), L->d_red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+    MRB_HIP(hipStreamSynchronize(s));
+    for (int n = 0; n < 3; ++n) out[n] = 0.0;
+    for (unsigned b = 0; b < nb; ++b)
+        for (int n = 0; n < 3; ++n) out[n] += part[size_t(5 * b + n)];
     return MRB_OK;
 }
 

@@ -242,6 +242,12 @@ class Level:
         check(self.lib.mrb_reduce_conservation(self.h, out, stream))
         return np.array(out[:])
 
+    def eb_force(self, stream=None) -> np.ndarray:
+        """Momentum-exchange force on the embedded boundary (LBM::compute_eb_forces)."""
+        out = (c_double * 3)()
+        check(self.lib.mrb_eb_force(self.h, out, stream))
+        return np.array(out[:])
+
     def hot_kernel_name(self) -> str:
         return self.lib.mrb_hot_kernel_name(self.h).decode()
 

@@ -1322,6 +1322,37 @@ void orc_ic_fields(
     }
 }
 
+/* LBM::compute_eb_forces (LBM.cpp:994-1044), one level with no finer level
+ * (mask = 0): over valid cells with is_fluid comp1 == 1 (solid cells with a
+ * fluid face neighbour), F += 2 c_q f_q(x - c_q) is_fluid(x - c_q, 0).
+ * Summed in (box, k, j, i, q) order. */
+void orc_eb_force(const orc_level *L, double out[3])
+{
+    out[0] = out[1] = out[2] = 0.0;
+    for (int b = 0; b < L->nbox; b++) {
+        const fabdim *fd = &L->fdg[b];
+        const double *f = L->f[b];
+        const int32_t *isf = L->isf[b];
+        for (int k = L->vlo[b][2]; k <= L->vhi[b][2]; k++)
+            for (int j = L->vlo[b][1]; j <= L->vhi[b][1]; j++)
+                for (int i = L->vlo[b][0]; i <= L->vhi[b][0]; i++) {
+                    if (isf[fidx(fd, 1, i, j, k)] != 1) continue;
+                    double fs[3] = {0.0, 0.0, 0.0};
+                    for (int q = 0; q < ORC_NQ; q++) {
+                        const int *ev = EV[q];
+                        const int *eb = EV[BOUNCE[q]];
+                        const int a = i + eb[0], c = j + eb[1], e = k + eb[2];
+                        for (int idir = 0; idir < 3; idir++)
+                            fs[idir] += 2.0 * ev[idir] * f[fidx(fd, q, a, c, e)] *
+                                        isf[fidx(fd, 0, a, c, e)];
+                    }
+                    out[0] += fs[0];
+                    out[1] += fs[1];
+                    out[2] += fs[2];
+                }
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Two levels, refinement ratio 2.                                           */
 /* ------------------------------------------------------------------------ */

@@ -118,6 +118,9 @@ void orc_collide(orc_level *L);                     /* LBM::collide */
 void orc_evolve(orc_level *L, int nsteps);
 /* conservation sums over valid fluid cells: mass, x/y/z momentum, 2rhoE */
 void orc_conservation(const orc_level *L, double out[5]);
+/* LBM::compute_eb_forces (LBM.cpp:994-1044) on one level: momentum-exchange
+ * force on the embedded boundary from the current (post-collision) f */
+void orc_eb_force(const orc_level *L, double out[3]);
 
 /* ---- two levels, refinement ratio 2 -------------------------------------
  * The coarse level must cover its domain; the fine level's domain is the

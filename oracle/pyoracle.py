@@ -88,6 +88,7 @@ def load() -> ctypes.CDLL:
         "orc_get_temperature_vec": (c_double, [c_double, c_double, D3, c_double]),
         "orc_masked_avgdown8": (c_double, [D3]),
         "orc_sphere_covered": (c_int, [D3, D3, D3, c_double, c_int]),
+        "orc_eb_force": (None, [c_void_p, D3]),
         "orc_ic_fields": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_double, c_double]),
         "orc_amr_coalesce": (None, [c_void_p, c_void_p, c_int]),
@@ -284,6 +285,11 @@ class OracleLevel:
     def conservation(self):
         out = (c_double * 5)()
         self.lib.orc_conservation(self.h, out)
+        return np.array(out[:])
+
+    def eb_force(self):
+        out = (c_double * 3)()
+        self.lib.orc_eb_force(self.h, out)
         return np.array(out[:])
 
 

@@ -242,3 +242,32 @@ def test_f32_shifted_storage_tracks_f64(oracle, engine, case):
     dr = np.abs(r32 - r64)[fluid].max() / r64[fluid].max()
     print(f"{case}: {n} steps, max rel velocity error {du:.2e}, density {dr:.2e}")
     assert du <= 1e-6 and dr <= 1e-6, (du, dr)
+
+
+@pytest.mark.parametrize("layout", ["compact", "fab", "compact_f32s"])
+def test_eb_force_matches_oracle(oracle, engine, layout):
+    """Momentum-exchange force on the channel's sphere after 20 steps (FP64
+    sums in a different order: 1e-12 relative; FP32-shifted storage: the
+    populations differ by ~1e-7 relative, so 1e-5)."""
+    from oracle.pyoracle import build_oracle_level
+
+    cfg = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0))
+    O = build_oracle_level(cfg)
+    O.evolve(20)
+    ref = O.eb_force()
+    if layout == "fab":
+        O0 = build_oracle_level(cfg)
+        E = engine.Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=O0.boxes, ng=3,
+                         layout="fab")
+        E.set_phys(cfg["nu"], cfg.get("alpha"))
+        E.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+        E.flags_upload(O0.flat("is_fluid"))
+        E.upload("f", O0.flat("f"))
+        E.upload("g", O0.flat("g"))
+    else:
+        E = configs.build_level(cfg, layout="compact", storage="f32s" if layout == "compact_f32s" else "f64")
+    E.advance(20)
+    got = E.eb_force()
+    tol = 1e-5 if layout == "compact_f32s" else 1e-12
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (layout, got, ref)
+    E.close()

@@ -241,6 +241,33 @@ int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream);
  * Results are identical to mrb_amr_advance.  Asynchronous. */
 int mrb_amr_advance_graph(mrb_level *crse, mrb_level *fine, int nsteps, void *stream);
 
+/* ---- isothermal variant with partial bounce-back, and its adjoint ---------
+ * No reference counterpart as a whole: the f update of LBM::collide
+ * (Source/LBM.cpp:621-807) with the temperature held at T0 (g not stored,
+ * Q-correction kept), LBM::stream's pull (LBM.cpp:558-604) with a
+ * constant-velocity inlet at x-lo (BC.H:395-419) and an outflow at x-hi
+ * (BC.H:435-447), y and z periodic, followed by the Walsh-Burwinkle-Saar
+ * partial bounce-back f_q <- (1 - n_s) f_q + n_s f*_qbar with a per-cell
+ * solid fraction n_s in [0, 1] (n_s = 0: the reference's fluid update).
+ * Loss J = mean outlet-plane u_x + 1e-3 sum rho |u|^2 / 2 of the final
+ * state.  Arrays (n_s: N doubles; u0, du0: 3N, component-major) are device
+ * pointers, cell = (z*ny + y)*nx + x. */
+typedef struct mrb_iso mrb_iso;
+int mrb_iso_create(const int ncell[3], double T0, double rho0, double u_in, mrb_iso **out);
+int mrb_iso_destroy(mrb_iso *iso);
+/* nsteps steps from f = f_eq(rho0, u0(x), R T0); *J receives the loss. */
+int mrb_iso_forward(
+    mrb_iso *iso, const double *ns, double tau, const double *u0, int nsteps, double *J, void *stream);
+/* The loss and its exact gradient with respect to tau, n_s (per cell) and
+ * u0 (per cell, 3 comps) by the hand-written adjoint of the step, with
+ * checkpoints every `every` steps (<= 0: ceil(sqrt(nsteps))) and the
+ * states between checkpoints recomputed.  Synchronous. */
+int mrb_iso_gradient(
+    mrb_iso *iso, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
+    double *dtau, double *dns, double *du0, void *stream);
+/* Final populations of the last mrb_iso_forward (27 N doubles, host). */
+int mrb_iso_state(const mrb_iso *iso, double *host_f);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 const char *mrb_last_error(void);
 /* Name of the kernel variant used by mrb_stream_collide. */

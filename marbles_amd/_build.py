@@ -26,6 +26,7 @@ SOURCES = [
     "mrb_fused.hip",
     "mrb_amr.hip",
     "mrb_batched.hip",
+    "mrb_adjoint.hip",
 ]
 
 COMMON_FLAGS = [

@@ -107,6 +107,12 @@ SIGNATURES = {
     "mrb_amr_advance_graph": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "mrb_autotune_count": (c_int, [c_void_p]),
     "mrb_eb_force": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
+    "mrb_iso_create": (c_int, [POINTER(c_int), c_double, c_double, c_double, POINTER(c_void_p)]),
+    "mrb_iso_destroy": (c_int, [c_void_p]),
+    "mrb_iso_forward": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_int, POINTER(c_double), c_void_p]),
+    "mrb_iso_gradient": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_int, c_int, POINTER(c_double),
+                                 POINTER(c_double), c_void_p, c_void_p, c_void_p]),
+    "mrb_iso_state": (c_int, [c_void_p, POINTER(c_double)]),
     "mrb_autotune_name": (ctypes.c_char_p, [c_void_p, c_int]),
 }
 

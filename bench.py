@@ -177,6 +177,80 @@ def run_amr(args, torch) -> int:
     return 0
 
 
+def run_adjoint(args, torch) -> int:
+    """Gradient of the flow objective (outlet-plane mean u_x + 1e-3 KE) with
+    respect to n_s, tau and u0 on an n^3 box with an n_s field of seeded
+    smoothed blobs (15 % mean), inflow 0.04: forward time, gradient time
+    (recompute + adjoint), and central differences (h = 1e-6) on 32 seeded
+    cells, tau and 4 initial-velocity components."""
+    import numpy as np
+
+    from marbles_amd.adjoint import IsoSolver
+
+    n = 128 if args.n == 256 else args.n
+    nsteps = args.steps
+    shape = (n, n, n)
+    rng = np.random.default_rng(5)
+    z, y, x = np.meshgrid(*[np.arange(n)] * 3, indexing="ij")
+    s = np.zeros(shape)
+    for _ in range(24):
+        c = rng.uniform(0, n, 3)
+        s += np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (2 * (n / 16) ** 2))
+    s = np.clip(s, 0, 1)
+    s = np.clip(s * 0.15 / s.mean(), 0, 1)
+    ns = torch.tensor(s, device="cuda")
+    u0 = torch.zeros((3,) + shape, dtype=torch.float64, device="cuda")
+    u0[0] = 0.04
+    tau = 0.6
+    S = IsoSolver((n, n, n), T0=1.0 / 3.0, u_in=0.04)
+    S.forward(ns, tau, u0, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = S.forward(ns, tau, u0, nsteps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J2, dtau, dns, du0 = S.gradient(ns, tau, u0, nsteps)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    h = 1e-6
+    checks = [(dtau, (S.forward(ns, tau + h, u0, nsteps) - S.forward(ns, tau - h, u0, nsteps)) / (2 * h))]
+    flat = ns.view(-1)
+    for c in rng.choice(ns.numel(), 32, replace=False):
+        keep = flat[int(c)].item()
+        flat[int(c)] = keep + h
+        jp = S.forward(ns, tau, u0, nsteps)
+        flat[int(c)] = keep - h
+        jm = S.forward(ns, tau, u0, nsteps)
+        flat[int(c)] = keep
+        checks.append((dns.view(-1)[int(c)].item(), (jp - jm) / (2 * h)))
+    uf = u0.view(-1)
+    for c in rng.choice(u0.numel(), 4, replace=False):
+        keep = uf[int(c)].item()
+        uf[int(c)] = keep + h
+        jp = S.forward(ns, tau, u0, nsteps)
+        uf[int(c)] = keep - h
+        jm = S.forward(ns, tau, u0, nsteps)
+        uf[int(c)] = keep
+        checks.append((du0.view(-1)[int(c)].item(), (jp - jm) / (2 * h)))
+    scale = max(abs(g) for g, _ in checks)
+    err = max(abs(g - d) for g, d in checks) / scale
+    S.close()
+    log(f"J={J:.12g} forward {t_fwd:.3f}s gradient {t_grad:.3f}s ({t_grad / t_fwd:.2f}x) FD max rel err {err:.2e}")
+    print(json.dumps({
+        "metric": "adjoint gradient cost / forward cost (isothermal D3Q27, partial bounce-back)",
+        "value": round(t_grad / t_fwd, 3), "unit": "x", "n_gpus": 1, "steps": nsteps, "warmup": 0,
+        "ms_per_step": round(t_fwd / nsteps * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 box, n_s blobs (15 % mean), inflow 0.04, outflow, tau {tau}, "
+                               f"{nsteps} steps, checkpoints every ceil(sqrt(steps))",
+                   "J": J, "forward_s": t_fwd, "gradient_s": t_grad,
+                   "mlups_forward": n ** 3 * nsteps / t_fwd / 1e6,
+                   "fd_max_rel_error": err, "fd_points": len(checks)},
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,8 +263,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the shifted-FP32 comparison run")
-    ap.add_argument("--workload", default="channel", choices=["channel", "amr"],
-                    help="channel: the single-level 256^3 channel (default); amr: the two-level workload")
+    ap.add_argument("--workload", default="channel", choices=["channel", "amr", "adjoint"],
+                    help="channel: the single-level 256^3 channel (default); amr: the two-level workload; "
+                         "adjoint: gradient cost and finite-difference check at 128^3")
     ap.add_argument("--graph", action="store_true", help="amr: replay captured hipGraphs")
     args = ap.parse_args()
 
@@ -206,6 +281,10 @@ def main() -> int:
     if not torch.cuda.is_available():
         log("bench.py: no GPU visible")
         return 1
+    if args.workload == "adjoint":
+        if "--steps" not in sys.argv:
+            args.steps = 200
+        return run_adjoint(args, torch)
     if args.workload == "amr":
         if "--steps" not in sys.argv:
             args.steps = 10

@@ -18,13 +18,18 @@
 // Loss J = mean over the outlet plane (x = nx-1) of u_x + 1e-3 sum over
 // cells of rho |u|^2 / 2, from the final populations.
 //
-// Adjoint of one step (lambda = dJ/d populations), three passes:
-//   a: per cell, the transpose of collide + partial bounce-back: lambda at
-//      the pulled populations (without the Q-correction term), dJ/d(dQ),
-//      dJ/dn_s (accumulated), dJ/dtau (block partial sums)
-//   b: per cell, the transpose of dQ (a 3-point gather along each axis) and
-//      of Q(rho, u) added to the pulled populations' lambda
-//   c: the transpose of the pull (a push, written as a gather)
+// Adjoint of one step (lambda = dJ/d populations), two passes:
+//   adj: per cell, lambda of the post-collision populations gathered through
+//        the transpose of the pull from the previous pass's output, then the
+//        transpose of collide + partial bounce-back: lambda at the pulled
+//        populations (without the Q-correction path), dJ/d(dQ), dJ/dn_s
+//        (accumulated), dJ/dtau (block partial sums)
+//   adj_q: per cell, the transpose of dQ (a 3-point gather along each axis)
+//        and of Q(rho, u), kept as four coefficients that the next gather
+//        adds to each pulled population's lambda
+// Checkpoints: the forward sweep writes a checkpoint every `every` steps in
+// place and keeps the last segment's states; the reverse sweep recomputes
+// each earlier segment from its checkpoint.
 // Layout: f[q*N + cell], cell = (z*ny + y)*nx + x, FP64.
 #include <algorithm>
 #include <cmath>
@@ -40,13 +45,16 @@ struct mrb_iso {
     double inlet[mrb::NQ] = {};
     // device work arrays
     double *f[2] = {nullptr, nullptr};
-    double *lam[2] = {nullptr, nullptr};
-    double *lams = nullptr;   // 27 N: lambda of the pulled populations
-    double *q = nullptr;      // 3 N: Q of the current state
-    double *dqbar = nullptr;  // 3 N
-    double *ru = nullptr;     // 4 N: rho, u of the pulled state
-    double *part = nullptr;   // one partial sum per block of the cell grid
+    double *lam[2] = {nullptr, nullptr};  // lambda ping-pong (27 N each)
+    double *q = nullptr;                  // 3 N: Q of the current state
+    double *dqbar = nullptr;              // 3 N
+    double *uo = nullptr;                 // 3 N: u of the pulled state
+    double *bu = nullptr;                 // 4 N: shared lambda terms of one adjoint pass
+    double *ab = nullptr;                 // 4 N: per-cell gather coefficients alpha, beta
+    double *part = nullptr;               // one partial sum per block of the cell grid
     int64_t nparts = 0;
+    // checkpoints (one per segment), the states and Q arrays of one segment;
+    // kept between gradient calls of the same segment shape
     std::vector<double *> ckpt, seg, segq;
 };
 
@@ -72,19 +80,31 @@ __device__ __forceinline__ bool iso_cell(const IsoArgs &A, int &x, int &y, int &
 
 __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// pull of one cell (see the file comment)
+// pull of one cell (see the file comment), without branches: source
+// coordinates per velocity component (index c + 1 holds the source of c),
+// every load issued unconditionally, the inlet constant selected afterwards
 __device__ __forceinline__ void iso_pull(const double *__restrict__ f, const IsoArgs &A, int x, int y, int z,
                                          double fs[NQ])
 {
+    const uint32_t nxy = uint32_t(A.nx) * uint32_t(A.ny);
+    const uint32_t xs[3] = {uint32_t(x + 1 >= A.nx ? A.nx - 1 : x + 1), uint32_t(x), uint32_t(x > 0 ? x - 1 : 0)};
+    const uint32_t yo[3] = {uint32_t(wrap(y + 1, A.ny)) * uint32_t(A.nx), uint32_t(y) * uint32_t(A.nx),
+                            uint32_t(wrap(y - 1, A.ny)) * uint32_t(A.nx)};
+    const uint32_t zo[3] = {uint32_t(wrap(z + 1, A.nz)) * nxy, uint32_t(z) * nxy, uint32_t(wrap(z - 1, A.nz)) * nxy};
+    double r[NQ];
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-        const int sy = wrap(y - cy, A.ny), sz = wrap(z - cz, A.nz);
-        int sx = x - cx;
-        const bool inlet = sx < 0;
-        sx = sx >= A.nx ? A.nx - 1 : (sx < 0 ? 0 : sx);
-        const double v = f[int64_t(q) * A.N + (int64_t(sz) * A.ny + sy) * A.nx + sx];
-        fs[q] = inlet ? A.inlet[q] : v;
+        const double *fq = f + int64_t(q) * A.N;
+        r[q] = fq[zo[cz + 1] + yo[cy + 1] + xs[cx + 1]];
+    });
+    const bool inlet = x == 0;
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (ev(q, 0) == 1)
+            fs[q] = inlet ? A.inlet[q] : r[q];
+        else
+            fs[q] = r[q];
     });
 }
 
@@ -107,18 +127,24 @@ __device__ __forceinline__ double iso_phi(int c, double ua, double P)
     return ((c * 0.5) * ua + (c != 0 ? 1.0 : 0.0) * (1.5 * P - 1.0)) - P + 1.0;
 }
 
-// dQ_a at a cell from the Q array (domain-box rule on every axis)
+// dQ_a at a cell from the Q array (domain-box rule on every axis); the
+// neighbour loads are clamped into the box and issued unconditionally
 __device__ __forceinline__ void iso_dq(const double *__restrict__ Q, const IsoArgs &A, int x, int y, int z,
                                        int64_t cell, double dq[3])
 {
     const int pos[3] = {x, y, z}, n[3] = {A.nx, A.ny, A.nz};
     const int64_t st[3] = {1, A.nx, int64_t(A.nx) * A.ny};
+    double qp[3], q0[3], qm[3];
     for (int a = 0; a < 3; ++a) {
         const double *Qa = Q + a * A.N;
         const bool p_ok = pos[a] + 1 < n[a], m_ok = pos[a] - 1 >= 0;
-        const double qp = p_ok ? Qa[cell + st[a]] : 0.0;
-        const double qm = m_ok ? Qa[cell - st[a]] : 0.0;
-        dq[a] = one_sided_gradient(p_ok, m_ok, qp, Qa[cell], qm, 1.0);
+        qp[a] = Qa[cell + (p_ok ? st[a] : 0)];
+        q0[a] = Qa[cell];
+        qm[a] = Qa[cell - (m_ok ? st[a] : 0)];
+    }
+    for (int a = 0; a < 3; ++a) {
+        const bool p_ok = pos[a] + 1 < n[a], m_ok = pos[a] - 1 >= 0;
+        dq[a] = one_sided_gradient(p_ok, m_ok, p_ok ? qp[a] : 0.0, q0[a], m_ok ? qm[a] : 0.0, 1.0);
     }
 }
 
@@ -222,11 +248,66 @@ __global__ void __launch_bounds__(256) k_iso_seed(const double *__restrict__ f, 
     });
 }
 
-// pass a: transpose of collide + partial bounce-back at one cell
-__global__ void __launch_bounds__(256) k_iso_adj_a(
-    const double *__restrict__ f, const double *__restrict__ Q, const double *__restrict__ lam_in,
-    const double *__restrict__ ns, double *__restrict__ lams, double *__restrict__ dqbar, double *__restrict__ ru,
-    double *__restrict__ dns, double *__restrict__ part, IsoArgs A)
+// lambda of one post-collision population read through the pull by the cell
+// t: the pulled populations' lambda there plus the Q-path term
+// alpha(t) + beta(t) . c_q (ab holds alpha, beta_x, beta_y, beta_z)
+template <int q>
+__device__ __forceinline__ double lam_at(const double *__restrict__ lams, const double *__restrict__ ab, int64_t N,
+                                         int64_t t)
+{
+    double v = lams[int64_t(q) * N + t] + ab[t];
+    static_for<3>([&](auto ac) {
+        constexpr int a = decltype(ac)::value;
+        if constexpr (ev(q, a) == 1) v += ab[(a + 1) * N + t];
+        if constexpr (ev(q, a) == -1) v -= ab[(a + 1) * N + t];
+    });
+    return v;
+}
+
+// transpose of the pull, as a gather: f_q(x) was read by the cell x + c_q
+// (when inside the domain in x) and, on the x-hi boundary for c_x = -1, also
+// through the outflow ghost by the boundary cell at (x, y + c_y, z + c_z)
+template <int q>
+__device__ __forceinline__ double gather_one(const double *__restrict__ lams, const double *__restrict__ ab,
+                                             const IsoArgs &A, int x, int y, int z)
+{
+    constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+    const int ty = wrap(y + cy, A.ny), tz = wrap(z + cz, A.nz);
+    const int tx = x + cx;
+    const bool in = tx >= 0 && tx < A.nx;
+    const int64_t row = (int64_t(tz) * A.ny + ty) * A.nx;
+    const double v = lam_at<q>(lams, ab, A.N, row + (tx < 0 ? 0 : (tx >= A.nx ? A.nx - 1 : tx)));
+    double l = in ? v : 0.0;
+    if constexpr (cx == -1)
+        if (x == A.nx - 1) l += lam_at<q>(lams, ab, A.N, row + x);
+    return l;
+}
+
+__device__ __forceinline__ void iso_gather_lambda(const double *__restrict__ lams, const double *__restrict__ ab,
+                                                  const IsoArgs &A, int x, int y, int z, double lo[NQ])
+{
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        lo[q] = gather_one<q>(lams, ab, A, x, y, z);
+    });
+}
+
+// one adjoint step at one cell: lambda_{n+1} (read directly from lin for the
+// last step, else gathered from the previous pass's output) through the
+// transpose of collide + partial bounce-back.  Per bounce pair (q, qbar
+// adjacent in the ordering) the pulled populations' lambda is stored as soon
+// as the pair's lambda_{n+1} arrive: (1 - w)(1 - n_s) lambda_q + n_s
+// lambda_qbar.  The terms shared by all q (through rho, u and the
+// Q-correction) are left to the next gather, as four per-cell coefficients
+// in bu (base, ms ubar / rho), together with dJ/d(dQ) times dt w_corr
+// (dqbar) and u of the pulled state (uo); dJ/dn_s is accumulated and dJ/dw
+// summed per block.
+template <bool GATHER>
+__global__ void __launch_bounds__(256) k_iso_adj(
+    const double *__restrict__ f, const double *__restrict__ Q, const double *__restrict__ lin,
+    const double *__restrict__ ab, const double *__restrict__ ns, double *__restrict__ lams,
+    double *__restrict__ dqbar, double *__restrict__ uo, double *__restrict__ bu, double *__restrict__ dns,
+    double *__restrict__ part, IsoArgs A)
 {
     __shared__ double sh[256];
     int x, y, z;
@@ -242,11 +323,10 @@ __global__ void __launch_bounds__(256) k_iso_adj_a(
         double P[3];
         for (int a = 0; a < 3; ++a) P[a] = u[a] * u[a] + A.RT + A.dt * wc * dq[a];
         const double s = ns[cell];
-        double lo[NQ];
-        for (int q = 0; q < NQ; ++q) lo[q] = lam_in[int64_t(q) * A.N + cell];
+        const double keep = (1.0 - w) * (1.0 - s);
         double rbar = 0.0, ubar[3] = {0.0, 0.0, 0.0}, pbar[3] = {0.0, 0.0, 0.0}, gns = 0.0;
-        double ls[NQ];
-        static_for<NQ>([&](auto qc) {
+        // contribution of one population, given its lambda_{n+1}
+        auto one = [&](auto qc, double lq) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
             const double px = iso_phi(cx, u[0], P[0]);
@@ -254,24 +334,47 @@ __global__ void __launch_bounds__(256) k_iso_adj_a(
             const double pz = iso_phi(cz, u[2], P[2]);
             const double feq = ((rho * px) * py) * pz;
             const double fc = fs[q] + w * (feq - fs[q]);
-            const double lc = (1.0 - s) * lo[q];
-            gns += lo[q] * (fs[kBounce[q]] - fc);
+            const double lc = (1.0 - s) * lq;
+            gns += lq * (fs[kBounce[q]] - fc);
             gw += lc * (feq - fs[q]);
             const double mu = w * lc;
             rbar += mu * (px * py * pz);
-            const double gx = mu * rho * py * pz, gy = mu * rho * px * pz, gz = mu * rho * px * py;
-            ubar[0] += gx * (cx * 0.5);
-            ubar[1] += gy * (cy * 0.5);
-            ubar[2] += gz * (cz * 0.5);
-            pbar[0] += gx * ((cx != 0 ? 1.5 : 0.0) - 1.0);
-            pbar[1] += gy * ((cy != 0 ? 1.5 : 0.0) - 1.0);
-            pbar[2] += gz * ((cz != 0 ? 1.5 : 0.0) - 1.0);
-            ls[q] = (1.0 - w) * lc;
-        });
-        // partial bounce-back's reflected term: lambda*_q += n_s lambda_qbar
-        static_for<NQ>([&](auto qc) {
+            const double mr = mu * rho;
+            const double g[3] = {mr * (py * pz), mr * (px * pz), mr * (px * py)};
+            // dphi/du = c / 2, dphi/dP = 1/2 (c != 0) or -1 (c = 0)
+            static_for<3>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                constexpr int c = ev(q, a);
+                if constexpr (c != 0) {
+                    const double h = 0.5 * g[a];
+                    ubar[a] += c > 0 ? h : -h;
+                    pbar[a] += h;
+                } else {
+                    pbar[a] -= g[a];
+                }
+            });
+        };
+        auto lam_in = [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            ls[q] += s * lo[kBounce[q]];
+            if constexpr (GATHER)
+                return gather_one<q>(lin, ab, A, x, y, z);
+            else
+                return lin[int64_t(q) * A.N + cell];
+        };
+        {
+            const double l0 = lam_in(std::integral_constant<int, 0>{});
+            one(std::integral_constant<int, 0>{}, l0);
+            lams[cell] = keep * l0 + s * l0;
+        }
+        static_for<(NQ - 1) / 2>([&](auto pc) {
+            constexpr int q = 2 * decltype(pc)::value + 1, qb = q + 1;
+            static_assert(kBounce[q] == qb, "bounce pairs are adjacent");
+            const double lq = lam_in(std::integral_constant<int, q>{});
+            const double lb = lam_in(std::integral_constant<int, qb>{});
+            one(std::integral_constant<int, q>{}, lq);
+            one(std::integral_constant<int, qb>{}, lb);
+            lams[int64_t(q) * A.N + cell] = keep * lq + s * lb;
+            lams[int64_t(qb) * A.N + cell] = keep * lb + s * lq;
         });
         double wcbar = 0.0;
         for (int a = 0; a < 3; ++a) {
@@ -281,18 +384,16 @@ __global__ void __launch_bounds__(256) k_iso_adj_a(
         }
         rbar += wcbar * (-wc / rho);
         gw += wcbar * (-1.0 / (w * w * rho));
-        static_for<NQ>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            double v = ls[q] + rbar;
-            v += ubar[0] * (A.ms * ev(q, 0) - u[0]) / rho;
-            v += ubar[1] * (A.ms * ev(q, 1) - u[1]) / rho;
-            v += ubar[2] * (A.ms * ev(q, 2) - u[2]) / rho;
-            lams[int64_t(q) * A.N + cell] = v;
-        });
-        ru[cell] = rho;
-        ru[A.N + cell] = u[0];
-        ru[2 * A.N + cell] = u[1];
-        ru[3 * A.N + cell] = u[2];
+        // du_a/df*_q = (ms c_qa - u_a) / rho: a shared part and a signed sum
+        // of ms ubar_a / rho over the components of c_q
+        double base = rbar;
+        for (int a = 0; a < 3; ++a) {
+            const double ub = ubar[a] / rho;
+            base -= ub * u[a];
+            bu[(a + 1) * A.N + cell] = ub * A.ms;
+            uo[a * A.N + cell] = u[a];
+        }
+        bu[cell] = base;
         dns[cell] += gns;
     }
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
@@ -305,77 +406,50 @@ __global__ void __launch_bounds__(256) k_iso_adj_a(
     if (t == 0) part[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] += sh[0];
 }
 
-// pass b: transpose of dQ (gather) and of Q(rho, u)
-__global__ void __launch_bounds__(256) k_iso_adj_b(double *__restrict__ lams, const double *__restrict__ dqbar,
-                                                   const double *__restrict__ ru, IsoArgs A)
+// transpose of dQ (a 3-point gather of dqbar along each axis) and of
+// Q_a(rho, u) = rho u_a (k - u_a^2), whose derivative with respect to the
+// pulled population q is 2 u_a^3 + (k - 3 u_a^2) ms c_qa.  Stores the
+// per-cell coefficients of the term that the next gather adds to every
+// pulled population's lambda, alpha + beta . c_q, with alpha = base +
+// sum_a Qbar_a 2 u_a^3 and beta_a = ms ubar_a / rho + Qbar_a (k - 3 u_a^2) ms
+// (base and ms ubar / rho from bu).
+__global__ void __launch_bounds__(256) k_iso_adj_q(const double *__restrict__ dqbar, const double *__restrict__ uo,
+                                                   const double *__restrict__ bu, double *__restrict__ ab, IsoArgs A)
 {
     int x, y, z;
     int64_t cell;
     if (!iso_cell(A, x, y, z, cell)) return;
     const int pos[3] = {x, y, z}, n[3] = {A.nx, A.ny, A.nz};
     const int64_t st[3] = {1, A.nx, int64_t(A.nx) * A.ny};
-    double qbar[3];
+    double alpha = bu[cell];
     for (int a = 0; a < 3; ++a) {
         const double *g = dqbar + a * A.N;
         const bool p_ok = pos[a] + 1 < n[a], m_ok = pos[a] - 1 >= 0;
-        // own gradient: vc = -Q (only p), +Q (only m), 0 otherwise
-        double v = (p_ok && !m_ok) ? -g[cell] : ((m_ok && !p_ok) ? g[cell] : 0.0);
-        // the neighbour below uses this cell as its p
-        if (m_ok) {
-            const bool mm_ok = pos[a] - 2 >= 0;
-            v += (mm_ok ? 0.5 : 1.0) * g[cell - st[a]];
-        }
-        // the neighbour above uses this cell as its m
-        if (p_ok) {
-            const bool pp_ok = pos[a] + 2 < n[a];
-            v -= (pp_ok ? 0.5 : 1.0) * g[cell + st[a]];
-        }
-        qbar[a] = v;  // idx = 1 (unit cells)
+        const double gc = g[cell];
+        const double gm = g[cell - (m_ok ? st[a] : 0)];
+        const double gp = g[cell + (p_ok ? st[a] : 0)];
+        // own gradient: -Q (only p), +Q (only m), 0 otherwise
+        double v = (p_ok && !m_ok) ? -gc : ((m_ok && !p_ok) ? gc : 0.0);
+        // the neighbour below uses this cell as its p, the one above as its m
+        if (m_ok) v += (pos[a] - 2 >= 0 ? 0.5 : 1.0) * gm;
+        if (p_ok) v -= (pos[a] + 2 < n[a] ? 0.5 : 1.0) * gp;
+        const double ua = uo[a * A.N + cell];
+        alpha += v * (2.0 * ua * ua * ua);
+        ab[(a + 1) * A.N + cell] = bu[(a + 1) * A.N + cell] + v * ((A.k - 3.0 * ua * ua) * A.ms);
     }
-    const double u[3] = {ru[A.N + cell], ru[2 * A.N + cell], ru[3 * A.N + cell]};
-    double cq[3], du[3];
-    for (int a = 0; a < 3; ++a) {
-        // dQ_a/df_q = u k - u^3 + (k - 3u^2)(ms c - u)
-        cq[a] = u[a] * A.k - u[a] * u[a] * u[a] - (A.k - 3.0 * u[a] * u[a]) * u[a];
-        du[a] = (A.k - 3.0 * u[a] * u[a]) * A.ms;
-    }
-    static_for<NQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        double v = 0.0;
-        v += qbar[0] * (cq[0] + du[0] * ev(q, 0));
-        v += qbar[1] * (cq[1] + du[1] * ev(q, 1));
-        v += qbar[2] * (cq[2] + du[2] * ev(q, 2));
-        lams[int64_t(q) * A.N + cell] += v;
-    });
+    ab[cell] = alpha;
 }
 
-// pass c: transpose of the pull
-__global__ void __launch_bounds__(256) k_iso_adj_c(const double *__restrict__ lams, double *__restrict__ lam_out,
-                                                   IsoArgs A)
+// dJ/du0 from lambda_0 (gathered) through the initial equilibrium
+__global__ void __launch_bounds__(256) k_iso_u0grad(const double *__restrict__ lams, const double *__restrict__ ab,
+                                                    const double *__restrict__ u0, double *__restrict__ du0, IsoArgs A,
+                                                    double rho0)
 {
     int x, y, z;
     int64_t cell;
     if (!iso_cell(A, x, y, z, cell)) return;
-    static_for<NQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-        const int ty = wrap(y + cy, A.ny), tz = wrap(z + cz, A.nz);
-        const int tx = x + cx;
-        double v = 0.0;
-        if (tx >= 0 && tx < A.nx) v += lams[int64_t(q) * A.N + (int64_t(tz) * A.ny + ty) * A.nx + tx];
-        if (cx == -1 && x == A.nx - 1)  // read again through the outflow ghost
-            v += lams[int64_t(q) * A.N + (int64_t(tz) * A.ny + ty) * A.nx + x];
-        lam_out[int64_t(q) * A.N + cell] = v;
-    });
-}
-
-// dJ/du0 from lambda_0 through the initial equilibrium
-__global__ void __launch_bounds__(256) k_iso_u0grad(const double *__restrict__ lam, const double *__restrict__ u0,
-                                                    double *__restrict__ du0, IsoArgs A, double rho0)
-{
-    int x, y, z;
-    int64_t cell;
-    if (!iso_cell(A, x, y, z, cell)) return;
+    double lo[NQ];
+    iso_gather_lambda(lams, ab, A, x, y, z, lo);
     const double u[3] = {u0[cell], u0[A.N + cell], u0[2 * A.N + cell]};
     const double P[3] = {u[0] * u[0] + A.RT, u[1] * u[1] + A.RT, u[2] * u[2] + A.RT};
     double g[3] = {0.0, 0.0, 0.0};
@@ -383,7 +457,7 @@ __global__ void __launch_bounds__(256) k_iso_u0grad(const double *__restrict__ l
         constexpr int q = decltype(qc)::value;
         constexpr int c[3] = {ev(q, 0), ev(q, 1), ev(q, 2)};
         const double ph[3] = {iso_phi(c[0], u[0], P[0]), iso_phi(c[1], u[1], P[1]), iso_phi(c[2], u[2], P[2])};
-        const double l = lam[int64_t(q) * A.N + cell] * rho0;
+        const double l = lo[q] * rho0;
         for (int a = 0; a < 3; ++a) {
             const double dphi = c[a] * 0.5 + ((c[a] != 0 ? 1.5 : 0.0) - 1.0) * 2.0 * u[a];
             g[a] += l * dphi * ph[(a + 1) % 3] * ph[(a + 2) % 3];
@@ -461,10 +535,11 @@ int mrb_iso_create(const int ncell[3], double T0, double rho0, double u_in, mrb_
         MRB_HIP(hipMalloc(&S->f[i], pop));
         MRB_HIP(hipMalloc(&S->lam[i], pop));
     }
-    MRB_HIP(hipMalloc(&S->lams, pop));
     MRB_HIP(hipMalloc(&S->q, sizeof(double) * 3 * size_t(S->N)));
     MRB_HIP(hipMalloc(&S->dqbar, sizeof(double) * 3 * size_t(S->N)));
-    MRB_HIP(hipMalloc(&S->ru, sizeof(double) * 4 * size_t(S->N)));
+    MRB_HIP(hipMalloc(&S->uo, sizeof(double) * 3 * size_t(S->N)));
+    MRB_HIP(hipMalloc(&S->bu, sizeof(double) * 4 * size_t(S->N)));
+    MRB_HIP(hipMalloc(&S->ab, sizeof(double) * 4 * size_t(S->N)));
     {
         IsoArgs A{};
         A.nx = S->n[0];
@@ -488,6 +563,23 @@ static void iso_free_states(mrb_iso *S)
     S->segq.clear();
 }
 
+// nseg checkpoints, every - 1 segment states and every Q arrays
+static int iso_ensure_states(mrb_iso *S, int nseg, int every)
+{
+    if (S->ckpt.size() == size_t(nseg) && S->seg.size() == size_t(every - 1) && S->segq.size() == size_t(every))
+        return MRB_OK;
+    MRB_HIP(hipDeviceSynchronize());
+    iso_free_states(S);
+    const size_t pop = sizeof(double) * NQ * size_t(S->N);
+    S->ckpt.assign(size_t(nseg), nullptr);
+    S->seg.assign(size_t(every - 1), nullptr);
+    S->segq.assign(size_t(every), nullptr);
+    for (auto &p : S->ckpt) MRB_HIP(hipMalloc(&p, pop));
+    for (auto &p : S->seg) MRB_HIP(hipMalloc(&p, pop));
+    for (auto &p : S->segq) MRB_HIP(hipMalloc(&p, sizeof(double) * 3 * size_t(S->N)));
+    return MRB_OK;
+}
+
 int mrb_iso_destroy(mrb_iso *S)
 {
     if (!S) return MRB_OK;
@@ -497,10 +589,11 @@ int mrb_iso_destroy(mrb_iso *S)
         (void)hipFree(S->f[i]);
         (void)hipFree(S->lam[i]);
     }
-    (void)hipFree(S->lams);
     (void)hipFree(S->q);
     (void)hipFree(S->dqbar);
-    (void)hipFree(S->ru);
+    (void)hipFree(S->uo);
+    (void)hipFree(S->bu);
+    (void)hipFree(S->ab);
     (void)hipFree(S->part);
     delete S;
     return MRB_OK;
@@ -531,64 +624,77 @@ int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0,
     if (!S || !ns || !u0 || !J || !dtau || !dns || !du0) return set_error(MRB_E_ARG, "null argument");
     if (!(tau > 0.5) || nsteps < 1) return set_error(MRB_E_ARG, "tau must exceed 1/2 and nsteps be >= 1");
     if (every < 1) every = std::max(1, int(std::ceil(std::sqrt(double(nsteps)))));
+    every = std::min(every, nsteps);
     hipStream_t s = as_stream(stream);
     const IsoArgs A = iso_args(S, tau);
     const dim3 g = iso_grid(A);
     const dim3 b(64, 4, 1);
-    const size_t pop = sizeof(double) * NQ * size_t(S->N);
     const int nseg = (nsteps + every - 1) / every;
-    // checkpoints and one segment of states (sqrt(N) memory)
-    iso_free_states(S);
-    S->ckpt.assign(size_t(nseg), nullptr);
-    S->seg.assign(size_t(every), nullptr);
-    S->segq.assign(size_t(every), nullptr);
-    for (auto &p : S->ckpt) MRB_HIP(hipMalloc(&p, pop));
-    for (auto &p : S->seg) MRB_HIP(hipMalloc(&p, pop));
-    for (auto &p : S->segq) MRB_HIP(hipMalloc(&p, sizeof(double) * 3 * size_t(S->N)));
-    // forward sweep, keeping every `every`-th state
-    k_iso_init<<<g, b, 0, s>>>(S->f[0], u0, A, S->rho0);
-    MRB_HIP(hipGetLastError());
-    int cur = 0;
-    for (int n = 0; n < nsteps; ++n) {
-        if (n % every == 0)
-            MRB_HIP(hipMemcpyAsync(S->ckpt[size_t(n / every)], S->f[cur], pop, hipMemcpyDeviceToDevice, s));
-        const int rc = iso_step(S, A, S->f[cur], S->f[1 - cur], ns, s);
+    const int last0 = (nseg - 1) * every;  // first step of the last segment
+    {
+        const int rc = iso_ensure_states(S, nseg, every);
         if (rc) return rc;
-        cur = 1 - cur;
     }
-    int rc = iso_loss(S, A, S->f[cur], s, J);
+    // state i of segment sg: its checkpoint for i = 0, else a segment buffer
+    auto state = [&](int sg, int i) { return i == 0 ? S->ckpt[size_t(sg)] : S->seg[size_t(i - 1)]; };
+    // forward sweep: checkpoints written in place, the last segment's states
+    // and Q arrays kept, the other states in the ping-pong pair
+    auto fwd_state = [&](int n) {
+        if (n >= last0) return state(nseg - 1, n - last0);
+        if (n % every == 0) return S->ckpt[size_t(n / every)];
+        return S->f[n & 1];
+    };
+    k_iso_init<<<g, b, 0, s>>>(fwd_state(0), u0, A, S->rho0);
+    MRB_HIP(hipGetLastError());
+    for (int n = 0; n < nsteps; ++n) {
+        const double *fn = fwd_state(n);
+        double *fo = n + 1 < nsteps ? fwd_state(n + 1) : S->f[nsteps & 1];
+        double *qn = n >= last0 ? S->segq[size_t(n - last0)] : S->q;
+        k_iso_q<<<g, b, 0, s>>>(fn, qn, A);
+        MRB_HIP(hipGetLastError());
+        k_iso_step<<<g, b, 0, s>>>(fn, fo, qn, ns, A);
+        MRB_HIP(hipGetLastError());
+    }
+    const double *ffin = S->f[nsteps & 1];
+    int rc = iso_loss(S, A, ffin, s, J);
     if (rc) return rc;
-    // reverse sweep
+    // reverse sweep: two passes per step
     MRB_HIP(hipMemsetAsync(dns, 0, sizeof(double) * size_t(S->N), s));
     MRB_HIP(hipMemsetAsync(S->part, 0, sizeof(double) * size_t(S->nparts), s));
-    k_iso_seed<<<g, b, 0, s>>>(S->f[cur], S->lam[0], A);
+    k_iso_seed<<<g, b, 0, s>>>(ffin, S->lam[0], A);
     MRB_HIP(hipGetLastError());
     int lc = 0;
+    bool first = true;
     for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int s0 = sg * every, s1 = std::min(nsteps, s0 + every);
-        // recompute the segment's states and their Q arrays
-        MRB_HIP(hipMemcpyAsync(S->seg[0], S->ckpt[size_t(sg)], pop, hipMemcpyDeviceToDevice, s));
-        for (int i = 0; i < s1 - s0; ++i) {
-            k_iso_q<<<g, b, 0, s>>>(S->seg[size_t(i)], S->segq[size_t(i)], A);
-            MRB_HIP(hipGetLastError());
-            if (i + 1 < s1 - s0) {
-                k_iso_step<<<g, b, 0, s>>>(S->seg[size_t(i)], S->seg[size_t(i + 1)], S->segq[size_t(i)], ns, A);
+        const int s0 = sg * every, len = std::min(nsteps, s0 + every) - s0;
+        if (sg < nseg - 1) {
+            // recompute the segment's states and their Q arrays from its checkpoint
+            for (int i = 0; i < len; ++i) {
+                k_iso_q<<<g, b, 0, s>>>(state(sg, i), S->segq[size_t(i)], A);
                 MRB_HIP(hipGetLastError());
+                if (i + 1 < len) {
+                    k_iso_step<<<g, b, 0, s>>>(state(sg, i), state(sg, i + 1), S->segq[size_t(i)], ns, A);
+                    MRB_HIP(hipGetLastError());
+                }
             }
         }
-        for (int n = s1 - 1; n >= s0; --n) {
-            const double *fn = S->seg[size_t(n - s0)];
-            const double *qn = S->segq[size_t(n - s0)];
-            k_iso_adj_a<<<g, b, 0, s>>>(fn, qn, S->lam[lc], ns, S->lams, S->dqbar, S->ru, dns, S->part, A);
+        for (int i = len - 1; i >= 0; --i) {
+            const double *fn = state(sg, i);
+            const double *qn = S->segq[size_t(i)];
+            if (first)
+                k_iso_adj<false><<<g, b, 0, s>>>(fn, qn, S->lam[lc], S->ab, ns, S->lam[1 - lc], S->dqbar, S->uo,
+                                                  S->bu, dns, S->part, A);
+            else
+                k_iso_adj<true><<<g, b, 0, s>>>(fn, qn, S->lam[lc], S->ab, ns, S->lam[1 - lc], S->dqbar, S->uo,
+                                                 S->bu, dns, S->part, A);
             MRB_HIP(hipGetLastError());
-            k_iso_adj_b<<<g, b, 0, s>>>(S->lams, S->dqbar, S->ru, A);
-            MRB_HIP(hipGetLastError());
-            k_iso_adj_c<<<g, b, 0, s>>>(S->lams, S->lam[1 - lc], A);
+            k_iso_adj_q<<<g, b, 0, s>>>(S->dqbar, S->uo, S->bu, S->ab, A);
             MRB_HIP(hipGetLastError());
             lc = 1 - lc;
+            first = false;
         }
     }
-    k_iso_u0grad<<<g, b, 0, s>>>(S->lam[lc], u0, du0, A, S->rho0);
+    k_iso_u0grad<<<g, b, 0, s>>>(S->lam[lc], S->ab, u0, du0, A, S->rho0);
     MRB_HIP(hipGetLastError());
     std::vector<double> part(size_t(S->nparts));
     MRB_HIP(hipMemcpyAsync(part.data(), S->part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
@@ -596,7 +702,6 @@ int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0,
     double gw = 0.0;
     for (double v : part) gw += v;
     *dtau = gw * (-1.0 / (tau * tau));  // w = 1/tau
-    iso_free_states(S);
     return MRB_OK;
 }
 

@@ -113,7 +113,7 @@ def measured_traffic(storage: str, kernels) -> float | None:
         data = json.loads(block)[storage]["kernels"]
     except (OSError, AttributeError, KeyError, ValueError):
         return None
-    elem = {"f64": "double", "f32s": "float"}[storage]
+    elem = {"f64": "double", "f32s": "float", "f16s": "_Float16"}[storage]
     total = 0.0
     for k in kernels:
         rec = data.get(f"{k}<{elem}>")
@@ -258,21 +258,107 @@ def run_adjoint(args, torch) -> int:
     return 0
 
 
+def run_precision(args, torch) -> int:
+    """Precision study: periodic n^3 box (default 512^3) with a seeded
+    random-phase divergence-free velocity field (E(k) ~ k^-5/3, k <= 32,
+    max |u| = 0.04), T0 = 0.2, run for the given steps with FP64,
+    shifted-FP32 and scaled-FP16 population storage (FP64 arithmetic in
+    all); velocity, density and kinetic energy of each compared with the
+    FP64 run on the device (relative L2 and max errors).  tau = 0.55: at
+    tau = 0.52 the thermal f+g model diverges on this field at 512^3
+    between steps 200 and 400 (finite at 200, NaN at 400); tau = 0.55 stays
+    finite through 2000 steps."""
+    from marbles_amd import configs
+    from marbles_amd.level import Level
+
+    n = args.n if "--n" in sys.argv else 512
+    nsteps = args.steps
+    tau = 0.55
+    cfg = configs.spectral_turbulence(n, tau=tau)
+    t0 = time.perf_counter()
+    fields = configs.compact_fields(cfg)
+    log(f"spectral field {n}^3 built in {time.perf_counter() - t0:.1f}s")
+    sh = torch.cuda.current_stream().cuda_stream
+    N = n ** 3
+    ref = None
+    legs = {}
+    chunk = 100
+    for st in ("f64", "f32s", "f16s"):
+        t0 = time.perf_counter()
+        lvl = Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], layout="compact", storage=st)
+        lvl.set_phys(cfg["nu"], cfg.get("alpha"))
+        lvl.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+        lvl.flags_all_regular()
+        lvl.init_fields(*fields, adiabatic_exponent=cfg["ic"]["adiabatic_exponent"])
+        torch.cuda.synchronize()
+        log(f"{st}: level initialised in {time.perf_counter() - t0:.1f}s")
+        lvl.advance(args.warmup, stream=sh)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            lvl.advance(k, stream=sh)
+            torch.cuda.synchronize()
+            done += k
+            log(f"{st}: {done}/{nsteps} steps, {time.perf_counter() - t0:.1f}s")
+        secs = time.perf_counter() - t0
+        mac = torch.empty(5 * N, dtype=torch.float64, device="cuda")
+        lvl.macro_fields(mac, stream=sh)
+        cons = lvl.conservation(stream=sh)
+        sat = lvl.storage_saturations()
+        lvl.close()
+        rho, u = mac[:N], mac[N:4 * N].view(3, N)
+        ke = float(0.5 * (rho * (u * u).sum(0)).sum())
+        leg = {"mlups": round(N * nsteps / secs / 1e6, 1), "seconds": round(secs, 2), "kinetic_energy": ke,
+               "mass": float(cons[0]), "energy_2rhoE": float(cons[4])}
+        if st == "f64":
+            ref = (rho, u, ke, cons)
+        else:
+            r0, u0, ke0, c0 = ref
+            du = u - u0
+            leg.update({
+                "velocity_rel_l2": float(du.norm() / u0.norm()),
+                "velocity_rel_max": float(du.abs().max() / u0.abs().max()),
+                "density_rel_l2": float((rho - r0).norm() / r0.norm()),
+                "density_rel_max": float((rho - r0).abs().max() / r0.abs().max()),
+                "kinetic_energy_rel": abs(ke - ke0) / ke0,
+                "mass_rel": abs(float(cons[0]) - float(c0[0])) / abs(float(c0[0])),
+                "saturated_values": sat,
+            })
+        legs[st] = leg
+        log(f"{st}: " + ", ".join(f"{k}={v:.3g}" if isinstance(v, float) else f"{k}={v}" for k, v in leg.items()))
+        del mac
+        torch.cuda.empty_cache()
+    print(json.dumps({
+        "metric": "D3Q27 MLUPS per population storage with errors against FP64 (precision study)",
+        "value": legs["f32s"]["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": nsteps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * legs["f32s"]["seconds"] / nsteps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"spectral_{n}: periodic {n}^3, random-phase solenoidal field E(k)~k^-5/3 "
+                               f"(k<=32), max|u|=0.04, tau {tau}, T0 0.2, thermal D3Q27 f+g; value = "
+                               f"shifted-FP32 MLUPS",
+                   "storages": legs},
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--storage", default="f64", choices=["f64", "f32s"])
+    ap.add_argument("--storage", default="f64", choices=["f64", "f32s", "f16s"])
     ap.add_argument("--n", type=int, default=256, help="domain edge (256 = the named workload)")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle timing")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-f32", action="store_true", help="skip the shifted-FP32 comparison run")
-    ap.add_argument("--workload", default="channel", choices=["channel", "amr", "adjoint"],
+    ap.add_argument("--no-f32", action="store_true", help="skip the shifted-FP32 and scaled-FP16 comparison runs")
+    ap.add_argument("--workload", default="channel", choices=["channel", "amr", "adjoint", "precision"],
                     help="channel: the single-level 256^3 channel (default); amr: the two-level workload; "
-                         "adjoint: gradient cost and finite-difference check at 128^3")
+                         "adjoint: gradient cost and finite-difference check at 128^3; precision: FP64, "
+                         "shifted FP32 and scaled FP16 storage on a 512^3 spectral field, errors vs FP64")
     ap.add_argument("--graph", action="store_true", help="amr: replay captured hipGraphs")
     args = ap.parse_args()
 
@@ -288,6 +374,12 @@ def main() -> int:
     if not torch.cuda.is_available():
         log("bench.py: no GPU visible")
         return 1
+    if args.workload == "precision":
+        if "--steps" not in sys.argv:
+            args.steps = 2000
+        if "--warmup" not in sys.argv:
+            args.warmup = 0
+        return run_precision(args, torch)
     if args.workload == "adjoint":
         if "--steps" not in sys.argv:
             args.steps = 200
@@ -322,39 +414,43 @@ def main() -> int:
     cons = lvl.conservation(stream=sh)
     log(f"conservation (mass, mom x/y/z, 2rhoE): {cons.tolist()}")
 
-    # the same run with shifted FP32 populations, compared at equal step count
-    f32 = None
+    # the same run with shifted FP32 and with scaled FP16 populations,
+    # compared at equal step count
+    legs = {"f32s": None, "f16s": None}
     if args.storage == "f64" and not args.no_f32:
-        (u64, rho64) = velocity_density(lvl, cfg)
-        lvl.close()
-        l32 = configs.build_level(cfg, layout="compact", storage="f32s")
-        r32 = time_level(l32, args.steps, args.warmup, args.profile_steps, sh, torch)
-        (u32, rho32) = velocity_density(l32, cfg)
-        l32.close()
         import numpy as np
 
+        (u64, rho64) = velocity_density(lvl, cfg)
+        lvl.close()
         fluid = rho64 > 0
         umax = max(float(np.abs(c[fluid]).max()) for c in u64)
-        du = max(float(np.abs(a[fluid] - b[fluid]).max()) for a, b in zip(u32, u64)) / umax
-        dr = float(np.abs(rho32[fluid] - rho64[fluid]).max() / np.abs(rho64[fluid]).max())
-        k32 = sum(r32["per_kernel_ms"].values())
-        m32 = cells * args.steps / r32["secs"] / 1e6
-        b32 = configs.bytes_per_update("f32s", 2)
-        f32 = {
-            "value": round(m32, 2),
-            "unit": "MLUPS",
-            "speedup_vs_f64": round(m32 / mlups, 3),
-            "max_rel_velocity_error": du,
-            "max_rel_density_error": dr,
-            "compared_after_steps": args.warmup + args.steps + args.profile_steps,
-            "arithmetic": "f64",
-            "bytes_per_update": b32,
-            "achieved_gbs": round(b32 * cells / (k32 * 1e-3) / 1e9, 1),
-            "traffic": measured_traffic("f32s", r32["per_kernel_ms"]) if n == 256 else None,
-            "kernel_ms_per_step": {k: round(v, 5) for k, v in r32["per_kernel_ms"].items()},
-        }
-        log(f"f32-shifted: {m32:.1f} MLUPS ({m32 / mlups:.2f}x), max rel velocity error {du:.2e}, "
-            f"density {dr:.2e}")
+        for st in legs:
+            lr = configs.build_level(cfg, layout="compact", storage=st)
+            rr = time_level(lr, args.steps, args.warmup, args.profile_steps, sh, torch)
+            (ur, rhor) = velocity_density(lr, cfg)
+            sat = lr.storage_saturations()
+            lr.close()
+            du = max(float(np.abs(a[fluid] - b[fluid]).max()) for a, b in zip(ur, u64)) / umax
+            dr = float(np.abs(rhor[fluid] - rho64[fluid]).max() / np.abs(rho64[fluid]).max())
+            kms = sum(rr["per_kernel_ms"].values())
+            mr = cells * args.steps / rr["secs"] / 1e6
+            br = configs.bytes_per_update(st, 2)
+            legs[st] = {
+                "value": round(mr, 2),
+                "unit": "MLUPS",
+                "speedup_vs_f64": round(mr / mlups, 3),
+                "max_rel_velocity_error": du,
+                "max_rel_density_error": dr,
+                "compared_after_steps": args.warmup + args.steps + args.profile_steps,
+                "arithmetic": "f64",
+                "bytes_per_update": br,
+                "achieved_gbs": round(br * cells / (kms * 1e-3) / 1e9, 1),
+                "traffic": measured_traffic(st, rr["per_kernel_ms"]) if n == 256 else None,
+                "kernel_ms_per_step": {k: round(v, 5) for k, v in rr["per_kernel_ms"].items()},
+            }
+            if st == "f16s":
+                legs[st]["saturated_values"] = sat
+            log(f"{st}: {mr:.1f} MLUPS ({mr / mlups:.2f}x), max rel velocity error {du:.2e}, density {dr:.2e}")
     else:
         lvl.close()
     copy_gbs = copy_ceiling(torch)
@@ -371,7 +467,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32s": "f32"}[args.storage],
+        "dtype": {"f64": "f64", "f32s": "f32", "f16s": "f16"}[args.storage],
         "data": "synthetic",
         "config": {
             "workload": f"{cfg['name']}: thermal D3Q27 f+g, velocity inlet u=0.05, outflow, periodic y/z, "
@@ -394,7 +490,8 @@ def main() -> int:
             "frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
         },
         "cpu_baseline": None,
-        "f32_shifted": f32,
+        "f32_shifted": legs["f32s"],
+        "f16_scaled": legs["f16s"],
     }
     if not args.no_cpu_baseline:
         try:

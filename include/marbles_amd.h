@@ -43,7 +43,8 @@ extern "C" {
 /* Population storage precision. */
 #define MRB_STORAGE_F64 0         /* FP64 */
 #define MRB_STORAGE_F32_SHIFTED 1 /* FP32 of f - f_eq(rho0, 0, T0) */
-#define MRB_STORAGE_F16_SCALED 2  /* FP16 of (f - shift) / scale (per box) */
+#define MRB_STORAGE_F16_SCALED 2  /* FP16 of (f - shift) * 2^e, e per (box, lattice);
+                                     compact layout, boxes = z-slabs of 8 planes */
 
 /* Boundary types, identical values to Source/BC.H:13-20. */
 #define MRB_BC_PERIODIC 0
@@ -268,6 +269,12 @@ int mrb_iso_gradient(
 /* Final populations of the last mrb_iso_forward (27 N doubles, host). */
 int mrb_iso_state(const mrb_iso *iso, double *host_f);
 
+/* Density, velocity (x, y, z) and temperature of the current state per
+ * cell into the device array out (5 N doubles, field-major; zeros in solid
+ * cells), as LBM::f_to_macrodata (Source/LBM.cpp:810-906) computes them.
+ * Compact layout, every storage.  Asynchronous. */
+int mrb_macro_fields(const mrb_level *lvl, double *out, void *stream);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 const char *mrb_last_error(void);
 /* Name of the kernel variant used by mrb_stream_collide. */
@@ -275,6 +282,15 @@ const char *mrb_hot_kernel_name(const mrb_level *lvl);
 /* Device pointers of the current population buffers (for zero-copy use by
  * the Python/torch wrappers).  lattice 0 = f, 1 = g. */
 int mrb_population_ptr(const mrb_level *lvl, int lattice, void **dev_ptr, int64_t *bytes);
+/* FP16 storage: number of block writes (256 cells, one lattice) since the
+ * level was created in which some value exceeded the FP16 range at its
+ * box's scale and was clamped (the per-box scale leaves a factor 16 of
+ * headroom; 0 for the other storages).  No reference counterpart. */
+int mrb_storage_saturations(const mrb_level *lvl, int64_t *count);
+/* The rest-state values subtracted by the reduced-precision storages, per
+ * direction (zero for FP64), for callers that read the raw buffers of
+ * mrb_population_ptr.  No reference counterpart. */
+int mrb_storage_shift(const mrb_level *lvl, int lattice, double out[27]);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,9 @@ SIGNATURES = {
     "mrb_last_error": (c_char_p, []),
     "mrb_hot_kernel_name": (c_char_p, [c_void_p]),
     "mrb_population_ptr": (c_int, [c_void_p, c_int, POINTER(c_void_p), POINTER(c_int64)]),
+    "mrb_storage_saturations": (c_int, [c_void_p, POINTER(c_int64)]),
+    "mrb_storage_shift": (c_int, [c_void_p, c_int, POINTER(c_double)]),
+    "mrb_macro_fields": (c_int, [c_void_p, c_void_p, c_void_p]),
     "mrb_init_fields": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
                                 c_double]),
     "mrb_amr_link": (c_int, [c_void_p, c_void_p]),

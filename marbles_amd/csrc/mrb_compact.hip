@@ -5,7 +5,8 @@
 // HBM layout: populations SoA, cell = (z*ny + y)*nx + x.  FP64: f and g
 // interleaved by component, pop[(2q + lattice)*N + cell], so f_q =
 // f[q*cs + cell], g_q = g[q*cs + cell] with g = f + N and cs = 2N.  Shifted
-// FP32: pop[(q*N + cell)*2 + lattice], f_q = f[q*cs + 2*cell] with g = f + 1.
+// FP32 and scaled FP16: pop[(q*N + cell)*2 + lattice], f_q = f[q*cs +
+// 2*cell] with g = f + 1 (one 8- or 4-byte access moves f_q and g_q).
 // Two buffers (A/B); a 32-bit stream mask per cell (bit q = the pull
 // source x - c_q is fluid).
 //
@@ -69,13 +70,18 @@ struct ShiftTab {
 // of a cell side by side (pop[(q*N + cell)*2 + lattice]) so that one
 // 8-byte load or store moves both.
 template <class T>
-constexpr int kCellStride = std::is_same<T, float>::value ? 2 : 1;
+constexpr int kCellStride = std::is_same<T, double>::value ? 1 : 2;
+template <class T>
+constexpr bool kIsHalf = std::is_same<T, _Float16>::value;
 
 // store of a cell's f_q and g_q
 template <class T>
 __device__ __forceinline__ void store_pair(T *__restrict__ fo, T *__restrict__ go, int64_t e, T fv, T gv)
 {
-    if constexpr (kCellStride<T> == 2) {
+    if constexpr (kIsHalf<T>) {
+        const uint32_t lo = __builtin_bit_cast(uint16_t, fv), hi = __builtin_bit_cast(uint16_t, gv);
+        *reinterpret_cast<uint32_t *>(fo + e) = lo | (hi << 16);
+    } else if constexpr (kCellStride<T> == 2) {
         *reinterpret_cast<float2 *>(fo + e) = make_float2(fv, gv);
     } else {
         fo[e] = fv;
@@ -83,7 +89,28 @@ __device__ __forceinline__ void store_pair(T *__restrict__ fo, T *__restrict__ g
     }
 }
 
-// storage <-> FP64 working value
+// FP16 storage: scale exponents and box maxima of the read (cur) and the
+// written (next) buffers, per (scale box, lattice); unused by the other
+// storages
+struct Half16 {
+    const int *__restrict__ e_cur;
+    int *__restrict__ e_next;
+    const uint32_t *__restrict__ amax_cur;
+    uint32_t *__restrict__ amax_next;
+    uint32_t *__restrict__ ovf;
+};
+
+// FP16 value of (v - shift) * 2^e: the difference is taken in FP64, then
+// rounded to FP32, scaled there (exact for a power of two) and clamped to
+// the FP16 range; amax tracks |v - shift|
+__device__ __forceinline__ _Float16 to_half(double v, double shift, float scale, float &amax)
+{
+    const float d = float(v - shift);
+    amax = fmaxf(amax, fabsf(d));
+    return _Float16(__builtin_amdgcn_fmed3f(d * scale, -65504.0f, 65504.0f));
+}
+
+// storage <-> FP64 working value (FP64 and shifted FP32 storage)
 template <class T>
 __device__ __forceinline__ double widen(T v, double shift)
 {
@@ -131,7 +158,8 @@ __device__ __forceinline__ void targets3(int x, int n, int periodic, int t[3], b
 template <class T>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
-    const ShiftTab &sh, int x, int y, int z, int64_t cell, uint32_t mask, double fv[NQ], double gv[NQ])
+    const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, double fv[NQ],
+    double gv[NQ])
 {
     int tx[3], ty[3], tz[3];
     bool gx[3], gy[3], gz[3];
@@ -140,7 +168,7 @@ __device__ __forceinline__ void pull_cell(
     targets3(z, A.nz, A.per[2], tz, gz);
     const uint32_t c32 = uint32_t(cell);
     T rf[NQ], rg[NQ];
-    uint32_t cmask = 0;
+    uint32_t cmask = 0, dmask = 0;
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
@@ -159,7 +187,12 @@ __device__ __forceinline__ void pull_cell(
         comp = (ghost && t == MRB_BC_SLIPY) ? reflect(q, 1) : comp;
         comp = (ghost && t == MRB_BC_SLIPZ) ? reflect(q, 2) : comp;
         const int64_t e = int64_t(comp) * A.cs + int64_t(direct ? src : c32) * kCellStride<T>;
-        if constexpr (kCellStride<T> == 2) {
+        dmask |= uint32_t(direct) << q;
+        if constexpr (kIsHalf<T>) {
+            const uint32_t v = *reinterpret_cast<const uint32_t *>(f + e);
+            rf[q] = __builtin_bit_cast(_Float16, uint16_t(v & 0xffffu));
+            rg[q] = __builtin_bit_cast(_Float16, uint16_t(v >> 16));
+        } else if constexpr (kCellStride<T> == 2) {
             // f_q and g_q side by side: one 8-byte load (g == f + 1)
             const float2 v = *reinterpret_cast<const float2 *>(f + e);
             rf[q] = v.x;
@@ -170,12 +203,37 @@ __device__ __forceinline__ void pull_cell(
         }
         cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
     });
-    static_for<NQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const bool use_const = (cmask >> q) & 1u;
-        fv[q] = use_const ? bc.fval[q] : widen(rf[q], sh.f(q));
-        gv[q] = use_const ? bc.gval[q] : widen(rg[q], sh.g(q));
-    });
+    if constexpr (kIsHalf<T>) {
+        // inverse scales of the source planes z - c_z (index c_z + 1); a
+        // source that is not the pull's (bounce-back, wall ghosts) is the
+        // cell itself, in plane z (index 1).  Wave-uniform values.
+        float invf[3], invg[3];
+        for (int k = 0; k < 3; ++k) {
+            const int b = tz[k] / kScaleSlab;
+            invf[k] = ldexpf(1.0f, -h16.e_cur[2 * b]);
+            invg[k] = ldexpf(1.0f, -h16.e_cur[2 * b + 1]);
+        }
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int k = ev(q, 2) + 1;
+            const bool use_const = (cmask >> q) & 1u;
+            const bool direct = (dmask >> q) & 1u;
+            const float sf = direct ? invf[k] : invf[1], sg = direct ? invg[k] : invg[1];
+            // unscaling in FP32 is exact (a power of two on an FP16 value)
+            const double vf = double(float(rf[q]) * sf) + sh.f(q);
+            const double vg = double(float(rg[q]) * sg) + sh.g(q);
+            fv[q] = use_const ? bc.fval[q] : vf;
+            gv[q] = use_const ? bc.gval[q] : vg;
+        });
+    } else {
+        (void)dmask;
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const bool use_const = (cmask >> q) & 1u;
+            fv[q] = use_const ? bc.fval[q] : widen(rf[q], sh.f(q));
+            gv[q] = use_const ? bc.gval[q] : widen(rg[q], sh.g(q));
+        });
+    }
 }
 
 __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y, int &z, int64_t &cell)
@@ -188,19 +246,28 @@ __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y
     return true;
 }
 
-// pass 1: Q_corr of the post-stream state
+// pass 1: Q_corr of the post-stream state.  With FP16 storage it also
+// clears the box maxima that pass 2 accumulates for the buffer it writes.
 template <class T>
 __global__ void __launch_bounds__(256) k_qpass(
     const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask,
-    double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, CollideParams P)
+    double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, Half16 h16, CollideParams P)
 {
+    if constexpr (kIsHalf<T>) {
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0 &&
+            blockIdx.z % kScaleSlab == 0) {
+            const int b = int(blockIdx.z) / kScaleSlab;
+            h16.amax_next[2 * b] = 0u;
+            h16.amax_next[2 * b + 1] = 0u;
+        }
+    }
     int x, y, z;
     int64_t cell;
     if (!cell_coords(A, x, y, z, cell)) return;
     const uint32_t m = mask[cell];
     if (!(m & 1u)) return;
     double fv[NQ], gv[NQ];
-    pull_cell(f, g, A, bc, sh, x, y, z, cell, m, fv, gv);
+    pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
     double rho, jx, jy, jz, E;
     accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
     const Macro M = macro_from(rho, jx, jy, jz, E, P);
@@ -209,47 +276,103 @@ __global__ void __launch_bounds__(256) k_qpass(
     Q[2 * A.N + cell] = M.Qz;
 }
 
+// FP16 storage: the block's largest |population - shift| per lattice goes
+// to the written buffer's box entries (every block lies in one z-plane, so
+// in one scale box); a block whose largest value exceeded the FP16 range
+// at its scale (so that some value was clamped) is counted; the box's new
+// exponent is stored by each of its blocks (the same value)
+__device__ __forceinline__ void half_block_stats(const Half16 &h16, int ef, int eg, float mf, float mg)
+{
+    __shared__ float smf[4], smg[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        mf = fmaxf(mf, __shfl_xor(mf, o));
+        mg = fmaxf(mg, __shfl_xor(mg, o));
+    }
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    const int wv = t / 64;
+    if ((t & 63) == 0) {
+        smf[wv] = mf;
+        smg[wv] = mg;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int nw = int(blockDim.x * blockDim.y) / 64;
+        for (int i = 1; i < nw; ++i) {
+            mf = fmaxf(mf, smf[i]);
+            mg = fmaxf(mg, smg[i]);
+        }
+        const int b = int(blockIdx.z) / kScaleSlab;
+        atomicMax(&h16.amax_next[2 * b], __float_as_uint(mf));
+        atomicMax(&h16.amax_next[2 * b + 1], __float_as_uint(mg));
+        const uint32_t nsat = uint32_t(ldexpf(mf, ef) > 65504.0f) + uint32_t(ldexpf(mg, eg) > 65504.0f);
+        if (nsat) atomicAdd(h16.ovf, nsat);
+        h16.e_next[2 * b] = ef;
+        h16.e_next[2 * b + 1] = eg;
+    }
+}
+
 // pass 2: stream + collide, writes the B buffers
 template <class T>
 __device__ __forceinline__ void collide_body(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
-    const ShiftTab &sh, const CollideParams &P)
+    const ShiftTab &sh, const Half16 &h16, const CollideParams &P)
 {
     int x, y, z;
     int64_t cell;
-    if (!cell_coords(A, x, y, z, cell)) return;
-    const uint32_t m = mask[cell];
-    if (!(m & 1u)) {
-        // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
+    const bool in = cell_coords(A, x, y, z, cell);
+    // FP16: scales of the written buffer from the read buffer's box maxima
+    int ef = 0, eg = 0;
+    float scf = 1.0f, scg = 1.0f;
+    float mf = 0.0f, mg = 0.0f;
+    if constexpr (kIsHalf<T>) {
+        const int b = int(blockIdx.z) / kScaleSlab;
+        ef = f16_exponent(__uint_as_float(h16.amax_cur[2 * b]));
+        eg = f16_exponent(__uint_as_float(h16.amax_cur[2 * b + 1]));
+        scf = ldexpf(1.0f, ef);
+        scg = ldexpf(1.0f, eg);
+    }
+    auto put = [&](auto qc, double fnew, double gnew) {
+        constexpr int q = decltype(qc)::value;
+        const int64_t e = int64_t(q) * A.cs + cell * kCellStride<T>;
+        if constexpr (kIsHalf<T>) {
+            store_pair(fo, go, e, to_half(fnew, sh.f(q), scf, mf), to_half(gnew, sh.g(q), scg, mg));
+        } else {
+            store_pair(fo, go, e, narrow<T>(fnew, sh.f(q)), narrow<T>(gnew, sh.g(q)));
+        }
+    };
+    const uint32_t m = in ? mask[cell] : 0u;
+    if (in && !(m & 1u)) {
+        // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565); FP16
+        // storage holds 0 there (downloads return the sentinel)
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(-1.0, sh.f(q)),
-                       narrow<T>(-1.0, sh.g(q)));
+            const int64_t e = int64_t(q) * A.cs + cell * kCellStride<T>;
+            if constexpr (kIsHalf<T>)
+                store_pair(fo, go, e, T(0.0f), T(0.0f));
+            else
+                store_pair(fo, go, e, narrow<T>(-1.0, sh.f(q)), narrow<T>(-1.0, sh.g(q)));
         });
-        return;
+    } else if (in) {
+        double fv[NQ], gv[NQ];
+        pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
+        Moments mo;
+        accumulate_moments(fv, gv, mo);
+        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        // gradient(): neighbour inside the domain box and fluid
+        const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+        const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
+        const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
+        const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
+        const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0,
+                                              P.inv_dx[0]);
+        const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
+                                              ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
+        const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
+                                              zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
+        collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) { put(qc, fnew, gnew); });
     }
-    double fv[NQ], gv[NQ];
-    pull_cell(f, g, A, bc, sh, x, y, z, cell, m, fv, gv);
-    Moments mo;
-    accumulate_moments(fv, gv, mo);
-    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
-    // gradient(): neighbour inside the domain box and fluid
-    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
-    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
-    const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
-    const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
-    const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0,
-                                          P.inv_dx[0]);
-    const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
-                                          ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
-    const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
-                                          zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
-    collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
-        constexpr int q = decltype(qc)::value;
-        store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(fnew, sh.f(q)),
-                   narrow<T>(gnew, sh.g(q)));
-    });
+    if constexpr (kIsHalf<T>) half_block_stats(h16, ef, eg, mf, mg);
 }
 
 // register budget as the compiler chooses (2 waves per SIMD)
@@ -257,9 +380,9 @@ template <class T>
 __global__ void __launch_bounds__(256) k_collide(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
-    CollideParams P)
+    Half16 h16, CollideParams P)
 {
-    collide_body(f, g, fo, go, mask, Q, A, bc, sh, P);
+    collide_body(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
 // at most 168 registers (3 waves per SIMD), some values spilled to scratch
@@ -267,12 +390,13 @@ template <class T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_collide_w3(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
-    CollideParams P)
+    Half16 h16, CollideParams P)
 {
-    collide_body(f, g, fo, go, mask, Q, A, bc, sh, P);
+    collide_body(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
-// stream only (parity of the pull with the reference push)
+// stream only (parity of the pull with the reference push); FP64 and
+// shifted FP32 storage
 template <class T>
 __global__ void __launch_bounds__(256) k_stream_only(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
@@ -284,7 +408,7 @@ __global__ void __launch_bounds__(256) k_stream_only(
     const uint32_t m = mask[cell];
     double fv[NQ], gv[NQ];
     if (m & 1u) {
-        pull_cell(f, g, A, bc, sh, x, y, z, cell, m, fv, gv);
+        pull_cell(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
     } else {
         for (int q = 0; q < NQ; ++q) fv[q] = gv[q] = -1.0;
     }
@@ -363,24 +487,39 @@ static ShiftTab shift_tab(const mrb_level *L)
     return t;
 }
 
+static Half16 half16_of(mrb_level *L)
+{
+    Half16 h{};
+    if (L->storage == MRB_STORAGE_F16_SCALED) {
+        h.e_cur = L->d_e16[L->cur];
+        h.e_next = L->d_e16[1 - L->cur];
+        h.amax_cur = L->d_amax16[L->cur];
+        h.amax_next = L->d_amax16[1 - L->cur];
+        h.ovf = L->d_ovf16;
+    }
+    return h;
+}
+
 template <class T>
 static int two_pass_t(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
     const CompactArgs A = make_args(L);
     const ShiftTab sh = shift_tab(L);
+    const Half16 h16 = half16_of(L);
     const T *f = (const T *)L->pop[0][L->cur];
     const T *g = (const T *)L->pop[1][L->cur];
     T *fo = (T *)L->pop[0][1 - L->cur];
     T *go = (T *)L->pop[1][1 - L->cur];
     if (ev) MRB_HIP(hipEventRecord(ev[0], s));
-    k_qpass<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, L->cp);
+    k_qpass<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[1], s));
     if (variant == 1)
         k_collide_w3<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh,
-                                                              L->cp);
+                                                              h16, L->cp);
     else
-        k_collide<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, L->cp);
+        k_collide<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
+                                                           L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[2], s));
     return MRB_OK;
@@ -390,6 +529,7 @@ static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
     if (L->storage == MRB_STORAGE_F64) return two_pass_t<double>(L, variant, s, ev);
     if (L->storage == MRB_STORAGE_F32_SHIFTED) return two_pass_t<float>(L, variant, s, ev);
+    if (L->storage == MRB_STORAGE_F16_SCALED) return two_pass_t<_Float16>(L, variant, s, ev);
     return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
 }
 
@@ -460,6 +600,170 @@ int compact_stream_only(mrb_level *L, hipStream_t s)
 const char *compact_kernel_name(const mrb_level *L)
 {
     return compact_variant_name(L->hot);
+}
+
+// ---------------------------------------------------------------------------
+// FP16 storage: FP64 populations (component stride cs) <-> the scaled FP16
+// pair layout of one lattice of the current buffer
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ float block_max(float m)
+{
+    __shared__ float sm[4];
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((t & 63) == 0) sm[t / 64] = m;
+    __syncthreads();
+    if (t == 0)
+        for (int i = 1; i < int(blockDim.x * blockDim.y) / 64; ++i) m = fmaxf(m, sm[i]);
+    return m;
+}
+
+__global__ void k_clear_lattice(uint32_t *__restrict__ a, int n, int lattice)
+{
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) a[2 * i + lattice] = 0u;
+}
+
+// largest |population - shift| over the fluid cells of each scale box
+__global__ void __launch_bounds__(256) k_amax16(
+    const double *__restrict__ src, int64_t cs, const double *__restrict__ shift, const int32_t *__restrict__ isf,
+    uint32_t *__restrict__ amax, int lattice, CompactArgs A)
+{
+    int x, y, z;
+    int64_t cell;
+    float m = 0.0f;
+    if (cell_coords(A, x, y, z, cell) && isf[cell] == 1)
+        for (int q = 0; q < NQ; ++q) m = fmaxf(m, float(fabs(src[q * cs + cell] - shift[q])));
+    m = block_max(m);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        atomicMax(&amax[2 * (int(blockIdx.z) / kScaleSlab) + lattice], __float_as_uint(m));
+}
+
+__global__ void __launch_bounds__(256) k_pack16(
+    const double *__restrict__ src, int64_t cs, const double *__restrict__ shift, const int32_t *__restrict__ isf,
+    const uint32_t *__restrict__ amax, int *__restrict__ e, _Float16 *__restrict__ dst, uint32_t *__restrict__ ovf,
+    int lattice, CompactArgs A)
+{
+    const int b = int(blockIdx.z) / kScaleSlab;
+    const int ee = f16_exponent(__uint_as_float(amax[2 * b + lattice]));
+    if (threadIdx.x == 0 && threadIdx.y == 0) e[2 * b + lattice] = ee;
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const bool solid = isf[cell] != 1;
+    const float scale = ldexpf(1.0f, ee);
+    float mx = 0.0f;
+    for (int q = 0; q < NQ; ++q)
+        dst[(int64_t(q) * A.N + cell) * 2] = solid ? _Float16(0.0f) : to_half(src[q * cs + cell], shift[q], scale, mx);
+    if (ldexpf(mx, ee) > 65504.0f) atomicAdd(ovf, 1u);
+}
+
+__global__ void __launch_bounds__(256) k_unpack16(
+    const _Float16 *__restrict__ src, const double *__restrict__ shift, const int *__restrict__ e,
+    const int32_t *__restrict__ isf, double *__restrict__ dst, int lattice, CompactArgs A)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const bool solid = isf[cell] != 1;
+    const float inv = ldexpf(1.0f, -e[2 * (z / kScaleSlab) + lattice]);
+    for (int q = 0; q < NQ; ++q) {
+        const float h = float(src[(int64_t(q) * A.N + cell) * 2]);
+        dst[int64_t(q) * A.N + cell] = solid ? -1.0 : double(h * inv) + shift[q];
+    }
+}
+
+// density, velocity and temperature of the stored state per cell (the
+// macrodata of LBM::f_to_macrodata, LBM.cpp:810-906, that the precision
+// comparisons use); solid cells get zeros
+template <class T>
+__global__ void __launch_bounds__(256) k_macro(
+    const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask, CompactArgs A,
+    ShiftTab sh, const int *__restrict__ e16, CollideParams P, double *__restrict__ out)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (mask[cell] & 1u) {
+        float sf = 1.0f, sg = 1.0f;
+        if constexpr (kIsHalf<T>) {
+            sf = ldexpf(1.0f, -e16[2 * (z / kScaleSlab)]);
+            sg = ldexpf(1.0f, -e16[2 * (z / kScaleSlab) + 1]);
+        }
+        double fv[NQ], gv[NQ];
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int64_t e = int64_t(q) * A.cs + cell * kCellStride<T>;
+            if constexpr (kIsHalf<T>) {
+                fv[q] = double(float(f[e]) * sf) + sh.f(q);
+                gv[q] = double(float(g[e]) * sg) + sh.g(q);
+            } else {
+                fv[q] = widen(f[e], sh.f(q));
+                gv[q] = widen(g[e], sh.g(q));
+            }
+        });
+        double rho, jx, jy, jz, E;
+        accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
+        const Macro M = macro_from(rho, jx, jy, jz, E, P);
+        o[0] = rho;
+        o[1] = M.u;
+        o[2] = M.v;
+        o[3] = M.w;
+        o[4] = M.T;
+    }
+    for (int k = 0; k < 5; ++k) out[k * A.N + cell] = o[k];
+}
+
+}  // namespace
+
+int compact_macro(const mrb_level *L, double *out, hipStream_t s)
+{
+    const CompactArgs A = make_args(L);
+    const ShiftTab sh = shift_tab(L);
+    const dim3 g = grid_of(A), b(64, 4, 1);
+    if (L->storage == MRB_STORAGE_F64)
+        k_macro<double><<<g, b, 0, s>>>((const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur],
+                                        L->d_mask, A, sh, nullptr, L->cp, out);
+    else if (L->storage == MRB_STORAGE_F32_SHIFTED)
+        k_macro<float><<<g, b, 0, s>>>((const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur],
+                                       L->d_mask, A, sh, nullptr, L->cp, out);
+    else
+        k_macro<_Float16><<<g, b, 0, s>>>((const _Float16 *)L->pop[0][L->cur],
+                                          (const _Float16 *)L->pop[1][L->cur], L->d_mask, A, sh,
+                                          L->d_e16[L->cur], L->cp, out);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+int f16_pack(mrb_level *L, int lattice, const double *src, int64_t src_cs, hipStream_t s)
+{
+    if (L->storage != MRB_STORAGE_F16_SCALED) return set_error(MRB_E_STATE, "not an FP16 level");
+    const CompactArgs A = make_args(L);
+    uint32_t *amax = L->d_amax16[L->cur];
+    const double *shift = L->d_small + lattice * NQ;
+    k_clear_lattice<<<(L->nsb16 + 255) / 256, 256, 0, s>>>(amax, L->nsb16, lattice);
+    MRB_HIP(hipGetLastError());
+    k_amax16<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(src, src_cs, shift, L->d_isf, amax, lattice, A);
+    MRB_HIP(hipGetLastError());
+    k_pack16<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(src, src_cs, shift, L->d_isf, amax, L->d_e16[L->cur],
+                                                    static_cast<_Float16 *>(L->pop[lattice][L->cur]), L->d_ovf16,
+                                                    lattice, A);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+int f16_unpack(const mrb_level *L, int lattice, double *dst, hipStream_t s)
+{
+    if (L->storage != MRB_STORAGE_F16_SCALED) return set_error(MRB_E_STATE, "not an FP16 level");
+    const CompactArgs A = make_args(L);
+    k_unpack16<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(static_cast<const _Float16 *>(L->pop[lattice][L->cur]),
+                                                      L->d_small + lattice * NQ, L->d_e16[L->cur], L->d_isf, dst,
+                                                      lattice, A);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
 }
 
 }  // namespace mrb

@@ -88,6 +88,31 @@ struct FusedVariant {
 extern const FusedVariant kFusedVariants[];
 extern const int kNumFusedVariants;
 
+// FP16 storage with a per-box scale: stored value = (population - shift) *
+// 2^e, with e per (scale box, lattice).  A scale box of the compact layout
+// is a slab of kScaleSlab z-planes over the whole x-y extent, so that the
+// cells a wave pulls from (one z-plane and its two neighbours) use at most
+// three boxes' scales, read as wave-uniform values.  e is chosen from the
+// largest |population - shift| m of the box's fluid cells in the previous
+// state so that m * 2^e < 2^kF16Target: the FP16 maximum (65504) then
+// leaves a factor 16 of headroom for growth within one step; larger values
+// are clamped to the FP16 range, and each (block, lattice) write that
+// clamped is counted.  Solid cells hold 0 (downloads return the stream's
+// -1 sentinel there).
+constexpr int kScaleSlab = 8;
+constexpr int kF16Target = 12;
+__host__ __device__ inline int f16_exponent(float m)
+{
+    if (!(m > 0.0f)) return 40;
+    int k;
+    (void)frexpf(m, &k);  // m < 2^k
+    const int e = kF16Target - k;
+    return e < -24 ? -24 : (e > 40 ? 40 : e);
+}
+int f16_pack(mrb_level *L, int lattice, const double *src, int64_t src_cs, hipStream_t s);
+int f16_unpack(const mrb_level *L, int lattice, double *dst, hipStream_t s);
+int compact_macro(const mrb_level *L, double *out, hipStream_t s);
+
 }  // namespace mrb
 
 struct mrb_level {
@@ -129,6 +154,13 @@ struct mrb_level {
     bool flags_ready = false;
     double shift_rho0 = 1.0, shift_T0 = 1.0 / 3.0;
     double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
+    // FP16 storage (compact layout): power-of-two scale exponent and the
+    // largest |population - shift| over fluid cells per (scale box, lattice),
+    // one pair of arrays per population buffer; saturation counter
+    int *d_e16[2] = {nullptr, nullptr};
+    uint32_t *d_amax16[2] = {nullptr, nullptr};  // float bits
+    uint32_t *d_ovf16 = nullptr;
+    int nsb16 = 0;  // scale boxes (z-slabs of kScaleSlab planes)
     // timing events (created with the level)
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // hot-path kernel variant of the compact layout (compact_variant_name)

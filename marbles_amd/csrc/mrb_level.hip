@@ -414,10 +414,11 @@ __global__ void k_conservation(
     const PopT *__restrict__ f, const PopT *__restrict__ g, const int32_t *__restrict__ isf,
     const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total,
     const double *__restrict__ fshift, const double *__restrict__ gshift, double *__restrict__ part,
-    int64_t cs, int cst)
+    int64_t cs, int cst, const int *__restrict__ e16, int64_t nxy)
 {
     __shared__ double sh[5][kBlock];
     double acc[5] = {0, 0, 0, 0, 0};
+    constexpr bool half = std::is_same<PopT, _Float16>::value;
     for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < total;
          c += int64_t(gridDim.x) * blockDim.x) {
         const int b = find_box(base, nbox, c);
@@ -430,14 +431,28 @@ __global__ void k_conservation(
         const PopT *fp = f + B.base * NQ + loc * cst;
         const PopT *gp = g + B.base * NQ + loc * cst;
         const int64_t stride = cs ? cs : B.ncell;
+        // FP16 storage (compact layout): per-box scale of the cell's z-slab
+        float sf = 1.0f, sg = 1.0f;
+        if constexpr (half) {
+            const int b = int(c / nxy) / kScaleSlab;
+            sf = ldexpf(1.0f, -e16[2 * b]);
+            sg = ldexpf(1.0f, -e16[2 * b + 1]);
+        }
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const double fq = double(fp[q * stride]) + fshift[q];
+            double fq, gq;
+            if constexpr (half) {
+                fq = double(float(fp[q * stride]) * sf) + fshift[q];
+                gq = double(float(gp[q * stride]) * sg) + gshift[q];
+            } else {
+                fq = double(fp[q * stride]) + fshift[q];
+                gq = double(gp[q * stride]) + gshift[q];
+            }
             acc[0] += fq;
             if constexpr (ev(q, 0) != 0) acc[1] += ev(q, 0) * fq;
             if constexpr (ev(q, 1) != 0) acc[2] += ev(q, 1) * fq;
             if constexpr (ev(q, 2) != 0) acc[3] += ev(q, 2) * fq;
-            acc[4] += double(gp[q * stride]) + gshift[q];
+            acc[4] += gq;
         });
     }
     for (int n = 0; n < 5; ++n) sh[n][threadIdx.x] = acc[n];
@@ -590,12 +605,12 @@ static int64_t comp_stride(const mrb_level *L)
     return L->layout == MRB_LAYOUT_COMPACT ? 2 * L->total_cells : 0;
 }
 
-// Element stride between neighbouring cells: shifted FP32 storage on the
-// compact layout keeps f_q and g_q of a cell side by side
-// (pop[(q*N + cell)*2 + lattice]) so that one 8-byte load fetches both.
+// Element stride between neighbouring cells: the reduced-precision storages
+// on the compact layout keep f_q and g_q of a cell side by side
+// (pop[(q*N + cell)*2 + lattice]) so that one load fetches both.
 static int cell_stride(const mrb_level *L)
 {
-    return (L->layout == MRB_LAYOUT_COMPACT && L->storage == MRB_STORAGE_F32_SHIFTED) ? 2 : 1;
+    return (L->layout == MRB_LAYOUT_COMPACT && L->storage != MRB_STORAGE_F64) ? 2 : 1;
 }
 
 // Host <-> device copy of one lattice's 27 components (host: contiguous
@@ -842,6 +857,17 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
     MRB_HIP(hipMalloc(&L->d_small, sizeof(double) * 6 * NQ));
     for (auto &e : L->ev) MRB_HIP(hipEventCreate(&e));
     MRB_HIP(hipMemset(L->d_small, 0, sizeof(double) * 6 * NQ));
+    if (L->storage == MRB_STORAGE_F16_SCALED) {
+        L->nsb16 = (L->ncell[2] + kScaleSlab - 1) / kScaleSlab;
+        for (int ab = 0; ab < 2; ++ab) {
+            MRB_HIP(hipMalloc(&L->d_e16[ab], sizeof(int) * 2 * L->nsb16));
+            MRB_HIP(hipMemset(L->d_e16[ab], 0, sizeof(int) * 2 * L->nsb16));
+            MRB_HIP(hipMalloc(&L->d_amax16[ab], sizeof(uint32_t) * 2 * L->nsb16));
+            MRB_HIP(hipMemset(L->d_amax16[ab], 0, sizeof(uint32_t) * 2 * L->nsb16));
+        }
+        MRB_HIP(hipMalloc(&L->d_ovf16, sizeof(uint32_t)));
+        MRB_HIP(hipMemset(L->d_ovf16, 0, sizeof(uint32_t)));
+    }
     if (L->layout == MRB_LAYOUT_FAB) {
         MRB_HIP(hipMalloc(&L->d_md, sizeof(double) * 19 * T));
         MRB_HIP(hipMemset(L->d_md, 0, sizeof(double) * 19 * T));
@@ -927,6 +953,11 @@ int mrb_level_destroy(mrb_level *L)
     (void)hipFree(L->d_box_base);
     (void)hipFree(L->d_isf);
     (void)hipFree(L->d_mask);
+    for (int ab = 0; ab < 2; ++ab) {
+        (void)hipFree(L->d_e16[ab]);
+        (void)hipFree(L->d_amax16[ab]);
+    }
+    (void)hipFree(L->d_ovf16);
     (void)hipFree(L->d_md);
     (void)hipFree(L->d_dq);
     (void)hipFree(L->d_q);
@@ -1066,7 +1097,18 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
             for (int64_t c = 0; c < N; ++c) buf[size_t(q * N + c)] = float(h[q * N + c] - sh[q]);
         return copy_lattice(L, arr, buf.data(), true);
     }
-    return set_error(MRB_E_UNSUPPORTED, "fp16 storage upload not available");
+    // FP16 with per-box scale: scaled on the device (the box maxima skip
+    // solid cells, so the flags come first)
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "fp16 storage: set the flags before the populations");
+    double *d = nullptr;
+    MRB_HIP(hipMalloc(&d, sizeof(double) * size_t(n)));
+    hipError_t e = hipMemcpy(d, h, sizeof(double) * size_t(n), hipMemcpyHostToDevice);
+    int rc = MRB_OK;
+    if (e == hipSuccess) rc = f16_pack(L, arr, d, N, nullptr);
+    if (e == hipSuccess && rc == MRB_OK) e = hipDeviceSynchronize();
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "fp16 upload");
+    return rc;
 }
 
 int mrb_download(const mrb_level *L, int arr, void *host)
@@ -1091,7 +1133,14 @@ int mrb_download(const mrb_level *L, int arr, void *host)
             for (int64_t c = 0; c < N; ++c) h[q * N + c] = double(buf[size_t(q * N + c)]) + sh[q];
         return MRB_OK;
     }
-    return set_error(MRB_E_UNSUPPORTED, "fp16 storage download not available");
+    double *d = nullptr;
+    MRB_HIP(hipMalloc(&d, sizeof(double) * size_t(n)));
+    int rc = f16_unpack(L, arr, d, nullptr);
+    hipError_t e = rc == MRB_OK ? hipDeviceSynchronize() : hipSuccess;
+    if (rc == MRB_OK && e == hipSuccess) e = hipMemcpy(h, d, sizeof(double) * size_t(n), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "fp16 download");
+    return rc;
 }
 
 int mrb_compute_flags_sphere(mrb_level *L, const double center[3], double radius, int has_fluid_inside)
@@ -1309,7 +1358,6 @@ int mrb_init_fields(
     if (!L || !rho || !u || !v || !w || !T) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
     if (!(m_bar > 0.0) || !(adiabatic_exponent > 1.0)) return set_error(MRB_E_ARG, "invalid gas parameters");
-    if (L->storage == MRB_STORAGE_F16_SCALED) return set_error(MRB_E_UNSUPPORTED, "fp16 storage initial state");
     // ic::Initializer::initialize (IC.H:460-525) with the per-cell state
     // given for every fab cell (box order, local index x fastest); the
     // equilibria are evaluated on the device with the same expressions
@@ -1320,6 +1368,28 @@ int mrb_init_fields(
     hipError_t e = hipSuccess;
     for (int k = 0; k < 5 && e == hipSuccess; ++k)
         e = hipMemcpy(d + k * n, src[k], sizeof(double) * size_t(n), hipMemcpyHostToDevice);
+    if (e == hipSuccess && L->storage == MRB_STORAGE_F16_SCALED) {
+        // FP64 equilibria into a temporary compact FP64 pair of lattices
+        // (component stride 2N, zero shift), then scaled per box into FP16
+        const double R = kAirMolarMass / m_bar;
+        double *t = nullptr;
+        e = hipMalloc(&t, sizeof(double) * (2 * NQ * size_t(n) + 2 * NQ));
+        if (e == hipSuccess) e = hipMemset(t + 2 * NQ * n, 0, sizeof(double) * 2 * NQ);
+        if (e == hipSuccess) {
+            k_init_fields<double><<<nblocks(n), kBlock>>>(t, t + n, L->d_isf, L->d_boxes, L->d_box_base, L->nbox, n,
+                                                          d, R, adiabatic_exponent, t + 2 * NQ * n, 2 * n, 1);
+            e = hipGetLastError();
+        }
+        int rc = MRB_OK;
+        if (e == hipSuccess) rc = f16_pack(L, 0, t, 2 * n, nullptr);
+        if (e == hipSuccess && rc == MRB_OK) rc = f16_pack(L, 1, t + n, 2 * n, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(t);
+        (void)hipFree(d);
+        if (e != hipSuccess) return hip_fail(e, "mrb_init_fields");
+        L->fresh_depth = 0;
+        return rc;
+    }
     if (e == hipSuccess) {
         // d_small[0 .. 2NQ) holds the storage shift (zero for FP64)
         const double R = kAirMolarMass / m_bar;
@@ -1599,16 +1669,22 @@ int mrb_reduce_conservation(const mrb_level *L, double out[5], void *stream)
     const unsigned nb = std::min<unsigned>(4096, nblocks(L->total_cells));
     // shift tables live at d_small[0..2*NQ) (zero for FP64 storage)
     const double *shifts = L->d_small;
+    const int64_t nxy = int64_t(L->ncell[0]) * L->ncell[1];
     if (L->storage == MRB_STORAGE_F64) {
         k_conservation<double><<<nb, kBlock, 0, s>>>(
             (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L));
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L),
+            nullptr, nxy);
     } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
         k_conservation<float><<<nb, kBlock, 0, s>>>(
             (const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L));
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L),
+            nullptr, nxy);
     } else {
-        return set_error(MRB_E_UNSUPPORTED, "conservation sums for this storage");
+        k_conservation<_Float16><<<nb, kBlock, 0, s>>>(
+            (const _Float16 *)L->pop[0][L->cur], (const _Float16 *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L),
+            L->d_e16[L->cur], nxy);
     }
     MRB_HIP(hipGetLastError());
     std::vector<double> part(size_t(5 * nb));
@@ -1648,6 +1724,35 @@ int mrb_eb_force(const mrb_level *L, double out[3], void *stream)
     for (int n = 0; n < 3; ++n) out[n] = 0.0;
     for (unsigned b = 0; b < nb; ++b)
         for (int n = 0; n < 3; ++n) out[n] += part[size_t(5 * b + n)];
+    return MRB_OK;
+}
+
+int mrb_storage_saturations(const mrb_level *L, int64_t *count)
+{
+    if (!L || !count) return set_error(MRB_E_ARG, "null argument");
+    *count = 0;
+    if (L->storage != MRB_STORAGE_F16_SCALED) return MRB_OK;
+    uint32_t v = 0;
+    MRB_HIP(hipDeviceSynchronize());
+    MRB_HIP(hipMemcpy(&v, L->d_ovf16, sizeof(v), hipMemcpyDeviceToHost));
+    *count = v;
+    return MRB_OK;
+}
+
+int mrb_macro_fields(const mrb_level *L, double *out, void *stream)
+{
+    if (!L || !out) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
+    if (!L->phys_set) return set_error(MRB_E_STATE, "physics not set");
+    return compact_macro(L, out, as_stream(stream));
+}
+
+int mrb_storage_shift(const mrb_level *L, int lattice, double out[27])
+{
+    if (!L || !out || lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "bad argument");
+    for (int q = 0; q < NQ; ++q)
+        out[q] = L->storage == MRB_STORAGE_F64 ? 0.0 : (lattice == 0 ? L->fshift[q] : L->gshift[q]);
     return MRB_OK;
 }
 

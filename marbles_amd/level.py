@@ -248,6 +248,26 @@ class Level:
         check(self.lib.mrb_eb_force(self.h, out, stream))
         return np.array(out[:])
 
+    def macro_fields(self, out, stream=None) -> None:
+        """Density, velocity and temperature per cell into a device tensor of
+        5 * ncells doubles (compact layout)."""
+        n = int(np.prod(self.ncell))
+        if out.numel() != 5 * n or out.dtype.itemsize != 8 or not out.is_cuda:
+            raise ValueError("out must be a device float64 tensor of 5 * ncells elements")
+        check(self.lib.mrb_macro_fields(self.h, c_void_p(out.data_ptr()), stream))
+
+    def storage_saturations(self) -> int:
+        """FP16 storage: values clamped to the FP16 maximum so far (0 otherwise)."""
+        n = ctypes.c_int64()
+        check(self.lib.mrb_storage_saturations(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def shift(self, which: str) -> np.ndarray:
+        """Rest-state values subtracted by the reduced-precision storages."""
+        out = (c_double * NQ)()
+        check(self.lib.mrb_storage_shift(self.h, _ARRS[which], out))
+        return np.array(out[:])
+
     def hot_kernel_name(self) -> str:
         return self.lib.mrb_hot_kernel_name(self.h).decode()
 

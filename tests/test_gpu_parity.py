@@ -244,6 +244,60 @@ def test_f32_shifted_storage_tracks_f64(oracle, engine, case):
     assert du <= 1e-6 and dr <= 1e-6, (du, dr)
 
 
+@pytest.mark.parametrize("case", ["tg_tau1.0", "channel"])
+def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
+    """FP16 storage of (f - f_eq(rho0, 0, T0)) * 2^e, e per (z-slab box,
+    lattice), FP64 arithmetic: velocity and density against the FP64 run
+    after 100 steps (FP16 keeps 11 significant bits, so the bars are set by
+    that, not by 1e-6); no stored value saturates; conservation sums in FP64
+    agree with the FP64 run's."""
+    if case == "tg_tau1.0":
+        cfg, n = configs.taylor_green(32, tau=1.0), 100
+    else:
+        cfg, n = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 100
+    runs, cons = {}, {}
+    for storage in ("f64", "f16s"):
+        E = configs.build_level(cfg, layout="compact", storage=storage)
+        E.advance(n)
+        runs[storage] = _velocity(E)
+        cons[storage] = E.conservation()
+        if storage == "f16s":
+            assert E.storage_saturations() == 0
+        E.close()
+    (u64, r64), (u16, r16) = runs["f64"], runs["f16s"]
+    fluid = r64 > 0
+    du = np.abs(u16 - u64)[:, fluid].max() / np.abs(u64[:, fluid]).max()
+    dr = np.abs(r16 - r64)[fluid].max() / r64[fluid].max()
+    dm = abs(cons["f16s"][0] - cons["f64"][0]) / abs(cons["f64"][0])
+    print(f"{case}: {n} steps, FP16 max rel velocity error {du:.2e}, density {dr:.2e}, mass {dm:.2e}")
+    assert du <= 2e-2 and dr <= 2e-3 and dm <= 1e-4, (du, dr, dm)
+
+
+def test_f16_scaled_round_trip(oracle, engine):
+    """Upload then download through the FP16 storage: every fluid value
+    within 2^-11 (FP16 unit roundoff) of its box's largest |f - shift|;
+    solid cells read back as the stream's -1 sentinel."""
+    cfg = configs.channel_sphere((32, 16, 24), radius=4.0, center=(10.0, 8.0, 12.0))
+    E64 = configs.build_level(cfg, layout="compact", storage="f64")
+    E16 = configs.build_level(cfg, layout="compact", storage="f16s")
+    isf = E64.valid("is_fluid")[0]
+    for lat in ("f", "g"):
+        a = E64.valid(lat)
+        E16.upload(lat, a)
+        b = E16.valid(lat)
+        shift = np.array(E16.shift(lat))[:, None, None, None]
+        fluid = isf == 1
+        for z0 in range(0, a.shape[1], 8):
+            sl = slice(z0, z0 + 8)
+            fz = fluid[sl]
+            amax = np.abs(a[:, sl] - shift)[:, fz].max()
+            err = np.abs(b[:, sl] - a[:, sl])[:, fz].max()
+            assert err <= amax * 2.0 ** -11 * (1 + 1e-12), (lat, z0, err, amax)
+        assert np.all(b[:, ~fluid] == -1.0)
+    E64.close()
+    E16.close()
+
+
 @pytest.mark.parametrize("layout", ["compact", "fab", "compact_f32s"])
 def test_eb_force_matches_oracle(oracle, engine, layout):
     """Momentum-exchange force on the channel's sphere after 20 steps (FP64

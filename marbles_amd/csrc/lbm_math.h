@@ -86,22 +86,25 @@ struct CollideParams {
 
 // Raw moments of the post-stream populations (LBM.cpp:834-862), summed in
 // direction order.
-struct Moments {
-    double rho, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz, E, qx, qy, qz;
+// R = the arithmetic type of the sums (double; float for the FP16 storage)
+template <class R>
+struct MomentsR {
+    R rho, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz, E, qx, qy, qz;
 };
+using Moments = MomentsR<double>;
 
-template <class T>
+template <class T, class R>
 __host__ __device__ __forceinline__ void accumulate_moments(
-    const T* __restrict__ f, const T* __restrict__ g, Moments& m)
+    const T* __restrict__ f, const T* __restrict__ g, MomentsR<R>& m)
 {
-    m.rho = m.jx = m.jy = m.jz = 0.0;
-    m.pxx = m.pyy = m.pzz = m.pxy = m.pxz = m.pyz = 0.0;
-    m.E = m.qx = m.qy = m.qz = 0.0;
+    m.rho = m.jx = m.jy = m.jz = R(0);
+    m.pxx = m.pyy = m.pzz = m.pxy = m.pxz = m.pyz = R(0);
+    m.E = m.qx = m.qy = m.qz = R(0);
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-        const double fq = double(f[q]);
-        const double gq = double(g[q]);
+        const R fq = R(f[q]);
+        const R gq = R(g[q]);
         m.rho += fq;
         if constexpr (cx == 1) m.jx += fq;
         if constexpr (cx == -1) m.jx += -fq;
@@ -129,16 +132,15 @@ __host__ __device__ __forceinline__ void accumulate_moments(
 }
 
 // Only the sums the Q-correction needs (rho, j, 2 rho E).
-template <class T>
+template <class T, class R>
 __host__ __device__ __forceinline__ void accumulate_q_moments(
-    const T* __restrict__ f, const T* __restrict__ g, double& rho, double& jx, double& jy,
-    double& jz, double& E)
+    const T* __restrict__ f, const T* __restrict__ g, R& rho, R& jx, R& jy, R& jz, R& E)
 {
-    rho = jx = jy = jz = E = 0.0;
+    rho = jx = jy = jz = E = R(0);
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-        const double fq = double(f[q]);
+        const R fq = R(f[q]);
         rho += fq;
         if constexpr (cx == 1) jx += fq;
         if constexpr (cx == -1) jx += -fq;
@@ -146,7 +148,7 @@ __host__ __device__ __forceinline__ void accumulate_q_moments(
         if constexpr (cy == -1) jy += -fq;
         if constexpr (cz == 1) jz += fq;
         if constexpr (cz == -1) jz += -fq;
-        E += double(g[q]);
+        E += R(g[q]);
     });
 }
 
@@ -289,27 +291,63 @@ __host__ __device__ __forceinline__ CollideCoeffs collide_coeffs(
     return C;
 }
 
-// f equilibrium of direction q (Utilities.H:42-63, product order kept)
-template <int q>
-__host__ __device__ __forceinline__ double feq_of(const CollideCoeffs& C)
+// The per-cell coefficients rounded to FP32, for the FP16 storage's FP32
+// per-direction arithmetic
+struct CollideCoeffsF {
+    float omega;
+    float rphx[3], phy[3], phz[3];
+    float E, a1x, a1y, a1z;
+    float tx[2], ty[2], tz[2];
+    float dxy, dxz, dyz;
+};
+
+__host__ __device__ __forceinline__ CollideCoeffsF to_float(const CollideCoeffs& C)
+{
+    CollideCoeffsF F;
+    F.omega = float(C.omega);
+    for (int k = 0; k < 3; ++k) {
+        F.rphx[k] = float(C.rphx[k]);
+        F.phy[k] = float(C.phy[k]);
+        F.phz[k] = float(C.phz[k]);
+    }
+    F.E = float(C.E);
+    F.a1x = float(C.a1x);
+    F.a1y = float(C.a1y);
+    F.a1z = float(C.a1z);
+    for (int k = 0; k < 2; ++k) {
+        F.tx[k] = float(C.tx[k]);
+        F.ty[k] = float(C.ty[k]);
+        F.tz[k] = float(C.tz[k]);
+    }
+    F.dxy = float(C.dxy);
+    F.dxz = float(C.dxz);
+    F.dyz = float(C.dyz);
+    return F;
+}
+
+// f equilibrium of direction q (Utilities.H:42-63, product order kept);
+// Co = CollideCoeffs (FP64) or CollideCoeffsF (FP32)
+template <int q, class Co>
+__host__ __device__ __forceinline__ auto feq_of(const Co& C)
 {
     constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
     return ((C.rphx[cx + 1]) * C.phy[cy + 1]) * C.phz[cz + 1];
 }
 
 // g equilibrium of direction q (Utilities.H:147-160, summation order kept)
-template <int q>
-__host__ __device__ __forceinline__ double geq_of(const CollideCoeffs& C)
+template <int q, class Co>
+__host__ __device__ __forceinline__ auto geq_of(const Co& C)
 {
+    using R = decltype(C.E);
     constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-    double ge = C.E;
+    R ge = C.E;
     if constexpr (cx == 1) ge = ge + C.a1x;
     if constexpr (cx == -1) ge = ge + -C.a1x;
     if constexpr (cy == 1) ge = ge + C.a1y;
     if constexpr (cy == -1) ge = ge + -C.a1y;
     if constexpr (cz == 1) ge = ge + C.a1z;
     if constexpr (cz == -1) ge = ge + -C.a1z;
-    double S = C.tx[cx * cx] + C.ty[cy * cy];
+    R S = C.tx[cx * cx] + C.ty[cy * cy];
     if constexpr (cx * cy == 1) S = S + C.dxy;
     if constexpr (cx * cy == -1) S = S + -C.dxy;
     S = S + C.tz[cz * cz];
@@ -317,8 +355,8 @@ __host__ __device__ __forceinline__ double geq_of(const CollideCoeffs& C)
     if constexpr (cx * cz == -1) S = S + -C.dxz;
     if constexpr (cy * cz == 1) S = S + C.dyz;
     if constexpr (cy * cz == -1) S = S + -C.dyz;
-    ge = ge + 0.5 * S;
-    return ge * weight(q);
+    ge = ge + R(0.5) * S;
+    return ge * R(weight(q));
 }
 
 // Equilibria of LBM::macrodata_to_equilibrium (LBM.cpp:659-758) hoisted per
@@ -336,6 +374,22 @@ __host__ __device__ __forceinline__ void collide_cell(
         const double fq = double(f[q]);
         const double gq = double(g[q]);
         out(qc, fq + omega * (feq_of<q>(C) - fq), gq + omega * (geq_of<q>(C) - gq));
+    });
+}
+
+// The same with FP32 populations, moments and per-direction arithmetic; the
+// per-cell coefficients are formed in FP64 from the FP32 moments.
+template <class FOut>
+__host__ __device__ __forceinline__ void collide_cell_f32(
+    const float* __restrict__ f, const float* __restrict__ g, const MomentsR<float>& mf, const Macro& M,
+    double dqx, double dqy, double dqz, const CollideParams& P, FOut&& out)
+{
+    const Moments m{mf.rho, mf.jx, mf.jy, mf.jz, mf.pxx, mf.pyy, mf.pzz, mf.pxy, mf.pxz, mf.pyz, mf.E, mf.qx, mf.qy, mf.qz};
+    const CollideCoeffsF C = to_float(collide_coeffs(m, M, dqx, dqy, dqz, P));
+    const float omega = C.omega;
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        out(qc, f[q] + omega * (feq_of<q>(C) - f[q]), g[q] + omega * (geq_of<q>(C) - g[q]));
     });
 }
 

@@ -29,6 +29,7 @@ struct CompactArgs {
     int64_t cs;  // component stride of the population arrays (2N)
     int bc[6];
     uint32_t bcpack;  // bc[face] in bits 4*face .. 4*face+3
+    int fast_ok;      // byte offsets of a lattice's cells fit 32 bits
 };
 
 // Source coordinates of the pull for one direction component c in
@@ -109,6 +110,13 @@ __device__ __forceinline__ _Float16 to_half(double v, double shift, float scale,
     amax = fmaxf(amax, fabsf(d));
     return _Float16(__builtin_amdgcn_fmed3f(d * scale, -65504.0f, 65504.0f));
 }
+// the same from an FP32 working value (difference taken in FP32)
+__device__ __forceinline__ _Float16 to_half(float v, double shift, float scale, float &amax)
+{
+    const float d = v - float(shift);
+    amax = fmaxf(amax, fabsf(d));
+    return _Float16(__builtin_amdgcn_fmed3f(d * scale, -65504.0f, 65504.0f));
+}
 
 // storage <-> FP64 working value (FP64 and shifted FP32 storage)
 template <class T>
@@ -136,6 +144,16 @@ __device__ __forceinline__ T narrow(double v, double shift)
 // (index c + 1): t = the coordinate the pull reads, x - c wrapped on a
 // periodic axis and, beyond a non-periodic face, the boundary cell next to
 // the ghost (the cell an outflow ghost copies); g = the source is a ghost.
+// the directions q with c_q[d] == c, as a bit set
+constexpr uint32_t qset(int d, int c)
+{
+    uint32_t m = 0;
+    for (int q = 0; q < NQ; ++q)
+        if (ev(q, d) == c) m |= 1u << q;
+    return m;
+}
+constexpr uint32_t kAllQ = (1u << NQ) - 1u;
+
 __device__ __forceinline__ void targets3(int x, int n, int periodic, int t[3], bool g[3])
 {
     const int p = x + 1, m = x - 1;
@@ -155,11 +173,13 @@ __device__ __forceinline__ void targets3(int x, int n, int periodic, int t[3], b
 // read the cell's own pre-stream populations, OUTFLOW the boundary cell
 // next to the ghost (at a corner of two non-periodic faces, clamped in
 // both), VELOCITY a constant.
-template <class T>
+// W = the working type of the pulled values: double, or float for the
+// FP16 storage (FP32 arithmetic)
+template <class T, class W>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
-    const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, double fv[NQ],
-    double gv[NQ])
+    const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, W fv[NQ],
+    W gv[NQ])
 {
     int tx[3], ty[3], tz[3];
     bool gx[3], gy[3], gz[3];
@@ -169,6 +189,58 @@ __device__ __forceinline__ void pull_cell(
     const uint32_t c32 = uint32_t(cell);
     T rf[NQ], rg[NQ];
     uint32_t cmask = 0, dmask = 0;
+    // The boundary handling of the 27 sources as bit sets over q, per cell:
+    // G = sources beyond a non-periodic face, the face being the first of
+    // z, y, x (BCFill's order) in which the source lies outside; V, O =
+    // those whose face is VELOCITY or OUTFLOW.  A source is read directly
+    // (from the wrapped or clamped coordinates) unless it is a solid cell
+    // or a ghost of a NOSLIP or SLIP face.
+    uint32_t G = 0, V = 0, O = 0;
+    static_for<3>([&](auto dc) {
+        constexpr int d = 2 - decltype(dc)::value;
+        const bool *gd = d == 0 ? gx : (d == 1 ? gy : gz);
+        const uint32_t Gd = (gd[2] ? qset(d, 1) : 0u) | (gd[0] ? qset(d, -1) : 0u);
+        const uint32_t Fd = Gd & ~G;
+        G |= Gd;
+        const uint32_t tl = (A.bcpack >> (4 * d)) & 15u, th = (A.bcpack >> (4 * (d + 3))) & 15u;
+        const uint32_t lo = Fd & qset(d, 1), hi = Fd & qset(d, -1);
+        V |= (tl == MRB_BC_VELOCITY ? lo : 0u) | (th == MRB_BC_VELOCITY ? hi : 0u);
+        O |= (tl == MRB_BC_OUTFLOW ? lo : 0u) | (th == MRB_BC_OUTFLOW ? hi : 0u);
+    });
+    const uint32_t nondirect = (G & ~(V | O)) | (~G & ~mask & kAllQ);
+    if (A.fast_ok && __all(nondirect == 0u)) {
+        // every source of every cell of the wave is read directly: one
+        // wave-uniform base per direction and a 32-bit byte offset per lane
+        constexpr uint32_t bytes = kCellStride<T> * sizeof(T);
+        const uint32_t nxb = uint32_t(A.nx) * bytes, nxyb = nxb * uint32_t(A.ny);
+        uint32_t xo[3], yo[3], zo[3];
+        for (int k = 0; k < 3; ++k) {
+            xo[k] = uint32_t(tx[k]) * bytes;
+            yo[k] = uint32_t(ty[k]) * nxb;
+            zo[k] = uint32_t(tz[k]) * nxyb;
+        }
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+            const uint32_t off = zo[cz + 1] + yo[cy + 1] + xo[cx + 1];
+            const char *bf = reinterpret_cast<const char *>(f) + int64_t(q) * A.cs * int64_t(sizeof(T));
+            if constexpr (kIsHalf<T>) {
+                const uint32_t v = *reinterpret_cast<const uint32_t *>(bf + off);
+                rf[q] = __builtin_bit_cast(_Float16, uint16_t(v & 0xffffu));
+                rg[q] = __builtin_bit_cast(_Float16, uint16_t(v >> 16));
+            } else if constexpr (kCellStride<T> == 2) {
+                const float2 v = *reinterpret_cast<const float2 *>(bf + off);
+                rf[q] = v.x;
+                rg[q] = v.y;
+            } else {
+                const char *bg = reinterpret_cast<const char *>(g) + int64_t(q) * A.cs * int64_t(sizeof(T));
+                rf[q] = *reinterpret_cast<const T *>(bf + off);
+                rg[q] = *reinterpret_cast<const T *>(bg + off);
+            }
+        });
+        cmask = V;
+        dmask = kAllQ;
+    } else {
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
@@ -203,6 +275,7 @@ __device__ __forceinline__ void pull_cell(
         }
         cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
     });
+    }
     if constexpr (kIsHalf<T>) {
         // inverse scales of the source planes z - c_z (index c_z + 1); a
         // source that is not the pull's (bounce-back, wall ghosts) is the
@@ -220,10 +293,10 @@ __device__ __forceinline__ void pull_cell(
             const bool direct = (dmask >> q) & 1u;
             const float sf = direct ? invf[k] : invf[1], sg = direct ? invg[k] : invg[1];
             // unscaling in FP32 is exact (a power of two on an FP16 value)
-            const double vf = double(float(rf[q]) * sf) + sh.f(q);
-            const double vg = double(float(rg[q]) * sg) + sh.g(q);
-            fv[q] = use_const ? bc.fval[q] : vf;
-            gv[q] = use_const ? bc.gval[q] : vg;
+            const W vf = W(float(rf[q]) * sf) + W(sh.f(q));
+            const W vg = W(float(rg[q]) * sg) + W(sh.g(q));
+            fv[q] = use_const ? W(bc.fval[q]) : vf;
+            gv[q] = use_const ? W(bc.gval[q]) : vg;
         });
     } else {
         (void)dmask;
@@ -266,11 +339,12 @@ __global__ void __launch_bounds__(256) k_qpass(
     if (!cell_coords(A, x, y, z, cell)) return;
     const uint32_t m = mask[cell];
     if (!(m & 1u)) return;
-    double fv[NQ], gv[NQ];
+    using V = std::conditional_t<kIsHalf<T>, float, double>;
+    V fv[NQ], gv[NQ];
     pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
-    double rho, jx, jy, jz, E;
+    V rho, jx, jy, jz, E;
     accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
-    const Macro M = macro_from(rho, jx, jy, jz, E, P);
+    const Macro M = macro_from(double(rho), double(jx), double(jy), double(jz), double(E), P);
     Q[cell] = M.Qx;
     Q[A.N + cell] = M.Qy;
     Q[2 * A.N + cell] = M.Qz;
@@ -332,7 +406,7 @@ __device__ __forceinline__ void collide_body(
         scf = ldexpf(1.0f, ef);
         scg = ldexpf(1.0f, eg);
     }
-    auto put = [&](auto qc, double fnew, double gnew) {
+    auto put = [&](auto qc, auto fnew, auto gnew) {
         constexpr int q = decltype(qc)::value;
         const int64_t e = int64_t(q) * A.cs + cell * kCellStride<T>;
         if constexpr (kIsHalf<T>) {
@@ -354,11 +428,12 @@ __device__ __forceinline__ void collide_body(
                 store_pair(fo, go, e, narrow<T>(-1.0, sh.f(q)), narrow<T>(-1.0, sh.g(q)));
         });
     } else if (in) {
-        double fv[NQ], gv[NQ];
+        using V = std::conditional_t<kIsHalf<T>, float, double>;
+        V fv[NQ], gv[NQ];
         pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
-        Moments mo;
+        MomentsR<V> mo;
         accumulate_moments(fv, gv, mo);
-        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        const Macro M = macro_from(double(mo.rho), double(mo.jx), double(mo.jy), double(mo.jz), double(mo.E), P);
         // gradient(): neighbour inside the domain box and fluid
         const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
         const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
@@ -370,7 +445,10 @@ __device__ __forceinline__ void collide_body(
                                               ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
         const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
                                               zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
-        collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) { put(qc, fnew, gnew); });
+        if constexpr (kIsHalf<T>)
+            collide_cell_f32(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, float fnew, float gnew) { put(qc, fnew, gnew); });
+        else
+            collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) { put(qc, fnew, gnew); });
     }
     if constexpr (kIsHalf<T>) half_block_stats(h16, ef, eg, mf, mg);
 }
@@ -457,6 +535,8 @@ CompactArgs make_args(const mrb_level *L)
     for (int n = 0; n < 6; ++n) A.bc[n] = L->bc.bc_type[n];
     A.bcpack = 0;
     for (int n = 0; n < 6; ++n) A.bcpack |= uint32_t(A.bc[n] & 15) << (4 * n);
+    const int64_t cell_bytes = L->storage == MRB_STORAGE_F64 ? 8 : (L->storage == MRB_STORAGE_F32_SHIFTED ? 8 : 4);
+    A.fast_ok = A.N * cell_bytes < (int64_t(1) << 32) ? 1 : 0;
     return A;
 }
 

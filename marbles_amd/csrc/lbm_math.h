@@ -86,7 +86,7 @@ struct CollideParams {
 
 // Raw moments of the post-stream populations (LBM.cpp:834-862), summed in
 // direction order.
-// R = the arithmetic type of the sums (double; float for the FP16 storage)
+// R = the arithmetic type of the sums
 template <class R>
 struct MomentsR {
     R rho, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz, E, qx, qy, qz;
@@ -291,42 +291,7 @@ __host__ __device__ __forceinline__ CollideCoeffs collide_coeffs(
     return C;
 }
 
-// The per-cell coefficients rounded to FP32, for the FP16 storage's FP32
-// per-direction arithmetic
-struct CollideCoeffsF {
-    float omega;
-    float rphx[3], phy[3], phz[3];
-    float E, a1x, a1y, a1z;
-    float tx[2], ty[2], tz[2];
-    float dxy, dxz, dyz;
-};
-
-__host__ __device__ __forceinline__ CollideCoeffsF to_float(const CollideCoeffs& C)
-{
-    CollideCoeffsF F;
-    F.omega = float(C.omega);
-    for (int k = 0; k < 3; ++k) {
-        F.rphx[k] = float(C.rphx[k]);
-        F.phy[k] = float(C.phy[k]);
-        F.phz[k] = float(C.phz[k]);
-    }
-    F.E = float(C.E);
-    F.a1x = float(C.a1x);
-    F.a1y = float(C.a1y);
-    F.a1z = float(C.a1z);
-    for (int k = 0; k < 2; ++k) {
-        F.tx[k] = float(C.tx[k]);
-        F.ty[k] = float(C.ty[k]);
-        F.tz[k] = float(C.tz[k]);
-    }
-    F.dxy = float(C.dxy);
-    F.dxz = float(C.dxz);
-    F.dyz = float(C.dyz);
-    return F;
-}
-
-// f equilibrium of direction q (Utilities.H:42-63, product order kept);
-// Co = CollideCoeffs (FP64) or CollideCoeffsF (FP32)
+// f equilibrium of direction q (Utilities.H:42-63, product order kept)
 template <int q, class Co>
 __host__ __device__ __forceinline__ auto feq_of(const Co& C)
 {
@@ -374,22 +339,6 @@ __host__ __device__ __forceinline__ void collide_cell(
         const double fq = double(f[q]);
         const double gq = double(g[q]);
         out(qc, fq + omega * (feq_of<q>(C) - fq), gq + omega * (geq_of<q>(C) - gq));
-    });
-}
-
-// The same with FP32 populations, moments and per-direction arithmetic; the
-// per-cell coefficients are formed in FP64 from the FP32 moments.
-template <class FOut>
-__host__ __device__ __forceinline__ void collide_cell_f32(
-    const float* __restrict__ f, const float* __restrict__ g, const MomentsR<float>& mf, const Macro& M,
-    double dqx, double dqy, double dqz, const CollideParams& P, FOut&& out)
-{
-    const Moments m{mf.rho, mf.jx, mf.jy, mf.jz, mf.pxx, mf.pyy, mf.pzz, mf.pxy, mf.pxz, mf.pyz, mf.E, mf.qx, mf.qy, mf.qz};
-    const CollideCoeffsF C = to_float(collide_coeffs(m, M, dqx, dqy, dqz, P));
-    const float omega = C.omega;
-    static_for<NQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        out(qc, f[q] + omega * (feq_of<q>(C) - f[q]), g[q] + omega * (geq_of<q>(C) - g[q]));
     });
 }
 

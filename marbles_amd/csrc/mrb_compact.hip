@@ -110,13 +110,6 @@ __device__ __forceinline__ _Float16 to_half(double v, double shift, float scale,
     amax = fmaxf(amax, fabsf(d));
     return _Float16(__builtin_amdgcn_fmed3f(d * scale, -65504.0f, 65504.0f));
 }
-// the same from an FP32 working value (difference taken in FP32)
-__device__ __forceinline__ _Float16 to_half(float v, double shift, float scale, float &amax)
-{
-    const float d = v - float(shift);
-    amax = fmaxf(amax, fabsf(d));
-    return _Float16(__builtin_amdgcn_fmed3f(d * scale, -65504.0f, 65504.0f));
-}
 
 // storage <-> FP64 working value (FP64 and shifted FP32 storage)
 template <class T>
@@ -173,8 +166,7 @@ __device__ __forceinline__ void targets3(int x, int n, int periodic, int t[3], b
 // read the cell's own pre-stream populations, OUTFLOW the boundary cell
 // next to the ghost (at a corner of two non-periodic faces, clamped in
 // both), VELOCITY a constant.
-// W = the working type of the pulled values: double, or float for the
-// FP16 storage (FP32 arithmetic)
+// W = the working type of the pulled values (double)
 template <class T, class W>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
@@ -339,12 +331,11 @@ __global__ void __launch_bounds__(256) k_qpass(
     if (!cell_coords(A, x, y, z, cell)) return;
     const uint32_t m = mask[cell];
     if (!(m & 1u)) return;
-    using V = std::conditional_t<kIsHalf<T>, float, double>;
-    V fv[NQ], gv[NQ];
+    double fv[NQ], gv[NQ];
     pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
-    V rho, jx, jy, jz, E;
+    double rho, jx, jy, jz, E;
     accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
-    const Macro M = macro_from(double(rho), double(jx), double(jy), double(jz), double(E), P);
+    const Macro M = macro_from(rho, jx, jy, jz, E, P);
     Q[cell] = M.Qx;
     Q[A.N + cell] = M.Qy;
     Q[2 * A.N + cell] = M.Qz;
@@ -406,7 +397,7 @@ __device__ __forceinline__ void collide_body(
         scf = ldexpf(1.0f, ef);
         scg = ldexpf(1.0f, eg);
     }
-    auto put = [&](auto qc, auto fnew, auto gnew) {
+    auto put = [&](auto qc, double fnew, double gnew) {
         constexpr int q = decltype(qc)::value;
         const int64_t e = int64_t(q) * A.cs + cell * kCellStride<T>;
         if constexpr (kIsHalf<T>) {
@@ -428,12 +419,11 @@ __device__ __forceinline__ void collide_body(
                 store_pair(fo, go, e, narrow<T>(-1.0, sh.f(q)), narrow<T>(-1.0, sh.g(q)));
         });
     } else if (in) {
-        using V = std::conditional_t<kIsHalf<T>, float, double>;
-        V fv[NQ], gv[NQ];
+        double fv[NQ], gv[NQ];
         pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
-        MomentsR<V> mo;
+        Moments mo;
         accumulate_moments(fv, gv, mo);
-        const Macro M = macro_from(double(mo.rho), double(mo.jx), double(mo.jy), double(mo.jz), double(mo.E), P);
+        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
         // gradient(): neighbour inside the domain box and fluid
         const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
         const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
@@ -445,10 +435,7 @@ __device__ __forceinline__ void collide_body(
                                               ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
         const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
                                               zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
-        if constexpr (kIsHalf<T>)
-            collide_cell_f32(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, float fnew, float gnew) { put(qc, fnew, gnew); });
-        else
-            collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) { put(qc, fnew, gnew); });
+        collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) { put(qc, fnew, gnew); });
     }
     if constexpr (kIsHalf<T>) half_block_stats(h16, ef, eg, mf, mg);
 }

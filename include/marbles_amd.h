@@ -275,6 +275,28 @@ int mrb_iso_state(const mrb_iso *iso, double *host_f);
  * Compact layout, every storage.  Asynchronous. */
 int mrb_macro_fields(const mrb_level *lvl, double *out, void *stream);
 
+/* LBM::compute_derived (Source/LBM.cpp:909-955): vorticity x, y, z and
+ * magnitude of the current state's velocity into the device array out
+ * (4 N doubles, field-major; 0 in solid cells), with the reference's
+ * gradient() (Utilities.H:279-312: central differences, one-sided next to a
+ * domain face or a solid cell).  The velocity is that of the stored
+ * (post-collision) populations, equal to the last collide's post-stream
+ * velocity up to rounding.  Compact layout.  Asynchronous. */
+int mrb_vorticity(mrb_level *lvl, double *out, void *stream);
+
+/* Refinement tags for LBM::ErrorEst (LBM.cpp:1597-1611), the criteria of
+ * tagging.<name>.value_greater / value_less / adjacent_difference_greater
+ * (LBM.cpp:351-377) evaluated on a device field of N doubles (for example a
+ * component of mrb_macro_fields or mrb_vorticity): tags[cell] = 1 where
+ * field > value, field < value, or the largest |difference| to a face
+ * neighbour > value (neighbours wrap on periodic axes and are skipped beyond
+ * non-periodic faces).  Compact layout.  Asynchronous. */
+#define MRB_TAG_GREATER 0
+#define MRB_TAG_LESS 1
+#define MRB_TAG_ADJACENT_DIFFERENCE 2
+int mrb_tag_cells(const mrb_level *lvl, const double *field, int criterion, double value, uint8_t *tags,
+                  void *stream);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 const char *mrb_last_error(void);
 /* Name of the kernel variant used by mrb_stream_collide. */

@@ -13,6 +13,9 @@ MRB_LAYOUT_COMPACT = 1
 MRB_STORAGE_F64 = 0
 MRB_STORAGE_F32_SHIFTED = 1
 MRB_STORAGE_F16_SCALED = 2
+MRB_TAG_GREATER = 0
+MRB_TAG_LESS = 1
+MRB_TAG_ADJACENT_DIFFERENCE = 2
 MRB_ARR_F = 0
 MRB_ARR_G = 1
 MRB_ARR_ISFLUID = 2
@@ -101,6 +104,8 @@ SIGNATURES = {
     "mrb_storage_saturations": (c_int, [c_void_p, POINTER(c_int64)]),
     "mrb_storage_shift": (c_int, [c_void_p, c_int, POINTER(c_double)]),
     "mrb_macro_fields": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "mrb_vorticity": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "mrb_tag_cells": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p]),
     "mrb_init_fields": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
                                 c_double]),
     "mrb_amr_link": (c_int, [c_void_p, c_void_p]),

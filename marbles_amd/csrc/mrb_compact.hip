@@ -784,7 +784,99 @@ __global__ void __launch_bounds__(256) k_macro(
     for (int k = 0; k < 5; ++k) out[k * A.N + cell] = o[k];
 }
 
+// vorticity (x, y, z, magnitude) of the velocity in mac (fields 1..3) with
+// the reference's gradient(): central inside, one-sided where the neighbour
+// lies outside the domain box or is solid (LBM::compute_derived,
+// LBM.cpp:909-955); 0 in solid cells
+__global__ void __launch_bounds__(256) k_vorticity(
+    const double *__restrict__ mac, const uint32_t *__restrict__ mask, CompactArgs A, CollideParams P,
+    double *__restrict__ out)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    if (m & 1u) {
+        // bits of the stream mask: 2/1 = x+1/x-1 fluid, 4/3 = y, 6/5 = z
+        const bool ok[3][2] = {{x < A.nx - 1 && ((m >> 2) & 1u), x > 0 && ((m >> 1) & 1u)},
+                               {y < A.ny - 1 && ((m >> 4) & 1u), y > 0 && ((m >> 3) & 1u)},
+                               {z < A.nz - 1 && ((m >> 6) & 1u), z > 0 && ((m >> 5) & 1u)}};
+        const int64_t st[3] = {1, A.nx, int64_t(A.nx) * A.ny};
+        auto grad = [&](int d, int comp) {
+            const double *u = mac + comp * A.N;
+            const bool p = ok[d][0], mm = ok[d][1];
+            return one_sided_gradient(p, mm, p ? u[cell + st[d]] : 0.0, u[cell], mm ? u[cell - st[d]] : 0.0,
+                                      P.inv_dx[d]);
+        };
+        const double vx = grad(0, 2), wx = grad(0, 3);
+        const double uy = grad(1, 1), wy = grad(1, 3);
+        const double uz = grad(2, 1), vz = grad(2, 2);
+        w[0] = wy - vz;
+        w[1] = uz - wx;
+        w[2] = vx - uy;
+        w[3] = sqrt((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
+    }
+    for (int k = 0; k < 4; ++k) out[k * A.N + cell] = w[k];
+}
+
+// refinement tags (AMReX AMRErrorTag criteria): 1 where the field is
+// greater / less than the value, or where the largest absolute difference
+// to a face neighbour (wrapped on periodic axes, skipped beyond a
+// non-periodic face) is greater than it
+__global__ void __launch_bounds__(256) k_tag(
+    const double *__restrict__ fld, CompactArgs A, int criterion, double value, uint8_t *__restrict__ tags)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const double v = fld[cell];
+    bool t;
+    if (criterion == MRB_TAG_GREATER) {
+        t = v > value;
+    } else if (criterion == MRB_TAG_LESS) {
+        t = v < value;
+    } else {
+        const int pos[3] = {x, y, z}, n[3] = {A.nx, A.ny, A.nz};
+        const int64_t st[3] = {1, A.nx, int64_t(A.nx) * A.ny};
+        double dmax = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                int p = pos[d] + sgn;
+                bool in = p >= 0 && p < n[d];
+                if (!in && A.per[d]) {
+                    p = p < 0 ? p + n[d] : p - n[d];
+                    in = true;
+                }
+                if (in) dmax = fmax(dmax, fabs(fld[cell + int64_t(p - pos[d]) * st[d]] - v));
+            }
+        }
+        t = dmax > value;
+    }
+    tags[cell] = t ? 1 : 0;
+}
+
 }  // namespace
+
+int compact_vorticity(mrb_level *L, double *out, hipStream_t s)
+{
+    const int64_t N = L->total_cells;
+    if (!L->d_mac) MRB_HIP(hipMalloc(&L->d_mac, sizeof(double) * 5 * size_t(N)));
+    const int rc = compact_macro(L, L->d_mac, s);
+    if (rc) return rc;
+    const CompactArgs A = make_args(L);
+    k_vorticity<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(L->d_mac, L->d_mask, A, L->cp, out);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+int compact_tag(const mrb_level *L, const double *field, int criterion, double value, uint8_t *tags, hipStream_t s)
+{
+    const CompactArgs A = make_args(L);
+    k_tag<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(field, A, criterion, value, tags);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
 
 int compact_macro(const mrb_level *L, double *out, hipStream_t s)
 {

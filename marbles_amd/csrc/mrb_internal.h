@@ -112,6 +112,9 @@ __host__ __device__ inline int f16_exponent(float m)
 int f16_pack(mrb_level *L, int lattice, const double *src, int64_t src_cs, hipStream_t s);
 int f16_unpack(const mrb_level *L, int lattice, double *dst, hipStream_t s);
 int compact_macro(const mrb_level *L, double *out, hipStream_t s);
+int compact_vorticity(mrb_level *L, double *out, hipStream_t s);
+int compact_tag(const mrb_level *L, const double *field, int criterion, double value, uint8_t *tags,
+                hipStream_t s);
 
 }  // namespace mrb
 
@@ -161,6 +164,7 @@ struct mrb_level {
     uint32_t *d_amax16[2] = {nullptr, nullptr};  // float bits
     uint32_t *d_ovf16 = nullptr;
     int nsb16 = 0;  // scale boxes (z-slabs of kScaleSlab planes)
+    double *d_mac = nullptr;  // compact: macro fields scratch (5 N) for derived quantities
     // timing events (created with the level)
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // hot-path kernel variant of the compact layout (compact_variant_name)

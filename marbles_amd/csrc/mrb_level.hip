@@ -958,6 +958,7 @@ int mrb_level_destroy(mrb_level *L)
         (void)hipFree(L->d_amax16[ab]);
     }
     (void)hipFree(L->d_ovf16);
+    (void)hipFree(L->d_mac);
     (void)hipFree(L->d_md);
     (void)hipFree(L->d_dq);
     (void)hipFree(L->d_q);
@@ -1746,6 +1747,24 @@ int mrb_macro_fields(const mrb_level *L, double *out, void *stream)
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags not set");
     if (!L->phys_set) return set_error(MRB_E_STATE, "physics not set");
     return compact_macro(L, out, as_stream(stream));
+}
+
+int mrb_vorticity(mrb_level *L, double *out, void *stream)
+{
+    if (!L || !out) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    if (!L->flags_ready || !L->phys_set) return set_error(MRB_E_STATE, "flags and physics must be set");
+    return compact_vorticity(L, out, as_stream(stream));
+}
+
+int mrb_tag_cells(const mrb_level *L, const double *field, int criterion, double value, uint8_t *tags,
+                  void *stream)
+{
+    if (!L || !field || !tags) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
+    if (criterion != MRB_TAG_GREATER && criterion != MRB_TAG_LESS && criterion != MRB_TAG_ADJACENT_DIFFERENCE)
+        return set_error(MRB_E_ARG, "unknown tagging criterion");
+    return compact_tag(L, field, criterion, value, tags, as_stream(stream));
 }
 
 int mrb_storage_shift(const mrb_level *L, int lattice, double out[27])

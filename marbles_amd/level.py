@@ -256,6 +256,25 @@ class Level:
             raise ValueError("out must be a device float64 tensor of 5 * ncells elements")
         check(self.lib.mrb_macro_fields(self.h, c_void_p(out.data_ptr()), stream))
 
+    def vorticity(self, out, stream=None) -> None:
+        """Vorticity x, y, z and magnitude per cell (LBM::compute_derived) into a
+        device tensor of 4 * ncells doubles (compact layout)."""
+        n = int(np.prod(self.ncell))
+        if out.numel() != 4 * n or out.dtype.itemsize != 8 or not out.is_cuda:
+            raise ValueError("out must be a device float64 tensor of 4 * ncells elements")
+        check(self.lib.mrb_vorticity(self.h, c_void_p(out.data_ptr()), stream))
+
+    def tag_cells(self, field, criterion: str, value: float, tags, stream=None) -> None:
+        """Refinement tags (uint8 device tensor of ncells) from a device float64
+        field: criterion 'greater', 'less' or 'adjacent_difference'."""
+        n = int(np.prod(self.ncell))
+        if field.numel() != n or tags.numel() != n or tags.dtype.itemsize != 1:
+            raise ValueError("field: ncells doubles, tags: ncells bytes")
+        crit = {"greater": _lib.MRB_TAG_GREATER, "less": _lib.MRB_TAG_LESS,
+                "adjacent_difference": _lib.MRB_TAG_ADJACENT_DIFFERENCE}[criterion]
+        check(self.lib.mrb_tag_cells(self.h, c_void_p(field.data_ptr()), crit, float(value),
+                                     c_void_p(tags.data_ptr()), stream))
+
     def storage_saturations(self) -> int:
         """FP16 storage: values clamped to the FP16 maximum so far (0 otherwise)."""
         n = ctypes.c_int64()

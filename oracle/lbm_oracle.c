@@ -873,6 +873,48 @@ static inline double gradient(
     return (vp + vc + vm) * L->inv_dx[dir];
 }
 
+/* ------------------------------------------------------------------------ */
+/* LBM::compute_derived (Source/LBM.cpp:909-955): vorticity of the velocity  */
+/* of the current populations (macrodata recomputed from them first).        */
+/* out: per box, the valid cells' 4 components [vort_x, vort_y, vort_z,      */
+/* vort_mag] component-major (x fastest), boxes back to back; 0 in solids.   */
+/* ------------------------------------------------------------------------ */
+void orc_vorticity(orc_level *L, double *out)
+{
+    orc_f_to_macrodata(L);
+    int64_t off = 0;
+    for (int b = 0; b < L->nbox; b++) {
+        const fabdim *fd = &L->fdg[b];
+        const fabdim *d0 = &L->fd0[b];
+        const int32_t *isf = L->isf[b];
+        const int nx = d0->hi[0] - d0->lo[0] + 1, ny = d0->hi[1] - d0->lo[1] + 1,
+                  nz = d0->hi[2] - d0->lo[2] + 1;
+        const int64_t nv = (int64_t)nx * ny * nz;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int k = d0->lo[2]; k <= d0->hi[2]; k++)
+            for (int j = d0->lo[1]; j <= d0->hi[1]; j++)
+                for (int i = d0->lo[0]; i <= d0->hi[0]; i++) {
+                    const int64_t c = ((int64_t)(k - d0->lo[2]) * ny + (j - d0->lo[1])) * nx + (i - d0->lo[0]);
+                    double w[4] = {0.0, 0.0, 0.0, 0.0};
+                    if (isf[fidx(fd, 0, i, j, k)] == 1) {
+                        const int iv[3] = {i, j, k};
+                        const double vx = gradient(L, b, 0, 2, iv);
+                        const double wx = gradient(L, b, 0, 3, iv);
+                        const double uy = gradient(L, b, 1, 1, iv);
+                        const double wy = gradient(L, b, 1, 3, iv);
+                        const double uz = gradient(L, b, 2, 1, iv);
+                        const double vz = gradient(L, b, 2, 2, iv);
+                        w[0] = wy - vz;
+                        w[1] = uz - wx;
+                        w[2] = vx - uy;
+                        w[3] = sqrt((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
+                    }
+                    for (int n = 0; n < 4; n++) out[off + n * nv + c] = w[n];
+                }
+        off += 4 * nv;
+    }
+}
+
 void orc_compute_q_corrections(orc_level *L)
 {
     for (int b = 0; b < L->nbox; b++) {

@@ -121,6 +121,10 @@ void orc_conservation(const orc_level *L, double out[5]);
 /* LBM::compute_eb_forces (LBM.cpp:994-1044) on one level: momentum-exchange
  * force on the embedded boundary from the current (post-collision) f */
 void orc_eb_force(const orc_level *L, double out[3]);
+/* LBM::compute_derived (LBM.cpp:909-955): vorticity (x, y, z, magnitude) of
+ * the current populations' velocity over every box's valid cells (per box:
+ * 4 components, component-major, x fastest); recomputes the macrodata. */
+void orc_vorticity(orc_level *L, double *out);
 
 /* ---- two levels, refinement ratio 2 -------------------------------------
  * The coarse level must cover its domain; the fine level's domain is the

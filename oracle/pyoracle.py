@@ -89,6 +89,7 @@ def load() -> ctypes.CDLL:
         "orc_masked_avgdown8": (c_double, [D3]),
         "orc_sphere_covered": (c_int, [D3, D3, D3, c_double, c_int]),
         "orc_eb_force": (None, [c_void_p, D3]),
+        "orc_vorticity": (None, [c_void_p, c_void_p]),
         "orc_ic_fields": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_double, c_double]),
         "orc_amr_coalesce": (None, [c_void_p, c_void_p, c_int]),
@@ -291,6 +292,19 @@ class OracleLevel:
         out = (c_double * 3)()
         self.lib.orc_eb_force(self.h, out)
         return np.array(out[:])
+
+    def vorticity(self):
+        """LBM::compute_derived's vorticity (x, y, z, magnitude) of the current
+        populations over the domain, shaped (4, nz, ny, nx)."""
+        sizes = [int(np.prod(sh)) for sh in self.shapes0]
+        out = np.zeros(4 * sum(sizes))
+        self.lib.orc_vorticity(self.h, out.ctypes.data_as(c_void_p))
+        dom = np.zeros((4,) + tuple(reversed(self.ncell)))
+        off = 0
+        for (lo, hi), sh, n in zip(self.boxes, self.shapes0, sizes):
+            dom[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = out[off:off + 4 * n].reshape((4,) + sh)
+            off += 4 * n
+        return dom
 
 
 def amr_coalesce(fine: OracleLevel, crse: OracleLevel, lattice: int) -> None:

@@ -93,13 +93,15 @@ typedef struct {
  * operator Source/VelocityBC.H:235-281 with VelocityBC.cpp:6-21). */
 typedef struct {
     int bc_type[6];     /* m_bc_type: index idir + lohi*3 */
-    int velbc_kind;     /* 0 = "noop", 1 = "constant" */
-    int vel_dir;        /* velocity_bc_constant.dir */
-    double mach_ref;    /* velocity_bc_constant.Mach_ref */
-    double density;     /* velocity_bc_constant.initial_density */
-    double temperature; /* velocity_bc_constant.initial_temperature */
+    int velbc_kind;     /* 0 = "noop", 1 = "constant", 2 = "channel" (VelocityBC.H:90-129),
+                           3 = "parabolic" (VelocityBC.H:139-192) */
+    int vel_dir;        /* velocity_bc_constant.dir; parabolic: tangential_dir */
+    double mach_ref;    /* velocity_bc_<kind>.Mach_ref */
+    double density;     /* velocity_bc_<kind>.initial_density */
+    double temperature; /* velocity_bc_<kind>.initial_temperature */
     double adiabatic_exponent;
     double m_bar;
+    int normal_dir;     /* velocity_bc_parabolic.normal_dir */
 } mrb_bc;
 
 /* ---- lifetime ----------------------------------------------------------

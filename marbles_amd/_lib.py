@@ -67,6 +67,7 @@ class BC(ctypes.Structure):
         ("temperature", c_double),
         ("adiabatic_exponent", c_double),
         ("m_bar", c_double),
+        ("normal_dir", c_int),
     ]
 
 

@@ -75,6 +75,43 @@ def channel_sphere(ncell=(256, 256, 256), radius: float = 24.0, center=(64.0, 12
     }
 
 
+def channel_profile(ncell=(64, 32, 32), kind: str = "channel", tau: float = 0.55, u: float = 0.05,
+                    temperature: float = 0.2, max_box: int | None = None) -> dict:
+    """Walled channel with a velocity-profile inlet at x-lo and zero-gradient
+    outflow at x-hi (Channel / Parabolic velocity operators, VelocityBC.H:
+    90-192).  kind "channel": no-slip walls in y and z, u_x = 16 u_ref
+    y (ny-1-y) z (nz-1-z) / ny^4 on the ghost cells; kind "parabolic":
+    no-slip walls in y, periodic z, u_x = 4 u_m y (H - y) / H^2 at the ghost
+    cell centres (normal_dir 1, tangential_dir 0).  u_ref / u_m = u; the
+    domain starts at rest."""
+    gamma = 5.0 / 3.0
+    mach = u / math.sqrt(gamma * temperature)
+    vbc = {"kind": kind, "Mach_ref": mach, "initial_density": 1.0, "initial_temperature": temperature,
+           "adiabatic_exponent": gamma, "mean_molecular_mass": AIR_MOLAR_MASS}
+    if kind == "parabolic":
+        vbc.update({"normal_dir": 1, "tangential_dir": 0})
+        periodic, bc_lo, bc_hi = (0, 0, 1), (2, 1, 0), (5, 1, 0)
+    else:
+        periodic, bc_lo, bc_hi = (0, 0, 0), (2, 1, 1), (5, 1, 1)
+    return {
+        "name": "channel_{}_{}x{}x{}".format(kind, *ncell),
+        "ncell": tuple(ncell),
+        "periodic": periodic,
+        "prob_lo": (0.0, 0.0, 0.0),
+        "prob_hi": tuple(float(n) for n in ncell),
+        "bc_lo": bc_lo,
+        "bc_hi": bc_hi,
+        "velocity_bc": vbc,
+        "nu": (tau - 0.5) * temperature,
+        "tau": tau,
+        "T0": temperature,
+        "ic": {"type": "constant", "density": 1.0, "mach": (0.0, 0.0, 0.0),
+               "temperature": temperature, "adiabatic_exponent": gamma},
+        "eb": {"type": "all_regular"},
+        "max_box": max_box,
+    }
+
+
 def chop_boxes(ncell, max_box: int | None):
     """Domain chopped into boxes of at most max_box cells per side (x fastest)."""
     if not max_box:

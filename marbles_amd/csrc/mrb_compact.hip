@@ -159,6 +159,26 @@ __device__ __forceinline__ void targets3(int x, int n, int periodic, int t[3], b
     t[2] = lo ? (periodic ? m + n : 0) : m;
 }
 
+// VELOCITY ghost value of direction q for the cell (x, y, z) from the
+// face tables of a Channel or Parabolic profile: the face is the first of
+// z, y, x (BCFill's loop order) in which the source x - c_q is a ghost, the
+// ghost cell the unwrapped x - c_q
+template <int q>
+__device__ __forceinline__ void profile_ghost(const BCDev &bc, const bool gx[3], const bool gy[3], const bool gz[3],
+                                              int x, int y, int z, double &fv, double &gv)
+{
+    constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+    const int iv[3] = {x - cx, y - cy, z - cz};
+    const bool hz = gz[cz + 1], hy = gy[cy + 1];
+    const int dd = hz ? 2 : (hy ? 1 : 0);
+    const int c = dd == 2 ? cz : (dd == 1 ? cy : cx);
+    const int face = c > 0 ? dd : dd + 3;
+    const int d1 = dd == 0 ? 1 : 0, d2 = dd == 2 ? 1 : 2;
+    const double *t = bc.tab[face] + (int64_t(iv[d1] + 1) * bc.tabw[face] + (iv[d2] + 1)) * (2 * NQ);
+    fv = t[q];
+    gv = t[NQ + q];
+}
+
 // Pull of one cell, without branches so that all 54 loads are in flight
 // together.  For a pull source beyond a physical face the reference's
 // BCFill value is resolved: the face applied is the last non-periodic face
@@ -287,16 +307,20 @@ __device__ __forceinline__ void pull_cell(
             // unscaling in FP32 is exact (a power of two on an FP16 value)
             const W vf = W(float(rf[q]) * sf) + W(sh.f(q));
             const W vg = W(float(rg[q]) * sg) + W(sh.g(q));
-            fv[q] = use_const ? W(bc.fval[q]) : vf;
-            gv[q] = use_const ? W(bc.gval[q]) : vg;
+            double cf = bc.fval[q], cg = bc.gval[q];
+            if (bc.prof.kind && use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
+            fv[q] = use_const ? W(cf) : vf;
+            gv[q] = use_const ? W(cg) : vg;
         });
     } else {
         (void)dmask;
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const bool use_const = (cmask >> q) & 1u;
-            fv[q] = use_const ? bc.fval[q] : widen(rf[q], sh.f(q));
-            gv[q] = use_const ? bc.gval[q] : widen(rg[q], sh.g(q));
+            double cf = bc.fval[q], cg = bc.gval[q];
+            if (bc.prof.kind && use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
+            fv[q] = use_const ? cf : widen(rf[q], sh.f(q));
+            gv[q] = use_const ? cg : widen(rg[q], sh.g(q));
         });
     }
 }

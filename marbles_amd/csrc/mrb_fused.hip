@@ -371,6 +371,8 @@ int fused_launch(mrb_level *L, int variant, hipStream_t s)
         if (L->bc.bc_type[n] == MRB_BC_SLIPX || L->bc.bc_type[n] == MRB_BC_SLIPY ||
             L->bc.bc_type[n] == MRB_BC_SLIPZ)
             return set_error(MRB_E_UNSUPPORTED, "fused kernel: slip walls use the two-pass step");
+    if (L->bcdev.prof.kind)
+        return set_error(MRB_E_UNSUPPORTED, "fused kernel: velocity profiles use the two-pass step");
     if (variant < 0 || variant >= kNumFusedVariants) return set_error(MRB_E_ARG, "bad fused variant");
     const FusedVariant &v = kFusedVariants[variant];
     if (v.hold) {

@@ -30,10 +30,42 @@ struct GhostPair {
 };
 
 // Velocity-inlet ghost values and boundary types.
+// Velocity-inflow profile of the Channel and Parabolic velocity operators
+// (VelocityBC.H:90-192): the ghost cell's velocity from its index.
+struct VelProfile {
+    int kind;     // 0: constant values (fval/gval); 2: channel; 3: parabolic
+    int nd, td;   // parabolic: normal and tangential directions
+    int domhi[3];
+    double vref;  // channel u_ref, parabolic um (Mach_ref * speed of sound)
+    double rho, R, T, gamma;
+    double plo, phi, dxn;  // parabolic: prob_lo, prob_hi and dx along nd
+};
+
+__host__ __device__ inline void profile_velocity(const VelProfile &P, const int iv[3], double vel[3])
+{
+    vel[0] = vel[1] = vel[2] = 0.0;
+    if (P.kind == 2) {
+        const double c1 = double(iv[1] * (P.domhi[1] - iv[1]));
+        const double c2 = double(iv[2] * (P.domhi[2] - iv[2]));
+        const double n = double(P.domhi[1] + 1);
+        vel[0] = 16.0 * P.vref * c1 * c2 / (((n * n) * n) * n);
+    } else if (P.kind == 3) {
+        const double height = P.phi - P.plo;
+        const double x = P.plo + (iv[P.nd] + 0.5) * P.dxn;
+        vel[P.td] = 4.0 * P.vref * x * (height - x) / (height * height);
+    }
+}
+
 struct BCDev {
     int type[6];
     double fval[NQ];
     double gval[NQ];
+    VelProfile prof;
+    // compact layout, profile kinds: per face, the f and g ghost values over
+    // the face's transverse coordinates t1 < t2 (each from -1 to n), entry
+    // ((t1 + 1) * tabw + t2 + 1) * 2NQ + [q | NQ + q]
+    const double *tab[6];
+    int tabw[6];
 };
 
 // Geometry of the compact (single box, no ghost) layout.
@@ -165,6 +197,7 @@ struct mrb_level {
     uint32_t *d_ovf16 = nullptr;
     int nsb16 = 0;  // scale boxes (z-slabs of kScaleSlab planes)
     double *d_mac = nullptr;  // compact: macro fields scratch (5 N) for derived quantities
+    double *d_bctab = nullptr;  // compact: velocity-profile ghost tables of the faces
     // timing events (created with the level)
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // hot-path kernel variant of the compact layout (compact_variant_name)

@@ -172,7 +172,16 @@ __global__ void k_physbc_fab(
                     if (t_bc == MRB_BC_NOSLIP) {
                         *dq = data[kBounce[q] * B.ncell + loc_of(B, n0, n1, n2)];
                     } else if (t_bc == MRB_BC_VELOCITY) {
-                        *dq = is_energy ? bc.gval[q] : bc.fval[q];
+                        if (bc.prof.kind) {
+                            // Channel / Parabolic: the velocity of this ghost cell
+                            double v[3];
+                            profile_velocity(bc.prof, iv, v);
+                            const VelProfile &P = bc.prof;
+                            *dq = is_energy ? thermal_g_equilibrium(P.rho, v, P.T, P.R, P.gamma, q)
+                                            : set_equilibrium_value(P.rho, v, P.R * P.T, q);
+                        } else {
+                            *dq = is_energy ? bc.gval[q] : bc.fval[q];
+                        }
                     } else if (t_bc == MRB_BC_OUTFLOW) {
                         int s[3] = {iv[0], iv[1], iv[2]};
                         s[idir] += normal_dir;
@@ -726,6 +735,63 @@ static void compute_bc_values(mrb_level *L)
         (void)hipMemcpy(L->d_small + 4 * NQ, d.fval, sizeof(double) * NQ, hipMemcpyHostToDevice);
         (void)hipMemcpy(L->d_small + 5 * NQ, d.gval, sizeof(double) * NQ, hipMemcpyHostToDevice);
     }
+    // Channel / Parabolic (VelocityBC.cpp:23-54): the velocity depends on
+    // the ghost cell
+    VelProfile &P = d.prof;
+    P = VelProfile{};
+    for (int f = 0; f < 6; ++f) {
+        d.tab[f] = nullptr;
+        d.tabw[f] = 0;
+    }
+    (void)hipFree(L->d_bctab);
+    L->d_bctab = nullptr;
+    if (L->bc.velbc_kind == 2 || L->bc.velbc_kind == 3) {
+        P.kind = L->bc.velbc_kind;
+        const double sos = std::sqrt(
+            L->bc.adiabatic_exponent * (kAirMolarMass / L->bc.m_bar) * L->bc.temperature);
+        P.vref = L->bc.mach_ref * sos;
+        P.rho = L->bc.density;
+        P.R = kAirMolarMass / L->bc.m_bar;
+        P.T = L->bc.temperature;
+        P.gamma = L->bc.adiabatic_exponent;
+        P.nd = L->bc.normal_dir;
+        P.td = L->bc.vel_dir;
+        for (int k = 0; k < 3; ++k) P.domhi[k] = L->ncell[k] - 1;
+        P.plo = L->prob_lo[P.nd];
+        P.phi = L->prob_hi[P.nd];
+        P.dxn = L->dx[P.nd];
+        if (L->layout == MRB_LAYOUT_COMPACT) {
+            // ghost tables of the VELOCITY faces for the compact pull
+            std::vector<double> host;
+            int64_t offs[6];
+            for (int f = 0; f < 6; ++f) {
+                offs[f] = -1;
+                if (L->bc.bc_type[f] != MRB_BC_VELOCITY) continue;
+                const int dd = f % 3, d1 = dd == 0 ? 1 : 0, d2 = dd == 2 ? 1 : 2;
+                const int n1 = L->ncell[d1] + 2, n2 = L->ncell[d2] + 2;
+                offs[f] = int64_t(host.size());
+                d.tabw[f] = n2;
+                for (int a = 0; a < n1; ++a)
+                    for (int b = 0; b < n2; ++b) {
+                        int iv[3];
+                        iv[dd] = f < 3 ? -1 : L->ncell[dd];
+                        iv[d1] = a - 1;
+                        iv[d2] = b - 1;
+                        double v[3];
+                        profile_velocity(P, iv, v);
+                        for (int q = 0; q < NQ; ++q) host.push_back(set_equilibrium_value(P.rho, v, P.R * P.T, q));
+                        for (int q = 0; q < NQ; ++q)
+                            host.push_back(thermal_g_equilibrium(P.rho, v, P.T, P.R, P.gamma, q));
+                    }
+            }
+            if (!host.empty() && hipMalloc(&L->d_bctab, sizeof(double) * host.size()) == hipSuccess &&
+                hipMemcpy(L->d_bctab, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) ==
+                    hipSuccess) {
+                for (int f = 0; f < 6; ++f)
+                    if (offs[f] >= 0) d.tab[f] = L->d_bctab + offs[f];
+            }
+        }
+    }
 }
 
 static void compute_shift(mrb_level *L)
@@ -959,6 +1025,7 @@ int mrb_level_destroy(mrb_level *L)
     }
     (void)hipFree(L->d_ovf16);
     (void)hipFree(L->d_mac);
+    (void)hipFree(L->d_bctab);
     (void)hipFree(L->d_md);
     (void)hipFree(L->d_dq);
     (void)hipFree(L->d_q);
@@ -1012,10 +1079,18 @@ int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
                 return set_error(MRB_E_ARG, "unknown bc type");
         }
     }
-    if (bc->velbc_kind == 1 && (bc->vel_dir < 0 || bc->vel_dir > 2))
+    if (bc->velbc_kind < 0 || bc->velbc_kind > 3) return set_error(MRB_E_ARG, "unknown velocity bc kind");
+    if ((bc->velbc_kind == 1 || bc->velbc_kind == 3) && (bc->vel_dir < 0 || bc->vel_dir > 2))
         return set_error(MRB_E_ARG, "velocity bc dir must be 0, 1 or 2");
+    if (bc->velbc_kind == 3 && (bc->normal_dir < 0 || bc->normal_dir > 2))
+        return set_error(MRB_E_ARG, "parabolic velocity bc normal_dir must be 0, 1 or 2");
     L->bc = *bc;
     compute_bc_values(L);
+    if (L->bc.velbc_kind >= 2 && L->layout == MRB_LAYOUT_COMPACT) {
+        for (int f = 0; f < 6; ++f)
+            if (L->bc.bc_type[f] == MRB_BC_VELOCITY && !L->bcdev.tab[f])
+                return set_error(MRB_E_HIP, "velocity profile tables could not be allocated");
+    }
     return MRB_OK;
 }
 

@@ -96,14 +96,18 @@ class Level:
         check(self.lib.mrb_set_phys(self.h, ctypes.byref(p)))
 
     def set_bc(self, bc_lo: Sequence[int], bc_hi: Sequence[int], velocity_bc: dict | None = None) -> None:
-        """velocity_bc: {"kind": "constant"|"noop", "dir", "Mach_ref", "initial_density",
-        "initial_temperature", "adiabatic_exponent", "mean_molecular_mass"} (VelocityBC.cpp:6-21)."""
+        """velocity_bc: {"kind": "constant"|"channel"|"parabolic"|"noop", "dir" (constant),
+        "normal_dir", "tangential_dir" (parabolic), "Mach_ref", "initial_density",
+        "initial_temperature", "adiabatic_exponent", "mean_molecular_mass"} (VelocityBC.cpp:6-54;
+        Mach_ref defaults 0.005, parabolic 0.05)."""
         v = dict(velocity_bc or {})
-        kind = {"noop": 0, "constant": 1}[v.get("kind", "noop")]
-        b = BC((c_int * 6)(*list(bc_lo), *list(bc_hi)), kind, int(v.get("dir", 1)),
-               float(v.get("Mach_ref", 0.005)), float(v.get("initial_density", 1.0)),
+        kind = {"noop": 0, "constant": 1, "channel": 2, "parabolic": 3}[v.get("kind", "noop")]
+        mach = 0.05 if kind == 3 else 0.005
+        b = BC((c_int * 6)(*list(bc_lo), *list(bc_hi)), kind,
+               int(v.get("tangential_dir" if kind == 3 else "dir", 1)),
+               float(v.get("Mach_ref", mach)), float(v.get("initial_density", 1.0)),
                float(v.get("initial_temperature", 1.0 / 3.0)), float(v.get("adiabatic_exponent", 5.0 / 3.0)),
-               float(v.get("mean_molecular_mass", AIR_MOLAR_MASS)))
+               float(v.get("mean_molecular_mass", AIR_MOLAR_MASS)), int(v.get("normal_dir", 0)))
         check(self.lib.mrb_set_bc(self.h, ctypes.byref(b)))
 
     def set_shift_state(self, rho0: float, T0: float) -> None:

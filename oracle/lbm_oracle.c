@@ -595,9 +595,35 @@ void orc_fill_boundary(orc_level *L, int kind)
 /* ------------------------------------------------------------------------ */
 
 static void velbc_op(
-    const orc_bc *bc, double *rho, double vel[3], double *R, double *T, double *gamma)
+    const orc_level *L, const int iv[3], const orc_bc *bc, double *rho, double vel[3], double *R, double *T,
+    double *gamma)
 {
-    if (bc->velbc_kind == 1) {
+    if (bc->velbc_kind == 2) {
+        /* Channel: VelocityBC.cpp:23-37 and VelocityBC.H:90-129 */
+        const double speed_of_sound_ref =
+            sqrt(bc->adiabatic_exponent * (AIR_MOLAR_MASS / bc->m_bar) * bc->temperature);
+        const double u_ref = bc->mach_ref * speed_of_sound_ref;
+        *rho = bc->density;
+        const double c1 = (double)(iv[1] * (L->dom_hi[1] - iv[1]));
+        const double c2 = (double)(iv[2] * (L->dom_hi[2] - iv[2]));
+        vel[0] = 16.0 * u_ref * c1 * c2 / pow((double)(L->dom_hi[1] + 1), 4.0);
+        *R = (AIR_MOLAR_MASS / bc->m_bar);
+        *T = bc->temperature;
+        *gamma = bc->adiabatic_exponent;
+    } else if (bc->velbc_kind == 3) {
+        /* Parabolic: VelocityBC.cpp:39-54 and VelocityBC.H:139-192 */
+        const double speed_of_sound_ref =
+            sqrt(bc->adiabatic_exponent * (AIR_MOLAR_MASS / bc->m_bar) * bc->temperature);
+        const double um = bc->mach_ref * speed_of_sound_ref;
+        const int nd = bc->normal_dir, td = bc->vel_dir;
+        *rho = bc->density;
+        const double height = L->prob_hi[nd] - L->prob_lo[nd];
+        const double x = L->prob_lo[nd] + (iv[nd] + 0.5) * L->dx[nd];
+        vel[td] = 4.0 * um * x * (height - x) / (height * height);
+        *R = (AIR_MOLAR_MASS / bc->m_bar);
+        *T = bc->temperature;
+        *gamma = bc->adiabatic_exponent;
+    } else if (bc->velbc_kind == 1) {
         /* VelocityBC.cpp:6-21 and VelocityBC.H:253-271 */
         const double speed_of_sound_ref =
             sqrt(bc->adiabatic_exponent * (AIR_MOLAR_MASS / bc->m_bar) * bc->temperature);
@@ -634,7 +660,7 @@ static void bcfill_cell(orc_level *L, int b, double *data, const int iv[3], int 
                             double R = 1.0, temperature = 1.0 / 3.0;
                             double gamma = 5.0 / 3.0;
                             double rho_bc = 0.0;
-                            velbc_op(&L->bc, &rho_bc, vel_bc, &R, &temperature, &gamma);
+                            velbc_op(L, iv, &L->bc, &rho_bc, vel_bc, &R, &temperature, &gamma);
                             if (is_energy) {
                                 /* BC.H:141-189 velocity_bc_g */
                                 const double cv = R / (gamma - 1.0);

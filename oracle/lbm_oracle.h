@@ -63,13 +63,14 @@ typedef struct {
 /* Physical-boundary description (Source/BC.H:22-63, VelocityBC.H:235-281). */
 typedef struct {
     int bc_type[6];   /* m_bc_type: index idir + lohi*3 */
-    int velbc_kind;   /* 0 = "noop", 1 = "constant" */
-    int vel_dir;      /* velocity_bc_constant.dir (reference default 1) */
-    double mach_ref;  /* velocity_bc_constant.Mach_ref */
-    double density;   /* velocity_bc_constant.initial_density */
+    int velbc_kind;   /* 0 = "noop", 1 = "constant", 2 = "channel", 3 = "parabolic" */
+    int vel_dir;      /* velocity_bc_constant.dir (reference default 1); parabolic: tangential_dir */
+    double mach_ref;  /* velocity_bc_<kind>.Mach_ref */
+    double density;   /* velocity_bc_<kind>.initial_density */
     double temperature;
     double adiabatic_exponent;
     double m_bar;
+    int normal_dir;   /* velocity_bc_parabolic.normal_dir (reference default 0) */
 } orc_bc;
 
 /* ---- level lifetime ---------------------------------------------------- */

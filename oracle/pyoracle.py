@@ -30,7 +30,7 @@ class Phys(ctypes.Structure):
 class BC(ctypes.Structure):
     _fields_ = [("bc_type", c_int * 6), ("velbc_kind", c_int), ("vel_dir", c_int),
                 ("mach_ref", c_double), ("density", c_double), ("temperature", c_double),
-                ("adiabatic_exponent", c_double), ("m_bar", c_double)]
+                ("adiabatic_exponent", c_double), ("m_bar", c_double), ("normal_dir", c_int)]
 
 
 _lib = None
@@ -171,11 +171,13 @@ class OracleLevel:
 
     def set_bc(self, bc_lo, bc_hi, velocity_bc=None):
         v = dict(velocity_bc or {})
-        kind = {"noop": 0, "constant": 1}[v.get("kind", "noop")]
-        b = BC((c_int * 6)(*list(bc_lo), *list(bc_hi)), kind, int(v.get("dir", 1)),
-               float(v.get("Mach_ref", 0.005)), float(v.get("initial_density", 1.0)),
+        kind = {"noop": 0, "constant": 1, "channel": 2, "parabolic": 3}[v.get("kind", "noop")]
+        mach = {3: 0.05}.get(kind, 0.005)  # constructors' Mach_ref defaults
+        b = BC((c_int * 6)(*list(bc_lo), *list(bc_hi)), kind,
+               int(v.get("tangential_dir" if kind == 3 else "dir", 1)),
+               float(v.get("Mach_ref", mach)), float(v.get("initial_density", 1.0)),
                float(v.get("initial_temperature", 1.0 / 3.0)), float(v.get("adiabatic_exponent", 5.0 / 3.0)),
-               float(v.get("mean_molecular_mass", AIR_MOLAR_MASS)))
+               float(v.get("mean_molecular_mass", AIR_MOLAR_MASS)), int(v.get("normal_dir", 0)))
         self.lib.orc_set_bc(self.h, ctypes.byref(b))
 
     # data

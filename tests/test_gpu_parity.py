@@ -325,3 +325,36 @@ def test_eb_force_matches_oracle(oracle, engine, layout):
     tol = 1e-5 if layout == "compact_f32s" else 1e-12
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (layout, got, ref)
     E.close()
+
+
+@pytest.mark.parametrize("kind", ["channel", "parabolic"])
+@pytest.mark.parametrize("layout", ["compact", "fab", "fab_batched"])
+def test_velocity_profile_inlet_matches_oracle(oracle, engine, kind, layout):
+    """Walled channel driven by a Channel or Parabolic inlet profile (ghost
+    values from the ghost cell's index): compact two-pass step, fab stages
+    and the batched fab step against the oracle after 30 steps."""
+    from oracle.pyoracle import build_oracle_level
+
+    cfg = configs.channel_profile((40, 20, 24), kind=kind, max_box=None if layout == "compact" else 12)
+    O = build_oracle_level(cfg)
+    if layout == "compact":
+        E = _compact_from(O, engine, cfg)
+    else:
+        O, E = _fab_pair(oracle, engine, cfg)
+    n = 30
+    if layout == "fab":
+        for _ in range(n):
+            for lat in (0, 1):
+                E.fillpatch(lat)
+                E.stream(lat)
+            E.collide()
+    else:
+        E.advance(n)
+    O.evolve(n)
+    for name in ("f", "g"):
+        a, b = E.valid(name), O.valid(name)
+        err = _rel_err(a, b)
+        assert err <= REL_TOL, (kind, layout, name, err)
+    u = O.valid("f")
+    assert np.abs(np.tensordot(np.array([c[0] for c in oracle.stencil()[0]], float), u, axes=(0, 0))).max() > 1e-3
+    E.close()

@@ -112,6 +112,54 @@ def channel_profile(ncell=(64, 32, 32), kind: str = "channel", tau: float = 0.55
     }
 
 
+def plugin_case(kind: str, ncell, params: dict, nu: float, alpha: float | None = None,
+                periodic=(1, 1, 1), bc_lo=(0, 0, 0), bc_hi=(0, 0, 0), prob_lo=(0.0, 0.0, 0.0),
+                prob_hi=None, max_box: int | None = None) -> dict:
+    """Workload initialised by one of the reference's IC plugins
+    (marbles_amd.ic: viscosity_test, thermaldiffusivity_test, sod), regular
+    geometry; the level physics take the plugin's adiabatic exponent and
+    molecular mass, as the decks set lbm.* to the ic_*.* values."""
+    gamma = params.get("adiabatic_exponent", 5.0 / 3.0)
+    m_bar = params.get("mean_molecular_mass", AIR_MOLAR_MASS)
+    return {
+        "name": "{}_{}x{}x{}".format(kind, *ncell),
+        "ncell": tuple(ncell),
+        "periodic": tuple(periodic),
+        "prob_lo": tuple(prob_lo),
+        "prob_hi": tuple(prob_hi) if prob_hi else tuple(float(n) for n in ncell),
+        "bc_lo": tuple(bc_lo),
+        "bc_hi": tuple(bc_hi),
+        "velocity_bc": None,
+        "nu": nu,
+        "alpha": alpha,
+        "adiabatic_exponent": gamma,
+        "m_bar": m_bar,
+        "ic": {"type": "plugin", "kind": kind, "params": dict(params), "adiabatic_exponent": gamma,
+               "m_bar": m_bar},
+        "eb": {"type": "all_regular"},
+        "max_box": max_box,
+    }
+
+
+def shear_wave(ny: int = 64, tau: float = 0.8, temperature: float = 0.2) -> dict:
+    """ViscosityTest plugin on a periodic 4 x ny x 4 box: u_x = 0.01 c_s
+    sin(2 pi y / ny) decays as exp(-nu k^2 t), nu = (tau - 1/2) R T0."""
+    return plugin_case("viscosity_test", (4, ny, 4),
+                       {"wave_length": float(ny), "initial_temperature": temperature},
+                       nu=(tau - 0.5) * temperature)
+
+
+def sod_tube(nx: int = 300, tau: float = 0.55, temperature: float = 0.2) -> dict:
+    """The sod deck (Tests/test_files/sod/sod.inp) scaled to nx cells:
+    density 0.5 and ratio 4, temperature ratio 1/8 across x = nx/2,
+    gamma = 2, zero-gradient outflow at both x faces, periodic y and z."""
+    nu = (tau - 0.5) * temperature
+    return plugin_case("sod", (nx, 2, 2),
+                       {"density": 0.5, "x_discontinuity": nx / 2.0, "initial_temperature": temperature,
+                        "adiabatic_exponent": 2.0, "density_ratio": 4.0, "temperature_ratio": 0.125},
+                       nu=nu, alpha=nu, periodic=(0, 1, 1), bc_lo=(5, 0, 0), bc_hi=(5, 0, 0))
+
+
 def chop_boxes(ncell, max_box: int | None):
     """Domain chopped into boxes of at most max_box cells per side (x fastest)."""
     if not max_box:
@@ -136,7 +184,8 @@ def build_level(cfg: dict, layout: str = "compact", storage: str = "f64", ng: in
         boxes, ngv = chop_boxes(cfg["ncell"], cfg.get("max_box")), (3 if ng is None else ng)
     lvl = Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=boxes, ng=ngv,
                 layout=layout, storage=storage)
-    lvl.set_phys(cfg["nu"], cfg.get("alpha"))
+    lvl.set_phys(cfg["nu"], cfg.get("alpha"), adiabatic_exponent=cfg.get("adiabatic_exponent", 5.0 / 3.0),
+                 m_bar=cfg.get("m_bar", AIR_MOLAR_MASS))
     lvl.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
     eb = cfg["eb"]
     if eb["type"] == "sphere":
@@ -146,6 +195,12 @@ def build_level(cfg: dict, layout: str = "compact", storage: str = "f64", ng: in
     ic = cfg["ic"]
     if ic["type"] == "taylorgreen":
         lvl.init_taylorgreen(ic["rho0"], ic["v0"], ic["omega"])
+    elif ic["type"] == "plugin":
+        from .ic import fab_fields
+
+        fields = fab_fields(ic["kind"], ic["params"], cfg["ncell"], cfg["prob_lo"], cfg["prob_hi"],
+                            boxes or [((0, 0, 0), tuple(n - 1 for n in cfg["ncell"]))], ngv)
+        lvl.init_fields(*fields, adiabatic_exponent=ic["adiabatic_exponent"], m_bar=ic["m_bar"])
     elif ic["type"] == "spectral":
         if layout != "compact":
             raise ValueError("spectral initial condition: compact layout only")

@@ -88,6 +88,8 @@ int64_t local_of(const BoxDev &B, int i, int j, int k)
     return (int64_t(k - B.lo[2]) * B.n[1] + (j - B.lo[1])) * B.n[0] + (i - B.lo[0]);
 }
 
+// image one period away (AMReX's periodic shifts span one period); a cell
+// more than one period outside a short periodic direction has none
 bool periodic_image(const mrb_level *L, const int iv[3], int s[3])
 {
     for (int d = 0; d < 3; ++d) {
@@ -99,6 +101,7 @@ bool periodic_image(const mrb_level *L, const int iv[3], int s[3])
             if (!L->periodic[d]) return false;
             s[d] -= L->ncell[d];
         }
+        if (s[d] < 0 || s[d] >= L->ncell[d]) return false;
     }
     return true;
 }

@@ -652,6 +652,8 @@ static void build_ghost_map(const mrb_level *L, int depth, std::vector<GhostPair
                     if (i >= B.vlo[0] && i <= B.vhi[0] && j >= B.vlo[1] && j <= B.vhi[1] &&
                         k >= B.vlo[2] && k <= B.vhi[2])
                         continue;
+                    // image one period away (AMReX's periodic shifts span
+                    // one period; with ng > n a ghost can have none)
                     int s[3] = {i, j, k};
                     bool ok = true;
                     for (int d = 0; d < 3; ++d) {
@@ -662,6 +664,7 @@ static void build_ghost_map(const mrb_level *L, int depth, std::vector<GhostPair
                             if (!L->periodic[d]) ok = false;
                             s[d] -= L->ncell[d];
                         }
+                        if (s[d] < 0 || s[d] >= L->ncell[d]) ok = false;
                     }
                     if (!ok) continue;
                     const int ob = owner[size_t((int64_t(s[2]) * nd1 + s[1]) * nd0 + s[0])];

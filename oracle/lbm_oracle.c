@@ -524,6 +524,9 @@ static inline double gas_constant(const orc_level *L)
 /* ghost cells over uncovered index space are left untouched.                */
 /* ------------------------------------------------------------------------ */
 
+/* The image of a ghost cell one period away (AMReX's periodic shifts span
+ * one period); a ghost more than one period outside a short periodic
+ * direction (ng > n) has none and is left as it is. */
 static int periodic_image(const orc_level *L, const int iv[3], int s[3])
 {
     for (int d = 0; d < 3; d++) {
@@ -535,6 +538,7 @@ static int periodic_image(const orc_level *L, const int iv[3], int s[3])
             if (!L->periodic[d]) return 0;
             s[d] -= L->nd[d];
         }
+        if (s[d] < L->dom_lo[d] || s[d] > L->dom_hi[d]) return 0;
     }
     return 1;
 }

@@ -264,10 +264,22 @@ int mrb_iso_forward(
 /* The loss and its exact gradient with respect to tau, n_s (per cell) and
  * u0 (per cell, 3 comps) by the hand-written adjoint of the step, with
  * checkpoints every `every` steps (<= 0: ceil(sqrt(nsteps))) and the
- * states between checkpoints recomputed.  Synchronous. */
+ * states between checkpoints recomputed: mrb_iso_record followed by
+ * mrb_iso_reverse.  Synchronous. */
 int mrb_iso_gradient(
     mrb_iso *iso, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
     double *dtau, double *dns, double *du0, void *stream);
+/* The forward sweep of mrb_iso_gradient alone: nsteps steps keeping the
+ * checkpoints; *J receives the loss.  ns and u0 must stay allocated and
+ * unchanged until mrb_iso_reverse; mrb_iso_forward in between discards the
+ * recording.  Synchronous. */
+int mrb_iso_record(
+    mrb_iso *iso, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
+    void *stream);
+/* The reverse sweep of the last mrb_iso_record (recomputing each earlier
+ * segment from its checkpoint): the gradients as in mrb_iso_gradient.  Uses
+ * the recording up.  Synchronous. */
+int mrb_iso_reverse(mrb_iso *iso, double *dtau, double *dns, double *du0, void *stream);
 /* Final populations of the last mrb_iso_forward (27 N doubles, host). */
 int mrb_iso_state(const mrb_iso *iso, double *host_f);
 

@@ -121,6 +121,8 @@ SIGNATURES = {
     "mrb_iso_forward": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_int, POINTER(c_double), c_void_p]),
     "mrb_iso_gradient": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_int, c_int, POINTER(c_double),
                                  POINTER(c_double), c_void_p, c_void_p, c_void_p]),
+    "mrb_iso_record": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_int, c_int, POINTER(c_double), c_void_p]),
+    "mrb_iso_reverse": (c_int, [c_void_p, POINTER(c_double), c_void_p, c_void_p, c_void_p]),
     "mrb_iso_state": (c_int, [c_void_p, POINTER(c_double)]),
     "mrb_autotune_name": (ctypes.c_char_p, [c_void_p, c_int]),
 }

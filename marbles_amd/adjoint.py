@@ -46,6 +46,7 @@ class IsoSolver:
         ns = _dev(ns, self.shape)
         u0 = _dev(u0, (3,) + self.shape)
         J = c_double()
+        self._rec = None  # the engine's forward replaces a recording
         check(self.lib.mrb_iso_forward(self.h, c_void_p(ns.data_ptr()), float(tau), c_void_p(u0.data_ptr()),
                                        int(nsteps), ctypes.byref(J), self._stream()))
         return J.value
@@ -62,6 +63,32 @@ class IsoSolver:
                                         int(nsteps), int(every), ctypes.byref(J), ctypes.byref(dtau),
                                         c_void_p(dns.data_ptr()), c_void_p(du0.data_ptr()), self._stream()))
         return J.value, dtau.value, dns, du0
+
+    def record(self, ns: torch.Tensor, tau: float, u0: torch.Tensor, nsteps: int, every: int = 0):
+        """Forward sweep keeping the checkpoints for reverse(); returns (J, token).
+        The token names this recording: a later record() or forward() replaces it."""
+        ns = _dev(ns, self.shape)
+        u0 = _dev(u0, (3,) + self.shape)
+        J = c_double()
+        check(self.lib.mrb_iso_record(self.h, c_void_p(ns.data_ptr()), float(tau), c_void_p(u0.data_ptr()),
+                                      int(nsteps), int(every), ctypes.byref(J), self._stream()))
+        self._token = getattr(self, "_token", 0) + 1
+        self._rec = (ns, u0)  # the engine reads them again in reverse()
+        return J.value, self._token
+
+    def reverse(self, token: int):
+        """(dJ/dtau, dJ/dn_s, dJ/du0) of the recording `token`, or None when a
+        later record() or forward() has replaced it."""
+        if getattr(self, "_token", None) != token or getattr(self, "_rec", None) is None:
+            return None
+        ns, u0 = self._rec
+        dns = torch.empty_like(ns)
+        du0 = torch.empty_like(u0)
+        dtau = c_double()
+        check(self.lib.mrb_iso_reverse(self.h, ctypes.byref(dtau), c_void_p(dns.data_ptr()),
+                                       c_void_p(du0.data_ptr()), self._stream()))
+        self._rec = None
+        return dtau.value, dns, du0
 
     def state(self) -> np.ndarray:
         """Final populations of the last forward run, shape (27, nz, ny, nx)."""
@@ -82,20 +109,30 @@ class IsoSolver:
 
 
 class FlowLoss(torch.autograd.Function):
-    """J(n_s, tau, u0); backward returns the adjoint gradients."""
+    """J(n_s, tau, u0).  The forward pass records the checkpointed forward
+    sweep; backward runs the reverse sweep (or, if another evaluation on the
+    same solver has replaced the recording, the whole gradient again)."""
 
     @staticmethod
     def forward(ctx, ns, tau, u0, solver: IsoSolver, nsteps: int, every: int = 0):
         if any(ctx.needs_input_grad[:3]):
-            J, dtau, dns, du0 = solver.gradient(ns.detach(), float(tau), u0.detach(), nsteps, every)
-            ctx.save_for_backward(dns, du0, torch.tensor(dtau, dtype=torch.float64, device=ns.device))
+            J, token = solver.record(ns.detach(), float(tau), u0.detach(), nsteps, every)
+            ctx.solver, ctx.token, ctx.args = solver, token, (float(tau), nsteps, every)
+            ctx.save_for_backward(ns.detach(), u0.detach())
         else:
             J = solver.forward(ns.detach(), float(tau), u0.detach(), nsteps)
         return torch.tensor(J, dtype=torch.float64, device=ns.device)
 
     @staticmethod
     def backward(ctx, gJ):
-        dns, du0, dtau = ctx.saved_tensors
+        ns, u0 = ctx.saved_tensors
+        got = ctx.solver.reverse(ctx.token)
+        if got is None:
+            tau, nsteps, every = ctx.args
+            _, dtau, dns, du0 = ctx.solver.gradient(ns, tau, u0, nsteps, every)
+        else:
+            dtau, dns, du0 = got
+        dtau = torch.tensor(dtau, dtype=torch.float64, device=ns.device)
         return gJ * dns, (gJ * dtau).reshape(()), gJ * du0, None, None, None
 
 

@@ -56,6 +56,11 @@ struct mrb_iso {
     // checkpoints (one per segment), the states and Q arrays of one segment;
     // kept between gradient calls of the same segment shape
     std::vector<double *> ckpt, seg, segq;
+    // the recorded forward sweep that mrb_iso_reverse differentiates
+    bool rec_valid = false;
+    int rec_nsteps = 0, rec_every = 0;
+    double rec_tau = 0.0;
+    const double *rec_ns = nullptr, *rec_u0 = nullptr;
 };
 
 namespace mrb {
@@ -604,6 +609,7 @@ int mrb_iso_forward(mrb_iso *S, const double *ns, double tau, const double *u0, 
     if (!S || !ns || !u0 || !J) return set_error(MRB_E_ARG, "null argument");
     if (!(tau > 0.5) || nsteps < 0) return set_error(MRB_E_ARG, "tau must exceed 1/2 and nsteps be >= 0");
     hipStream_t s = as_stream(stream);
+    S->rec_valid = false;  // the ping-pong pair holds the recording's final state
     const IsoArgs A = iso_args(S, tau);
     k_iso_init<<<iso_grid(A), dim3(64, 4, 1), 0, s>>>(S->f[0], u0, A, S->rho0);
     MRB_HIP(hipGetLastError());
@@ -618,13 +624,14 @@ int mrb_iso_forward(mrb_iso *S, const double *ns, double tau, const double *u0, 
     return iso_loss(S, A, S->f[0], s, J);
 }
 
-int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
-                     double *dtau, double *dns, double *du0, void *stream)
+int mrb_iso_record(mrb_iso *S, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
+                   void *stream)
 {
-    if (!S || !ns || !u0 || !J || !dtau || !dns || !du0) return set_error(MRB_E_ARG, "null argument");
+    if (!S || !ns || !u0 || !J) return set_error(MRB_E_ARG, "null argument");
     if (!(tau > 0.5) || nsteps < 1) return set_error(MRB_E_ARG, "tau must exceed 1/2 and nsteps be >= 1");
     if (every < 1) every = std::max(1, int(std::ceil(std::sqrt(double(nsteps)))));
     every = std::min(every, nsteps);
+    S->rec_valid = false;
     hipStream_t s = as_stream(stream);
     const IsoArgs A = iso_args(S, tau);
     const dim3 g = iso_grid(A);
@@ -635,7 +642,6 @@ int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0,
         const int rc = iso_ensure_states(S, nseg, every);
         if (rc) return rc;
     }
-    // state i of segment sg: its checkpoint for i = 0, else a segment buffer
     auto state = [&](int sg, int i) { return i == 0 ? S->ckpt[size_t(sg)] : S->seg[size_t(i - 1)]; };
     // forward sweep: checkpoints written in place, the last segment's states
     // and Q arrays kept, the other states in the ping-pong pair
@@ -655,13 +661,34 @@ int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0,
         k_iso_step<<<g, b, 0, s>>>(fn, fo, qn, ns, A);
         MRB_HIP(hipGetLastError());
     }
-    const double *ffin = S->f[nsteps & 1];
-    int rc = iso_loss(S, A, ffin, s, J);
+    const int rc = iso_loss(S, A, S->f[nsteps & 1], s, J);
     if (rc) return rc;
+    S->rec_valid = true;
+    S->rec_nsteps = nsteps;
+    S->rec_every = every;
+    S->rec_tau = tau;
+    S->rec_ns = ns;
+    S->rec_u0 = u0;
+    return MRB_OK;
+}
+
+int mrb_iso_reverse(mrb_iso *S, double *dtau, double *dns, double *du0, void *stream)
+{
+    if (!S || !dtau || !dns || !du0) return set_error(MRB_E_ARG, "null argument");
+    if (!S->rec_valid) return set_error(MRB_E_STATE, "no recorded forward sweep (mrb_iso_record)");
+    hipStream_t s = as_stream(stream);
+    const int nsteps = S->rec_nsteps, every = S->rec_every;
+    const double tau = S->rec_tau;
+    const double *ns = S->rec_ns, *u0 = S->rec_u0;
+    const IsoArgs A = iso_args(S, tau);
+    const dim3 g = iso_grid(A);
+    const dim3 b(64, 4, 1);
+    const int nseg = (nsteps + every - 1) / every;
+    auto state = [&](int sg, int i) { return i == 0 ? S->ckpt[size_t(sg)] : S->seg[size_t(i - 1)]; };
     // reverse sweep: two passes per step
     MRB_HIP(hipMemsetAsync(dns, 0, sizeof(double) * size_t(S->N), s));
     MRB_HIP(hipMemsetAsync(S->part, 0, sizeof(double) * size_t(S->nparts), s));
-    k_iso_seed<<<g, b, 0, s>>>(ffin, S->lam[0], A);
+    k_iso_seed<<<g, b, 0, s>>>(S->f[nsteps & 1], S->lam[0], A);
     MRB_HIP(hipGetLastError());
     int lc = 0;
     bool first = true;
@@ -694,6 +721,9 @@ int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0,
             first = false;
         }
     }
+    // the segment buffers now hold recomputed states of the first segment;
+    // the recording is used up
+    S->rec_valid = false;
     k_iso_u0grad<<<g, b, 0, s>>>(S->lam[lc], S->ab, u0, du0, A, S->rho0);
     MRB_HIP(hipGetLastError());
     std::vector<double> part(size_t(S->nparts));
@@ -703,6 +733,15 @@ int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0,
     for (double v : part) gw += v;
     *dtau = gw * (-1.0 / (tau * tau));  // w = 1/tau
     return MRB_OK;
+}
+
+int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
+                     double *dtau, double *dns, double *du0, void *stream)
+{
+    if (!S || !ns || !u0 || !J || !dtau || !dns || !du0) return set_error(MRB_E_ARG, "null argument");
+    const int rc = mrb_iso_record(S, ns, tau, u0, nsteps, every, J, stream);
+    if (rc) return rc;
+    return mrb_iso_reverse(S, dtau, dns, du0, stream);
 }
 
 int mrb_iso_state(const mrb_iso *S, double *host_f)

@@ -115,3 +115,35 @@ def test_autograd_function_returns_the_adjoint_gradients(solver):
     _, dtau, dns, du0 = solver.gradient(ns.detach(), TAU, u0.detach(), NSTEPS)
     assert tau.grad.item() == 2.0 * dtau
     assert torch.equal(ns.grad, 2.0 * dns) and torch.equal(u0.grad, 2.0 * du0)
+
+
+def test_record_then_reverse_equals_the_gradient(solver):
+    ns, u0 = _inputs()
+    ns, u0 = ns.cuda(), u0.cuda()
+    J, dtau, dns, du0 = solver.gradient(ns, TAU, u0, NSTEPS, every=4)
+    J2, token = solver.record(ns, TAU, u0, NSTEPS, every=4)
+    dtau2, dns2, du02 = solver.reverse(token)
+    assert J == J2 and dtau == dtau2
+    assert torch.equal(dns, dns2) and torch.equal(du0, du02)
+    # the recording is used up, and a forward run replaces a new one
+    assert solver.reverse(token) is None
+    _, token = solver.record(ns, TAU, u0, NSTEPS)
+    solver.forward(ns, TAU, u0, NSTEPS)
+    assert solver.reverse(token) is None
+
+
+def test_autograd_with_two_evaluations_before_backward(solver):
+    """The second evaluation replaces the first one's recording; the first
+    one's backward then recomputes its gradient from scratch."""
+    from marbles_amd.adjoint import flow_loss
+
+    ns, u0 = _inputs()
+    ns = ns.cuda().requires_grad_(True)
+    u0 = u0.cuda().requires_grad_(True)
+    t1 = torch.tensor(TAU, dtype=torch.float64, device="cuda", requires_grad=True)
+    t2 = torch.tensor(0.7, dtype=torch.float64, device="cuda", requires_grad=True)
+    (flow_loss(solver, ns, t1, u0, NSTEPS) + flow_loss(solver, ns, t2, u0, NSTEPS)).backward()
+    _, d1, n1, v1 = solver.gradient(ns.detach(), TAU, u0.detach(), NSTEPS)
+    _, d2, n2, v2 = solver.gradient(ns.detach(), 0.7, u0.detach(), NSTEPS)
+    assert t1.grad.item() == d1 and t2.grad.item() == d2
+    assert torch.allclose(ns.grad, n1 + n2, rtol=1e-14, atol=0) and torch.allclose(u0.grad, v1 + v2, rtol=1e-14, atol=0)

@@ -305,6 +305,8 @@ def run_precision(args, torch) -> int:
         lvl = Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], layout="compact", storage=st)
         lvl.set_phys(cfg["nu"], cfg.get("alpha"))
         lvl.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+        if st != "f64":
+            lvl.set_shift_state(*configs.reference_state(cfg))
         lvl.flags_all_regular()
         lvl.init_fields(*fields, adiabatic_exponent=cfg["ic"]["adiabatic_exponent"])
         torch.cuda.synchronize()

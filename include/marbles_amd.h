@@ -113,7 +113,10 @@ int mrb_level_destroy(mrb_level *lvl);
 int mrb_set_phys(mrb_level *lvl, const mrb_phys *phys);
 int mrb_set_bc(mrb_level *lvl, const mrb_bc *bc);
 /* Reference shift state used by the reduced-precision storages:
- * rho0, T0 (f_eq and g_eq at rest).  Defaults: 1, 1/3. */
+ * rho0, T0 (f_eq and g_eq at rest).  Defaults: 1, 1/3.  Set it to the
+ * flow's reference density and temperature before the initial state: the
+ * stored values are the differences to it, and their rounding error scales
+ * with their size. */
 int mrb_set_shift_state(mrb_level *lvl, double rho0, double T0);
 /* Element count of one box's host fab for `arr`. */
 int64_t mrb_fab_size(const mrb_level *lvl, int arr, int box);

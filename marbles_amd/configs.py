@@ -187,6 +187,9 @@ def build_level(cfg: dict, layout: str = "compact", storage: str = "f64", ng: in
     lvl.set_phys(cfg["nu"], cfg.get("alpha"), adiabatic_exponent=cfg.get("adiabatic_exponent", 5.0 / 3.0),
                  m_bar=cfg.get("m_bar", AIR_MOLAR_MASS))
     lvl.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+    if storage != "f64":
+        # the shifted storages subtract the rest equilibrium of this state
+        lvl.set_shift_state(*reference_state(cfg))
     eb = cfg["eb"]
     if eb["type"] == "sphere":
         lvl.flags_sphere(eb["center"], eb["radius"], eb.get("has_fluid_inside", 0))
@@ -208,6 +211,18 @@ def build_level(cfg: dict, layout: str = "compact", storage: str = "f64", ng: in
     else:
         lvl.init_constant(ic["density"], ic["mach"], ic["temperature"], ic["adiabatic_exponent"])
     return lvl
+
+
+def reference_state(cfg: dict) -> tuple[float, float]:
+    """(rho0, T0) of a workload's initial state: the rest state the shifted
+    population storages subtract."""
+    ic = cfg["ic"]
+    if ic["type"] == "taylorgreen":
+        return float(ic["rho0"]), 1.0 / 3.0
+    if ic["type"] == "plugin":
+        p = ic["params"]
+        return float(p.get("density", 1.0)), float(p.get("initial_temperature", 1.0 / 3.0))
+    return float(ic.get("density", 1.0)), float(ic.get("temperature", cfg.get("T0", 1.0 / 3.0)))
 
 
 def mlups(cells: int, steps: int, seconds: float) -> float:

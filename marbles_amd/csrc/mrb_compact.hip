@@ -186,8 +186,9 @@ __device__ __forceinline__ void profile_ghost(const BCDev &bc, const bool gx[3],
 // read the cell's own pre-stream populations, OUTFLOW the boundary cell
 // next to the ghost (at a corner of two non-periodic faces, clamped in
 // both), VELOCITY a constant.
-// W = the working type of the pulled values (double)
-template <class T, class W>
+// W = the working type of the pulled values (double); PROF: the level has a
+// Channel or Parabolic inlet profile (ghost values from the face tables)
+template <bool PROF, class T, class W>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
     const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, W fv[NQ],
@@ -308,7 +309,8 @@ __device__ __forceinline__ void pull_cell(
             const W vf = W(float(rf[q]) * sf) + W(sh.f(q));
             const W vg = W(float(rg[q]) * sg) + W(sh.g(q));
             double cf = bc.fval[q], cg = bc.gval[q];
-            if (bc.prof.kind && use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
+            if constexpr (PROF)
+                if (use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
             fv[q] = use_const ? W(cf) : vf;
             gv[q] = use_const ? W(cg) : vg;
         });
@@ -318,7 +320,8 @@ __device__ __forceinline__ void pull_cell(
             constexpr int q = decltype(qc)::value;
             const bool use_const = (cmask >> q) & 1u;
             double cf = bc.fval[q], cg = bc.gval[q];
-            if (bc.prof.kind && use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
+            if constexpr (PROF)
+                if (use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
             fv[q] = use_const ? cf : widen(rf[q], sh.f(q));
             gv[q] = use_const ? cg : widen(rg[q], sh.g(q));
         });
@@ -337,7 +340,7 @@ __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y
 
 // pass 1: Q_corr of the post-stream state.  With FP16 storage it also
 // clears the box maxima that pass 2 accumulates for the buffer it writes.
-template <class T>
+template <class T, bool PROF>
 __global__ void __launch_bounds__(256) k_qpass(
     const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask,
     double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, Half16 h16, CollideParams P)
@@ -356,7 +359,7 @@ __global__ void __launch_bounds__(256) k_qpass(
     const uint32_t m = mask[cell];
     if (!(m & 1u)) return;
     double fv[NQ], gv[NQ];
-    pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
+    pull_cell<PROF>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
     double rho, jx, jy, jz, E;
     accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
     const Macro M = macro_from(rho, jx, jy, jz, E, P);
@@ -401,7 +404,7 @@ __device__ __forceinline__ void half_block_stats(const Half16 &h16, int ef, int 
 }
 
 // pass 2: stream + collide, writes the B buffers
-template <class T>
+template <class T, bool PROF>
 __device__ __forceinline__ void collide_body(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
@@ -444,7 +447,7 @@ __device__ __forceinline__ void collide_body(
         });
     } else if (in) {
         double fv[NQ], gv[NQ];
-        pull_cell(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
+        pull_cell<PROF>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
         Moments mo;
         accumulate_moments(fv, gv, mo);
         const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
@@ -465,28 +468,28 @@ __device__ __forceinline__ void collide_body(
 }
 
 // register budget as the compiler chooses (2 waves per SIMD)
-template <class T>
+template <class T, bool PROF>
 __global__ void __launch_bounds__(256) k_collide(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
     Half16 h16, CollideParams P)
 {
-    collide_body(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+    collide_body<T, PROF>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
 // at most 168 registers (3 waves per SIMD), some values spilled to scratch
-template <class T>
+template <class T, bool PROF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_collide_w3(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
     Half16 h16, CollideParams P)
 {
-    collide_body(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+    collide_body<T, PROF>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
 // stream only (parity of the pull with the reference push); FP64 and
 // shifted FP32 storage
-template <class T>
+template <class T, bool PROF>
 __global__ void __launch_bounds__(256) k_stream_only(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, CompactArgs A, BCDev bc, ShiftTab sh)
@@ -497,7 +500,7 @@ __global__ void __launch_bounds__(256) k_stream_only(
     const uint32_t m = mask[cell];
     double fv[NQ], gv[NQ];
     if (m & 1u) {
-        pull_cell(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
+        pull_cell<PROF>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
     } else {
         for (int q = 0; q < NQ; ++q) fv[q] = gv[q] = -1.0;
     }
@@ -591,8 +594,8 @@ static Half16 half16_of(mrb_level *L)
     return h;
 }
 
-template <class T>
-static int two_pass_t(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
+template <class T, bool PROF>
+static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
     const CompactArgs A = make_args(L);
     const ShiftTab sh = shift_tab(L);
@@ -602,18 +605,26 @@ static int two_pass_t(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     T *fo = (T *)L->pop[0][1 - L->cur];
     T *go = (T *)L->pop[1][1 - L->cur];
     if (ev) MRB_HIP(hipEventRecord(ev[0], s));
-    k_qpass<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+    k_qpass<T, PROF><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[1], s));
     if (variant == 1)
-        k_collide_w3<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh,
-                                                              h16, L->cp);
+        k_collide_w3<T, PROF><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev,
+                                                                    sh, h16, L->cp);
     else
-        k_collide<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
-                                                           L->cp);
+        k_collide<T, PROF><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh,
+                                                                 h16, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[2], s));
     return MRB_OK;
+}
+
+// the inlet-profile lookups are compiled only into the kernels of levels
+// that have one
+template <class T>
+static int two_pass_t(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
+{
+    return L->bcdev.prof.kind ? two_pass_tp<T, true>(L, variant, s, ev) : two_pass_tp<T, false>(L, variant, s, ev);
 }
 
 static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
@@ -672,14 +683,21 @@ int compact_stream_only(mrb_level *L, hipStream_t s)
 {
     const CompactArgs A = make_args(L);
     const ShiftTab sh = shift_tab(L);
+    const bool prof = L->bcdev.prof.kind != 0;
     if (L->storage == MRB_STORAGE_F64) {
-        k_stream_only<double><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(
-            (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur],
-            (double *)L->pop[0][1 - L->cur], (double *)L->pop[1][1 - L->cur], L->d_mask, A, L->bcdev, sh);
+        const double *f = (const double *)L->pop[0][L->cur], *g = (const double *)L->pop[1][L->cur];
+        double *fo = (double *)L->pop[0][1 - L->cur], *go = (double *)L->pop[1][1 - L->cur];
+        if (prof)
+            k_stream_only<double, true><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
+        else
+            k_stream_only<double, false><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
     } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
-        k_stream_only<float><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(
-            (const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur], (float *)L->pop[0][1 - L->cur],
-            (float *)L->pop[1][1 - L->cur], L->d_mask, A, L->bcdev, sh);
+        const float *f = (const float *)L->pop[0][L->cur], *g = (const float *)L->pop[1][L->cur];
+        float *fo = (float *)L->pop[0][1 - L->cur], *go = (float *)L->pop[1][1 - L->cur];
+        if (prof)
+            k_stream_only<float, true><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
+        else
+            k_stream_only<float, false><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
     } else {
         return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
     }

@@ -188,6 +188,7 @@ struct mrb_level {
     bool phys_set = false;
     bool flags_ready = false;
     double shift_rho0 = 1.0, shift_T0 = 1.0 / 3.0;
+    bool pops_set = false;  // populations uploaded or initialised
     double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
     // FP16 storage (compact layout): power-of-two scale exponent and the
     // largest |population - shift| over fluid cells per (scale box, lattice),

@@ -823,6 +823,17 @@ static void compute_shift(mrb_level *L)
     }
 }
 
+// A reduced-precision level holds population - shift: once populations
+// are stored, a different shift would silently reinterpret them.
+static bool shift_locked(const mrb_level *L, double m_bar, double gamma, double rho0, double T0)
+{
+    if (L->storage == MRB_STORAGE_F64 || !L->pops_set) return false;
+    const double R_new = kAirMolarMass / m_bar;
+    const double R_old = kAirMolarMass / (L->phys_set ? L->phys.m_bar : kAirMolarMass);
+    const double g_old = L->phys_set ? L->phys.adiabatic_exponent : 5.0 / 3.0;
+    return R_new != R_old || gamma != g_old || rho0 != L->shift_rho0 || T0 != L->shift_T0;
+}
+
 static size_t pop_elem_size(int storage)
 {
     return storage == MRB_STORAGE_F64 ? 8 : storage == MRB_STORAGE_F32_SHIFTED ? 4 : 2;
@@ -1050,6 +1061,8 @@ int mrb_set_phys(mrb_level *L, const mrb_phys *p)
     if (!L || !p) return set_error(MRB_E_ARG, "null argument");
     if (!(p->dt > 0.0) || !(p->mesh_speed > 0.0) || !(p->adiabatic_exponent > 1.0) || !(p->m_bar > 0.0))
         return set_error(MRB_E_ARG, "invalid physics parameters");
+    if (shift_locked(L, p->m_bar, p->adiabatic_exponent, L->shift_rho0, L->shift_T0))
+        return set_error(MRB_E_STATE, "reduced-precision level: R and gamma must be set before the populations");
     L->phys = *p;
     L->phys_set = true;
     ++L->gen;
@@ -1100,6 +1113,11 @@ int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
 int mrb_set_shift_state(mrb_level *L, double rho0, double T0)
 {
     if (!L) return set_error(MRB_E_ARG, "null argument");
+    if (!(rho0 > 0.0) || !(T0 > 0.0)) return set_error(MRB_E_ARG, "rho0 and T0 must be positive");
+    const double m_bar = L->phys_set ? L->phys.m_bar : kAirMolarMass;
+    const double gamma = L->phys_set ? L->phys.adiabatic_exponent : 5.0 / 3.0;
+    if (shift_locked(L, m_bar, gamma, rho0, T0))
+        return set_error(MRB_E_STATE, "reduced-precision level: set the shift state before the populations");
     L->shift_rho0 = rho0;
     L->shift_T0 = T0;
     compute_shift(L);
@@ -1164,6 +1182,7 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
     }
     if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
     L->fresh_depth = 0;
+    L->pops_set = true;
     const int64_t n = NQ * L->total_cells;
     const double *h = static_cast<const double *>(host);
     if (L->storage == MRB_STORAGE_F64) return copy_lattice(L, arr, const_cast<void *>(host), true);
@@ -1369,6 +1388,7 @@ static int init_from_state(
 
 int mrb_init_taylorgreen(mrb_level *L, double rho0, double v0, const double omega[3])
 {
+    if (L) L->pops_set = true;
     if (!L || !omega) return set_error(MRB_E_ARG, "null argument");
     const double Lw = 1.0 / M_PI;
     const double o0 = omega[0], o1 = omega[1], o2 = omega[2];
@@ -1393,6 +1413,7 @@ int mrb_init_constant(
     mrb_level *L, double density, const double mach[3], double temperature, double adiabatic_exponent,
     double m_bar)
 {
+    if (L) L->pops_set = true;
     if (!L || !mach) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
     // ic::Constant (IC.cpp:4-32): velocity = mach * speed_of_sound_ref
@@ -1434,6 +1455,7 @@ int mrb_init_fields(
     mrb_level *L, const double *rho, const double *u, const double *v, const double *w, const double *T,
     double adiabatic_exponent, double m_bar)
 {
+    if (L) L->pops_set = true;
     if (!L || !rho || !u || !v || !w || !T) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
     if (!(m_bar > 0.0) || !(adiabatic_exponent > 1.0)) return set_error(MRB_E_ARG, "invalid gas parameters");

@@ -116,7 +116,9 @@ int mrb_set_bc(mrb_level *lvl, const mrb_bc *bc);
  * rho0, T0 (f_eq and g_eq at rest).  Defaults: 1, 1/3.  Set it to the
  * flow's reference density and temperature before the initial state: the
  * stored values are the differences to it, and their rounding error scales
- * with their size. */
+ * with their size.  On a reduced-precision level with populations set, a
+ * call (or an mrb_set_phys) that would change the shift returns
+ * MRB_E_STATE. */
 int mrb_set_shift_state(mrb_level *lvl, double rho0, double T0);
 /* Element count of one box's host fab for `arr`. */
 int64_t mrb_fab_size(const mrb_level *lvl, int arr, int box);

@@ -62,25 +62,38 @@ __global__ void __launch_bounds__(kAmrBlock) k_explode(
     const BoxDev &CB = cboxes[e.crse_box];
     const double *c0 = crse + CB.base * NQ + e.crse_local;
     double *fo = fine + FB.base * NQ + e.fine_local;
-    const int64_t stride[3] = {1, CB.n[0], int64_t(CB.n[0]) * CB.n[1]};
+    // a direction without both neighbours reads the cell itself (offset 0):
+    // then dl = dr = 0 and the limiter gives the zero slope, with every load
+    // in bounds and issued unconditionally
+    int64_t stride[3] = {1, CB.n[0], int64_t(CB.n[0]) * CB.n[1]};
     double o[3];
-    for (int d = 0; d < 3; ++d) o[d] = ((e.bits >> d) & 1) ? 0.25 : -0.25;
-    for (int q = 0; q < NQ; ++q) {
-        const double *cq = c0 + q * CB.ncell;
-        const double c = cq[0];
+    for (int d = 0; d < 3; ++d) {
+        o[d] = ((e.bits >> d) & 1) ? 0.25 : -0.25;
+        if (!((e.bits >> (3 + d)) & 1)) stride[d] = 0;
+    }
+    double v[NQ][7];
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double *cq = c0 + int64_t(q) * CB.ncell;
+        v[q][0] = cq[0];
+        for (int d = 0; d < 3; ++d) {
+            v[q][1 + 2 * d] = cq[-stride[d]];
+            v[q][2 + 2 * d] = cq[stride[d]];
+        }
+    });
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double c = v[q][0];
         double s[3];
         for (int d = 0; d < 3; ++d) {
-            s[d] = 0.0;
-            if (!((e.bits >> (3 + d)) & 1)) continue;
-            const double cm = cq[-stride[d]];
-            const double cp = cq[stride[d]];
+            const double cm = v[q][1 + 2 * d], cp = v[q][2 + 2 * d];
             const double dc = 0.5 * (cp - cm);
             const double dl = c - cm;
             const double dr = cp - c;
-            if (dl * dr > 0.0) s[d] = copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc);
+            s[d] = dl * dr > 0.0 ? copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc) : 0.0;
         }
-        fo[q * FB.ncell] = ((c + s[0] * o[0]) + s[1] * o[1]) + s[2] * o[2];
-    }
+        fo[int64_t(q) * FB.ncell] = ((c + s[0] * o[0]) + s[1] * o[1]) + s[2] * o[2];
+    });
 }
 
 int64_t local_of(const BoxDev &B, int i, int j, int k)

@@ -305,20 +305,29 @@ __global__ void __launch_bounds__(256) k_stream_listed(
     double *fob = fo + B.base * NQ;
     double *gob = go + B.base * NQ;
     const int64_t sy = B.n[0], sz = nxy;
+    // every load issued unconditionally (a source outside the fab, or any
+    // source of a solid cell, reads the cell itself), the -1 sentinel
+    // selected afterwards
+    double vf[NQ], vg[NQ];
+    uint32_t keep = 0;
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
         const int a = ii - cx, b = jj - cy, c = kk - cz;
         const bool in_fab = a >= 0 && a < B.n[0] && b >= 0 && b < B.n[1] && c >= 0 && c < B.n[2];
-        double vf = -1.0, vg = -1.0;
-        if (fluid && in_fab) {
-            const int64_t idx = ((m >> q) & 1u) ? int64_t(q) * B.ncell + loc - (cx + cy * sy + cz * sz)
-                                                : int64_t(kBounce[q]) * B.ncell + loc;
-            vf = fb[idx];
-            vg = gb[idx];
-        }
-        fob[q * B.ncell + loc] = vf;
-        gob[q * B.ncell + loc] = vg;
+        const bool use = fluid && in_fab;
+        const bool direct = use && ((m >> q) & 1u);
+        const int64_t idx = direct ? int64_t(q) * B.ncell + loc - (cx + cy * sy + cz * sz)
+                                   : int64_t(kBounce[q]) * B.ncell + loc;
+        vf[q] = fb[idx];
+        vg[q] = gb[idx];
+        keep |= uint32_t(use) << q;
+    });
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const bool use = (keep >> q) & 1u;
+        fob[q * B.ncell + loc] = use ? vf[q] : -1.0;
+        gob[q * B.ncell + loc] = use ? vg[q] : -1.0;
     });
 }
 

@@ -87,19 +87,20 @@ __device__ __forceinline__ int64_t loc_of(const BoxDev &B, int i, int j, int k)
 // ---------------------------------------------------------------------------
 // FillBoundary: ghost <- owner valid cell, all components
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, int NC>
 __global__ void k_fill_map(
-    T *__restrict__ arr, int ncomp, const BoxDev *__restrict__ boxes,
-    const GhostPair *__restrict__ map, int64_t nmap)
+    T *__restrict__ arr, const BoxDev *__restrict__ boxes, const GhostPair *__restrict__ map, int64_t nmap)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= nmap) return;
     const GhostPair p = map[t];
-    const BoxDev &D = boxes[p.dst_box];
-    const BoxDev &S = boxes[p.src_box];
-    T *dst = arr + D.base * ncomp + p.dst_local;
-    const T *src = arr + S.base * ncomp + p.src_local;
-    for (int c = 0; c < ncomp; ++c) dst[c * D.ncell] = src[c * S.ncell];
+    const int64_t dn = boxes[p.dst_box].ncell, sn = boxes[p.src_box].ncell;
+    T *dst = arr + boxes[p.dst_box].base * NC + p.dst_local;
+    const T *src = arr + boxes[p.src_box].base * NC + p.src_local;
+    // all NC loads in flight before the stores
+    T v[NC];
+    static_for<NC>([&](auto cc) { v[decltype(cc)::value] = src[decltype(cc)::value * sn]; });
+    static_for<NC>([&](auto cc) { dst[decltype(cc)::value * dn] = v[decltype(cc)::value]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -686,10 +687,16 @@ static int fill_boundary(mrb_level *L, void *arr, int ncomp, bool is_int, bool o
     const GhostPair *map = one_layer ? L->d_gmap1 : L->d_gmap;
     const int64_t n = one_layer ? L->n_gmap1 : L->n_gmap;
     if (n == 0) return MRB_OK;
-    if (is_int)
-        k_fill_map<int32_t><<<nblocks(n), kBlock, 0, s>>>((int32_t *)arr, ncomp, L->d_boxes, map, n);
+    if (is_int && ncomp == 2)
+        k_fill_map<int32_t, 2><<<nblocks(n), kBlock, 0, s>>>((int32_t *)arr, L->d_boxes, map, n);
+    else if (!is_int && ncomp == NQ)
+        k_fill_map<double, NQ><<<nblocks(n), kBlock, 0, s>>>((double *)arr, L->d_boxes, map, n);
+    else if (!is_int && ncomp == 3)
+        k_fill_map<double, 3><<<nblocks(n), kBlock, 0, s>>>((double *)arr, L->d_boxes, map, n);
+    else if (!is_int && ncomp == 19)
+        k_fill_map<double, 19><<<nblocks(n), kBlock, 0, s>>>((double *)arr, L->d_boxes, map, n);
     else
-        k_fill_map<double><<<nblocks(n), kBlock, 0, s>>>((double *)arr, ncomp, L->d_boxes, map, n);
+        return set_error(MRB_E_ARG, "fill_boundary: unsupported component count");
     MRB_HIP(hipGetLastError());
     return MRB_OK;
 }

@@ -249,8 +249,9 @@ def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
     """FP16 storage of (f - f_eq(rho0, 0, T0)) * 2^e, e per (z-slab box,
     lattice), FP64 arithmetic: velocity and density against the FP64 run
     after 100 steps (FP16 keeps 11 significant bits, so the bars are set by
-    that, not by 1e-6); no stored value saturates; conservation sums in FP64
-    agree with the FP64 run's."""
+    that, not by 1e-6; measured 3.6e-4 / 8.7e-4 velocity); no stored value
+    saturates; conservation sums in FP64 agree with the FP64 run's; the
+    shift cannot change once populations are stored."""
     if case == "tg_tau1.0":
         cfg, n = configs.taylor_green(32, tau=1.0), 100
     else:
@@ -263,6 +264,10 @@ def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
         cons[storage] = E.conservation()
         if storage == "f16s":
             assert E.storage_saturations() == 0
+            from marbles_amd._lib import MrbError
+
+            with pytest.raises(MrbError):
+                E.set_shift_state(1.0, 0.25)
         E.close()
     (u64, r64), (u16, r16) = runs["f64"], runs["f16s"]
     fluid = r64 > 0
@@ -270,7 +275,7 @@ def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
     dr = np.abs(r16 - r64)[fluid].max() / r64[fluid].max()
     dm = abs(cons["f16s"][0] - cons["f64"][0]) / abs(cons["f64"][0])
     print(f"{case}: {n} steps, FP16 max rel velocity error {du:.2e}, density {dr:.2e}, mass {dm:.2e}")
-    assert du <= 2e-2 and dr <= 2e-3 and dm <= 1e-4, (du, dr, dm)
+    assert du <= 5e-3 and dr <= 1e-3 and dm <= 1e-4, (du, dr, dm)
 
 
 def test_f16_scaled_round_trip(oracle, engine):

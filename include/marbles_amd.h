@@ -50,7 +50,9 @@ extern "C" {
 #define MRB_BC_PERIODIC 0
 #define MRB_BC_NOSLIP 1
 #define MRB_BC_VELOCITY 2
-#define MRB_BC_PRESSURE 3 /* not supported */
+#define MRB_BC_PRESSURE 3 /* pressure_bc_f/g (BC.H:222-321); mrb_set_bc requires the face's
+                             tangential axes periodic or NOSLIP on both sides, and two
+                             cells or more along its own axis */
 #define MRB_BC_OUTFLOW 5
 #define MRB_BC_SLIPX 6
 #define MRB_BC_SLIPY 7

@@ -112,6 +112,64 @@ def channel_profile(ncell=(64, 32, 32), kind: str = "channel", tau: float = 0.55
     }
 
 
+def pressure_case(kind: str, ncell=(40, 20, 24), tau: float = 0.6, u: float = 0.04, temperature: float = 0.2,
+                  max_box: int | None = None) -> dict:
+    """Workloads with PRESSURE faces (pressure_bc_f / pressure_bc_g,
+    BC.H:222-321); the boundary density and temperature come from the
+    velocity operator (1 and 1/3 with none).
+      "outlet":  velocity inlet u at x-lo, pressure outlet at x-hi (rho_bc 1),
+                 y/z periodic, a sphere upstream of the outlet; starts at rest.
+      "outlet_body": the same with the sphere reaching into the two layers
+                 next to the outlet: the outlet's sums then read the solid
+                 cells' contents (0, then the stream's -1), and the state
+                 blows up within a few steps, as in the reference.
+      "walled":  velocity inlet u along z at z-lo, pressure outlet at z-hi,
+                 no-slip x walls, y periodic; starts at rest.
+      "both":    pressure at both x faces (no operator: rho_bc 1, T 1/3),
+                 no-slip y walls, z periodic; starts at rho 1.01, T0.
+      "channel": the Channel profile inlet at x-lo, pressure outlet at
+                 x-hi, no-slip y and z walls."""
+    gamma = 5.0 / 3.0
+    mach = u / math.sqrt(gamma * temperature)
+    vbc = {"kind": "constant", "dir": 0, "Mach_ref": mach, "initial_density": 1.0,
+           "initial_temperature": temperature, "adiabatic_exponent": gamma, "mean_molecular_mass": AIR_MOLAR_MASS}
+    eb = {"type": "all_regular"}
+    density = 1.0
+    if kind in ("outlet", "outlet_body"):
+        periodic, bc_lo, bc_hi = (0, 1, 1), (2, 0, 0), (3, 0, 0)
+        cx = ncell[0] - (2.5 if kind == "outlet_body" else 10.0)
+        eb = {"type": "sphere", "center": (cx, ncell[1] / 2, ncell[2] / 2), "radius": 4.0, "has_fluid_inside": 0}
+    elif kind == "walled":
+        periodic, bc_lo, bc_hi = (0, 1, 0), (1, 0, 2), (1, 0, 3)
+        vbc["dir"] = 2
+    elif kind == "both":
+        periodic, bc_lo, bc_hi = (0, 0, 1), (3, 1, 0), (3, 1, 0)
+        vbc = None
+        density = 1.01
+    elif kind == "channel":
+        periodic, bc_lo, bc_hi = (0, 0, 0), (2, 1, 1), (3, 1, 1)
+        vbc = dict(vbc, kind="channel")
+    else:
+        raise ValueError(kind)
+    return {
+        "name": "pressure_{}_{}x{}x{}".format(kind, *ncell),
+        "ncell": tuple(ncell),
+        "periodic": periodic,
+        "prob_lo": (0.0, 0.0, 0.0),
+        "prob_hi": tuple(float(n) for n in ncell),
+        "bc_lo": bc_lo,
+        "bc_hi": bc_hi,
+        "velocity_bc": vbc,
+        "nu": (tau - 0.5) * temperature,
+        "tau": tau,
+        "T0": temperature,
+        "ic": {"type": "constant", "density": density, "mach": (0.0, 0.0, 0.0),
+               "temperature": temperature, "adiabatic_exponent": gamma},
+        "eb": eb,
+        "max_box": max_box,
+    }
+
+
 def plugin_case(kind: str, ncell, params: dict, nu: float, alpha: float | None = None,
                 periodic=(1, 1, 1), bc_lo=(0, 0, 0), bc_hi=(0, 0, 0), prob_lo=(0.0, 0.0, 0.0),
                 prob_hi=None, max_box: int | None = None) -> dict:

@@ -179,6 +179,63 @@ __device__ __forceinline__ void profile_ghost(const BCDev &bc, const bool gx[3],
     gv = t[NQ + q];
 }
 
+// Compile-time boundary features of a level's pull (template argument BCX):
+// a Channel or Parabolic inlet profile, PRESSURE faces
+constexpr int kBcxProfile = 1;
+constexpr int kBcxPressure = 2;
+
+// pressure_bc_f (BC.H:222-271) for the fluid cell (x, y, z) next to a
+// PRESSURE face of axis pd: the ghosts of the directions in Pm take
+// f_eq(rho_bc, u, R T) with u along pd from the populations the cell
+// receives.  The reference reads them as data(x + c_qq, qqbar): for the
+// tangential and outgoing directions those are the pulled values (a
+// periodic image, or a NOSLIP ghost whose value is the cell's own
+// population), except from a solid cell, which holds its stored value (0
+// in the initial state, the -1 of the stream afterwards; FP16 storage keeps
+// 0 there, so -1 is used)
+template <class T>
+__device__ __forceinline__ void pressure_ghosts(
+    const T *__restrict__ f, const CompactArgs &A, const BCDev &bc, const ShiftTab &sh, const int tx[3],
+    const int ty[3], const int tz[3], int x, int y, int z, uint32_t mask, uint32_t Pm, double fv[NQ],
+    double gv[NQ])
+{
+    const int pd = bc.pdir;
+    const int xc = pd == 0 ? x : (pd == 1 ? y : z);
+    const int ndir = xc == 0 ? 1 : -1;
+    double rho_out = 0.0, rho_tan = 0.0;
+    static_for<NQ>([&](auto qc) {
+        constexpr int qq = decltype(qc)::value;
+        constexpr int p = kBounce[qq];
+        constexpr int cx = ev(p, 0), cy = ev(p, 1), cz = ev(p, 2);
+        const int cin = pd == 0 ? cx : (pd == 1 ? cy : cz);
+        if (cin == -ndir || cin == 0) {
+            double v = fv[p];
+            if (!((mask >> p) & 1u)) {
+                if constexpr (kIsHalf<T>) {
+                    v = -1.0;
+                } else {
+                    const int64_t src =
+                        (int64_t(tz[cz + 1]) * A.ny + ty[cy + 1]) * A.nx + tx[cx + 1];
+                    v = widen(f[int64_t(p) * A.cs + src * kCellStride<T>], sh.f(p));
+                }
+            }
+            if (cin == -ndir)
+                rho_out += 2.0 * v;
+            else
+                rho_tan += 1.0 * v;
+        }
+    });
+    const double un = ndir * (1.0 - (rho_out + rho_tan) / bc.prho);
+    const double vel[3] = {pd == 0 ? un : 0.0, pd == 1 ? un : 0.0, pd == 2 ? un : 0.0};
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if ((Pm >> q) & 1u) {
+            fv[q] = set_equilibrium_value(bc.prho, vel, bc.prt, q);
+            gv[q] = bc.pgval[q];
+        }
+    });
+}
+
 // Pull of one cell, without branches so that all 54 loads are in flight
 // together.  For a pull source beyond a physical face the reference's
 // BCFill value is resolved: the face applied is the last non-periodic face
@@ -186,9 +243,10 @@ __device__ __forceinline__ void profile_ghost(const BCDev &bc, const bool gx[3],
 // read the cell's own pre-stream populations, OUTFLOW the boundary cell
 // next to the ghost (at a corner of two non-periodic faces, clamped in
 // both), VELOCITY a constant.
-// W = the working type of the pulled values (double); PROF: the level has a
-// Channel or Parabolic inlet profile (ghost values from the face tables)
-template <bool PROF, class T, class W>
+// W = the working type of the pulled values (double); BCX: the level's
+// boundary features (kBcxProfile: Channel or Parabolic inlet profile, ghost
+// values from the face tables; kBcxPressure: PRESSURE faces)
+template <int BCX, class T, class W>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
     const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, W fv[NQ],
@@ -208,7 +266,7 @@ __device__ __forceinline__ void pull_cell(
     // those whose face is VELOCITY or OUTFLOW.  A source is read directly
     // (from the wrapped or clamped coordinates) unless it is a solid cell
     // or a ghost of a NOSLIP or SLIP face.
-    uint32_t G = 0, V = 0, O = 0;
+    uint32_t G = 0, V = 0, O = 0, Pm = 0;
     static_for<3>([&](auto dc) {
         constexpr int d = 2 - decltype(dc)::value;
         const bool *gd = d == 0 ? gx : (d == 1 ? gy : gz);
@@ -219,6 +277,8 @@ __device__ __forceinline__ void pull_cell(
         const uint32_t lo = Fd & qset(d, 1), hi = Fd & qset(d, -1);
         V |= (tl == MRB_BC_VELOCITY ? lo : 0u) | (th == MRB_BC_VELOCITY ? hi : 0u);
         O |= (tl == MRB_BC_OUTFLOW ? lo : 0u) | (th == MRB_BC_OUTFLOW ? hi : 0u);
+        if constexpr ((BCX & kBcxPressure) != 0)
+            Pm |= (tl == MRB_BC_PRESSURE ? lo : 0u) | (th == MRB_BC_PRESSURE ? hi : 0u);
     });
     const uint32_t nondirect = (G & ~(V | O)) | (~G & ~mask & kAllQ);
     if (A.fast_ok && __all(nondirect == 0u)) {
@@ -309,7 +369,7 @@ __device__ __forceinline__ void pull_cell(
             const W vf = W(float(rf[q]) * sf) + W(sh.f(q));
             const W vg = W(float(rg[q]) * sg) + W(sh.g(q));
             double cf = bc.fval[q], cg = bc.gval[q];
-            if constexpr (PROF)
+            if constexpr ((BCX & kBcxProfile) != 0)
                 if (use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
             fv[q] = use_const ? W(cf) : vf;
             gv[q] = use_const ? W(cg) : vg;
@@ -320,11 +380,17 @@ __device__ __forceinline__ void pull_cell(
             constexpr int q = decltype(qc)::value;
             const bool use_const = (cmask >> q) & 1u;
             double cf = bc.fval[q], cg = bc.gval[q];
-            if constexpr (PROF)
+            if constexpr ((BCX & kBcxProfile) != 0)
                 if (use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
             fv[q] = use_const ? cf : widen(rf[q], sh.f(q));
             gv[q] = use_const ? cg : widen(rg[q], sh.g(q));
         });
+    }
+    if constexpr ((BCX & kBcxPressure) != 0) {
+        // (a cell with a PRESSURE ghost source never takes the fast path)
+        if (Pm) pressure_ghosts(f, A, bc, sh, tx, ty, tz, x, y, z, mask, Pm, fv, gv);
+    } else {
+        (void)Pm;
     }
 }
 
@@ -340,7 +406,7 @@ __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y
 
 // pass 1: Q_corr of the post-stream state.  With FP16 storage it also
 // clears the box maxima that pass 2 accumulates for the buffer it writes.
-template <class T, bool PROF>
+template <class T, int BCX>
 __global__ void __launch_bounds__(256) k_qpass(
     const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask,
     double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, Half16 h16, CollideParams P)
@@ -359,7 +425,7 @@ __global__ void __launch_bounds__(256) k_qpass(
     const uint32_t m = mask[cell];
     if (!(m & 1u)) return;
     double fv[NQ], gv[NQ];
-    pull_cell<PROF>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
+    pull_cell<BCX>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
     double rho, jx, jy, jz, E;
     accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
     const Macro M = macro_from(rho, jx, jy, jz, E, P);
@@ -404,7 +470,7 @@ __device__ __forceinline__ void half_block_stats(const Half16 &h16, int ef, int 
 }
 
 // pass 2: stream + collide, writes the B buffers
-template <class T, bool PROF>
+template <class T, int BCX>
 __device__ __forceinline__ void collide_body(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
@@ -447,7 +513,7 @@ __device__ __forceinline__ void collide_body(
         });
     } else if (in) {
         double fv[NQ], gv[NQ];
-        pull_cell<PROF>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
+        pull_cell<BCX>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
         Moments mo;
         accumulate_moments(fv, gv, mo);
         const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
@@ -468,28 +534,28 @@ __device__ __forceinline__ void collide_body(
 }
 
 // register budget as the compiler chooses (2 waves per SIMD)
-template <class T, bool PROF>
+template <class T, int BCX>
 __global__ void __launch_bounds__(256) k_collide(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
     Half16 h16, CollideParams P)
 {
-    collide_body<T, PROF>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+    collide_body<T, BCX>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
 // at most 168 registers (3 waves per SIMD), some values spilled to scratch
-template <class T, bool PROF>
+template <class T, int BCX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_collide_w3(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
     Half16 h16, CollideParams P)
 {
-    collide_body<T, PROF>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+    collide_body<T, BCX>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
 // stream only (parity of the pull with the reference push); FP64 and
 // shifted FP32 storage
-template <class T, bool PROF>
+template <class T, int BCX>
 __global__ void __launch_bounds__(256) k_stream_only(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, CompactArgs A, BCDev bc, ShiftTab sh)
@@ -500,7 +566,7 @@ __global__ void __launch_bounds__(256) k_stream_only(
     const uint32_t m = mask[cell];
     double fv[NQ], gv[NQ];
     if (m & 1u) {
-        pull_cell<PROF>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
+        pull_cell<BCX>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
     } else {
         for (int q = 0; q < NQ; ++q) fv[q] = gv[q] = -1.0;
     }
@@ -594,7 +660,7 @@ static Half16 half16_of(mrb_level *L)
     return h;
 }
 
-template <class T, bool PROF>
+template <class T, int BCX>
 static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
     const CompactArgs A = make_args(L);
@@ -605,26 +671,36 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     T *fo = (T *)L->pop[0][1 - L->cur];
     T *go = (T *)L->pop[1][1 - L->cur];
     if (ev) MRB_HIP(hipEventRecord(ev[0], s));
-    k_qpass<T, PROF><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+    k_qpass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[1], s));
     if (variant == 1)
-        k_collide_w3<T, PROF><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev,
+        k_collide_w3<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev,
                                                                     sh, h16, L->cp);
     else
-        k_collide<T, PROF><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh,
+        k_collide<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh,
                                                                  h16, L->cp);
     MRB_HIP(hipGetLastError());
     if (ev) MRB_HIP(hipEventRecord(ev[2], s));
     return MRB_OK;
 }
 
-// the inlet-profile lookups are compiled only into the kernels of levels
-// that have one
+// the inlet-profile lookups and the pressure ghosts are compiled only into
+// the kernels of levels that have them
+static int bcx_of(const mrb_level *L)
+{
+    return (L->bcdev.prof.kind ? kBcxProfile : 0) | (L->bcdev.pdir >= 0 ? kBcxPressure : 0);
+}
+
 template <class T>
 static int two_pass_t(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
-    return L->bcdev.prof.kind ? two_pass_tp<T, true>(L, variant, s, ev) : two_pass_tp<T, false>(L, variant, s, ev);
+    switch (bcx_of(L)) {
+    case kBcxProfile: return two_pass_tp<T, kBcxProfile>(L, variant, s, ev);
+    case kBcxPressure: return two_pass_tp<T, kBcxPressure>(L, variant, s, ev);
+    case kBcxProfile | kBcxPressure: return two_pass_tp<T, kBcxProfile | kBcxPressure>(L, variant, s, ev);
+    default: return two_pass_tp<T, 0>(L, variant, s, ev);
+    }
 }
 
 static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
@@ -679,28 +755,35 @@ int compact_stream_collide_events(mrb_level *L, hipStream_t s, int &nslots)
     return step_impl(L, s, true, nslots);
 }
 
+template <class T, int BCX>
+static void launch_stream_only(mrb_level *L, const CompactArgs &A, const ShiftTab &sh, hipStream_t s)
+{
+    const T *f = (const T *)L->pop[0][L->cur], *g = (const T *)L->pop[1][L->cur];
+    T *fo = (T *)L->pop[0][1 - L->cur], *go = (T *)L->pop[1][1 - L->cur];
+    k_stream_only<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
+}
+
+template <class T>
+static void stream_only_t(mrb_level *L, const CompactArgs &A, const ShiftTab &sh, hipStream_t s)
+{
+    switch (bcx_of(L)) {
+    case kBcxProfile: launch_stream_only<T, kBcxProfile>(L, A, sh, s); break;
+    case kBcxPressure: launch_stream_only<T, kBcxPressure>(L, A, sh, s); break;
+    case kBcxProfile | kBcxPressure: launch_stream_only<T, kBcxProfile | kBcxPressure>(L, A, sh, s); break;
+    default: launch_stream_only<T, 0>(L, A, sh, s); break;
+    }
+}
+
 int compact_stream_only(mrb_level *L, hipStream_t s)
 {
     const CompactArgs A = make_args(L);
     const ShiftTab sh = shift_tab(L);
-    const bool prof = L->bcdev.prof.kind != 0;
-    if (L->storage == MRB_STORAGE_F64) {
-        const double *f = (const double *)L->pop[0][L->cur], *g = (const double *)L->pop[1][L->cur];
-        double *fo = (double *)L->pop[0][1 - L->cur], *go = (double *)L->pop[1][1 - L->cur];
-        if (prof)
-            k_stream_only<double, true><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
-        else
-            k_stream_only<double, false><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
-    } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
-        const float *f = (const float *)L->pop[0][L->cur], *g = (const float *)L->pop[1][L->cur];
-        float *fo = (float *)L->pop[0][1 - L->cur], *go = (float *)L->pop[1][1 - L->cur];
-        if (prof)
-            k_stream_only<float, true><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
-        else
-            k_stream_only<float, false><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh);
-    } else {
+    if (L->storage == MRB_STORAGE_F64)
+        stream_only_t<double>(L, A, sh, s);
+    else if (L->storage == MRB_STORAGE_F32_SHIFTED)
+        stream_only_t<float>(L, A, sh, s);
+    else
         return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
-    }
     MRB_HIP(hipGetLastError());
     L->cur = 1 - L->cur;
     return MRB_OK;

@@ -373,6 +373,8 @@ int fused_launch(mrb_level *L, int variant, hipStream_t s)
             return set_error(MRB_E_UNSUPPORTED, "fused kernel: slip walls use the two-pass step");
     if (L->bcdev.prof.kind)
         return set_error(MRB_E_UNSUPPORTED, "fused kernel: velocity profiles use the two-pass step");
+    if (L->bcdev.pdir >= 0)
+        return set_error(MRB_E_UNSUPPORTED, "fused kernel: pressure boundaries use the two-pass step");
     if (variant < 0 || variant >= kNumFusedVariants) return set_error(MRB_E_ARG, "bad fused variant");
     const FusedVariant &v = kFusedVariants[variant];
     if (v.hold) {

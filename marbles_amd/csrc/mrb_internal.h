@@ -66,6 +66,13 @@ struct BCDev {
     // ((t1 + 1) * tabw + t2 + 1) * 2NQ + [q | NQ + q]
     const double *tab[6];
     int tabw[6];
+    // PRESSURE faces (BC.H:222-321), all on one axis pdir (-1: none): the
+    // boundary density and R T of the f ghost equilibrium, and per
+    // direction q the g ghost value of the face q enters through
+    // (c_q[pdir] = +1: the low face, -1: the high face)
+    int pdir;
+    double prho, prt;
+    double pgval[NQ];
 };
 
 // Geometry of the compact (single box, no ghost) layout.

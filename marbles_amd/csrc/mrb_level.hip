@@ -137,6 +137,39 @@ __global__ void k_stream_fab(
     });
 }
 
+// pressure_bc_f (BC.H:222-271) for the ghost whose direction q feeds the
+// fluid cell ivn = (n0, n1, n2): twice the populations that cell receives
+// travelling out through the face plus those it receives tangentially set
+// the normal velocity of the ghost's equilibrium at the boundary density.
+// A read beyond a (NOSLIP) tangential face, data(ivn + c_qq, qqbar), finds
+// the bounce prefill of FillPatchOps.H:92-110, which the wall's own fill
+// repeats: the population qq of ivn itself.  That prefill is not run here,
+// so the value is read from ivn.
+__device__ __noinline__ double pressure_f(const double *__restrict__ data, const BoxDev &B, int n0, int n1, int n2,
+                                          int idir, int ndir, const int domlo[3], const int domhi[3], const int *type,
+                                          double rho_bc, double rt, int q)
+{
+    const int64_t own = loc_of(B, n0, n1, n2);
+    double rho_out = 0.0, rho_tan = 0.0;
+    for (int qq = 0; qq < NQ; ++qq) {
+        const int bqq = kBounce[qq];
+        const int cin = kEV[bqq][idir];
+        const int a[3] = {n0 + kEV[qq][0], n1 + kEV[qq][1], n2 + kEV[qq][2]};
+        bool wall = false;
+        for (int d = 0; d < 3; ++d)
+            wall = wall || ((a[d] < domlo[d] || a[d] > domhi[d]) && type[d] != MRB_BC_PERIODIC);
+        const int64_t r = wall ? int64_t(qq) * B.ncell + own : int64_t(bqq) * B.ncell + loc_of(B, a[0], a[1], a[2]);
+        if (cin == -ndir) {
+            rho_out += 2.0 * data[r];
+        } else if (cin == 0) {
+            rho_tan += 1.0 * data[r];
+        }
+    }
+    double vel[3] = {0.0, 0.0, 0.0};
+    vel[idir] = ndir * (1.0 - (rho_out + rho_tan) / rho_bc);
+    return set_equilibrium_value(rho_bc, vel, rt, q);
+}
+
 // ---------------------------------------------------------------------------
 // Physical boundary functor BCFill::operator() (BC.H:344-471) for the listed
 // fab cells (outside the domain in a non-periodic direction).
@@ -183,6 +216,10 @@ __global__ void k_physbc_fab(
                         } else {
                             *dq = is_energy ? bc.gval[q] : bc.fval[q];
                         }
+                    } else if (t_bc == MRB_BC_PRESSURE) {
+                        *dq = is_energy ? bc.pgval[q]
+                                        : pressure_f(data, B, n0, n1, n2, idir, normal_dir, domlo, domhi,
+                                                     bc.type, bc.prho, bc.prt, q);
                     } else if (t_bc == MRB_BC_OUTFLOW) {
                         int s[3] = {iv[0], iv[1], iv[2]};
                         s[idir] += normal_dir;
@@ -741,6 +778,32 @@ static void compute_bc_values(mrb_level *L)
         d.fval[q] = set_equilibrium_value(rho, vel, R * T, q);
         d.gval[q] = thermal_g_equilibrium(rho, vel, T, R, gamma, q);
     }
+    // BCFill's PRESSURE case (BC.H:421-452): rho 1, R 1, T 1/3, gamma 5/3
+    // unless a velocity operator sets them; the g ghost is the Grad
+    // equilibrium at the normal velocity ndir (1 - 1 / rho_bc)
+    // (pressure_bc_g with rho_bc_out = 1), ndir = c_q along the axis
+    d.pdir = -1;
+    for (int f = 0; f < 6; ++f)
+        if (L->bc.bc_type[f] == MRB_BC_PRESSURE) d.pdir = f % 3;
+    {
+        double prho = 1.0, pR = 1.0, pT = 1.0 / 3.0, pgamma = 5.0 / 3.0;
+        if (L->bc.velbc_kind >= 1) {
+            prho = L->bc.density;
+            pR = kAirMolarMass / L->bc.m_bar;
+            pT = L->bc.temperature;
+            pgamma = L->bc.adiabatic_exponent;
+        }
+        d.prho = prho;
+        d.prt = pR * pT;
+        for (int q = 0; q < NQ; ++q) {
+            d.pgval[q] = 0.0;
+            if (d.pdir < 0 || kEV[q][d.pdir] == 0) continue;
+            const int ndir = kEV[q][d.pdir];
+            double v[3] = {0.0, 0.0, 0.0};
+            v[d.pdir] = ndir * (1.0 - (1.0) / prho);
+            d.pgval[q] = thermal_g_equilibrium(prho, v, pT, pR, pgamma, q);
+        }
+    }
     if (L->d_small) {
         (void)hipMemcpy(L->d_small + 4 * NQ, d.fval, sizeof(double) * NQ, hipMemcpyHostToDevice);
         (void)hipMemcpy(L->d_small + 5 * NQ, d.gval, sizeof(double) * NQ, hipMemcpyHostToDevice);
@@ -1096,10 +1159,26 @@ int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
                 return set_error(MRB_E_ARG, "periodic direction needs bc 0 on both sides");
             if (!L->periodic[d] && t == MRB_BC_PERIODIC)
                 return set_error(MRB_E_ARG, "bc 0 (interior) in a non-periodic direction");
-            if (t == MRB_BC_PRESSURE) return set_error(MRB_E_UNSUPPORTED, "pressure bc not supported");
             if (t != MRB_BC_PERIODIC && t != MRB_BC_NOSLIP && t != MRB_BC_VELOCITY && t != MRB_BC_OUTFLOW &&
-                t != MRB_BC_SLIPX && t != MRB_BC_SLIPY && t != MRB_BC_SLIPZ)
+                t != MRB_BC_PRESSURE && t != MRB_BC_SLIPX && t != MRB_BC_SLIPY && t != MRB_BC_SLIPZ)
                 return set_error(MRB_E_ARG, "unknown bc type");
+        }
+    }
+    // PRESSURE reads the populations around the fluid cell its ghost feeds
+    // (BC.H:247-260), tangential ghosts included, while the same fill writes
+    // ghost cells.  Those reads see values that do not depend on the fill
+    // order when every tangential axis is periodic (FillBoundary images) or
+    // NOSLIP on both sides (the prefill and the wall give the same value),
+    // and the face's own axis has two cells or more.
+    for (int f = 0; f < 6; ++f) {
+        if (bc->bc_type[f] != MRB_BC_PRESSURE) continue;
+        const int d = f % 3;
+        if (L->ncell[d] < 2) return set_error(MRB_E_UNSUPPORTED, "pressure bc needs two cells or more along its axis");
+        for (int t = 0; t < 3; ++t) {
+            if (t == d || L->periodic[t]) continue;
+            if (bc->bc_type[t] != MRB_BC_NOSLIP || bc->bc_type[t + 3] != MRB_BC_NOSLIP)
+                return set_error(MRB_E_UNSUPPORTED,
+                                 "pressure bc needs its tangential axes periodic or no-slip on both sides");
         }
     }
     if (bc->velbc_kind < 0 || bc->velbc_kind > 3) return set_error(MRB_E_ARG, "unknown velocity bc kind");

@@ -695,8 +695,57 @@ static void bcfill_cell(orc_level *L, int b, double *data, const int iv[3], int 
                         } else if (t == ORC_BC_SLIPZ) {
                             data[o] = data[fidx(fd, BOUNCE_DIR[2][q], ivn[0], ivn[1], ivn[2])];
                         }
-                        /* PERIODIC and PRESSURE: nothing here (pressure is
-                         * outside the restated scope) */
+                        else if (t == ORC_BC_PRESSURE) {
+                            /* BCFill's PRESSURE case (BC.H:421-452): the
+                             * boundary state starts at rho 1, R 1, T 1/3,
+                             * gamma 5/3 and the velocity operator overrides
+                             * it; its velocity is not used */
+                            double vel_op[3] = {0.0, 0.0, 0.0};
+                            double R = 1.0, temperature = 1.0 / 3.0;
+                            double gamma = 5.0 / 3.0;
+                            double rho_bc = 1.0;
+                            velbc_op(L, iv, &L->bc, &rho_bc, vel_op, &R, &temperature, &gamma);
+                            if (is_energy) {
+                                /* pressure_bc_g (BC.H:273-321) with
+                                 * rho_bc_out = 1 as the caller passes it */
+                                const double rho_bc_out = 1.0;
+                                double vel_bc[3] = {0.0, 0.0, 0.0};
+                                vel_bc[idir] = normal_dir * (1.0 - (rho_bc_out) / rho_bc);
+                                const double cv = R / (gamma - 1.0);
+                                const double two_rho_e = get_energy_vec(temperature, rho_bc, vel_bc, cv);
+                                double heat_flux[3] = {0.0, 0.0, 0.0};
+                                double fl[6];
+                                get_equilibrium_moments_vec(
+                                    rho_bc, vel_bc, two_rho_e, cv, R, heat_flux, &fl[0], &fl[1], &fl[2],
+                                    &fl[3], &fl[4], &fl[5]);
+                                const double zero[3] = {0.0, 0.0, 0.0};
+                                data[o] = set_extended_grad_expansion_generic(
+                                    two_rho_e, heat_flux, fl, W[q], ev, THETA0, zero, 1.0);
+                            } else {
+                                /* pressure_bc_f (BC.H:222-271): the density
+                                 * leaving the fluid cell ivn through the face
+                                 * (twice the outgoing, once the tangential
+                                 * populations it receives) sets the normal
+                                 * velocity of the ghost's equilibrium */
+                                double rho_out = 0.0, rho_tan = 0.0;
+                                for (int qq = 0; qq < ORC_NQ; qq++) {
+                                    const int *evn_out = EV[qq];
+                                    const int bounce_qq = BOUNCE[qq];
+                                    const int *evn_in = EV[bounce_qq];
+                                    const int64_t r = fidx(fd, bounce_qq, ivn[0] + evn_out[0],
+                                                           ivn[1] + evn_out[1], ivn[2] + evn_out[2]);
+                                    if (evn_in[idir] == -normal_dir) {
+                                        rho_out += 2.0 * data[r];
+                                    } else if (evn_in[idir] == 0) {
+                                        rho_tan += 1.0 * data[r];
+                                    }
+                                }
+                                double vel_bc[3] = {0.0, 0.0, 0.0};
+                                vel_bc[idir] = normal_dir * (1.0 - (rho_out + rho_tan) / rho_bc);
+                                data[o] = set_equilibrium_value(rho_bc, vel_bc, R * temperature, ev);
+                            }
+                        }
+                        /* PERIODIC: nothing here */
                     } else {
                         data[o] = -1.0; /* BC.H:463-466 */
                     }

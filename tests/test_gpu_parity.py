@@ -365,6 +365,61 @@ def test_velocity_profile_inlet_matches_oracle(oracle, engine, kind, layout):
     E.close()
 
 
+PRESSURE_CASES = {"outlet": 30, "outlet_body": 2, "walled": 30, "both": 30, "channel": 30}
+
+
+@pytest.mark.parametrize("case", list(PRESSURE_CASES))
+@pytest.mark.parametrize("layout", ["compact", "fab", "fab_batched"])
+def test_pressure_boundary_matches_oracle(oracle, engine, case, layout):
+    """PRESSURE faces (pressure_bc_f / pressure_bc_g): an outlet behind a
+    velocity inlet, a walled outlet (edge ghosts beyond a wall and the face),
+    pressure at both ends, a Channel-profile inlet with a pressure outlet;
+    and two steps of an outlet whose sums reach solid cells (their contents
+    enter the sums: 0 in the initial state, -1 after a stream).  Compact
+    two-pass step, fab stages and the batched fab step against the oracle."""
+    from oracle.pyoracle import build_oracle_level
+
+    cfg = configs.pressure_case(case, max_box=None if layout == "compact" else 12)
+    O = build_oracle_level(cfg)
+    if layout == "compact":
+        E = _compact_from(O, engine, cfg)
+    else:
+        O, E = _fab_pair(oracle, engine, cfg)
+    n = PRESSURE_CASES[case]
+    if layout == "fab":
+        for _ in range(n):
+            for lat in (0, 1):
+                E.fillpatch(lat)
+                E.stream(lat)
+            E.collide()
+    else:
+        E.advance(n)
+    O.evolve(n)
+    for name in ("f", "g"):
+        a, b = E.valid(name), O.valid(name)
+        assert np.isfinite(b).all()
+        err = _rel_err(a, b)
+        assert err <= REL_TOL, (case, layout, name, err)
+    E.close()
+
+
+def test_pressure_bc_rejects_fill_order_dependent_faces(engine):
+    """A PRESSURE face whose tangential axis is neither periodic nor no-slip
+    on both sides, or whose own axis has one cell, is refused."""
+    from marbles_amd._lib import MrbError
+
+    E = engine.Level((8, 6, 4), (0, 0, 1), (0.0, 0.0, 0.0), (8.0, 6.0, 4.0), layout="compact")
+    E.set_bc((2, 1, 0), (3, 1, 0))
+    for lo, hi in (((2, 5, 0), (3, 1, 0)), ((2, 2, 0), (3, 1, 0)), ((2, 6, 0), (3, 6, 0)), ((2, 3, 0), (5, 1, 0))):
+        with pytest.raises(MrbError):
+            E.set_bc(lo, hi)
+    E.close()
+    F = engine.Level((1, 6, 4), (0, 1, 1), (0.0, 0.0, 0.0), (1.0, 6.0, 4.0), layout="compact")
+    with pytest.raises(MrbError):
+        F.set_bc((3, 0, 0), (1, 0, 0))
+    F.close()
+
+
 PLUGIN_CASES = {
     "viscosity_test": (lambda: configs.shear_wave(32, tau=0.8), 60),
     "thermaldiffusivity_test": (lambda: configs.plugin_case(

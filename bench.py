@@ -181,9 +181,10 @@ def run_adjoint(args, torch) -> int:
     """Gradient of the flow objective (outlet-plane mean u_x + 1e-3 KE) with
     respect to n_s, tau and u0 on an n^3 box with an n_s field of seeded
     smoothed blobs (15 % mean), inflow 0.04: forward time, gradient time
-    (forward sweep with checkpoints + recompute + adjoint; mean of three
-    calls after one untimed call of each), and central differences
-    (h = 1e-6) on 32 seeded cells, tau and 4 initial-velocity components."""
+    (recorded forward sweep + reverse sweep; mean of three calls after one
+    untimed call) with ceil(sqrt(N)) checkpoints and with the memory-aware
+    interval (value), and central differences (h = 1e-6) on 32 seeded
+    cells, tau and 4 initial-velocity components."""
     import numpy as np
 
     from marbles_amd.adjoint import IsoSolver
@@ -204,35 +205,46 @@ def run_adjoint(args, torch) -> int:
     u0[0] = 0.04
     tau = 0.6
     S = IsoSolver((n, n, n), T0=1.0 / 3.0, u_in=0.04)
-    # untimed first calls: the gradient's checkpoint buffers are allocated
-    # once and kept for later calls of the same length
-    S.forward(ns, tau, u0, nsteps)
-    S.gradient(ns, tau, u0, nsteps)
     reps = 3
+    S.forward(ns, tau, u0, nsteps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
         J = S.forward(ns, tau, u0, nsteps)
     torch.cuda.synchronize()
     t_fwd = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        J2, dtau, dns, du0 = S.gradient(ns, tau, u0, nsteps)
-    torch.cuda.synchronize()
-    t_grad = (time.perf_counter() - t0) / reps
-    # the two halves separately: the recorded forward sweep (the forward
-    # pass of the autograd Function) and the reverse sweep (its backward)
-    t_rec = t_rev = 0.0
-    for _ in range(reps):
+
+    def timed(every):
+        """Gradient time (mean of three calls after one untimed call, which
+        allocates the recording buffers), the recorded forward sweep and the
+        reverse sweep separately, and the interval used."""
+        S.gradient(ns, tau, u0, nsteps, every=every)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, token = S.record(ns, tau, u0, nsteps)
+        for _ in range(reps):
+            out = S.gradient(ns, tau, u0, nsteps, every=every)
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        S.reverse(token)
-        torch.cuda.synchronize()
-        t_rec += (t1 - t0) / reps
-        t_rev += (time.perf_counter() - t1) / reps
+        t_grad = (time.perf_counter() - t0) / reps
+        t_rec = t_rev = 0.0
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, token = S.record(ns, tau, u0, nsteps, every=every)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            S.reverse(token)
+            torch.cuda.synchronize()
+            t_rec += (t1 - t0) / reps
+            t_rev += (time.perf_counter() - t1) / reps
+        return out, {"every": S.record_interval(), "gradient_s": t_grad, "ratio": t_grad / t_fwd,
+                     "recorded_forward_s": t_rec, "reverse_s": t_rev, "reverse_over_forward": t_rev / t_fwd}
+
+    # sqrt(N) checkpoints (fewest buffers), then the memory-aware interval
+    # (every state kept when they fit in HBM: no recomputation)
+    (_, _, dns_sq, _), sq = timed(0)
+    (J2, dtau, dns, du0), kept = timed(-1)
+    identical = bool(torch.equal(dns, dns_sq))
+    t_grad, t_rec, t_rev = kept["gradient_s"], kept["recorded_forward_s"], kept["reverse_s"]
     h = 1e-6
     checks = [(dtau, (S.forward(ns, tau + h, u0, nsteps) - S.forward(ns, tau - h, u0, nsteps)) / (2 * h))]
     flat = ns.view(-1)
@@ -256,8 +268,10 @@ def run_adjoint(args, torch) -> int:
     scale = max(abs(g) for g, _ in checks)
     err = max(abs(g - d) for g, d in checks) / scale
     S.close()
-    log(f"J={J:.12g} forward {t_fwd:.3f}s gradient {t_grad:.3f}s ({t_grad / t_fwd:.2f}x; recorded forward "
-        f"{t_rec / t_fwd:.2f}x + reverse {t_rev / t_fwd:.2f}x) FD max rel err {err:.2e}")
+    log(f"J={J:.12g} forward {t_fwd:.3f}s; checkpoints every {sq['every']}: gradient {sq['gradient_s']:.3f}s "
+        f"({sq['ratio']:.2f}x); every {kept['every']}: gradient {t_grad:.3f}s ({t_grad / t_fwd:.2f}x; recorded "
+        f"forward {t_rec / t_fwd:.2f}x + reverse {t_rev / t_fwd:.2f}x); same gradient: {identical}; "
+        f"FD max rel err {err:.2e}")
     print(json.dumps({
         "metric": "adjoint gradient cost / forward cost (isothermal D3Q27, partial bounce-back; value = "
                   "recorded forward + reverse sweep)",
@@ -265,10 +279,12 @@ def run_adjoint(args, torch) -> int:
         "ms_per_step": round(t_fwd / nsteps * 1e3, 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n}^3 box, n_s blobs (15 % mean), inflow 0.04, outflow, tau {tau}, "
-                               f"{nsteps} steps, checkpoints every ceil(sqrt(steps))",
+                               f"{nsteps} steps, checkpoint interval chosen by device memory "
+                               f"(every {kept['every']})",
                    "J": J, "forward_s": t_fwd, "gradient_s": t_grad,
                    "recorded_forward_s": t_rec, "reverse_s": t_rev,
                    "reverse_over_forward": t_rev / t_fwd,
+                   "sqrt_checkpoints": sq, "memory_aware": kept, "gradients_identical": identical,
                    "mlups_forward": n ** 3 * nsteps / t_fwd / 1e6,
                    "fd_max_rel_error": err, "fd_points": len(checks)},
     }), flush=True)

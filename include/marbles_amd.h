@@ -270,9 +270,12 @@ int mrb_iso_forward(
     mrb_iso *iso, const double *ns, double tau, const double *u0, int nsteps, double *J, void *stream);
 /* The loss and its exact gradient with respect to tau, n_s (per cell) and
  * u0 (per cell, 3 comps) by the hand-written adjoint of the step, with
- * checkpoints every `every` steps (<= 0: ceil(sqrt(nsteps))) and the
- * states between checkpoints recomputed: mrb_iso_record followed by
- * mrb_iso_reverse.  Synchronous. */
+ * checkpoints every `every` steps and the states between checkpoints
+ * recomputed: mrb_iso_record followed by mrb_iso_reverse.  every = 0:
+ * ceil(sqrt(nsteps)) (fewest buffers); every < 0: the longest segment whose
+ * states fit half of the free device memory, the whole run (no
+ * recomputation) when they fit, never shorter than ceil(sqrt(nsteps)).
+ * Every interval gives the same gradient, bit for bit.  Synchronous. */
 int mrb_iso_gradient(
     mrb_iso *iso, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
     double *dtau, double *dns, double *du0, void *stream);
@@ -287,6 +290,9 @@ int mrb_iso_record(
  * segment from its checkpoint): the gradients as in mrb_iso_gradient.  Uses
  * the recording up.  Synchronous. */
 int mrb_iso_reverse(mrb_iso *iso, double *dtau, double *dns, double *du0, void *stream);
+/* The checkpoint interval of the last mrb_iso_record (after the every <= 0
+ * rules). */
+int mrb_iso_record_interval(const mrb_iso *iso, int *every);
 /* Final populations of the last mrb_iso_forward (27 N doubles, host). */
 int mrb_iso_state(const mrb_iso *iso, double *host_f);
 

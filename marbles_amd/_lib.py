@@ -123,6 +123,7 @@ SIGNATURES = {
                                  POINTER(c_double), c_void_p, c_void_p, c_void_p]),
     "mrb_iso_record": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_int, c_int, POINTER(c_double), c_void_p]),
     "mrb_iso_reverse": (c_int, [c_void_p, POINTER(c_double), c_void_p, c_void_p, c_void_p]),
+    "mrb_iso_record_interval": (c_int, [c_void_p, POINTER(c_int)]),
     "mrb_iso_state": (c_int, [c_void_p, POINTER(c_double)]),
     "mrb_autotune_name": (ctypes.c_char_p, [c_void_p, c_int]),
 }

@@ -51,9 +51,11 @@ class IsoSolver:
                                        int(nsteps), ctypes.byref(J), self._stream()))
         return J.value
 
-    def gradient(self, ns: torch.Tensor, tau: float, u0: torch.Tensor, nsteps: int, every: int = 0):
+    def gradient(self, ns: torch.Tensor, tau: float, u0: torch.Tensor, nsteps: int, every: int = -1):
         """(J, dJ/dtau, dJ/dn_s, dJ/du0) with checkpoints every `every` steps
-        (0: ceil(sqrt(nsteps)))."""
+        (0: ceil(sqrt(nsteps)); negative: the longest segment whose states fit
+        half of the free device memory, the whole run when they fit, so that
+        nothing is recomputed).  Every interval gives the same gradient."""
         ns = _dev(ns, self.shape)
         u0 = _dev(u0, (3,) + self.shape)
         dns = torch.empty_like(ns)
@@ -64,7 +66,7 @@ class IsoSolver:
                                         c_void_p(dns.data_ptr()), c_void_p(du0.data_ptr()), self._stream()))
         return J.value, dtau.value, dns, du0
 
-    def record(self, ns: torch.Tensor, tau: float, u0: torch.Tensor, nsteps: int, every: int = 0):
+    def record(self, ns: torch.Tensor, tau: float, u0: torch.Tensor, nsteps: int, every: int = -1):
         """Forward sweep keeping the checkpoints for reverse(); returns (J, token).
         The token names this recording: a later record() or forward() replaces it."""
         ns = _dev(ns, self.shape)
@@ -75,6 +77,12 @@ class IsoSolver:
         self._token = getattr(self, "_token", 0) + 1
         self._rec = (ns, u0)  # the engine reads them again in reverse()
         return J.value, self._token
+
+    def record_interval(self) -> int:
+        """Checkpoint interval of the last record() or gradient()."""
+        e = ctypes.c_int()
+        check(self.lib.mrb_iso_record_interval(self.h, ctypes.byref(e)))
+        return e.value
 
     def reverse(self, token: int):
         """(dJ/dtau, dJ/dn_s, dJ/du0) of the recording `token`, or None when a
@@ -114,7 +122,7 @@ class FlowLoss(torch.autograd.Function):
     same solver has replaced the recording, the whole gradient again)."""
 
     @staticmethod
-    def forward(ctx, ns, tau, u0, solver: IsoSolver, nsteps: int, every: int = 0):
+    def forward(ctx, ns, tau, u0, solver: IsoSolver, nsteps: int, every: int = -1):
         if any(ctx.needs_input_grad[:3]):
             J, token = solver.record(ns.detach(), float(tau), u0.detach(), nsteps, every)
             ctx.solver, ctx.token, ctx.args = solver, token, (float(tau), nsteps, every)
@@ -136,5 +144,5 @@ class FlowLoss(torch.autograd.Function):
         return gJ * dns, (gJ * dtau).reshape(()), gJ * du0, None, None, None
 
 
-def flow_loss(solver: IsoSolver, ns, tau, u0, nsteps: int, every: int = 0):
+def flow_loss(solver: IsoSolver, ns, tau, u0, nsteps: int, every: int = -1):
     return FlowLoss.apply(ns, tau, u0, solver, nsteps, every)

@@ -624,12 +624,39 @@ int mrb_iso_forward(mrb_iso *S, const double *ns, double tau, const double *u0, 
     return iso_loss(S, A, S->f[0], s, J);
 }
 
+// Device bytes of the recording buffers for segments of `every` steps:
+// one checkpoint per segment, every - 1 further states and every Q arrays
+static double iso_record_bytes(const mrb_iso *S, int nsteps, int every)
+{
+    const double pop = double(sizeof(double)) * NQ * double(S->N), q3 = double(sizeof(double)) * 3 * double(S->N);
+    const int nseg = (nsteps + every - 1) / every;
+    return double(nseg + every - 1) * pop + double(every) * q3;
+}
+
+// The longest segment whose recording fits half of the device memory that
+// is free (counting the recording buffers already held): the whole run
+// when it fits, so that the reverse sweep recomputes nothing; never
+// shorter than ceil(sqrt(nsteps)), whose buffers are the fewest
+static int iso_auto_every(const mrb_iso *S, int nsteps, int sqrt_every)
+{
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return sqrt_every;
+    const double pop = double(sizeof(double)) * NQ * double(S->N), q3 = double(sizeof(double)) * 3 * double(S->N);
+    const double held = double(S->ckpt.size() + S->seg.size()) * pop + double(S->segq.size()) * q3;
+    const double budget = 0.5 * (double(freeb) + held);
+    for (int e = nsteps; e > sqrt_every; --e)
+        if (iso_record_bytes(S, nsteps, e) <= budget) return e;
+    return sqrt_every;
+}
+
 int mrb_iso_record(mrb_iso *S, const double *ns, double tau, const double *u0, int nsteps, int every, double *J,
                    void *stream)
 {
     if (!S || !ns || !u0 || !J) return set_error(MRB_E_ARG, "null argument");
     if (!(tau > 0.5) || nsteps < 1) return set_error(MRB_E_ARG, "tau must exceed 1/2 and nsteps be >= 1");
-    if (every < 1) every = std::max(1, int(std::ceil(std::sqrt(double(nsteps)))));
+    const int sqrt_every = std::max(1, int(std::ceil(std::sqrt(double(nsteps)))));
+    if (every == 0) every = sqrt_every;
+    if (every < 0) every = iso_auto_every(S, nsteps, std::min(sqrt_every, nsteps));
     every = std::min(every, nsteps);
     S->rec_valid = false;
     hipStream_t s = as_stream(stream);
@@ -742,6 +769,13 @@ int mrb_iso_gradient(mrb_iso *S, const double *ns, double tau, const double *u0,
     const int rc = mrb_iso_record(S, ns, tau, u0, nsteps, every, J, stream);
     if (rc) return rc;
     return mrb_iso_reverse(S, dtau, dns, du0, stream);
+}
+
+int mrb_iso_record_interval(const mrb_iso *S, int *every)
+{
+    if (!S || !every) return set_error(MRB_E_ARG, "null argument");
+    *every = S->rec_every;
+    return MRB_OK;
 }
 
 int mrb_iso_state(const mrb_iso *S, double *host_f)

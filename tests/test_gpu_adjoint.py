@@ -63,11 +63,16 @@ def test_adjoint_matches_torch_autograd(solver, reference):
         assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("every", [1, 3, 5])
+@pytest.mark.parametrize("every", [1, 3, 5, NSTEPS, -1])
 def test_checkpoint_interval_does_not_change_the_gradient(solver, every):
+    """sqrt(N) checkpoints against other intervals: every state kept (no
+    recomputation) and the memory-aware choice, which keeps every state of
+    this small run."""
     ns, u0 = _inputs()
     a = solver.gradient(ns.cuda(), TAU, u0.cuda(), NSTEPS, every=0)
+    assert solver.record_interval() == 4
     b = solver.gradient(ns.cuda(), TAU, u0.cuda(), NSTEPS, every=every)
+    assert solver.record_interval() == (NSTEPS if every < 0 else every)
     assert a[0] == b[0] and a[1] == b[1]
     assert torch.equal(a[2], b[2]) and torch.equal(a[3], b[3])
 

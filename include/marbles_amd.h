@@ -296,6 +296,29 @@ int mrb_iso_record_interval(const mrb_iso *iso, int *every);
 /* Final populations of the last mrb_iso_forward (27 N doubles, host). */
 int mrb_iso_state(const mrb_iso *iso, double *host_f);
 
+/* ---- checkpoint files ----------------------------------------------------
+ * LBM::write_checkpoint_file (Source/LBM.cpp:1684-1750): <dir>/Header
+ * ("Checkpoint file for LBM", finest level, per level step, dt and time,
+ * then the BoxArrays) and per level the f and g MultiFabs in AMReX's VisMF
+ * format (<dir>/Level_<l>/f_00_H, f_00_D_00000, g_00_H, g_00_D_00000):
+ * fab levels with their ghost cells, compact levels as one box without
+ * ghosts.  Synchronous. */
+int mrb_write_checkpoint(const char *dir, mrb_level *const *levels, int nlevels, const int *steps,
+                         const double *dts, const double *times);
+/* LBM::read_checkpoint_file (LBM.cpp:1752-1800): the level count, and with
+ * non-null arrays (max_levels entries) each level's step, dt and time. */
+int mrb_checkpoint_header(const char *dir, int *nlevels, int *steps, double *dts, double *times,
+                          int max_levels);
+/* The BoxArray of one level of a checkpoint: *nbox, and with non-null
+ * arrays (3 ints per box, max_boxes boxes) the boxes' low and high cells. */
+int mrb_checkpoint_boxes(const char *dir, int level, int *nbox, int *box_lo, int *box_hi, int max_boxes);
+/* LBM::read_checkpoint_file (LBM.cpp:1800-1860): the f and g populations of
+ * every level into levels created with the checkpoint's boxes (fab layout,
+ * the same ghost width) or domain (compact), whose flags are already set;
+ * solid cells get 0 (fill_f_inside_eb, LBM.cpp:1278-1298) and the ghost
+ * cells are refilled before the next step.  Synchronous. */
+int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels);
+
 /* Density, velocity (x, y, z) and temperature of the current state per
  * cell into the device array out (5 N doubles, field-major; zeros in solid
  * cells), as LBM::f_to_macrodata (Source/LBM.cpp:810-906) computes them.

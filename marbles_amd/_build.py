@@ -27,6 +27,7 @@ SOURCES = [
     "mrb_amr.hip",
     "mrb_batched.hip",
     "mrb_adjoint.hip",
+    "mrb_io.hip",
 ]
 
 COMMON_FLAGS = [

@@ -80,6 +80,10 @@ SIGNATURES = {
     "mrb_set_shift_state": (c_int, [c_void_p, c_double, c_double]),
     "mrb_fab_size": (c_int64, [c_void_p, c_int, c_int]),
     "mrb_upload": (c_int, [c_void_p, c_int, c_void_p]),
+    "mrb_write_checkpoint": (c_int, [c_char_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "mrb_checkpoint_header": (c_int, [c_char_p, POINTER(c_int), c_void_p, c_void_p, c_void_p, c_int]),
+    "mrb_checkpoint_boxes": (c_int, [c_char_p, c_int, POINTER(c_int), c_void_p, c_void_p, c_int]),
+    "mrb_read_checkpoint": (c_int, [c_char_p, c_void_p, c_int]),
     "mrb_download": (c_int, [c_void_p, c_int, c_void_p]),
     "mrb_compute_flags_sphere": (c_int, [c_void_p, POINTER(c_double), c_double, c_int]),
     "mrb_finalize_flags": (c_int, [c_void_p]),

@@ -1,0 +1,133 @@
+"""Checkpoint and restart (LBM::write_checkpoint_file / read_checkpoint_file,
+Source/LBM.cpp:1684-1860) through the C ABI: a run interrupted by a
+checkpoint and restarted on fresh levels continues bit for bit like the
+uninterrupted run, and the files follow AMReX's checkpoint Header and VisMF
+layout, read here by an independent parser."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from marbles_amd import checkpoint, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _vismf(prefix):
+    """(ncomp, ngrow, boxes, fabs) of a VisMF MultiFab, parsed from its text
+    header and the FAB headers of its data file."""
+    lines = open(prefix + "_H").read().split("\n")
+    vers, how, ncomp, ngrow = int(lines[0]), int(lines[1]), int(lines[2]), int(lines[3])
+    assert (vers, how) == (1, 0)
+    m = re.match(r"\((\d+) 0", lines[4])
+    nbox = int(m.group(1))
+    boxre = re.compile(r"\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\) \(0,0,0\)\)")
+    boxes = [tuple(map(int, boxre.match(lines[5 + b]).groups())) for b in range(nbox)]
+    assert lines[5 + nbox] == ")" and int(lines[6 + nbox]) == nbox
+    fabs = []
+    for b in range(nbox):
+        tag, name, off = lines[7 + nbox + b].split()
+        assert tag == "FabOnDisk:"
+        with open(os.path.join(os.path.dirname(prefix), name), "rb") as fh:
+            fh.seek(int(off))
+            head = fh.readline().decode()
+            hm = re.match(r"FAB \(\(8, \(64 11 52 0 1 12 0 1023\)\),\(8, \(8 7 6 5 4 3 2 1\)\)\)"
+                          r"\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\) \(0,0,0\)\) (\d+)\n", head)
+            lo, hi, nc = hm.groups()[:3], hm.groups()[3:6], int(hm.group(7))
+            shape = tuple(int(h) - int(l) + 1 for l, h in zip(lo, hi))
+            assert nc == ncomp
+            assert tuple(int(l) + ngrow for l in lo) == boxes[b][:3]
+            data = np.fromfile(fh, dtype="<f8", count=nc * int(np.prod(shape)))
+            fabs.append(data.reshape((nc,) + shape[::-1]))
+    return ncomp, ngrow, boxes, fabs
+
+
+def _make(cfg, layout, boxes=None):
+    from oracle.pyoracle import build_oracle_level
+
+    import marbles_amd
+
+    O = build_oracle_level(cfg) if boxes is None else None
+    if layout == "compact":
+        E = marbles_amd.Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], layout="compact")
+    else:
+        boxes = boxes if boxes is not None else O.boxes
+        E = marbles_amd.Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=boxes, ng=3,
+                              layout="fab")
+    E.set_phys(cfg["nu"], cfg.get("alpha"))
+    E.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+    return E, O
+
+
+@pytest.mark.parametrize("layout", ["compact", "fab"])
+def test_restart_continues_bit_for_bit(oracle, engine, tmp_path, layout):
+    """Channel with an EB sphere (solid cells get 0 on restart, -1 again
+    after the next stream, which no fluid update reads): 12 steps,
+    checkpoint, 13 more on restarted levels == 25 uninterrupted steps."""
+    cfg = configs.channel_sphere((40, 24, 24), radius=5.0, center=(14.0, 12.0, 12.0),
+                                 max_box=None if layout == "compact" else 12)
+    E, O = _make(cfg, layout)
+    if layout == "compact":
+        flags, f0, g0 = O.valid("is_fluid"), O.valid("f"), O.valid("g")
+    else:
+        flags, f0, g0 = O.flat("is_fluid"), O.flat("f"), O.flat("g")
+    E.flags_upload(flags)
+    E.upload("f", f0)
+    E.upload("g", g0)
+    E.advance(12)
+    path = str(tmp_path / "chk00012")
+    checkpoint.write_checkpoint(path, [E], [12], [1.0], [12.0])
+    E.advance(13)
+    ref_f, ref_g = E.valid("f"), E.valid("g")
+    info = checkpoint.checkpoint_info(path)
+    assert info["nlevels"] == 1 and info["steps"] == [12] and info["times"] == [12.0]
+    assert [tuple(map(tuple, b)) for b in info["boxes"][0]] == [tuple(map(tuple, b)) for b in E.boxes]
+    R, _ = _make(cfg, layout, boxes=info["boxes"][0])
+    R.flags_upload(flags)
+    checkpoint.read_checkpoint(path, [R])
+    R.advance(13)
+    assert np.array_equal(R.valid("f"), ref_f) and np.array_equal(R.valid("g"), ref_g)
+    E.close()
+    R.close()
+
+
+def test_files_follow_the_amrex_layout(oracle, engine, tmp_path):
+    """Header lines of LBM::write_checkpoint_file; VisMF headers and FABs
+    (grown boxes, ghost width 3, 27 components) holding the downloaded
+    populations, solid cells included."""
+    cfg = configs.channel_sphere((32, 16, 16), radius=4.0, center=(10.0, 8.0, 8.0), max_box=8)
+    E, O = _make(cfg, "fab")
+    E.flags_upload(O.flat("is_fluid"))
+    E.upload("f", O.flat("f"))
+    E.upload("g", O.flat("g"))
+    E.advance(3)
+    path = str(tmp_path / "chk00003")
+    checkpoint.write_checkpoint(path, [E], [3], [0.5], [1.5])
+    head = open(os.path.join(path, "Header")).read().split("\n")
+    assert head[:5] == ["Checkpoint file for LBM", "0", "3 ", "0.5 ", "1.5 "]
+    for name in ("f", "g"):
+        ncomp, ngrow, boxes, fabs = _vismf(os.path.join(path, "Level_0", f"{name}_00"))
+        assert (ncomp, ngrow) == (27, 3)
+        assert boxes == [tuple(lo) + tuple(hi) for lo, hi in E.boxes]
+        flat = np.concatenate([a.ravel() for a in fabs])
+        assert np.array_equal(flat, E.download(name))
+    E.close()
+
+
+def test_restart_refuses_other_boxes(oracle, engine, tmp_path):
+    from marbles_amd._lib import MrbError
+
+    cfg = configs.taylor_green(16)
+    E, O = _make(cfg, "fab")
+    E.flags_upload(O.flat("is_fluid"))
+    E.upload("f", O.flat("f"))
+    E.upload("g", O.flat("g"))
+    path = str(tmp_path / "chk")
+    checkpoint.write_checkpoint(path, [E], [0], [1.0], [0.0])
+    R, _ = _make(cfg, "fab", boxes=[((0, 0, 0), (7, 15, 15)), ((8, 0, 0), (15, 15, 15))])
+    R.flags_all_regular()
+    with pytest.raises(MrbError):
+        checkpoint.read_checkpoint(path, [R])
+    E.close()
+    R.close()

@@ -319,6 +319,27 @@ int mrb_checkpoint_boxes(const char *dir, int level, int *nbox, int *box_lo, int
  * cells are refilled before the next step.  Synchronous. */
 int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels);
 
+/* ---- plotfile ------------------------------------------------------------
+ * LBM::write_plot_file (Source/LBM.cpp:1629-1682) for one compact level:
+ * the variables of LBM::plot_file_mf in its order -- the 19 macrodata
+ * components (rho, vel_x, vel_y, vel_z, vel_mag, two_rho_e, QCorrX..Z,
+ * pxx, pyy, pzz, pxy, pxz, pyz, qx, qy, qz, temperature) of the last step's
+ * post-stream state (what m_macrodata holds after the collide; the stored
+ * state before any step), with MRB_PLOT_STREAMING the 27 f and 27 g
+ * populations, with MRB_PLOT_DERIVED vort_x, vort_y, vort_z, vort_mag
+ * (compute_derived) and dQCorrX..Z (compute_q_corrections) of those
+ * macrodata, then is_fluid and eb_boundary -- written as AMReX's
+ * plotfile (<dir>/Header "HyperCLaw-V1.1", <dir>/Level_0/Cell_H and
+ * Cell_D_00000).  Synchronous. */
+#define MRB_PLOT_STREAMING 1 /* lbm.save_streaming */
+#define MRB_PLOT_DERIVED 2   /* lbm.save_derived */
+int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, int flags);
+/* The plotfile's variable count for these flags. */
+int mrb_plot_nvars(int flags);
+/* The plotfile's fields into host memory: mrb_plot_nvars(flags) x N
+ * doubles, variable-major (x fastest).  Synchronous. */
+int mrb_plot_fields(mrb_level *lvl, double *out, int flags);
+
 /* Density, velocity (x, y, z) and temperature of the current state per
  * cell into the device array out (5 N doubles, field-major; zeros in solid
  * cells), as LBM::f_to_macrodata (Source/LBM.cpp:810-906) computes them.

@@ -13,6 +13,8 @@ MRB_LAYOUT_COMPACT = 1
 MRB_STORAGE_F64 = 0
 MRB_STORAGE_F32_SHIFTED = 1
 MRB_STORAGE_F16_SCALED = 2
+MRB_PLOT_STREAMING = 1
+MRB_PLOT_DERIVED = 2
 MRB_TAG_GREATER = 0
 MRB_TAG_LESS = 1
 MRB_TAG_ADJACENT_DIFFERENCE = 2
@@ -84,6 +86,9 @@ SIGNATURES = {
     "mrb_checkpoint_header": (c_int, [c_char_p, POINTER(c_int), c_void_p, c_void_p, c_void_p, c_int]),
     "mrb_checkpoint_boxes": (c_int, [c_char_p, c_int, POINTER(c_int), c_void_p, c_void_p, c_int]),
     "mrb_read_checkpoint": (c_int, [c_char_p, c_void_p, c_int]),
+    "mrb_write_plotfile": (c_int, [c_char_p, c_void_p, c_double, c_int, c_int]),
+    "mrb_plot_nvars": (c_int, [c_int]),
+    "mrb_plot_fields": (c_int, [c_void_p, c_void_p, c_int]),
     "mrb_download": (c_int, [c_void_p, c_int, c_void_p]),
     "mrb_compute_flags_sphere": (c_int, [c_void_p, POINTER(c_double), c_double, c_int]),
     "mrb_finalize_flags": (c_int, [c_void_p]),

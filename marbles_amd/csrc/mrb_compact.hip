@@ -741,6 +741,7 @@ static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
     }
     if (rc) return rc;
     L->cur = 1 - L->cur;
+    L->prev_valid = true;
     return MRB_OK;
 }
 
@@ -786,6 +787,7 @@ int compact_stream_only(mrb_level *L, hipStream_t s)
         return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
     MRB_HIP(hipGetLastError());
     L->cur = 1 - L->cur;
+    L->prev_valid = false;  // a stream without a collide
     return MRB_OK;
 }
 
@@ -945,6 +947,98 @@ __global__ void __launch_bounds__(256) k_vorticity(
     for (int k = 0; k < 4; ++k) out[k * A.N + cell] = w[k];
 }
 
+// LBM::f_to_macrodata (LBM.cpp:810-906) for the plotfile: the 19 macrodata
+// components (rho, u, v, w, |u|, 2 rho e, QCorr, the pressure tensor, the
+// heat flux, T; 0 in solid cells) of the post-stream state of the last step,
+// which the reference's m_macrodata holds after the collide: the pull of
+// the buffer that held the state before the step (PULL), or, before any
+// step, of the stored populations themselves
+template <class T, int BCX, bool PULL>
+__global__ void __launch_bounds__(256) k_plot_md(
+    const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask, CompactArgs A, BCDev bc,
+    ShiftTab sh, Half16 h16, CollideParams P, double *__restrict__ out)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    double o[19];
+    for (int k = 0; k < 19; ++k) o[k] = 0.0;
+    if (m & 1u) {
+        double fv[NQ], gv[NQ];
+        if constexpr (PULL) {
+            pull_cell<BCX>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
+        } else {
+            float sf = 1.0f, sg = 1.0f;
+            if constexpr (kIsHalf<T>) {
+                sf = ldexpf(1.0f, -h16.e_cur[2 * (z / kScaleSlab)]);
+                sg = ldexpf(1.0f, -h16.e_cur[2 * (z / kScaleSlab) + 1]);
+            }
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const int64_t e = int64_t(q) * A.cs + cell * kCellStride<T>;
+                if constexpr (kIsHalf<T>) {
+                    fv[q] = double(float(f[e]) * sf) + sh.f(q);
+                    gv[q] = double(float(g[e]) * sg) + sh.g(q);
+                } else {
+                    fv[q] = widen(f[e], sh.f(q));
+                    gv[q] = widen(g[e], sh.g(q));
+                }
+            });
+        }
+        Moments mo;
+        accumulate_moments(fv, gv, mo);
+        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        o[0] = mo.rho;
+        o[1] = M.u;
+        o[2] = M.v;
+        o[3] = M.w;
+        o[4] = sqrt(M.u * M.u + M.v * M.v + M.w * M.w);
+        o[5] = mo.E;
+        o[6] = M.Qx;
+        o[7] = M.Qy;
+        o[8] = M.Qz;
+        o[9] = mo.pxx;
+        o[10] = mo.pyy;
+        o[11] = mo.pzz;
+        o[12] = mo.pxy;
+        o[13] = mo.pxz;
+        o[14] = mo.pyz;
+        o[15] = mo.qx;
+        o[16] = mo.qy;
+        o[17] = mo.qz;
+        o[18] = M.T;
+    }
+    for (int k = 0; k < 19; ++k) out[k * A.N + cell] = o[k];
+}
+
+// LBM::compute_q_corrections (LBM.cpp:959-991): dQCorr x, y, z, the
+// gradient() of the macrodata's QCorr components (6, 7, 8) along x, y, z;
+// 0 in solid cells
+__global__ void __launch_bounds__(256) k_plot_dq(
+    const double *__restrict__ md, const uint32_t *__restrict__ mask, CompactArgs A, CollideParams P,
+    double *__restrict__ out)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    double d[3] = {0.0, 0.0, 0.0};
+    if (m & 1u) {
+        const bool ok[3][2] = {{x < A.nx - 1 && ((m >> 2) & 1u), x > 0 && ((m >> 1) & 1u)},
+                               {y < A.ny - 1 && ((m >> 4) & 1u), y > 0 && ((m >> 3) & 1u)},
+                               {z < A.nz - 1 && ((m >> 6) & 1u), z > 0 && ((m >> 5) & 1u)}};
+        const int64_t st[3] = {1, A.nx, int64_t(A.nx) * A.ny};
+        for (int k = 0; k < 3; ++k) {
+            const double *u = md + (6 + k) * A.N;
+            const bool p = ok[k][0], mm = ok[k][1];
+            d[k] = one_sided_gradient(p, mm, p ? u[cell + st[k]] : 0.0, u[cell], mm ? u[cell - st[k]] : 0.0,
+                                      P.inv_dx[k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) out[k * A.N + cell] = d[k];
+}
+
 // refinement tags (AMReX AMRErrorTag criteria): 1 where the field is
 // greater / less than the value, or where the largest absolute difference
 // to a face neighbour (wrapped on periodic axes, skipped beyond a
@@ -991,6 +1085,53 @@ int compact_vorticity(mrb_level *L, double *out, hipStream_t s)
     if (rc) return rc;
     const CompactArgs A = make_args(L);
     k_vorticity<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(L->d_mac, L->d_mask, A, L->cp, out);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+template <class T, int BCX>
+static void launch_plot_md(mrb_level *L, const CompactArgs &A, const ShiftTab &sh, double *md, hipStream_t s)
+{
+    // the state the macrodata are taken from: the buffer before the last
+    // step (pulled), or the current one before any step
+    const int src = L->prev_valid ? 1 - L->cur : L->cur;
+    Half16 h{};
+    if (L->storage == MRB_STORAGE_F16_SCALED) h.e_cur = L->d_e16[src];
+    const T *f = (const T *)L->pop[0][src], *g = (const T *)L->pop[1][src];
+    if (L->prev_valid)
+        k_plot_md<T, BCX, true><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, A, L->bcdev, sh, h, L->cp,
+                                                                      md);
+    else
+        k_plot_md<T, 0, false><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, A, L->bcdev, sh, h, L->cp,
+                                                                     md);
+}
+
+template <class T>
+static void plot_md_t(mrb_level *L, const CompactArgs &A, const ShiftTab &sh, double *md, hipStream_t s)
+{
+    switch (bcx_of(L)) {
+    case kBcxProfile: launch_plot_md<T, kBcxProfile>(L, A, sh, md, s); break;
+    case kBcxPressure: launch_plot_md<T, kBcxPressure>(L, A, sh, md, s); break;
+    case kBcxProfile | kBcxPressure: launch_plot_md<T, kBcxProfile | kBcxPressure>(L, A, sh, md, s); break;
+    default: launch_plot_md<T, 0>(L, A, sh, md, s); break;
+    }
+}
+
+int compact_plot_fields(mrb_level *L, double *md, double *der, hipStream_t s)
+{
+    const CompactArgs A = make_args(L);
+    const ShiftTab sh = shift_tab(L);
+    if (L->storage == MRB_STORAGE_F64)
+        plot_md_t<double>(L, A, sh, md, s);
+    else if (L->storage == MRB_STORAGE_F32_SHIFTED)
+        plot_md_t<float>(L, A, sh, md, s);
+    else
+        plot_md_t<_Float16>(L, A, sh, md, s);
+    MRB_HIP(hipGetLastError());
+    // LBM::compute_derived (LBM.cpp:909-955) on the macrodata velocities
+    k_vorticity<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(md, L->d_mask, A, L->cp, der);
+    MRB_HIP(hipGetLastError());
+    k_plot_dq<<<grid_of(A), dim3(64, 4, 1), 0, s>>>(md, L->d_mask, A, L->cp, der + 4 * A.N);
     MRB_HIP(hipGetLastError());
     return MRB_OK;
 }

@@ -401,6 +401,7 @@ int fused_step(mrb_level *L, int variant, hipStream_t s)
     const int rc = fused_launch(L, variant, s);
     if (rc) return rc;
     L->cur = 1 - L->cur;
+    L->prev_valid = true;
     return MRB_OK;
 }
 

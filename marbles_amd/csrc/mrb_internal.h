@@ -152,6 +152,10 @@ int f16_pack(mrb_level *L, int lattice, const double *src, int64_t src_cs, hipSt
 int f16_unpack(const mrb_level *L, int lattice, double *dst, hipStream_t s);
 int compact_macro(const mrb_level *L, double *out, hipStream_t s);
 int compact_vorticity(mrb_level *L, double *out, hipStream_t s);
+// plotfile fields of a compact level: the 19 macrodata components of the
+// last step's post-stream state (or of the stored state before any step)
+// into md, and from them vorticity (4) and dQCorr (3) into der
+int compact_plot_fields(mrb_level *L, double *md, double *der, hipStream_t s);
 int compact_tag(const mrb_level *L, const double *field, int criterion, double value, uint8_t *tags,
                 hipStream_t s);
 
@@ -196,6 +200,9 @@ struct mrb_level {
     bool flags_ready = false;
     double shift_rho0 = 1.0, shift_T0 = 1.0 / 3.0;
     bool pops_set = false;  // populations uploaded or initialised
+    // compact layout: the spare buffer holds the state before the last
+    // step (set by a step, cleared when the populations are set)
+    bool prev_valid = false;
     double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
     // FP16 storage (compact layout): power-of-two scale exponent and the
     // largest |population - shift| over fluid cells per (scale box, lattice),

@@ -231,6 +231,30 @@ std::vector<int64_t> fab_cells(const mrb_level *L)
 
 std::string level_dir(const char *dir, int l) { return std::string(dir) + "/Level_" + std::to_string(l); }
 
+// LBM's plot variable names (LBM.cpp:14-82): macrodata, then f_00..f_26 and
+// g_00..g_26 (save_streaming), then the derived fields (save_derived), then
+// is_fluid and eb_boundary
+std::vector<std::string> plot_names(bool streaming, bool derived)
+{
+    std::vector<std::string> v = {"rho",  "vel_x", "vel_y", "vel_z", "vel_mag", "two_rho_e", "QCorrX",
+                                  "QCorrY", "QCorrZ", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "qx", "qy", "qz",
+                                  "temperature"};
+    if (streaming) {
+        for (const char *p : {"f_", "g_"})
+            for (int q = 0; q < NQ; ++q) {
+                char b[8];
+                std::snprintf(b, sizeof b, "%s%02d", p, q);
+                v.push_back(b);
+            }
+    }
+    if (derived)
+        for (const char *n : {"vort_x", "vort_y", "vort_z", "vort_mag", "dQCorrX", "dQCorrY", "dQCorrZ"})
+            v.push_back(n);
+    v.push_back("is_fluid");
+    v.push_back("eb_boundary");
+    return v;
+}
+
 }  // namespace
 }  // namespace mrb
 
@@ -370,6 +394,82 @@ int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels)
         }
     }
     return MRB_OK;
+}
+
+int mrb_plot_fields(mrb_level *L, double *out, int flags)
+{
+    if (!L || !out) return set_error(MRB_E_ARG, "null argument");
+    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_UNSUPPORTED, "plot fields: compact layout only");
+    if (!L->flags_ready || !L->pops_set) return set_error(MRB_E_STATE, "flags and populations must be set");
+    const bool streaming = flags & MRB_PLOT_STREAMING, derived = flags & MRB_PLOT_DERIVED;
+    const int64_t N = L->total_cells;
+    double *d = nullptr;
+    MRB_HIP(hipMalloc(&d, sizeof(double) * 26 * size_t(N)));
+    int rc = compact_plot_fields(L, d, d + 19 * N, nullptr);
+    hipError_t e = rc == MRB_OK ? hipDeviceSynchronize() : hipSuccess;
+    double *o = out;
+    if (rc == MRB_OK && e == hipSuccess) e = hipMemcpy(o, d, sizeof(double) * 19 * size_t(N), hipMemcpyDeviceToHost);
+    o += 19 * N;
+    if (rc == MRB_OK && e == hipSuccess && derived) {
+        const int64_t skip = streaming ? 2 * NQ * N : 0;
+        e = hipMemcpy(o + skip, d + 19 * N, sizeof(double) * 7 * size_t(N), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "plot fields");
+    if (rc) return rc;
+    if (streaming) {
+        if ((rc = mrb_download(L, MRB_ARR_F, o))) return rc;
+        if ((rc = mrb_download(L, MRB_ARR_G, o + NQ * N))) return rc;
+        o += 2 * NQ * N;
+    }
+    if (derived) o += 7 * N;
+    std::vector<int32_t> isf(size_t(2) * size_t(N));
+    if ((rc = mrb_download(L, MRB_ARR_ISFLUID, isf.data()))) return rc;
+    for (int64_t c = 0; c < 2 * N; ++c) o[c] = double(isf[size_t(c)]);
+    return MRB_OK;
+}
+
+int mrb_plot_nvars(int flags)
+{
+    return int(plot_names(flags & MRB_PLOT_STREAMING, flags & MRB_PLOT_DERIVED).size());
+}
+
+int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, int flags)
+{
+    if (!dir || !lvl) return set_error(MRB_E_ARG, "null argument");
+    const mrb_level *L = lvl;
+    const std::vector<std::string> names = plot_names(flags & MRB_PLOT_STREAMING, flags & MRB_PLOT_DERIVED);
+    const int nv = int(names.size());
+    const int64_t N = L->total_cells;
+    std::vector<double> data(size_t(nv) * size_t(N));
+    int rc = mrb_plot_fields(lvl, data.data(), flags);
+    if (rc) return rc;
+    if ((rc = make_dirs(level_dir(dir, 0)))) return rc;
+    // amrex::WriteMultiLevelPlotfile -> WriteGenericPlotfileHeader
+    std::ofstream h(std::string(dir) + "/Header", std::ios::trunc);
+    if (!h) return set_error(MRB_E_ARG, std::string("cannot write ") + dir + "/Header");
+    h.precision(17);
+    h << "HyperCLaw-V1.1" << '\n' << nv << '\n';
+    for (const auto &n : names) h << n << '\n';
+    h << 3 << '\n' << time << '\n' << 0 << '\n';
+    for (int k = 0; k < 3; ++k) h << L->prob_lo[k] << ' ';
+    h << '\n';
+    for (int k = 0; k < 3; ++k) h << L->prob_hi[k] << ' ';
+    h << '\n' << '\n';  // no refinement ratios with one level
+    const int dlo[3] = {0, 0, 0}, dhi[3] = {L->ncell[0] - 1, L->ncell[1] - 1, L->ncell[2] - 1};
+    h << box_text(dlo, dhi) << ' ' << '\n';
+    h << step << ' ' << '\n';
+    for (int k = 0; k < 3; ++k) h << L->dx[k] << ' ';
+    h << '\n' << 0 << '\n' << 0 << '\n';
+    const LevelBoxes B = level_boxes(L);
+    h << 0 << ' ' << B.valid.size() << ' ' << time << '\n' << step << '\n';
+    for (const auto &v : B.valid)
+        for (int k = 0; k < 3; ++k)
+            h << L->prob_lo[k] + v[size_t(k)] * L->dx[k] << ' ' << L->prob_lo[k] + (v[size_t(3 + k)] + 1) * L->dx[k]
+              << '\n';
+    h << "Level_0/Cell" << '\n';
+    if (!h) return set_error(MRB_E_ARG, std::string("write failed: ") + dir + "/Header");
+    return write_multifab(level_dir(dir, 0) + "/Cell", B, nv, data.data(), fab_cells(L));
 }
 
 }  // extern "C"

@@ -1269,6 +1269,7 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
     if (arr != MRB_ARR_F && arr != MRB_ARR_G) return set_error(MRB_E_ARG, "unknown array");
     L->fresh_depth = 0;
     L->pops_set = true;
+    L->prev_valid = false;
     const int64_t n = NQ * L->total_cells;
     const double *h = static_cast<const double *>(host);
     if (L->storage == MRB_STORAGE_F64) return copy_lattice(L, arr, const_cast<void *>(host), true);
@@ -1474,7 +1475,10 @@ static int init_from_state(
 
 int mrb_init_taylorgreen(mrb_level *L, double rho0, double v0, const double omega[3])
 {
-    if (L) L->pops_set = true;
+    if (L) {
+        L->pops_set = true;
+        L->prev_valid = false;
+    }
     if (!L || !omega) return set_error(MRB_E_ARG, "null argument");
     const double Lw = 1.0 / M_PI;
     const double o0 = omega[0], o1 = omega[1], o2 = omega[2];
@@ -1499,7 +1503,10 @@ int mrb_init_constant(
     mrb_level *L, double density, const double mach[3], double temperature, double adiabatic_exponent,
     double m_bar)
 {
-    if (L) L->pops_set = true;
+    if (L) {
+        L->pops_set = true;
+        L->prev_valid = false;
+    }
     if (!L || !mach) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
     // ic::Constant (IC.cpp:4-32): velocity = mach * speed_of_sound_ref
@@ -1541,7 +1548,10 @@ int mrb_init_fields(
     mrb_level *L, const double *rho, const double *u, const double *v, const double *w, const double *T,
     double adiabatic_exponent, double m_bar)
 {
-    if (L) L->pops_set = true;
+    if (L) {
+        L->pops_set = true;
+        L->prev_valid = false;
+    }
     if (!L || !rho || !u || !v || !w || !T) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
     if (!(m_bar > 0.0) || !(adiabatic_exponent > 1.0)) return set_error(MRB_E_ARG, "invalid gas parameters");

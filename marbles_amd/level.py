@@ -268,6 +268,25 @@ class Level:
             raise ValueError("out must be a device float64 tensor of 4 * ncells elements")
         check(self.lib.mrb_vorticity(self.h, c_void_p(out.data_ptr()), stream))
 
+    def plot_fields(self, streaming: bool = False, derived: bool = False) -> np.ndarray:
+        """The plotfile variables of LBM::plot_file_mf (compact layout) as a host
+        array (nvars, nz, ny, nx): macrodata of the last step's post-stream
+        state, optionally f and g, optionally vorticity and dQCorr, then
+        is_fluid and eb_boundary."""
+        flags = (_lib.MRB_PLOT_STREAMING if streaming else 0) | (_lib.MRB_PLOT_DERIVED if derived else 0)
+        nv = self.lib.mrb_plot_nvars(flags)
+        out = np.empty((nv,) + tuple(reversed(self.ncell)))
+        check(self.lib.mrb_plot_fields(self.h, out.ctypes.data_as(c_void_p), flags))
+        return out
+
+    def write_plotfile(self, path: str, time: float, step: int, streaming: bool = False,
+                       derived: bool = False) -> None:
+        """LBM::write_plot_file: an AMReX plotfile of this level at `path`."""
+        import os
+
+        flags = (_lib.MRB_PLOT_STREAMING if streaming else 0) | (_lib.MRB_PLOT_DERIVED if derived else 0)
+        check(self.lib.mrb_write_plotfile(os.fsencode(path), self.h, float(time), int(step), flags))
+
     def tag_cells(self, field, criterion: str, value: float, tags, stream=None) -> None:
         """Refinement tags (uint8 device tensor of ncells) from a device float64
         field: criterion 'greater', 'less' or 'adjacent_difference'."""

@@ -952,6 +952,65 @@ static inline double gradient(
     return (vp + vc + vm) * L->inv_dx[dir];
 }
 
+/* the vorticity LBM::compute_derived (LBM.cpp:927-950) computes at one
+ * cell from the level's macrodata; 0 where the cell is not fluid */
+static void vorticity_at(const orc_level *L, int b, int fluid, const int iv[3], double w[4])
+{
+    w[0] = w[1] = w[2] = w[3] = 0.0;
+    if (!fluid) return;
+    const double vx = gradient(L, b, 0, 2, iv);
+    const double wx = gradient(L, b, 0, 3, iv);
+    const double uy = gradient(L, b, 1, 1, iv);
+    const double wy = gradient(L, b, 1, 3, iv);
+    const double uz = gradient(L, b, 2, 1, iv);
+    const double vz = gradient(L, b, 2, 2, iv);
+    w[0] = wy - vz;
+    w[1] = uz - wx;
+    w[2] = vx - uy;
+    w[3] = sqrt((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
+}
+
+/* LBM::plot_file_mf (LBM.cpp:1629-1669): per box, the valid cells'
+ * variables [19 macrodata, (27 f, 27 g if streaming), (vort_x, vort_y,
+ * vort_z, vort_mag, dQCorrX..Z if derived), is_fluid, eb_boundary],
+ * component-major (x fastest), boxes back to back.  The macrodata and
+ * dQCorr are those of the last collide (or of the initial state), the
+ * vorticity is compute_derived's of those macrodata; the level is not
+ * modified. */
+void orc_plot_vars(const orc_level *L, int streaming, int derived, double *out)
+{
+    const int nc = 19 + (streaming ? 2 * ORC_NQ : 0) + (derived ? 7 : 0) + 2;
+    int64_t off = 0;
+    for (int b = 0; b < L->nbox; b++) {
+        const fabdim *fd = &L->fdg[b];
+        const fabdim *m1 = &L->fd1[b];
+        const fabdim *d0 = &L->fd0[b];
+        const int nx = d0->hi[0] - d0->lo[0] + 1, ny = d0->hi[1] - d0->lo[1] + 1,
+                  nz = d0->hi[2] - d0->lo[2] + 1;
+        const int64_t nv = (int64_t)nx * ny * nz;
+        for (int k = d0->lo[2]; k <= d0->hi[2]; k++)
+            for (int j = d0->lo[1]; j <= d0->hi[1]; j++)
+                for (int i = d0->lo[0]; i <= d0->hi[0]; i++) {
+                    const int64_t c = ((int64_t)(k - d0->lo[2]) * ny + (j - d0->lo[1])) * nx + (i - d0->lo[0]);
+                    int n = 0;
+                    for (int m = 0; m < 19; m++) out[off + (n++) * nv + c] = L->md[b][fidx(m1, m, i, j, k)];
+                    if (streaming) {
+                        for (int q = 0; q < ORC_NQ; q++) out[off + (n++) * nv + c] = L->f[b][fidx(fd, q, i, j, k)];
+                        for (int q = 0; q < ORC_NQ; q++) out[off + (n++) * nv + c] = L->g[b][fidx(fd, q, i, j, k)];
+                    }
+                    if (derived) {
+                        double w[4];
+                        const int iv[3] = {i, j, k};
+                        vorticity_at(L, b, L->isf[b][fidx(fd, 0, i, j, k)] == 1, iv, w);
+                        for (int m = 0; m < 4; m++) out[off + (n++) * nv + c] = w[m];
+                        for (int m = 0; m < 3; m++) out[off + (n++) * nv + c] = L->dq[b][fidx(d0, m, i, j, k)];
+                    }
+                    for (int m = 0; m < 2; m++) out[off + (n++) * nv + c] = (double)L->isf[b][fidx(fd, m, i, j, k)];
+                }
+        off += nc * nv;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* LBM::compute_derived (Source/LBM.cpp:909-955): vorticity of the velocity  */
 /* of the current populations (macrodata recomputed from them first).        */
@@ -974,20 +1033,9 @@ void orc_vorticity(orc_level *L, double *out)
             for (int j = d0->lo[1]; j <= d0->hi[1]; j++)
                 for (int i = d0->lo[0]; i <= d0->hi[0]; i++) {
                     const int64_t c = ((int64_t)(k - d0->lo[2]) * ny + (j - d0->lo[1])) * nx + (i - d0->lo[0]);
-                    double w[4] = {0.0, 0.0, 0.0, 0.0};
-                    if (isf[fidx(fd, 0, i, j, k)] == 1) {
-                        const int iv[3] = {i, j, k};
-                        const double vx = gradient(L, b, 0, 2, iv);
-                        const double wx = gradient(L, b, 0, 3, iv);
-                        const double uy = gradient(L, b, 1, 1, iv);
-                        const double wy = gradient(L, b, 1, 3, iv);
-                        const double uz = gradient(L, b, 2, 1, iv);
-                        const double vz = gradient(L, b, 2, 2, iv);
-                        w[0] = wy - vz;
-                        w[1] = uz - wx;
-                        w[2] = vx - uy;
-                        w[3] = sqrt((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
-                    }
+                    double w[4];
+                    const int iv[3] = {i, j, k};
+                    vorticity_at(L, b, isf[fidx(fd, 0, i, j, k)] == 1, iv, w);
                     for (int n = 0; n < 4; n++) out[off + n * nv + c] = w[n];
                 }
         off += 4 * nv;

@@ -126,6 +126,10 @@ void orc_eb_force(const orc_level *L, double out[3]);
  * the current populations' velocity over every box's valid cells (per box:
  * 4 components, component-major, x fastest); recomputes the macrodata. */
 void orc_vorticity(orc_level *L, double *out);
+/* LBM::plot_file_mf (LBM.cpp:1629-1669) of the level's current data: per
+ * box, the valid cells' [19 macrodata, (f, g), (vorticity, dQCorr),
+ * is_fluid, eb_boundary] component-major, boxes back to back. */
+void orc_plot_vars(const orc_level *L, int streaming, int derived, double *out);
 
 /* ---- two levels, refinement ratio 2 -------------------------------------
  * The coarse level must cover its domain; the fine level's domain is the

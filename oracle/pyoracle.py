@@ -90,6 +90,7 @@ def load() -> ctypes.CDLL:
         "orc_sphere_covered": (c_int, [D3, D3, D3, c_double, c_int]),
         "orc_eb_force": (None, [c_void_p, D3]),
         "orc_vorticity": (None, [c_void_p, c_void_p]),
+        "orc_plot_vars": (None, [c_void_p, c_int, c_int, c_void_p]),
         "orc_ic_fields": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_double, c_double]),
         "orc_amr_coalesce": (None, [c_void_p, c_void_p, c_int]),
@@ -307,6 +308,22 @@ class OracleLevel:
             dom[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = out[off:off + 4 * n].reshape((4,) + sh)
             off += 4 * n
         return dom
+
+
+def plot_vars(O: "OracleLevel", streaming: bool = False, derived: bool = False) -> np.ndarray:
+    """LBM::plot_file_mf of the oracle level over the domain, shaped
+    (nvars, nz, ny, nx) (macrodata, f and g, vorticity and dQCorr, is_fluid
+    and eb_boundary)."""
+    nc = 19 + (54 if streaming else 0) + (7 if derived else 0) + 2
+    sizes = [int(np.prod(sh)) for sh in O.shapes0]
+    out = np.zeros(nc * sum(sizes))
+    O.lib.orc_plot_vars(O.h, int(streaming), int(derived), out.ctypes.data_as(c_void_p))
+    dom = np.zeros((nc,) + tuple(reversed(O.ncell)))
+    off = 0
+    for (lo, hi), sh, n in zip(O.boxes, O.shapes0, sizes):
+        dom[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = out[off:off + nc * n].reshape((nc,) + sh)
+        off += nc * n
+    return dom
 
 
 def amr_coalesce(fine: OracleLevel, crse: OracleLevel, lattice: int) -> None:

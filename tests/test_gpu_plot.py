@@ -1,0 +1,85 @@
+"""Plotfile fields and files (LBM::write_plot_file, LBM.cpp:1629-1682)
+through the C ABI: the engine's plot variables equal the oracle's
+(macrodata of the last collide's post-stream state, f, g, vorticity and
+dQCorr of those macrodata, is_fluid), and the plotfile on disk follows
+AMReX's layout (Header read line by line, Cell MultiFab by an independent
+VisMF reader)."""
+import os
+
+import numpy as np
+import pytest
+from vismf import read_vismf
+
+from marbles_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+# LBM.cpp:14-82 (macrodata, f, g, derived, is_fluid names)
+NAMES = (["rho", "vel_x", "vel_y", "vel_z", "vel_mag", "two_rho_e", "QCorrX", "QCorrY", "QCorrZ", "pxx", "pyy",
+          "pzz", "pxy", "pxz", "pyz", "qx", "qy", "qz", "temperature"]
+         + [f"f_{q:02d}" for q in range(27)] + [f"g_{q:02d}" for q in range(27)]
+         + ["vort_x", "vort_y", "vort_z", "vort_mag", "dQCorrX", "dQCorrY", "dQCorrZ", "is_fluid", "eb_boundary"])
+
+
+def _pair(oracle, engine, cfg):
+    from oracle.pyoracle import build_oracle_level
+
+    O = build_oracle_level(cfg)
+    E = engine.Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], layout="compact")
+    E.set_phys(cfg["nu"], cfg.get("alpha"))
+    E.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+    E.flags_upload(O.valid("is_fluid"))
+    E.upload("f", O.valid("f"))
+    E.upload("g", O.valid("g"))
+    return O, E
+
+
+def _compare(got, ref):
+    for k in range(ref.shape[0]):
+        scale = max(np.abs(ref[k]).max(), 1e-300)
+        err = np.abs(got[k] - ref[k]).max() / scale
+        assert err <= 1e-12, (NAMES[k] if ref.shape[0] == len(NAMES) else k, err)
+
+
+@pytest.mark.parametrize("case", ["channel", "pressure"])
+def test_plot_fields_match_oracle(oracle, engine, case):
+    """Before any step (macrodata of the initial state) and after 15 steps
+    (the pull of the previous buffer), with an EB sphere, the velocity
+    inlet and outflow, or a pressure outlet behind a walled inlet."""
+    if case == "channel":
+        cfg = configs.channel_sphere((40, 24, 24), radius=5.0, center=(14.0, 12.0, 12.0))
+    else:
+        cfg = configs.pressure_case("walled", (24, 16, 20))
+    O, E = _pair(oracle, engine, cfg)
+    _compare(E.plot_fields(streaming=True, derived=True), oracle.plot_vars(O, streaming=True, derived=True))
+    E.advance(15)
+    O.evolve(15)
+    got = E.plot_fields(streaming=True, derived=True)
+    ref = oracle.plot_vars(O, streaming=True, derived=True)
+    assert np.abs(ref[19 + 54:19 + 54 + 4]).max() > 0  # a vorticity to compare
+    _compare(got, ref)
+    E.close()
+
+
+def test_plotfile_layout(oracle, engine, tmp_path):
+    cfg = configs.channel_sphere((32, 16, 12), radius=3.0, center=(10.0, 8.0, 6.0))
+    O, E = _pair(oracle, engine, cfg)
+    E.advance(4)
+    path = str(tmp_path / "plt00004")
+    E.write_plotfile(path, time=4.0, step=4, streaming=True, derived=True)
+    head = open(os.path.join(path, "Header")).read().split("\n")
+    nv = len(NAMES)
+    assert head[0] == "HyperCLaw-V1.1" and int(head[1]) == nv and head[2:2 + nv] == NAMES
+    rest = head[2 + nv:]
+    assert rest[0] == "3" and float(rest[1]) == 4.0 and rest[2] == "0"
+    assert [float(x) for x in rest[3].split()] == [0.0, 0.0, 0.0]
+    assert [float(x) for x in rest[4].split()] == [32.0, 16.0, 12.0]
+    assert rest[5] == "" and rest[6].strip() == "((0,0,0) (31,15,11) (0,0,0))" and rest[7].strip() == "4"
+    assert [float(x) for x in rest[8].split()] == [1.0, 1.0, 1.0] and rest[9] == "0" and rest[10] == "0"
+    assert rest[11].split() == ["0", "1", "4"] and rest[12] == "4"
+    assert [[float(x) for x in r.split()] for r in rest[13:16]] == [[0.0, 32.0], [0.0, 16.0], [0.0, 12.0]]
+    assert rest[16] == "Level_0/Cell"
+    ncomp, ngrow, boxes, fabs = read_vismf(os.path.join(path, "Level_0", "Cell"))
+    assert (ncomp, ngrow, boxes) == (nv, 0, [(0, 0, 0, 31, 15, 11)])
+    np.testing.assert_array_equal(fabs[0], E.plot_fields(streaming=True, derived=True))
+    E.close()

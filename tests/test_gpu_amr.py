@@ -25,10 +25,23 @@ def _nonperiodic(cfg):
     return cfg
 
 
+def _inlet_pressure(cfg):
+    """The outflow_x geometry with a constant velocity inlet (u = 0.04) at
+    x-lo, which the fine slab touches, and a pressure outlet at x-hi on the
+    coarse level."""
+    cfg = _nonperiodic(cfg)
+    cfg["bc_lo"] = (2, 0, 0)
+    cfg["bc_hi"] = (3, 0, 0)
+    cfg["velocity_bc"] = {"kind": "constant", "dir": 0, "Mach_ref": 0.04 / (5.0 / 3.0 * 0.2) ** 0.5,
+                          "initial_density": 1.0, "initial_temperature": 0.2}
+    return cfg
+
+
 CASES = {
     "periodic": lambda: small_amr(),
     "outflow_x": lambda: _nonperiodic(small_amr(n=32, nbox=80, seed=9, fraction=0.25)),
     "uneven_32": lambda: small_amr(n=32, nbox=80, seed=4, fraction=0.2),
+    "inlet_pressure": lambda: _inlet_pressure(small_amr(n=32, nbox=80, seed=9, fraction=0.25)),
 }
 
 
@@ -96,7 +109,7 @@ def test_amr_advance_matches_oracle(oracle, engine, case):
     H.close()
 
 
-@pytest.mark.parametrize("case", ["outflow_x", "uneven_32"])
+@pytest.mark.parametrize("case", ["outflow_x", "uneven_32", "inlet_pressure"])
 def test_batched_fab_step_on_uneven_boxes_matches_oracle(oracle, engine, case):
     """Single level over the coarse level's uneven boxes (edges 4 and 8):
     the batched fab step (one launch per pass over every box) against the
@@ -107,14 +120,14 @@ def test_batched_fab_step_on_uneven_boxes_matches_oracle(oracle, engine, case):
     cfg = CASES[case]()
     S = OracleLevel(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=cfg["coarse_boxes"], ng=3)
     S.set_phys(cfg["nu"], cfg["alpha"])
-    S.set_bc(cfg["bc_lo"], cfg["bc_hi"])
+    S.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
     S.flags_all_regular()
     S.ic_fields(*configs.level_fields(cfg, S.ncell, S.boxes, 3))
     S.init_finish()
     E = engine.Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=cfg["coarse_boxes"],
                      ng=3, layout="fab")
     E.set_phys(cfg["nu"], cfg["alpha"])
-    E.set_bc(cfg["bc_lo"], cfg["bc_hi"])
+    E.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
     E.flags_all_regular()
     E.init_fields(*configs.level_fields(cfg, E.ncell, E.boxes, 3))
     S.evolve(5)

@@ -370,6 +370,10 @@ int mrb_tag_cells(const mrb_level *lvl, const double *field, int criterion, doub
 
 /* ---- diagnostics ------------------------------------------------------------ */
 const char *mrb_last_error(void);
+/* lbm::stencil::Stencil (Source/Stencil.H:41-169): the 27 lattice
+ * velocities (q-major, 3 ints each), weights and opposite directions in
+ * the reference's order. */
+int mrb_stencil(int evs[81], double weights[27], int bounce_dirs[27]);
 /* Name of the kernel variant used by mrb_stream_collide. */
 const char *mrb_hot_kernel_name(const mrb_level *lvl);
 /* Device pointers of the current population buffers (for zero-copy use by

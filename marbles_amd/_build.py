@@ -73,5 +73,16 @@ def build_library() -> str:
     return LIB
 
 
+def build_cpp_test() -> str:
+    """Compiles tests/cpp/test_cpp_api.cpp (the C++17 host header against the
+    library and the CPU oracle) with g++; returns the program's path."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_cpp_api.cpp")
+    exe = os.path.splitext(src)[0]
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", INCLUDE, "-I", os.path.join(ROOT, "oracle"),
+          src, "-o", exe, "-L", HERE, "-lmarbles_amd", "-L", os.path.join(ROOT, "oracle"), "-llbm_oracle",
+          "-Wl,-rpath,$ORIGIN/../../marbles_amd", "-Wl,-rpath,$ORIGIN/../../oracle"])
+    return exe
+
+
 if __name__ == "__main__":
     print(build_library())

@@ -1371,6 +1371,17 @@ int mrb_compute_flags_sphere(mrb_level *L, const double center[3], double radius
     return mrb_finalize_flags(L);
 }
 
+int mrb_stencil(int evs[NQ * 3], double weights[NQ], int bounce_dirs[NQ])
+{
+    if (!evs || !weights || !bounce_dirs) return set_error(MRB_E_ARG, "null argument");
+    for (int q = 0; q < NQ; ++q) {
+        for (int d = 0; d < 3; ++d) evs[3 * q + d] = kEV[q][d];
+        weights[q] = weight(q);
+        bounce_dirs[q] = kBounce[q];
+    }
+    return MRB_OK;
+}
+
 int mrb_finalize_flags(mrb_level *L)
 {
     if (!L) return set_error(MRB_E_ARG, "null argument");

@@ -320,7 +320,8 @@ int mrb_checkpoint_boxes(const char *dir, int level, int *nbox, int *box_lo, int
 int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels);
 
 /* ---- plotfile ------------------------------------------------------------
- * LBM::write_plot_file (Source/LBM.cpp:1629-1682) for one compact level:
+ * LBM::write_plot_file (Source/LBM.cpp:1629-1682) for one level (compact,
+ * or fab outside a two-level hierarchy):
  * the variables of LBM::plot_file_mf in its order -- the 19 macrodata
  * components (rho, vel_x, vel_y, vel_z, vel_mag, two_rho_e, QCorrX..Z,
  * pxx, pyy, pzz, pxy, pxz, pyz, qx, qy, qz, temperature) of the last step's
@@ -336,8 +337,11 @@ int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels);
 int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, int flags);
 /* The plotfile's variable count for these flags. */
 int mrb_plot_nvars(int flags);
-/* The plotfile's fields into host memory: mrb_plot_nvars(flags) x N
- * doubles, variable-major (x fastest).  Synchronous. */
+/* The plotfile's fields into host memory: per box, mrb_plot_nvars(flags)
+ * variables over the box's valid cells (x fastest), boxes back to back.
+ * On a fab level the macrodata are those of the last stage collide, of the
+ * state before the last batched step pulled, or of the stored state before
+ * any step.  Synchronous. */
 int mrb_plot_fields(mrb_level *lvl, double *out, int flags);
 
 /* Density, velocity (x, y, z) and temperature of the current state per

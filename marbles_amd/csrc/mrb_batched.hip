@@ -74,25 +74,6 @@ __device__ __forceinline__ bool tile_cell(
     return true;
 }
 
-// pull of cell loc (fluid) in box B: f*_q = fluid(x - c_q) ? f_q(x - c_q) : f_qbar(x)
-__device__ __forceinline__ void pull_fab(
-    const double *__restrict__ f, const double *__restrict__ g, const BoxDev &B, int64_t loc,
-    uint32_t m, double fv[NQ], double gv[NQ])
-{
-    const double *fb = f + B.base * NQ;
-    const double *gb = g + B.base * NQ;
-    const int64_t sy = B.n[0], sz = int64_t(B.n[0]) * B.n[1];
-    static_for<NQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-        constexpr int qb = kBounce[q];
-        const int64_t src = loc - (cx + cy * sy + cz * sz);
-        const int64_t e = ((m >> q) & 1u) ? int64_t(q) * B.ncell + src : int64_t(qb) * B.ncell + loc;
-        fv[q] = fb[e];
-        gv[q] = gb[e];
-    });
-}
-
 // masked_avgdown (Utilities.H:315-350) of one population; false if all 8
 // children are the -1 sentinel
 __device__ __forceinline__ bool masked_mean(const double *__restrict__ c0, int64_t sy, int64_t sz, double &out)
@@ -406,6 +387,7 @@ int fab_batched_step(mrb_level *L, const mrb_level *coal_from, int out_depth, hi
     if (rc) return rc;
     L->cur = 1 - L->cur;
     L->fresh_depth = 0;
+    L->md_src = 2;
     if (out_depth > 0) return fab_fill_boundary_depth(L, out_depth, s);
     return MRB_OK;
 }

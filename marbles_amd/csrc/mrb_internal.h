@@ -152,10 +152,33 @@ int f16_pack(mrb_level *L, int lattice, const double *src, int64_t src_cs, hipSt
 int f16_unpack(const mrb_level *L, int lattice, double *dst, hipStream_t s);
 int compact_macro(const mrb_level *L, double *out, hipStream_t s);
 int compact_vorticity(mrb_level *L, double *out, hipStream_t s);
+// pull of cell loc (fluid) in box B: f*_q = fluid(x - c_q) ? f_q(x - c_q) : f_qbar(x)
+__device__ __forceinline__ void pull_fab(
+    const double *__restrict__ f, const double *__restrict__ g, const BoxDev &B, int64_t loc,
+    uint32_t m, double fv[NQ], double gv[NQ])
+{
+    const double *fb = f + B.base * NQ;
+    const double *gb = g + B.base * NQ;
+    const int64_t sy = B.n[0], sz = int64_t(B.n[0]) * B.n[1];
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+        constexpr int qb = kBounce[q];
+        const int64_t src = loc - (cx + cy * sy + cz * sz);
+        const int64_t e = ((m >> q) & 1u) ? int64_t(q) * B.ncell + src : int64_t(qb) * B.ncell + loc;
+        fv[q] = fb[e];
+        gv[q] = gb[e];
+    });
+}
+
 // plotfile fields of a compact level: the 19 macrodata components of the
 // last step's post-stream state (or of the stored state before any step)
 // into md, and from them vorticity (4) and dQCorr (3) into der
 int compact_plot_fields(mrb_level *L, double *md, double *der, hipStream_t s);
+// fab layout: the plotfile macrodata into L->d_md (valid cells, one ghost
+// layer filled) and vorticity (4) and dQCorr (3) into der (7 fab comps)
+int fab_plot_fields(mrb_level *L, double *der, hipStream_t s);
+int fab_plot_md_pull(mrb_level *L, hipStream_t s);
 int compact_tag(const mrb_level *L, const double *field, int criterion, double value, uint8_t *tags,
                 hipStream_t s);
 
@@ -203,6 +226,12 @@ struct mrb_level {
     // compact layout: the spare buffer holds the state before the last
     // step (set by a step, cleared when the populations are set)
     bool prev_valid = false;
+    // fab layout: where the plotfile's macrodata of the current state come
+    // from -- 0: the moments of the stored populations (initial state), 1:
+    // d_md of the last stage collide, 2: the pull of the spare buffer,
+    // which holds the state before the last batched step; -1: none
+    int md_src = 0;
+    bool in_hierarchy = false;  // linked by mrb_amr_link
     double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
     // FP16 storage (compact layout): power-of-two scale exponent and the
     // largest |population - shift| over fluid cells per (scale box, lattice),

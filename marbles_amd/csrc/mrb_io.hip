@@ -396,12 +396,68 @@ int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels)
     return MRB_OK;
 }
 
+// the plot variables of a fab level: per box, every variable over the
+// valid cells, boxes back to back
+static int fab_plot(mrb_level *L, double *out, bool streaming, bool derived)
+{
+    const int64_t T = L->total_cells;
+    double *der = nullptr;
+    MRB_HIP(hipMalloc(&der, sizeof(double) * 7 * size_t(T)));
+    int rc = fab_plot_fields(L, der, nullptr);
+    hipError_t e = rc == MRB_OK ? hipDeviceSynchronize() : hipSuccess;
+    std::vector<double> md(size_t(19) * size_t(T)), dv(size_t(7) * size_t(T));
+    if (rc == MRB_OK && e == hipSuccess) e = hipMemcpy(md.data(), L->d_md, sizeof(double) * md.size(), hipMemcpyDeviceToHost);
+    if (rc == MRB_OK && e == hipSuccess) e = hipMemcpy(dv.data(), der, sizeof(double) * dv.size(), hipMemcpyDeviceToHost);
+    (void)hipFree(der);
+    if (e != hipSuccess) return hip_fail(e, "plot fields");
+    if (rc) return rc;
+    std::vector<double> f, g;
+    if (streaming) {
+        f.resize(size_t(NQ) * size_t(T));
+        g.resize(f.size());
+        if ((rc = mrb_download(L, MRB_ARR_F, f.data()))) return rc;
+        if ((rc = mrb_download(L, MRB_ARR_G, g.data()))) return rc;
+    }
+    std::vector<int32_t> isf(size_t(2) * size_t(T));
+    if ((rc = mrb_download(L, MRB_ARR_ISFLUID, isf.data()))) return rc;
+    double *o = out;
+    for (int b = 0; b < L->nbox; ++b) {
+        const BoxDev &B = L->boxes[size_t(b)];
+        auto emit = [&](const double *src, int64_t base_comps, int comp) {
+            const double *p = src + B.base * base_comps + int64_t(comp) * B.ncell;
+            for (int k = B.vlo[2]; k <= B.vhi[2]; ++k)
+                for (int j = B.vlo[1]; j <= B.vhi[1]; ++j)
+                    for (int i = B.vlo[0]; i <= B.vhi[0]; ++i)
+                        *o++ = p[(int64_t(k - B.lo[2]) * B.n[1] + (j - B.lo[1])) * B.n[0] + (i - B.lo[0])];
+        };
+        for (int c = 0; c < 19; ++c) emit(md.data(), 19, c);
+        if (streaming) {
+            for (int q = 0; q < NQ; ++q) emit(f.data(), NQ, q);
+            for (int q = 0; q < NQ; ++q) emit(g.data(), NQ, q);
+        }
+        if (derived)
+            for (int c = 0; c < 7; ++c) emit(dv.data(), 7, c);
+        for (int c = 0; c < 2; ++c) {
+            const int32_t *p = isf.data() + B.base * 2 + int64_t(c) * B.ncell;
+            for (int k = B.vlo[2]; k <= B.vhi[2]; ++k)
+                for (int j = B.vlo[1]; j <= B.vhi[1]; ++j)
+                    for (int i = B.vlo[0]; i <= B.vhi[0]; ++i)
+                        *o++ = double(p[(int64_t(k - B.lo[2]) * B.n[1] + (j - B.lo[1])) * B.n[0] + (i - B.lo[0])]);
+        }
+    }
+    return MRB_OK;
+}
+
 int mrb_plot_fields(mrb_level *L, double *out, int flags)
 {
     if (!L || !out) return set_error(MRB_E_ARG, "null argument");
-    if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_UNSUPPORTED, "plot fields: compact layout only");
     if (!L->flags_ready || !L->pops_set) return set_error(MRB_E_STATE, "flags and populations must be set");
     const bool streaming = flags & MRB_PLOT_STREAMING, derived = flags & MRB_PLOT_DERIVED;
+    if (L->layout == MRB_LAYOUT_FAB) {
+        if (L->in_hierarchy)
+            return set_error(MRB_E_UNSUPPORTED, "plot fields: levels of a two-level hierarchy are not supported");
+        return fab_plot(L, out, streaming, derived);
+    }
     const int64_t N = L->total_cells;
     double *d = nullptr;
     MRB_HIP(hipMalloc(&d, sizeof(double) * 26 * size_t(N)));
@@ -440,7 +496,14 @@ int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, i
     const mrb_level *L = lvl;
     const std::vector<std::string> names = plot_names(flags & MRB_PLOT_STREAMING, flags & MRB_PLOT_DERIVED);
     const int nv = int(names.size());
-    const int64_t N = L->total_cells;
+    LevelBoxes B = level_boxes(L);
+    B.ng = 0;
+    std::vector<int64_t> nvalid;
+    int64_t N = 0;
+    for (const auto &v : B.valid) {
+        nvalid.push_back(int64_t(v[3] - v[0] + 1) * (v[4] - v[1] + 1) * (v[5] - v[2] + 1));
+        N += nvalid.back();
+    }
     std::vector<double> data(size_t(nv) * size_t(N));
     int rc = mrb_plot_fields(lvl, data.data(), flags);
     if (rc) return rc;
@@ -461,7 +524,6 @@ int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, i
     h << step << ' ' << '\n';
     for (int k = 0; k < 3; ++k) h << L->dx[k] << ' ';
     h << '\n' << 0 << '\n' << 0 << '\n';
-    const LevelBoxes B = level_boxes(L);
     h << 0 << ' ' << B.valid.size() << ' ' << time << '\n' << step << '\n';
     for (const auto &v : B.valid)
         for (int k = 0; k < 3; ++k)
@@ -469,7 +531,7 @@ int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, i
               << '\n';
     h << "Level_0/Cell" << '\n';
     if (!h) return set_error(MRB_E_ARG, std::string("write failed: ") + dir + "/Header");
-    return write_multifab(level_dir(dir, 0) + "/Cell", B, nv, data.data(), fab_cells(L));
+    return write_multifab(level_dir(dir, 0) + "/Cell", B, nv, data.data(), nvalid);
 }
 
 }  // extern "C"

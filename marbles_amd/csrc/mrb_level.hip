@@ -331,6 +331,65 @@ __global__ void k_dq_fab(
     }
 }
 
+// gradient() (Utilities.H:279-312) of macrodata component comp along d at
+// the valid fluid cell iv of box B, as k_dq_fab takes it: a neighbour counts
+// where it lies inside the domain box and is fluid
+__device__ double fab_gradient(const double *__restrict__ m, const int32_t *__restrict__ fl, const BoxDev &B,
+                               const int iv[3], int d, int comp, const int nd[3], double inv_dx)
+{
+    int p[3] = {iv[0], iv[1], iv[2]}, q[3] = {iv[0], iv[1], iv[2]};
+    p[d] += 1;
+    q[d] -= 1;
+    bool p_ok = true, m_ok = true;
+    for (int k = 0; k < 3; ++k) {
+        p_ok = p_ok && p[k] >= 0 && p[k] < nd[k];
+        m_ok = m_ok && q[k] >= 0 && q[k] < nd[k];
+    }
+    p_ok = p_ok && fl[loc_of(B, p[0], p[1], p[2])] == 1;
+    m_ok = m_ok && fl[loc_of(B, q[0], q[1], q[2])] == 1;
+    const double vp = p_ok ? m[comp * B.ncell + loc_of(B, p[0], p[1], p[2])] : 0.0;
+    const double vm = m_ok ? m[comp * B.ncell + loc_of(B, q[0], q[1], q[2])] : 0.0;
+    const double vc = m[comp * B.ncell + loc_of(B, iv[0], iv[1], iv[2])];
+    return one_sided_gradient(p_ok, m_ok, vp, vc, vm, inv_dx);
+}
+
+// LBM::compute_derived (LBM.cpp:909-955) and compute_q_corrections
+// (959-991) on the macrodata: vort_x, vort_y, vort_z, vort_mag, dQCorrX..Z
+// per valid cell (fab comps of der); 0 in solid cells
+__global__ void k_plot_derived_fab(
+    const double *__restrict__ md, const int32_t *__restrict__ isf, double *__restrict__ der,
+    const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox, int64_t total, int n0, int n1,
+    int n2, CollideParams P)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int iv[3];
+    local_to_ijk(B, loc, iv[0], iv[1], iv[2]);
+    if (!in_valid(B, iv[0], iv[1], iv[2])) return;
+    const int32_t *fl = isf + B.base * 2;
+    const double *m = md + B.base * 19;
+    double o[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (fl[loc] == 1) {
+        const int nd[3] = {n0, n1, n2};
+        const double vx = fab_gradient(m, fl, B, iv, 0, 2, nd, P.inv_dx[0]);
+        const double wx = fab_gradient(m, fl, B, iv, 0, 3, nd, P.inv_dx[0]);
+        const double uy = fab_gradient(m, fl, B, iv, 1, 1, nd, P.inv_dx[1]);
+        const double wy = fab_gradient(m, fl, B, iv, 1, 3, nd, P.inv_dx[1]);
+        const double uz = fab_gradient(m, fl, B, iv, 2, 1, nd, P.inv_dx[2]);
+        const double vz = fab_gradient(m, fl, B, iv, 2, 2, nd, P.inv_dx[2]);
+        o[0] = wy - vz;
+        o[1] = uz - wx;
+        o[2] = vx - uy;
+        o[3] = sqrt((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
+        for (int d = 0; d < 3; ++d) o[4 + d] = fab_gradient(m, fl, B, iv, d, 6 + d, nd, P.inv_dx[d]);
+    }
+    double *out = der + B.base * 7 + loc;
+    for (int k = 0; k < 7; ++k) out[k * B.ncell] = o[k];
+}
+
 // ---------------------------------------------------------------------------
 // macrodata_to_equilibrium + relax_f_to_equilibrium (LBM.cpp:621-807),
 // valid fluid cells, in place
@@ -744,6 +803,87 @@ int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s)
 }
 
 int fab_fill_boundary_q(mrb_level *L, hipStream_t s) { return fill_boundary(L, L->d_dq, 3, false, true, s); }
+
+// LBM::f_to_macrodata (LBM.cpp:810-906) of the post-stream state of the
+// last batched step at the valid cells: the pull of the buffer that held
+// the state before the step (its ghost cells filled then); 0 in solid cells
+__global__ void __launch_bounds__(256) k_plot_md_fab(
+    const double *__restrict__ f, const double *__restrict__ g, const uint32_t *__restrict__ mask,
+    double *__restrict__ md, const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox,
+    int64_t total, CollideParams P)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int i, j, k;
+    local_to_ijk(B, loc, i, j, k);
+    if (!in_valid(B, i, j, k)) return;
+    const uint32_t m = mask[B.base + loc];
+    double o[19];
+    for (int n = 0; n < 19; ++n) o[n] = 0.0;
+    if (m & 1u) {
+        double fv[NQ], gv[NQ];
+        pull_fab(f, g, B, loc, m, fv, gv);
+        Moments mo;
+        accumulate_moments(fv, gv, mo);
+        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        o[0] = mo.rho;
+        o[1] = M.u;
+        o[2] = M.v;
+        o[3] = M.w;
+        o[4] = sqrt(M.u * M.u + M.v * M.v + M.w * M.w);
+        o[5] = mo.E;
+        o[6] = M.Qx;
+        o[7] = M.Qy;
+        o[8] = M.Qz;
+        o[9] = mo.pxx;
+        o[10] = mo.pyy;
+        o[11] = mo.pzz;
+        o[12] = mo.pxy;
+        o[13] = mo.pxz;
+        o[14] = mo.pyz;
+        o[15] = mo.qx;
+        o[16] = mo.qy;
+        o[17] = mo.qz;
+        o[18] = M.T;
+    }
+    double *out = md + B.base * 19 + loc;
+    for (int n = 0; n < 19; ++n) out[n * B.ncell] = o[n];
+}
+
+int fab_plot_md_pull(mrb_level *L, hipStream_t s)
+{
+    const int prev = 1 - L->cur;
+    k_plot_md_fab<<<unsigned((L->total_cells + 255) / 256), 256, 0, s>>>(
+        (const double *)L->pop[0][prev], (const double *)L->pop[1][prev], L->d_mask, L->d_md, L->d_boxes,
+        L->d_box_base, L->nbox, L->total_cells, L->cp);
+    MRB_HIP(hipGetLastError());
+    return fill_boundary(L, L->d_md, 19, false, true, s);
+}
+
+int fab_plot_fields(mrb_level *L, double *der, hipStream_t s)
+{
+    const unsigned nb = nblocks(L->total_cells);
+    if (L->md_src < 0) return set_error(MRB_E_STATE, "plot fields: the stored state has no macrodata (stream without a collide)");
+    if (L->md_src == 0) {
+        // MakeNewLevelFromScratch: f_to_macrodata of the stored state
+        k_moments_fab<<<nb, kBlock, 0, s>>>((const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur],
+                                            L->d_isf, L->d_md, L->d_boxes, L->d_box_base, L->nbox, L->total_cells,
+                                            L->cp);
+        MRB_HIP(hipGetLastError());
+        const int rc = fill_boundary(L, L->d_md, 19, false, true, s);
+        if (rc) return rc;
+    } else if (L->md_src == 2) {
+        const int rc = fab_plot_md_pull(L, s);
+        if (rc) return rc;
+    }
+    k_plot_derived_fab<<<nb, kBlock, 0, s>>>(L->d_md, L->d_isf, der, L->d_boxes, L->d_box_base, L->nbox,
+                                             L->total_cells, L->ncell[0], L->ncell[1], L->ncell[2], L->cp);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
 
 int fab_fill_boundary_depth(mrb_level *L, int depth, hipStream_t s)
 {
@@ -1270,6 +1410,7 @@ int mrb_upload(mrb_level *L, int arr, const void *host)
     L->fresh_depth = 0;
     L->pops_set = true;
     L->prev_valid = false;
+    L->md_src = 0;
     const int64_t n = NQ * L->total_cells;
     const double *h = static_cast<const double *>(host);
     if (L->storage == MRB_STORAGE_F64) return copy_lattice(L, arr, const_cast<void *>(host), true);
@@ -1489,6 +1630,7 @@ int mrb_init_taylorgreen(mrb_level *L, double rho0, double v0, const double omeg
     if (L) {
         L->pops_set = true;
         L->prev_valid = false;
+        L->md_src = 0;
     }
     if (!L || !omega) return set_error(MRB_E_ARG, "null argument");
     const double Lw = 1.0 / M_PI;
@@ -1517,6 +1659,7 @@ int mrb_init_constant(
     if (L) {
         L->pops_set = true;
         L->prev_valid = false;
+        L->md_src = 0;
     }
     if (!L || !mach) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
@@ -1562,6 +1705,7 @@ int mrb_init_fields(
     if (L) {
         L->pops_set = true;
         L->prev_valid = false;
+        L->md_src = 0;
     }
     if (!L || !rho || !u || !v || !w || !T) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready) return set_error(MRB_E_STATE, "flags must be set before the initial state");
@@ -1673,6 +1817,7 @@ int mrb_stream(mrb_level *L, int lattice, void *stream)
     MRB_HIP(hipGetLastError());
     std::swap(L->pop[lattice][0], L->pop[lattice][1]);
     L->fresh_depth = 0;
+    if (L->md_src == 2) L->md_src = -1;
     ++L->gen;
     return fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, s);
 }
@@ -1702,6 +1847,7 @@ int mrb_collide(mrb_level *L, void *stream)
     rc = fill_boundary(L, g, NQ, false, false, s);
     if (rc) return rc;
     L->fresh_depth = L->ng;
+    L->md_src = 1;
     return MRB_OK;
 }
 

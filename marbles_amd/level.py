@@ -269,14 +269,21 @@ class Level:
         check(self.lib.mrb_vorticity(self.h, c_void_p(out.data_ptr()), stream))
 
     def plot_fields(self, streaming: bool = False, derived: bool = False) -> np.ndarray:
-        """The plotfile variables of LBM::plot_file_mf (compact layout) as a host
-        array (nvars, nz, ny, nx): macrodata of the last step's post-stream
+        """The plotfile variables of LBM::plot_file_mf as a host array (nvars, nz,
+        ny, nx) over the domain: macrodata of the last step's post-stream
         state, optionally f and g, optionally vorticity and dQCorr, then
         is_fluid and eb_boundary."""
         flags = (_lib.MRB_PLOT_STREAMING if streaming else 0) | (_lib.MRB_PLOT_DERIVED if derived else 0)
         nv = self.lib.mrb_plot_nvars(flags)
-        out = np.empty((nv,) + tuple(reversed(self.ncell)))
-        check(self.lib.mrb_plot_fields(self.h, out.ctypes.data_as(c_void_p), flags))
+        shapes = [tuple(h - l + 1 for l, h in zip(reversed(lo), reversed(hi))) for lo, hi in self.boxes]
+        flat = np.empty(nv * sum(int(np.prod(s)) for s in shapes))
+        check(self.lib.mrb_plot_fields(self.h, flat.ctypes.data_as(c_void_p), flags))
+        out = np.zeros((nv,) + tuple(reversed(self.ncell)))
+        off = 0
+        for (lo, hi), sh in zip(self.boxes, shapes):
+            n = int(np.prod(sh))
+            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = flat[off:off + nv * n].reshape((nv,) + sh)
+            off += nv * n
         return out
 
     def write_plotfile(self, path: str, time: float, step: int, streaming: bool = False,

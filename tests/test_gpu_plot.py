@@ -83,3 +83,66 @@ def test_plotfile_layout(oracle, engine, tmp_path):
     assert (ncomp, ngrow, boxes) == (nv, 0, [(0, 0, 0, 31, 15, 11)])
     np.testing.assert_array_equal(fabs[0], E.plot_fields(streaming=True, derived=True))
     E.close()
+
+
+def _fab_pair(oracle, engine, cfg):
+    from oracle.pyoracle import build_oracle_level
+
+    O = build_oracle_level(cfg)
+    E = engine.Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=O.boxes, ng=3,
+                     layout="fab")
+    E.set_phys(cfg["nu"], cfg.get("alpha"))
+    E.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+    E.flags_upload(O.flat("is_fluid"))
+    E.upload("f", O.flat("f"))
+    E.upload("g", O.flat("g"))
+    return O, E
+
+
+@pytest.mark.parametrize("path", ["stages", "batched"])
+def test_fab_plot_fields_match_oracle(oracle, engine, path):
+    """Multi-box fab level (boxes of 12): the initial state's macrodata, then
+    after 9 steps through the stage calls (the collide's own macrodata) or
+    the batched step (the pull of the state before the step), with vorticity
+    across box edges."""
+    cfg = configs.channel_sphere((36, 24, 24), radius=5.0, center=(13.0, 12.0, 12.0), max_box=12)
+    O, E = _fab_pair(oracle, engine, cfg)
+    assert len(O.boxes) > 1
+    _compare(E.plot_fields(streaming=True, derived=True), oracle.plot_vars(O, streaming=True, derived=True))
+    n = 9
+    if path == "stages":
+        for _ in range(n):
+            for lat in (0, 1):
+                E.fillpatch(lat)
+                E.stream(lat)
+            E.collide()
+    else:
+        E.advance(n)
+    O.evolve(n)
+    ref = oracle.plot_vars(O, streaming=True, derived=True)
+    assert np.abs(ref[19 + 54:19 + 54 + 4]).max() > 0
+    _compare(E.plot_fields(streaming=True, derived=True), ref)
+    E.close()
+
+
+def test_fab_plotfile_boxes(oracle, engine, tmp_path):
+    cfg = configs.channel_sphere((24, 16, 16), radius=3.0, center=(8.0, 8.0, 8.0), max_box=8)
+    O, E = _fab_pair(oracle, engine, cfg)
+    E.advance(3)
+    path = str(tmp_path / "plt00003")
+    E.write_plotfile(path, time=3.0, step=3, derived=True)
+    head = open(os.path.join(path, "Header")).read().split("\n")
+    nv = 19 + 7 + 2
+    rest = head[2 + nv:]
+    nbox = len(E.boxes)
+    assert rest[11].split() == ["0", str(nbox), "3"]
+    for b, (lo, hi) in enumerate(E.boxes):
+        rows = [[float(x) for x in r.split()] for r in rest[13 + 3 * b:16 + 3 * b]]
+        assert rows == [[float(lo[d]), float(hi[d] + 1)] for d in range(3)]
+    assert rest[13 + 3 * nbox] == "Level_0/Cell"
+    ncomp, ngrow, boxes, fabs = read_vismf(os.path.join(path, "Level_0", "Cell"))
+    assert (ncomp, ngrow) == (nv, 0) and boxes == [tuple(lo) + tuple(hi) for lo, hi in E.boxes]
+    P = E.plot_fields(derived=True)
+    for (lo, hi), fab in zip(E.boxes, fabs):
+        np.testing.assert_array_equal(fab, P[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+    E.close()

@@ -320,8 +320,7 @@ int mrb_checkpoint_boxes(const char *dir, int level, int *nbox, int *box_lo, int
 int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels);
 
 /* ---- plotfile ------------------------------------------------------------
- * LBM::write_plot_file (Source/LBM.cpp:1629-1682) for one level (compact,
- * or fab outside a two-level hierarchy):
+ * LBM::write_plot_file (Source/LBM.cpp:1629-1682) for one level:
  * the variables of LBM::plot_file_mf in its order -- the 19 macrodata
  * components (rho, vel_x, vel_y, vel_z, vel_mag, two_rho_e, QCorrX..Z,
  * pxx, pyy, pzz, pxy, pxz, pyz, qx, qy, qz, temperature) of the last step's
@@ -335,13 +334,20 @@ int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels);
 #define MRB_PLOT_STREAMING 1 /* lbm.save_streaming */
 #define MRB_PLOT_DERIVED 2   /* lbm.save_derived */
 int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, int flags);
+/* amrex::WriteMultiLevelPlotfile of LBM::write_plot_file for levels 0 ..
+ * nlevels-1 (refinement ratio 2; a two-level hierarchy linked by
+ * mrb_amr_link after mrb_amr_advance, or one level): time = the coarse
+ * level's, steps = each level's step count; <dir>/Level_<l>/Cell. */
+int mrb_write_plotfile_levels(const char *dir, mrb_level *const *levels, int nlevels, double time, const int *steps,
+                              int flags);
 /* The plotfile's variable count for these flags. */
 int mrb_plot_nvars(int flags);
 /* The plotfile's fields into host memory: per box, mrb_plot_nvars(flags)
  * variables over the box's valid cells (x fastest), boxes back to back.
  * On a fab level the macrodata are those of the last stage collide, of the
- * state before the last batched step pulled, or of the stored state before
- * any step.  Synchronous. */
+ * state before the last batched step pulled (with the coalesced fine values
+ * on the coarse level of a hierarchy, and the ring-1 ghosts of its fine
+ * level), or of the stored state before any step.  Synchronous. */
 int mrb_plot_fields(mrb_level *lvl, double *out, int flags);
 
 /* Density, velocity (x, y, z) and temperature of the current state per

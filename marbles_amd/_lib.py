@@ -87,6 +87,7 @@ SIGNATURES = {
     "mrb_checkpoint_boxes": (c_int, [c_char_p, c_int, POINTER(c_int), c_void_p, c_void_p, c_int]),
     "mrb_read_checkpoint": (c_int, [c_char_p, c_void_p, c_int]),
     "mrb_write_plotfile": (c_int, [c_char_p, c_void_p, c_double, c_int, c_int]),
+    "mrb_write_plotfile_levels": (c_int, [c_char_p, c_void_p, c_int, c_double, c_void_p, c_int]),
     "mrb_plot_nvars": (c_int, [c_int]),
     "mrb_stencil": (c_int, [c_void_p, c_void_p, c_void_p]),
     "mrb_plot_fields": (c_int, [c_void_p, c_void_p, c_int]),

@@ -51,6 +51,17 @@ class Hierarchy:
     def coalesce(self, stream=None) -> None:
         check(_lib.load().mrb_coalesce(self.fine.h, self.crse.h, stream))
 
+    def write_plotfile(self, path: str, time: float, steps, streaming: bool = False, derived: bool = False) -> None:
+        """LBM::write_plot_file of both levels (AMReX multi-level plotfile);
+        steps = (coarse steps, fine steps)."""
+        import ctypes
+        import os
+
+        flags = (_lib.MRB_PLOT_STREAMING if streaming else 0) | (_lib.MRB_PLOT_DERIVED if derived else 0)
+        lv = (ctypes.c_void_p * 2)(self.crse.h.value, self.fine.h.value)
+        st = (ctypes.c_int * 2)(*[int(x) for x in steps])
+        check(_lib.load().mrb_write_plotfile_levels(os.fsencode(path), lv, 2, float(time), st, flags))
+
     def advance(self, nsteps: int, stream=None, graph: bool = False) -> None:
         """nsteps coarse steps; graph=True replays a captured hipGraph per step."""
         fn = _lib.load().mrb_amr_advance_graph if graph else _lib.load().mrb_amr_advance

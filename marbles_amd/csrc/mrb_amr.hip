@@ -201,6 +201,7 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
     int rc = check_pair(crse, fine);
     if (rc) return rc;
     crse->in_hierarchy = fine->in_hierarchy = true;
+    crse->amr_fine = fine;
     const std::vector<int32_t> cown = owner_map(crse);
     for (int32_t o : cown)
         if (o < 0) return set_error(MRB_E_ARG, "the coarse level must cover its domain");

@@ -376,6 +376,112 @@ int fab_build_mask(mrb_level *L, hipStream_t s)
     return MRB_OK;
 }
 
+// LBM::f_to_macrodata (LBM.cpp:810-906) for the plotfile: the 19
+// macrodata components of a cell's post-stream state (as pass 1 gathers
+// it) into md; 0 for a solid cell
+__device__ __forceinline__ void plot_md_store(const double fv[NQ], const double gv[NQ], bool fluid, double *out,
+                                              int64_t s, const CollideParams &P)
+{
+    double o[19];
+    for (int n = 0; n < 19; ++n) o[n] = 0.0;
+    if (fluid) {
+        Moments mo;
+        accumulate_moments(fv, gv, mo);
+        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        o[0] = mo.rho;
+        o[1] = M.u;
+        o[2] = M.v;
+        o[3] = M.w;
+        o[4] = sqrt(M.u * M.u + M.v * M.v + M.w * M.w);
+        o[5] = mo.E;
+        o[6] = M.Qx;
+        o[7] = M.Qy;
+        o[8] = M.Qz;
+        o[9] = mo.pxx;
+        o[10] = mo.pyy;
+        o[11] = mo.pzz;
+        o[12] = mo.pxy;
+        o[13] = mo.pxz;
+        o[14] = mo.pyz;
+        o[15] = mo.qx;
+        o[16] = mo.qy;
+        o[17] = mo.qz;
+        o[18] = M.T;
+    }
+    for (int n = 0; n < 19; ++n) out[n * s] = o[n];
+}
+
+// the valid cells: the gather of pass 1 from the buffer before the step
+template <bool COAL>
+__global__ void __launch_bounds__(kFabTile) k_fab_plot_md(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ md,
+    const FabTile *__restrict__ tiles, FabArgs A, CoalArgs C, CollideParams P)
+{
+    int step, end;
+    for (int t = xcd_tile_begin(A.ntiles, step, end); t < end; t += step) {
+        const FabTile tile = tiles[t];
+        const BoxDev &B = A.boxes[tile.box];
+        int i, j, k;
+        int64_t loc;
+        if (!tile_cell(A, tile, int(threadIdx.x), B, i, j, k, loc)) continue;
+        const uint32_t m = A.mask[B.base + loc];
+        double fv[NQ], gv[NQ];
+        if (m & 1u) gather_cell<COAL>(f, g, B, loc, m, C, fv, gv);
+        plot_md_store(fv, gv, (m & 1u) != 0, md + B.base * 19 + loc, B.ncell, P);
+    }
+}
+
+// the listed ring-1 ghost cells of a fine level (their pass 1)
+__global__ void __launch_bounds__(256) k_fab_plot_md_listed(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ md,
+    const BoxDev *__restrict__ boxes, const uint32_t *__restrict__ mask, const ExplodeEntry *__restrict__ list,
+    int64_t n, CollideParams P)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const ExplodeEntry e = list[t];
+    const BoxDev &B = boxes[e.fine_box];
+    const int64_t loc = e.fine_local;
+    const uint32_t m = mask[B.base + loc];
+    double fv[NQ], gv[NQ];
+    if (m & 1u) pull_fab(f, g, B, loc, m, fv, gv);
+    plot_md_store(fv, gv, (m & 1u) != 0, md + B.base * 19 + loc, B.ncell, P);
+}
+
+int fab_plot_md_pull(mrb_level *L, const mrb_level *coal_from, hipStream_t s)
+{
+    const int prev = 1 - L->cur;
+    const double *f = (const double *)L->pop[0][prev];
+    const double *g = (const double *)L->pop[1][prev];
+    FabArgs A{};
+    A.boxes = L->d_boxes;
+    A.mask = L->d_mask;
+    for (int d = 0; d < 3; ++d) A.dom[d] = L->ncell[d];
+    A.grow = 0;
+    A.ntiles = L->n_tiles_c;
+    CoalArgs C{};
+    if (coal_from) {
+        // the values the last coarse pass gathered (the fine level has not
+        // stepped since)
+        C.vals = coal_from->d_coal_vals;
+        C.has = coal_from->d_coal_has;
+        C.cell = coal_from->d_coal_cell;
+        C.n = coal_from->n_coal;
+    }
+    const unsigned gc = grid_for(A.ntiles, L->fab_blocks);
+    if (coal_from && C.n > 0)
+        k_fab_plot_md<true><<<gc, kFabTile, 0, s>>>(f, g, L->d_md, L->d_tiles_c, A, C, L->cp);
+    else
+        k_fab_plot_md<false><<<gc, kFabTile, 0, s>>>(f, g, L->d_md, L->d_tiles_c, A, C, L->cp);
+    MRB_HIP(hipGetLastError());
+    if (L->n_qring > 0) {
+        k_fab_plot_md_listed<<<unsigned((L->n_qring + 255) / 256), 256, 0, s>>>(
+            f, g, L->d_md, L->d_boxes, L->d_mask, (const ExplodeEntry *)L->d_qring, L->n_qring, L->cp);
+        MRB_HIP(hipGetLastError());
+    }
+    return MRB_OK;
+}
+
 const int kFabBlockChoices[] = {0, 1024, 2048, 4096, 8192};
 const int kNumFabBlockChoices = int(sizeof(kFabBlockChoices) / sizeof(kFabBlockChoices[0]));
 

@@ -178,7 +178,11 @@ int compact_plot_fields(mrb_level *L, double *md, double *der, hipStream_t s);
 // fab layout: the plotfile macrodata into L->d_md (valid cells, one ghost
 // layer filled) and vorticity (4) and dQCorr (3) into der (7 fab comps)
 int fab_plot_fields(mrb_level *L, double *der, hipStream_t s);
-int fab_plot_md_pull(mrb_level *L, hipStream_t s);
+// the plotfile macrodata of the last batched step (pull of the pre-step
+// buffer, gathering coal_from's coalesced values on a coarse level, and at
+// the listed ring-1 ghosts of a fine level) into L->d_md, valid cells and
+// ring; the caller fills the covered ghost layer
+int fab_plot_md_pull(mrb_level *L, const mrb_level *coal_from, hipStream_t s);
 int compact_tag(const mrb_level *L, const double *field, int criterion, double value, uint8_t *tags,
                 hipStream_t s);
 
@@ -232,6 +236,7 @@ struct mrb_level {
     // which holds the state before the last batched step; -1: none
     int md_src = 0;
     bool in_hierarchy = false;  // linked by mrb_amr_link
+    mrb_level *amr_fine = nullptr;  // on the coarse level of a hierarchy
     double fshift[mrb::NQ] = {}, gshift[mrb::NQ] = {};
     // FP16 storage (compact layout): power-of-two scale exponent and the
     // largest |population - shift| over fluid cells per (scale box, lattice),

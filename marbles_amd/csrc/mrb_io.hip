@@ -453,11 +453,7 @@ int mrb_plot_fields(mrb_level *L, double *out, int flags)
     if (!L || !out) return set_error(MRB_E_ARG, "null argument");
     if (!L->flags_ready || !L->pops_set) return set_error(MRB_E_STATE, "flags and populations must be set");
     const bool streaming = flags & MRB_PLOT_STREAMING, derived = flags & MRB_PLOT_DERIVED;
-    if (L->layout == MRB_LAYOUT_FAB) {
-        if (L->in_hierarchy)
-            return set_error(MRB_E_UNSUPPORTED, "plot fields: levels of a two-level hierarchy are not supported");
-        return fab_plot(L, out, streaming, derived);
-    }
+    if (L->layout == MRB_LAYOUT_FAB) return fab_plot(L, out, streaming, derived);
     const int64_t N = L->total_cells;
     double *d = nullptr;
     MRB_HIP(hipMalloc(&d, sizeof(double) * 26 * size_t(N)));
@@ -490,48 +486,81 @@ int mrb_plot_nvars(int flags)
     return int(plot_names(flags & MRB_PLOT_STREAMING, flags & MRB_PLOT_DERIVED).size());
 }
 
-int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, int flags)
+int mrb_write_plotfile_levels(const char *dir, mrb_level *const *levels, int nlevels, double time, const int *steps,
+                              int flags)
 {
-    if (!dir || !lvl) return set_error(MRB_E_ARG, "null argument");
-    const mrb_level *L = lvl;
+    if (!dir || !levels || nlevels < 1 || !steps) return set_error(MRB_E_ARG, "null argument");
+    for (int l = 0; l < nlevels; ++l) {
+        if (!levels[l]) return set_error(MRB_E_ARG, "null level");
+        if (l > 0)
+            for (int d = 0; d < 3; ++d)
+                if (levels[l]->ncell[d] != 2 * levels[l - 1]->ncell[d])
+                    return set_error(MRB_E_ARG, "plotfile levels must refine by 2");
+    }
     const std::vector<std::string> names = plot_names(flags & MRB_PLOT_STREAMING, flags & MRB_PLOT_DERIVED);
     const int nv = int(names.size());
-    LevelBoxes B = level_boxes(L);
-    B.ng = 0;
-    std::vector<int64_t> nvalid;
-    int64_t N = 0;
-    for (const auto &v : B.valid) {
-        nvalid.push_back(int64_t(v[3] - v[0] + 1) * (v[4] - v[1] + 1) * (v[5] - v[2] + 1));
-        N += nvalid.back();
-    }
-    std::vector<double> data(size_t(nv) * size_t(N));
-    int rc = mrb_plot_fields(lvl, data.data(), flags);
-    if (rc) return rc;
-    if ((rc = make_dirs(level_dir(dir, 0)))) return rc;
     // amrex::WriteMultiLevelPlotfile -> WriteGenericPlotfileHeader
+    const mrb_level *L0 = levels[0];
+    int rc = make_dirs(dir);
+    if (rc) return rc;
     std::ofstream h(std::string(dir) + "/Header", std::ios::trunc);
     if (!h) return set_error(MRB_E_ARG, std::string("cannot write ") + dir + "/Header");
     h.precision(17);
     h << "HyperCLaw-V1.1" << '\n' << nv << '\n';
     for (const auto &n : names) h << n << '\n';
-    h << 3 << '\n' << time << '\n' << 0 << '\n';
-    for (int k = 0; k < 3; ++k) h << L->prob_lo[k] << ' ';
+    h << 3 << '\n' << time << '\n' << nlevels - 1 << '\n';
+    for (int k = 0; k < 3; ++k) h << L0->prob_lo[k] << ' ';
     h << '\n';
-    for (int k = 0; k < 3; ++k) h << L->prob_hi[k] << ' ';
-    h << '\n' << '\n';  // no refinement ratios with one level
-    const int dlo[3] = {0, 0, 0}, dhi[3] = {L->ncell[0] - 1, L->ncell[1] - 1, L->ncell[2] - 1};
-    h << box_text(dlo, dhi) << ' ' << '\n';
-    h << step << ' ' << '\n';
-    for (int k = 0; k < 3; ++k) h << L->dx[k] << ' ';
-    h << '\n' << 0 << '\n' << 0 << '\n';
-    h << 0 << ' ' << B.valid.size() << ' ' << time << '\n' << step << '\n';
-    for (const auto &v : B.valid)
-        for (int k = 0; k < 3; ++k)
-            h << L->prob_lo[k] + v[size_t(k)] * L->dx[k] << ' ' << L->prob_lo[k] + (v[size_t(3 + k)] + 1) * L->dx[k]
-              << '\n';
-    h << "Level_0/Cell" << '\n';
+    for (int k = 0; k < 3; ++k) h << L0->prob_hi[k] << ' ';
+    h << '\n';
+    for (int l = 0; l + 1 < nlevels; ++l) h << 2 << ' ';
+    h << '\n';
+    for (int l = 0; l < nlevels; ++l) {
+        const int dlo[3] = {0, 0, 0};
+        const int dhi[3] = {levels[l]->ncell[0] - 1, levels[l]->ncell[1] - 1, levels[l]->ncell[2] - 1};
+        h << box_text(dlo, dhi) << ' ';
+    }
+    h << '\n';
+    for (int l = 0; l < nlevels; ++l) h << steps[l] << ' ';
+    h << '\n';
+    for (int l = 0; l < nlevels; ++l) {
+        for (int k = 0; k < 3; ++k) h << levels[l]->dx[k] << ' ';
+        h << '\n';
+    }
+    h << 0 << '\n' << 0 << '\n';
+    std::vector<LevelBoxes> LB(static_cast<size_t>(nlevels));
+    for (int l = 0; l < nlevels; ++l) {
+        const mrb_level *L = levels[l];
+        LB[size_t(l)] = level_boxes(L);
+        LB[size_t(l)].ng = 0;
+        h << l << ' ' << LB[size_t(l)].valid.size() << ' ' << time << '\n' << steps[l] << '\n';
+        for (const auto &v : LB[size_t(l)].valid)
+            for (int k = 0; k < 3; ++k)
+                h << L->prob_lo[k] + v[size_t(k)] * L->dx[k] << ' '
+                  << L->prob_lo[k] + (v[size_t(3 + k)] + 1) * L->dx[k] << '\n';
+        h << "Level_" << l << "/Cell" << '\n';
+    }
     if (!h) return set_error(MRB_E_ARG, std::string("write failed: ") + dir + "/Header");
-    return write_multifab(level_dir(dir, 0) + "/Cell", B, nv, data.data(), nvalid);
+    h.close();
+    for (int l = 0; l < nlevels; ++l) {
+        std::vector<int64_t> nvalid;
+        int64_t N = 0;
+        for (const auto &v : LB[size_t(l)].valid) {
+            nvalid.push_back(int64_t(v[3] - v[0] + 1) * (v[4] - v[1] + 1) * (v[5] - v[2] + 1));
+            N += nvalid.back();
+        }
+        std::vector<double> data(size_t(nv) * size_t(N));
+        if ((rc = mrb_plot_fields(levels[l], data.data(), flags))) return rc;
+        if ((rc = make_dirs(level_dir(dir, l)))) return rc;
+        if ((rc = write_multifab(level_dir(dir, l) + "/Cell", LB[size_t(l)], nv, data.data(), nvalid))) return rc;
+    }
+    return MRB_OK;
+}
+
+int mrb_write_plotfile(const char *dir, mrb_level *lvl, double time, int step, int flags)
+{
+    mrb_level *const one[1] = {lvl};
+    return mrb_write_plotfile_levels(dir, one, 1, time, &step, flags);
 }
 
 }  // extern "C"

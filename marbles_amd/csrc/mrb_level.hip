@@ -804,65 +804,6 @@ int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s)
 
 int fab_fill_boundary_q(mrb_level *L, hipStream_t s) { return fill_boundary(L, L->d_dq, 3, false, true, s); }
 
-// LBM::f_to_macrodata (LBM.cpp:810-906) of the post-stream state of the
-// last batched step at the valid cells: the pull of the buffer that held
-// the state before the step (its ghost cells filled then); 0 in solid cells
-__global__ void __launch_bounds__(256) k_plot_md_fab(
-    const double *__restrict__ f, const double *__restrict__ g, const uint32_t *__restrict__ mask,
-    double *__restrict__ md, const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox,
-    int64_t total, CollideParams P)
-{
-    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= total) return;
-    const int b = find_box(base, nbox, c);
-    const BoxDev B = boxes[b];
-    const int64_t loc = c - B.base;
-    int i, j, k;
-    local_to_ijk(B, loc, i, j, k);
-    if (!in_valid(B, i, j, k)) return;
-    const uint32_t m = mask[B.base + loc];
-    double o[19];
-    for (int n = 0; n < 19; ++n) o[n] = 0.0;
-    if (m & 1u) {
-        double fv[NQ], gv[NQ];
-        pull_fab(f, g, B, loc, m, fv, gv);
-        Moments mo;
-        accumulate_moments(fv, gv, mo);
-        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
-        o[0] = mo.rho;
-        o[1] = M.u;
-        o[2] = M.v;
-        o[3] = M.w;
-        o[4] = sqrt(M.u * M.u + M.v * M.v + M.w * M.w);
-        o[5] = mo.E;
-        o[6] = M.Qx;
-        o[7] = M.Qy;
-        o[8] = M.Qz;
-        o[9] = mo.pxx;
-        o[10] = mo.pyy;
-        o[11] = mo.pzz;
-        o[12] = mo.pxy;
-        o[13] = mo.pxz;
-        o[14] = mo.pyz;
-        o[15] = mo.qx;
-        o[16] = mo.qy;
-        o[17] = mo.qz;
-        o[18] = M.T;
-    }
-    double *out = md + B.base * 19 + loc;
-    for (int n = 0; n < 19; ++n) out[n * B.ncell] = o[n];
-}
-
-int fab_plot_md_pull(mrb_level *L, hipStream_t s)
-{
-    const int prev = 1 - L->cur;
-    k_plot_md_fab<<<unsigned((L->total_cells + 255) / 256), 256, 0, s>>>(
-        (const double *)L->pop[0][prev], (const double *)L->pop[1][prev], L->d_mask, L->d_md, L->d_boxes,
-        L->d_box_base, L->nbox, L->total_cells, L->cp);
-    MRB_HIP(hipGetLastError());
-    return fill_boundary(L, L->d_md, 19, false, true, s);
-}
-
 int fab_plot_fields(mrb_level *L, double *der, hipStream_t s)
 {
     const unsigned nb = nblocks(L->total_cells);
@@ -876,8 +817,11 @@ int fab_plot_fields(mrb_level *L, double *der, hipStream_t s)
         const int rc = fill_boundary(L, L->d_md, 19, false, true, s);
         if (rc) return rc;
     } else if (L->md_src == 2) {
-        const int rc = fab_plot_md_pull(L, s);
+        // the coarse level of a hierarchy gathered the coalesced fine values
+        const int rc = fab_plot_md_pull(L, L->amr_fine, s);
         if (rc) return rc;
+        const int rc2 = fill_boundary(L, L->d_md, 19, false, true, s);
+        if (rc2) return rc2;
     }
     k_plot_derived_fab<<<nb, kBlock, 0, s>>>(L->d_md, L->d_isf, der, L->d_boxes, L->d_box_base, L->nbox,
                                              L->total_cells, L->ncell[0], L->ncell[1], L->ncell[2], L->cp);

@@ -146,3 +146,50 @@ def test_fab_plotfile_boxes(oracle, engine, tmp_path):
     for (lo, hi), fab in zip(E.boxes, fabs):
         np.testing.assert_array_equal(fab, P[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
     E.close()
+
+
+@pytest.mark.parametrize("case", ["uneven_32", "inlet_pressure"])
+def test_two_level_plot_fields_and_plotfile(oracle, engine, tmp_path, case):
+    """Both levels of a hierarchy against the oracle's: the initial state,
+    then after 4 coarse steps (coarse macrodata with the coalesced fine
+    values of its last pass, fine macrodata with the ring-1 ghosts over
+    uncovered domain for the gradients at the coarse/fine edge); and the
+    two-level plotfile (Header, Level_0/Cell, Level_1/Cell)."""
+    from amr_cases import oracle_hierarchy
+    from test_gpu_amr import CASES
+
+    cfg = CASES[case]()
+    C, F = oracle_hierarchy(cfg)
+    H = engine.Hierarchy(cfg)
+    for lvl, O in ((H.crse, C), (H.fine, F)):
+        _compare(lvl.plot_fields(derived=True), oracle.plot_vars(O, derived=True))
+    n = 4
+    oracle.amr_evolve(C, F, n)
+    H.advance(n)
+    for lvl, O in ((H.crse, C), (H.fine, F)):
+        ref = oracle.plot_vars(O, streaming=True, derived=True)
+        _compare(lvl.plot_fields(streaming=True, derived=True), ref)
+    path = str(tmp_path / "plt00004")
+    H.write_plotfile(path, time=4.0, steps=(4, 8), derived=True)
+    head = open(os.path.join(path, "Header")).read().split("\n")
+    nv = 19 + 7 + 2
+    rest = head[2 + nv:]
+    nc, nf = cfg["ncell"], tuple(2 * x for x in cfg["ncell"])
+    assert rest[2] == "1" and rest[5].split() == ["2"]
+    assert rest[6].strip() == (f"((0,0,0) ({nc[0] - 1},{nc[1] - 1},{nc[2] - 1}) (0,0,0)) "
+                               f"((0,0,0) ({nf[0] - 1},{nf[1] - 1},{nf[2] - 1}) (0,0,0))")
+    assert rest[7].split() == ["4", "8"]
+    assert [float(x) for x in rest[8].split()] == [1.0, 1.0, 1.0]
+    assert [float(x) for x in rest[9].split()] == [0.5, 0.5, 0.5]
+    assert rest[10] == "0" and rest[11] == "0"
+    nb0, nb1 = len(H.crse.boxes), len(H.fine.boxes)
+    assert rest[12].split() == ["0", str(nb0), "4"] and rest[13] == "4"
+    j = 14 + 3 * nb0
+    assert rest[j] == "Level_0/Cell" and rest[j + 1].split() == ["1", str(nb1), "4"] and rest[j + 2] == "8"
+    assert rest[j + 3 + 3 * nb1] == "Level_1/Cell"
+    P = H.fine.plot_fields(derived=True)
+    ncomp, ngrow, boxes, fabs = read_vismf(os.path.join(path, "Level_1", "Cell"))
+    assert (ncomp, ngrow, len(fabs)) == (nv, 0, nb1)
+    for (lo, hi), fab in zip(H.fine.boxes, fabs):
+        np.testing.assert_array_equal(fab, P[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+    H.close()

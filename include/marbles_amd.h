@@ -245,6 +245,25 @@ int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *strea
  * stream f, g; coalesce; coarse collide.  The fine level's dt must be half
  * the coarse one (compute_dt, LBM.cpp:1047-1077).  Asynchronous. */
 int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream);
+/* Regrid (AmrCore::regrid -> LBM::RemakeLevel, Source/LBM.cpp:1302-1365):
+ * mrb_cluster_tags turns refinement tags on the coarse domain (one byte per
+ * coarse cell, x fastest; e.g. from mrb_tag_cells) into fine boxes (ratio
+ * 2): the tags grown by n_error_buf cells, coarsened to blocks of
+ * blocking_factor fine cells, clustered by Berger-Rigoutsos to grid_eff,
+ * chopped to max_grid_size (amr.n_error_buf, amr.blocking_factor,
+ * amr.max_grid_size, amr.grid_eff).  AMReX's own clustering is not part of
+ * the reference, so the boxes follow the published method, not AMReX's
+ * tie-breaking.  *nbox, and with non-null arrays (3 ints per box) the
+ * boxes.  Host only. */
+int mrb_cluster_tags(const uint8_t *tags, const int ncell[3], int n_error_buf, int blocking_factor,
+                     int max_grid_size, double grid_eff, int *nbox, int *box_lo, int *box_hi, int max_boxes);
+/* The new fine level's populations (FillPatchTwoLevels in RemakeLevel,
+ * FillPatchOps.H:118-129): each valid cell from the old fine level where
+ * the old level has it, else the coarse interpolation of the explosion;
+ * 0 in solid cells (fill_f_inside_eb).  new_fine: created with the new
+ * boxes, flags set; afterwards mrb_amr_link(crse, new_fine) and the old
+ * level can be destroyed.  Synchronous. */
+int mrb_amr_regrid_fill(mrb_level *crse, const mrb_level *old_fine, mrb_level *new_fine);
 /* Same as mrb_amr_advance, with each coarse step replayed from a hipGraph
  * captured on first use (one graph per coarse buffer parity; recaptured
  * after mrb_set_phys, mrb_set_bc, a stage-level mrb_stream or a relink).

@@ -39,6 +39,32 @@ class Hierarchy:
     def levels(self):
         return (self.crse, self.fine)
 
+    def regrid(self, tags=None, boxes=None, n_error_buf: int = 1, blocking_factor: int = 8,
+               max_grid_size: int = 32, grid_eff: float = 0.7) -> list:
+        """AmrCore::regrid -> LBM::RemakeLevel for the fine level: new fine
+        boxes from refinement tags on the coarse domain (uint8 array
+        (nz, ny, nx); cluster_tags) or given directly, the new level filled
+        from the old one and the coarse interpolation, then linked; returns
+        the new boxes."""
+        from . import configs
+
+        if boxes is None:
+            boxes = cluster_tags(tags, n_error_buf, blocking_factor, max_grid_size, grid_eff)
+        if not boxes:
+            raise ValueError("regrid: no fine boxes")
+        cfg, old = self.cfg, self.fine
+        new = Level(old.ncell, cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=boxes, ng=old.ng,
+                    layout="fab")
+        new.set_phys(cfg["nu"], cfg.get("alpha"), dt=0.5)
+        new.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+        new.flags_all_regular()
+        lib = _lib.load()
+        check(lib.mrb_amr_regrid_fill(self.crse.h, old.h, new.h))
+        check(lib.mrb_amr_link(self.crse.h, new.h))
+        old.close()
+        self.fine = new
+        return new.boxes
+
     def updates_per_step(self) -> int:
         """Lattice updates per coarse step: N_coarse + 2 N_fine."""
         def cells(lvl):
@@ -71,3 +97,22 @@ class Hierarchy:
         # the fine level holds maps into the coarse one: release it first
         self.fine.close()
         self.crse.close()
+
+
+def cluster_tags(tags, n_error_buf: int = 1, blocking_factor: int = 8, max_grid_size: int = 32,
+                 grid_eff: float = 0.7) -> list:
+    """Fine boxes (ratio 2; (lo, hi) cell triples) covering the tagged cells
+    of a coarse-domain uint8 array (nz, ny, nx) (mrb_cluster_tags)."""
+    import ctypes
+
+    import numpy as np
+
+    t = np.ascontiguousarray(tags, dtype=np.uint8)
+    ncell = (ctypes.c_int * 3)(t.shape[2], t.shape[1], t.shape[0])
+    lib = _lib.load()
+    nb = ctypes.c_int()
+    args = (int(n_error_buf), int(blocking_factor), int(max_grid_size), float(grid_eff))
+    check(lib.mrb_cluster_tags(t.ctypes.data_as(ctypes.c_void_p), ncell, *args, ctypes.byref(nb), None, None, 0))
+    lo, hi = (ctypes.c_int * (3 * nb.value))(), (ctypes.c_int * (3 * nb.value))()
+    check(lib.mrb_cluster_tags(t.ctypes.data_as(ctypes.c_void_p), ncell, *args, ctypes.byref(nb), lo, hi, nb.value))
+    return [(tuple(lo[3 * b:3 * b + 3]), tuple(hi[3 * b:3 * b + 3])) for b in range(nb.value)]

@@ -9,6 +9,9 @@
 #include <cmath>
 #include <vector>
 
+#include <array>
+#include <tuple>
+
 #include "mrb_internal.h"
 
 namespace mrb {
@@ -94,6 +97,46 @@ __global__ void __launch_bounds__(kAmrBlock) k_explode(
         }
         fo[int64_t(q) * FB.ncell] = ((c + s[0] * o[0]) + s[1] * o[1]) + s[2] * o[2];
     });
+}
+
+// regrid: new fine cell <- the old fine level's cell (NQ components)
+struct LevelCopy {
+    int64_t dst_local, src_local;
+    int32_t dst_box, src_box;
+};
+
+__global__ void __launch_bounds__(kAmrBlock) k_copy_levels(
+    const double *__restrict__ src, double *__restrict__ dst, const BoxDev *__restrict__ sboxes,
+    const BoxDev *__restrict__ dboxes, const LevelCopy *__restrict__ map, int64_t n)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const LevelCopy m = map[t];
+    const BoxDev &S = sboxes[m.src_box];
+    const BoxDev &D = dboxes[m.dst_box];
+    const double *sp = src + S.base * NQ + m.src_local;
+    double *dp = dst + D.base * NQ + m.dst_local;
+    double v[NQ];
+    static_for<NQ>([&](auto qc) { v[decltype(qc)::value] = sp[int64_t(decltype(qc)::value) * S.ncell]; });
+    static_for<NQ>([&](auto qc) { dp[int64_t(decltype(qc)::value) * D.ncell] = v[decltype(qc)::value]; });
+}
+
+// fill_f_inside_eb (LBM.cpp:1278-1298): 0 in every solid fab cell
+__global__ void __launch_bounds__(kAmrBlock) k_zero_solid(
+    double *__restrict__ f, const int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes,
+    const int64_t *__restrict__ base, int nbox, int64_t total)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (base[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    const BoxDev &B = boxes[lo];
+    const int64_t loc = c - B.base;
+    if (isf[B.base * 2 + loc] != 0) return;
+    for (int q = 0; q < NQ; ++q) f[B.base * NQ + int64_t(q) * B.ncell + loc] = 0.0;
 }
 
 int64_t local_of(const BoxDev &B, int i, int j, int k)
@@ -454,6 +497,230 @@ static int amr_step(mrb_level *crse, mrb_level *fine, void *stream)
     // advance(0): stream, average_down_to, collide, with the coalescence
     // fused into the coarse passes
     return fab_batched_step(crse, fine, 1, s);
+}
+
+int mrb_amr_regrid_fill(mrb_level *crse, const mrb_level *old_fine, mrb_level *fine)
+{
+    int rc = check_pair(crse, fine);
+    if (rc) return rc;
+    if (!old_fine) return set_error(MRB_E_ARG, "null level");
+    for (int d = 0; d < 3; ++d)
+        if (old_fine->ncell[d] != fine->ncell[d]) return set_error(MRB_E_ARG, "old and new fine domains differ");
+    if (old_fine->layout != MRB_LAYOUT_FAB || old_fine->storage != MRB_STORAGE_F64)
+        return set_error(MRB_E_UNSUPPORTED, "regrid needs fab levels with FP64 storage");
+    if (!fine->flags_ready) return set_error(MRB_E_STATE, "the new fine level's flags are not set");
+    // the interpolation reads the coarse parents' neighbours
+    if (crse->fresh_depth < 1 && (rc = fab_fill_boundary_depth(crse, 1, nullptr))) return rc;
+    const std::vector<int32_t> cown = owner_map(crse);
+    const std::vector<int32_t> oown = owner_map(old_fine);
+    // FillPatchTwoLevels (FillPatchOps.H:118-129) over the new valid cells:
+    // the old fine level where it has the cell, else the interpolation from
+    // the coarse parent
+    std::vector<LevelCopy> copy;
+    std::vector<ExplodeEntry> interp;
+    for (int b = 0; b < fine->nbox; ++b) {
+        const BoxDev &FB = fine->boxes[size_t(b)];
+        for (int k = FB.vlo[2]; k <= FB.vhi[2]; ++k)
+            for (int j = FB.vlo[1]; j <= FB.vhi[1]; ++j)
+                for (int i = FB.vlo[0]; i <= FB.vhi[0]; ++i) {
+                    const int iv[3] = {i, j, k};
+                    const int ob = oown[size_t(dom_index(fine, iv))];
+                    if (ob >= 0) {
+                        LevelCopy m;
+                        m.dst_box = b;
+                        m.dst_local = local_of(FB, i, j, k);
+                        m.src_box = ob;
+                        m.src_local = local_of(old_fine->boxes[size_t(ob)], i, j, k);
+                        copy.push_back(m);
+                        continue;
+                    }
+                    const int P[3] = {i / 2, j / 2, k / 2};
+                    ExplodeEntry e{};
+                    e.fine_box = b;
+                    e.fine_local = local_of(FB, i, j, k);
+                    e.crse_box = cown[size_t(dom_index(crse, P))];
+                    e.crse_local = local_of(crse->boxes[size_t(e.crse_box)], P[0], P[1], P[2]);
+                    for (int d = 0; d < 3; ++d) {
+                        const bool hm = crse->periodic[d] || P[d] - 1 >= 0;
+                        const bool hp = crse->periodic[d] || P[d] + 1 < crse->ncell[d];
+                        e.bits |= (iv[d] & 1) << d;
+                        e.bits |= int(hm && hp) << (3 + d);
+                    }
+                    interp.push_back(e);
+                }
+    }
+    void *d_copy = nullptr, *d_interp = nullptr;
+    if (!copy.empty()) {
+        MRB_HIP(hipMalloc(&d_copy, sizeof(LevelCopy) * copy.size()));
+        MRB_HIP(hipMemcpy(d_copy, copy.data(), sizeof(LevelCopy) * copy.size(), hipMemcpyHostToDevice));
+    }
+    if (!interp.empty()) {
+        MRB_HIP(hipMalloc(&d_interp, sizeof(ExplodeEntry) * interp.size()));
+        MRB_HIP(hipMemcpy(d_interp, interp.data(), sizeof(ExplodeEntry) * interp.size(), hipMemcpyHostToDevice));
+    }
+    for (int l = 0; l < 2; ++l) {
+        double *dst = (double *)fine->pop[l][fine->cur];
+        if (!copy.empty())
+            k_copy_levels<<<blocks_for(int64_t(copy.size())), kAmrBlock>>>(
+                (const double *)old_fine->pop[l][old_fine->cur], dst, old_fine->d_boxes, fine->d_boxes,
+                (const LevelCopy *)d_copy, int64_t(copy.size()));
+        if (!interp.empty())
+            k_explode<<<blocks_for(int64_t(interp.size())), kAmrBlock>>>(
+                (const double *)crse->pop[l][crse->cur], dst, fine->d_boxes, crse->d_boxes,
+                (const ExplodeEntry *)d_interp, int64_t(interp.size()));
+        k_zero_solid<<<blocks_for(fine->total_cells), kAmrBlock>>>(dst, fine->d_isf, fine->d_boxes,
+                                                                     fine->d_box_base, fine->nbox, fine->total_cells);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d_copy);
+    (void)hipFree(d_interp);
+    if (e != hipSuccess) return hip_fail(e, "mrb_amr_regrid_fill");
+    fine->fresh_depth = 0;
+    fine->pops_set = true;
+    fine->prev_valid = false;
+    fine->md_src = 0;
+    ++fine->gen;
+    return MRB_OK;
+}
+
+// Berger-Rigoutsos clustering of tagged points (integer coordinates in a
+// block grid): a cluster whose bounding box is filled to at least grid_eff
+// is kept; otherwise it is cut at a plane without points (the one nearest
+// the middle of the longest such direction), else at the strongest sign
+// change of the second difference of the point counts per plane, else in
+// the middle of its longest side.
+static void br_cluster(std::vector<std::array<int, 3>> pts, double grid_eff, std::vector<std::array<int, 6>> &out)
+{
+    std::vector<std::vector<std::array<int, 3>>> stack;
+    stack.push_back(std::move(pts));
+    while (!stack.empty()) {
+        std::vector<std::array<int, 3>> P = std::move(stack.back());
+        stack.pop_back();
+        if (P.empty()) continue;
+        int lo[3] = {P[0][0], P[0][1], P[0][2]}, hi[3] = {lo[0], lo[1], lo[2]};
+        for (const auto &p : P)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], p[d]);
+                hi[d] = std::max(hi[d], p[d]);
+            }
+        const double vol = double(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+        if (double(P.size()) >= grid_eff * vol || (lo[0] == hi[0] && lo[1] == hi[1] && lo[2] == hi[2])) {
+            out.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
+            continue;
+        }
+        std::vector<int> sig[3];
+        for (int d = 0; d < 3; ++d) sig[d].assign(size_t(hi[d] - lo[d] + 1), 0);
+        for (const auto &p : P)
+            for (int d = 0; d < 3; ++d) ++sig[d][size_t(p[d] - lo[d])];
+        int dims[3] = {0, 1, 2};
+        std::stable_sort(dims, dims + 3, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+        int cut_d = -1, cut_at = 0;  // points with coordinate < cut_at go left
+        for (int di = 0; di < 3 && cut_d < 0; ++di) {
+            const int d = dims[di], n = int(sig[d].size());
+            int best = -1;
+            for (int i = 1; i + 1 < n; ++i)
+                if (sig[d][size_t(i)] == 0 && (best < 0 || std::abs(2 * i - (n - 1)) < std::abs(2 * best - (n - 1))))
+                    best = i;
+            if (best >= 0) {
+                cut_d = d;
+                cut_at = lo[d] + best;
+            }
+        }
+        if (cut_d < 0) {
+            int strength = 0;
+            for (int di = 0; di < 3; ++di) {
+                const int d = dims[di], n = int(sig[d].size());
+                for (int i = 1; i + 2 < n; ++i) {
+                    const int l0 = sig[d][size_t(i - 1)] - 2 * sig[d][size_t(i)] + sig[d][size_t(i + 1)];
+                    const int l1 = sig[d][size_t(i)] - 2 * sig[d][size_t(i + 1)] + sig[d][size_t(i + 2)];
+                    if ((l0 < 0 && l1 > 0) || (l0 > 0 && l1 < 0)) {
+                        const int st = std::abs(l1 - l0);
+                        if (st > strength) {
+                            strength = st;
+                            cut_d = d;
+                            cut_at = lo[d] + i + 1;
+                        }
+                    }
+                }
+            }
+        }
+        if (cut_d < 0) {
+            cut_d = dims[0];
+            cut_at = lo[cut_d] + (hi[cut_d] - lo[cut_d] + 1) / 2;
+        }
+        std::vector<std::array<int, 3>> L, R;
+        for (const auto &p : P) (p[cut_d] < cut_at ? L : R).push_back(p);
+        stack.push_back(std::move(R));
+        stack.push_back(std::move(L));
+    }
+}
+
+int mrb_cluster_tags(const uint8_t *tags, const int ncell[3], int n_error_buf, int blocking_factor,
+                     int max_grid_size, double grid_eff, int *nbox, int *box_lo, int *box_hi, int max_boxes)
+{
+    if (!tags || !ncell || !nbox) return set_error(MRB_E_ARG, "null argument");
+    if (blocking_factor < 2 || (blocking_factor & 1) || max_grid_size < blocking_factor ||
+        max_grid_size % blocking_factor || n_error_buf < 0 || !(grid_eff > 0.0 && grid_eff <= 1.0))
+        return set_error(MRB_E_ARG, "need an even blocking factor, max_grid_size a multiple of it, 0 < grid_eff <= 1");
+    const int bc = blocking_factor / 2;  // block edge in coarse cells
+    for (int d = 0; d < 3; ++d)
+        if (ncell[d] < 1 || ncell[d] % bc) return set_error(MRB_E_ARG, "blocking factor must divide the fine domain");
+    const int nb[3] = {ncell[0] / bc, ncell[1] / bc, ncell[2] / bc};
+    // tags grown by n_error_buf cells (TagBoxArray::buffer), then coarsened
+    // to blocks of blocking_factor fine cells
+    std::vector<uint8_t> blk(size_t(nb[0]) * nb[1] * nb[2], 0);
+    const int64_t n0 = ncell[0], n1 = ncell[1];
+    for (int k = 0; k < ncell[2]; ++k)
+        for (int j = 0; j < ncell[1]; ++j)
+            for (int i = 0; i < ncell[0]; ++i) {
+                if (!tags[size_t((k * n1 + j) * n0 + i)]) continue;
+                const int lo[3] = {std::max(0, i - n_error_buf) / bc, std::max(0, j - n_error_buf) / bc,
+                                   std::max(0, k - n_error_buf) / bc};
+                const int hi[3] = {std::min(ncell[0] - 1, i + n_error_buf) / bc,
+                                   std::min(ncell[1] - 1, j + n_error_buf) / bc,
+                                   std::min(ncell[2] - 1, k + n_error_buf) / bc};
+                for (int c = lo[2]; c <= hi[2]; ++c)
+                    for (int b = lo[1]; b <= hi[1]; ++b)
+                        for (int a = lo[0]; a <= hi[0]; ++a) blk[size_t((c * nb[1] + b) * nb[0] + a)] = 1;
+            }
+    std::vector<std::array<int, 3>> pts;
+    for (int c = 0; c < nb[2]; ++c)
+        for (int b = 0; b < nb[1]; ++b)
+            for (int a = 0; a < nb[0]; ++a)
+                if (blk[size_t((c * nb[1] + b) * nb[0] + a)]) pts.push_back({a, b, c});
+    std::vector<std::array<int, 6>> clusters;
+    br_cluster(std::move(pts), grid_eff, clusters);
+    // blocks -> fine cells (ratio 2), chopped to max_grid_size along each axis
+    std::vector<std::array<int, 6>> boxes;
+    const int step = max_grid_size / blocking_factor;  // blocks per piece
+    for (const auto &cb : clusters) {
+        for (int z = cb[2]; z <= cb[5]; z += step)
+            for (int y = cb[1]; y <= cb[4]; y += step)
+                for (int x = cb[0]; x <= cb[3]; x += step) {
+                    const int l[3] = {x, y, z};
+                    const int h[3] = {std::min(cb[3], x + step - 1), std::min(cb[4], y + step - 1),
+                                      std::min(cb[5], z + step - 1)};
+                    std::array<int, 6> fb;
+                    for (int d = 0; d < 3; ++d) {
+                        fb[size_t(d)] = l[d] * blocking_factor;
+                        fb[size_t(3 + d)] = (h[d] + 1) * blocking_factor - 1;
+                    }
+                    boxes.push_back(fb);
+                }
+    }
+    std::sort(boxes.begin(), boxes.end(), [](const std::array<int, 6> &a, const std::array<int, 6> &b) {
+        return std::make_tuple(a[2], a[1], a[0]) < std::make_tuple(b[2], b[1], b[0]);
+    });
+    *nbox = int(boxes.size());
+    if (!box_lo && !box_hi) return MRB_OK;
+    if (!box_lo || !box_hi || max_boxes < int(boxes.size())) return set_error(MRB_E_ARG, "box arrays too short");
+    for (size_t b = 0; b < boxes.size(); ++b)
+        for (int d = 0; d < 3; ++d) {
+            box_lo[3 * b + size_t(d)] = boxes[b][size_t(d)];
+            box_hi[3 * b + size_t(d)] = boxes[b][size_t(3 + d)];
+        }
+    return MRB_OK;
 }
 
 int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream)

@@ -1699,6 +1699,64 @@ void orc_amr_explode(const orc_level *crse, orc_level *fine, int lattice)
     orc_physbc(fine, lattice);
 }
 
+/* LBM::RemakeLevel (LBM.cpp:1302-1365) for the fine level: FillPatchTwoLevels
+ * (FillPatchOps.H:118-129) of the new level's valid cells -- the old fine
+ * level's value where it has the cell, else the coarse interpolation of
+ * orc_amr_explode -- then fill_f_inside_eb (0 in every solid fab cell).
+ * The new level's flags are set. */
+void orc_amr_regrid_fill(const orc_level *crse, const orc_level *old_fine, orc_level *fine)
+{
+    amr_check(crse, fine);
+    for (int lattice = 0; lattice < 2; lattice++) {
+        double **fa = lattice == 0 ? fine->f : fine->g;
+        double *const *oa = lattice == 0 ? old_fine->f : old_fine->g;
+        double *const *ca = lattice == 0 ? crse->f : crse->g;
+        for (int b = 0; b < fine->nbox; b++) {
+            const fabdim *fd = &fine->fdg[b];
+            for (int k = fine->vlo[b][2]; k <= fine->vhi[b][2]; k++)
+                for (int j = fine->vlo[b][1]; j <= fine->vhi[b][1]; j++)
+                    for (int i = fine->vlo[b][0]; i <= fine->vhi[b][0]; i++) {
+                        const int s[3] = {i, j, k};
+                        const int ob = owner_of(old_fine, s);
+                        if (ob >= 0) {
+                            const fabdim *od = &old_fine->fdg[ob];
+                            for (int q = 0; q < ORC_NQ; q++)
+                                fa[b][fidx(fd, q, i, j, k)] = oa[ob][fidx(od, q, i, j, k)];
+                            continue;
+                        }
+                        const int P[3] = {i / 2, j / 2, k / 2};
+                        const int cb = owner_of(crse, P);
+                        const fabdim *cd = &crse->fdg[cb];
+                        int has_m[3], has_p[3], hi_child[3];
+                        for (int d = 0; d < 3; d++) {
+                            has_m[d] = crse->periodic[d] || P[d] - 1 >= crse->dom_lo[d];
+                            has_p[d] = crse->periodic[d] || P[d] + 1 <= crse->dom_hi[d];
+                            hi_child[d] = s[d] & 1;
+                        }
+                        for (int q = 0; q < ORC_NQ; q++) {
+                            const double *cq = ca[cb];
+                            const double c = cq[fidx(cd, q, P[0], P[1], P[2])];
+                            double cm[3], cp[3];
+                            for (int d = 0; d < 3; d++) {
+                                int m[3] = {P[0], P[1], P[2]}, p[3] = {P[0], P[1], P[2]};
+                                m[d] -= 1;
+                                p[d] += 1;
+                                cm[d] = has_m[d] ? cq[fidx(cd, q, m[0], m[1], m[2])] : 0.0;
+                                cp[d] = has_p[d] ? cq[fidx(cd, q, p[0], p[1], p[2])] : 0.0;
+                            }
+                            fa[b][fidx(fd, q, i, j, k)] = orc_amr_interp(c, cm, cp, has_m, has_p, hi_child);
+                        }
+                    }
+            const int32_t *isf = fine->isf[b];
+            for (int k = fd->lo[2]; k <= fd->hi[2]; k++)
+                for (int j = fd->lo[1]; j <= fd->hi[1]; j++)
+                    for (int i = fd->lo[0]; i <= fd->hi[0]; i++)
+                        if (isf[fidx(fd, 0, i, j, k)] == 0)
+                            for (int q = 0; q < ORC_NQ; q++) fa[b][fidx(fd, q, i, j, k)] = 0.0;
+        }
+    }
+}
+
 /* LBM::evolve -> time_step(0) -> [fillpatch(1); 2 x (physbc(1); time_step(1)
  * -> advance(1))] -> advance(0) with average_down_to (LBM.cpp:416-419,
  * 498-512, 523-544). */

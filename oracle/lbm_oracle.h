@@ -148,6 +148,10 @@ void orc_amr_coalesce(const orc_level *fine, orc_level *crse, int lattice);
  * (own conservative, MC-limited linear scheme in place of AMReX
  * cell_cons_interp), ghost cells beyond non-periodic faces get physbc. */
 void orc_amr_explode(const orc_level *crse, orc_level *fine, int lattice);
+/* LBM::RemakeLevel's fill of a new fine level (valid cells from the old
+ * fine level where it has them, else the coarse interpolation; 0 in solid
+ * cells); the new level's flags must be set. */
+void orc_amr_regrid_fill(const orc_level *crse, const orc_level *old_fine, orc_level *fine);
 /* the interpolation formula alone for one fine child: coarse value c, its
  * -/+ neighbours per direction (has_m/has_p = 0 drops that direction's
  * slope), child offset sign per direction (0 = low child, 1 = high) */

@@ -91,6 +91,7 @@ def load() -> ctypes.CDLL:
         "orc_eb_force": (None, [c_void_p, D3]),
         "orc_vorticity": (None, [c_void_p, c_void_p]),
         "orc_plot_vars": (None, [c_void_p, c_int, c_int, c_void_p]),
+        "orc_amr_regrid_fill": (None, [c_void_p, c_void_p, c_void_p]),
         "orc_ic_fields": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_double, c_double]),
         "orc_amr_coalesce": (None, [c_void_p, c_void_p, c_int]),
@@ -324,6 +325,10 @@ def plot_vars(O: "OracleLevel", streaming: bool = False, derived: bool = False) 
         dom[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = out[off:off + nc * n].reshape((nc,) + sh)
         off += nc * n
     return dom
+
+
+def amr_regrid_fill(crse: "OracleLevel", old_fine: "OracleLevel", fine: "OracleLevel") -> None:
+    crse.lib.orc_amr_regrid_fill(crse.h, old_fine.h, fine.h)
 
 
 def amr_coalesce(fine: OracleLevel, crse: OracleLevel, lattice: int) -> None:

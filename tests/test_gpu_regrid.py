@@ -1,0 +1,43 @@
+"""Regrid of a two-level hierarchy (AmrCore::regrid -> LBM::RemakeLevel,
+Source/LBM.cpp:1302-1365) through the C ABI: tags from the coarse level's
+plot fields, boxes from the clustering, the new fine level filled from the
+old one and the coarse interpolation, bit for bit like the oracle's
+restatement of the fill, and the steps after the regrid still matching."""
+import numpy as np
+import pytest
+
+from amr_cases import oracle_hierarchy, small_amr, fine_ncell
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("seed", [4, 9])
+def test_regrid_fill_and_following_steps_match_oracle(oracle, engine, seed):
+    from marbles_amd.amr import cluster_tags
+
+    cfg = small_amr(n=32, nbox=80, seed=seed, fraction=0.2)
+    C, F = oracle_hierarchy(cfg)
+    H = engine.Hierarchy(cfg)
+    oracle.amr_evolve(C, F, 3)
+    H.advance(3)
+    speed = H.crse.plot_fields()[4]  # vel_mag of the coarse level
+    tags = (speed > np.percentile(speed, 85)).astype(np.uint8)
+    boxes = cluster_tags(tags, n_error_buf=1, blocking_factor=8, max_grid_size=16, grid_eff=0.7)
+    assert len(boxes) > 1 and sorted(boxes) != sorted(H.fine.boxes)
+    F2 = oracle.OracleLevel(fine_ncell(cfg), cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], boxes=boxes, ng=3)
+    F2.set_phys(cfg["nu"], cfg.get("alpha"), dt=0.5)
+    F2.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+    F2.flags_all_regular()
+    oracle.amr_regrid_fill(C, F, F2)
+    H.regrid(boxes=boxes)
+    for name in ("f", "g"):
+        assert np.array_equal(H.fine.valid(name), F2.valid(name)), name
+    # the new region holds interpolated values, the kept one the old level's
+    oracle.amr_evolve(C, F2, 3)
+    H.advance(3)
+    for lvl, O in ((H.crse, C), (H.fine, F2)):
+        for name in ("f", "g"):
+            a, b = lvl.valid(name), O.valid(name)
+            err = float(np.abs(a - b).max() / np.abs(b).max())
+            assert err <= 1e-12, (name, err)
+    H.close()

@@ -338,6 +338,15 @@ int mrb_checkpoint_boxes(const char *dir, int level, int *nbox, int *box_lo, int
  * cells are refilled before the next step.  Synchronous. */
 int mrb_read_checkpoint(const char *dir, mrb_level *const *levels, int nlevels);
 
+/* ---- forces file ---------------------------------------------------------
+ * LBM::open_forces_file / output_forces_file (Source/LBM.cpp:1925-1969)
+ * for the EB force of mrb_eb_force: open writes the header ("time", "fx",
+ * "fy", "fz" in 24-character columns) unless the file exists and
+ * initialize is 0 (a restart appends); append writes the time and the
+ * three components, width 24, 16 significant digits.  Host only. */
+int mrb_forces_file_open(const char *path, int initialize);
+int mrb_forces_file_append(const char *path, double time, const double force[3]);
+
 /* ---- plotfile ------------------------------------------------------------
  * LBM::write_plot_file (Source/LBM.cpp:1629-1682) for one level:
  * the variables of LBM::plot_file_mf in its order -- the 19 macrodata

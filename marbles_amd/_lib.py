@@ -89,6 +89,8 @@ SIGNATURES = {
     "mrb_write_plotfile": (c_int, [c_char_p, c_void_p, c_double, c_int, c_int]),
     "mrb_write_plotfile_levels": (c_int, [c_char_p, c_void_p, c_int, c_double, c_void_p, c_int]),
     "mrb_plot_nvars": (c_int, [c_int]),
+    "mrb_forces_file_open": (c_int, [c_char_p, c_int]),
+    "mrb_forces_file_append": (c_int, [c_char_p, c_double, c_void_p]),
     "mrb_stencil": (c_int, [c_void_p, c_void_p, c_void_p]),
     "mrb_cluster_tags": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, POINTER(c_int), c_void_p,
                                  c_void_p, c_int]),

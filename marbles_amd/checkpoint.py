@@ -1,7 +1,8 @@
-"""Checkpoint files of the reference solver (LBM::write_checkpoint_file and
-LBM::read_checkpoint_file, Source/LBM.cpp:1684-1860) for engine levels:
-AMReX's checkpoint Header plus the f and g MultiFabs in VisMF format,
-written and read by libmarbles_amd.so."""
+"""Files of the reference solver for engine levels, written and read by
+libmarbles_amd.so: checkpoints (LBM::write_checkpoint_file and
+LBM::read_checkpoint_file, Source/LBM.cpp:1684-1860: AMReX's checkpoint
+Header plus the f and g MultiFabs in VisMF format) and the forces file
+(LBM::open_forces_file / output_forces_file, LBM.cpp:1925-1969)."""
 from __future__ import annotations
 
 import ctypes
@@ -53,3 +54,15 @@ def read_checkpoint(path: str, levels: Sequence) -> None:
     flags already set; solid cells get 0 as fill_f_inside_eb gives them)."""
     lib = _lib.load()
     check(lib.mrb_read_checkpoint(os.fsencode(path), _levels_array(levels), len(levels)))
+
+
+def forces_file_open(path: str, initialize: bool = True) -> None:
+    """Creates the forces file with its header (or keeps an existing one when
+    initialize is False, as a restart does)."""
+    check(_lib.load().mrb_forces_file_open(os.fsencode(path), int(bool(initialize))))
+
+
+def forces_file_append(path: str, time: float, force) -> None:
+    """Appends one line: the time and the EB force components."""
+    f = (c_double * 3)(*[float(x) for x in force])
+    check(_lib.load().mrb_forces_file_append(os.fsencode(path), float(time), f))

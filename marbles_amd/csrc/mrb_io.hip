@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <limits>
 #include <sstream>
 #include <string>
@@ -479,6 +480,33 @@ int mrb_plot_fields(mrb_level *L, double *out, int flags)
     if ((rc = mrb_download(L, MRB_ARR_ISFLUID, isf.data()))) return rc;
     for (int64_t c = 0; c < 2 * N; ++c) o[c] = double(isf[size_t(c)]);
     return MRB_OK;
+}
+
+// LBM::open_forces_file / output_forces_file (LBM.cpp:1925-1969), columns
+// of width DATWIDTH = 24 and DATPRECISION = 16 significant digits
+// (Constants.H:63-64)
+int mrb_forces_file_open(const char *path, int initialize)
+{
+    if (!path) return set_error(MRB_E_ARG, "null argument");
+    std::ifstream probe(path);
+    if (probe.good() && !initialize) return MRB_OK;  // appended to
+    std::ofstream o(path, std::ios::out | std::ios::trunc);
+    if (!o) return set_error(MRB_E_ARG, std::string("cannot write ") + path);
+    o << std::setw(24) << "          time";
+    for (const char *c : {"          fx", "          fy", "          fz"}) o << std::setw(24) << c;
+    o << std::endl;
+    return o ? MRB_OK : set_error(MRB_E_ARG, std::string("write failed: ") + path);
+}
+
+int mrb_forces_file_append(const char *path, double time, const double force[3])
+{
+    if (!path || !force) return set_error(MRB_E_ARG, "null argument");
+    std::ofstream o(path, std::ios::app);
+    if (!o) return set_error(MRB_E_ARG, std::string("cannot append to ") + path);
+    o << std::setw(24) << std::setprecision(16) << time;
+    for (int d = 0; d < 3; ++d) o << std::setw(24) << std::setprecision(16) << force[d];
+    o << std::endl;
+    return o ? MRB_OK : set_error(MRB_E_ARG, std::string("write failed: ") + path);
 }
 
 int mrb_plot_nvars(int flags)

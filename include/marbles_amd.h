@@ -403,6 +403,13 @@ int mrb_vorticity(mrb_level *lvl, double *out, void *stream);
 #define MRB_TAG_GREATER 0
 #define MRB_TAG_LESS 1
 #define MRB_TAG_ADJACENT_DIFFERENCE 2
+/* LBM::ErrorEst (LBM.cpp:1597-1611) on any level: the plot variable named
+ * `field` (a macrodata or derived name of LBM.cpp:14-60, e.g. "rho",
+ * "vel_mag", "vort_mag") of the current state, evaluated on the device,
+ * and the criterion of mrb_tag_cells over the domain; host_tags: one byte
+ * per domain cell (x fastest), the input of mrb_cluster_tags.  The level's
+ * boxes must cover the domain (the levels ErrorEst tags do).  Synchronous. */
+int mrb_tag_level(mrb_level *lvl, const char *field, int criterion, double value, uint8_t *host_tags);
 int mrb_tag_cells(const mrb_level *lvl, const double *field, int criterion, double value, uint8_t *tags,
                   void *stream);
 

@@ -95,6 +95,7 @@ SIGNATURES = {
     "mrb_cluster_tags": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, POINTER(c_int), c_void_p,
                                  c_void_p, c_int]),
     "mrb_amr_regrid_fill": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "mrb_tag_level": (c_int, [c_void_p, c_char_p, c_int, c_double, c_void_p]),
     "mrb_plot_fields": (c_int, [c_void_p, c_void_p, c_int]),
     "mrb_download": (c_int, [c_void_p, c_int, c_void_p]),
     "mrb_compute_flags_sphere": (c_int, [c_void_p, POINTER(c_double), c_double, c_int]),

@@ -804,6 +804,23 @@ int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s)
 
 int fab_fill_boundary_q(mrb_level *L, hipStream_t s) { return fill_boundary(L, L->d_dq, 3, false, true, s); }
 
+// one fab component over the valid cells into a domain-sized array
+// (x fastest), as the dense field of the tagging criteria
+__global__ void k_fab_to_dense(const double *__restrict__ src, int ncomp, int comp, double *__restrict__ dense,
+                               const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base, int nbox,
+                               int64_t total, int nx, int ny)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int b = find_box(base, nbox, c);
+    const BoxDev B = boxes[b];
+    const int64_t loc = c - B.base;
+    int i, j, k;
+    local_to_ijk(B, loc, i, j, k);
+    if (!in_valid(B, i, j, k)) return;
+    dense[(int64_t(k) * ny + j) * nx + i] = src[B.base * ncomp + int64_t(comp) * B.ncell + loc];
+}
+
 int fab_plot_fields(mrb_level *L, double *der, hipStream_t s)
 {
     const unsigned nb = nblocks(L->total_cells);
@@ -2062,6 +2079,62 @@ int mrb_tag_cells(const mrb_level *L, const double *field, int criterion, double
     if (criterion != MRB_TAG_GREATER && criterion != MRB_TAG_LESS && criterion != MRB_TAG_ADJACENT_DIFFERENCE)
         return set_error(MRB_E_ARG, "unknown tagging criterion");
     return compact_tag(L, field, criterion, value, tags, as_stream(stream));
+}
+
+int mrb_tag_level(mrb_level *L, const char *field, int criterion, double value, uint8_t *host_tags)
+{
+    if (!L || !field || !host_tags) return set_error(MRB_E_ARG, "null argument");
+    if (criterion != MRB_TAG_GREATER && criterion != MRB_TAG_LESS && criterion != MRB_TAG_ADJACENT_DIFFERENCE)
+        return set_error(MRB_E_ARG, "unknown tagging criterion");
+    if (!L->flags_ready || !L->pops_set) return set_error(MRB_E_STATE, "flags and populations must be set");
+    // LBM::get_field by name: the macrodata and derived plot variables
+    static const char *const kNames[26] = {
+        "rho", "vel_x", "vel_y", "vel_z", "vel_mag", "two_rho_e", "QCorrX", "QCorrY", "QCorrZ",
+        "pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "qx", "qy", "qz", "temperature",
+        "vort_x", "vort_y", "vort_z", "vort_mag", "dQCorrX", "dQCorrY", "dQCorrZ"};
+    int idx = -1;
+    for (int n = 0; n < 26; ++n)
+        if (std::string(field) == kNames[n]) idx = n;
+    if (idx < 0) return set_error(MRB_E_ARG, std::string("unknown field ") + field);
+    const int64_t N = int64_t(L->ncell[0]) * L->ncell[1] * L->ncell[2];
+    // ErrorEst tags the levels below the finest, which cover the domain
+    int64_t covered = 0;
+    for (const BoxDev &B : L->boxes)
+        covered += int64_t(B.vhi[0] - B.vlo[0] + 1) * (B.vhi[1] - B.vlo[1] + 1) * (B.vhi[2] - B.vlo[2] + 1);
+    if (covered != N) return set_error(MRB_E_UNSUPPORTED, "mrb_tag_level: the level's boxes must cover the domain");
+    double *dense = nullptr, *der = nullptr, *md = nullptr;
+    uint8_t *tags = nullptr;
+    hipError_t e = hipMalloc(&dense, sizeof(double) * size_t(N));
+    if (e == hipSuccess) e = hipMalloc(&tags, size_t(N));
+    int rc = MRB_OK;
+    if (e == hipSuccess) {
+        if (L->layout == MRB_LAYOUT_COMPACT) {
+            e = hipMalloc(&md, sizeof(double) * 26 * size_t(N));
+            if (e == hipSuccess) rc = compact_plot_fields(L, md, md + 19 * N, nullptr);
+            if (e == hipSuccess && rc == MRB_OK)
+                e = hipMemcpy(dense, md + int64_t(idx) * N, sizeof(double) * size_t(N), hipMemcpyDeviceToDevice);
+        } else {
+            e = hipMalloc(&der, sizeof(double) * 7 * size_t(L->total_cells));
+            if (e == hipSuccess) rc = fab_plot_fields(L, der, nullptr);
+            if (e == hipSuccess && rc == MRB_OK) {
+                const bool is_md = idx < 19;
+                k_fab_to_dense<<<nblocks(L->total_cells), kBlock>>>(is_md ? L->d_md : der, is_md ? 19 : 7,
+                                                                    is_md ? idx : idx - 19, dense, L->d_boxes,
+                                                                    L->d_box_base, L->nbox, L->total_cells,
+                                                                    L->ncell[0], L->ncell[1]);
+                e = hipGetLastError();
+            }
+        }
+    }
+    if (e == hipSuccess && rc == MRB_OK) rc = compact_tag(L, dense, criterion, value, tags, nullptr);
+    if (e == hipSuccess && rc == MRB_OK) e = hipDeviceSynchronize();
+    if (e == hipSuccess && rc == MRB_OK) e = hipMemcpy(host_tags, tags, size_t(N), hipMemcpyDeviceToHost);
+    (void)hipFree(dense);
+    (void)hipFree(tags);
+    (void)hipFree(der);
+    (void)hipFree(md);
+    if (e != hipSuccess) return hip_fail(e, "mrb_tag_level");
+    return rc;
 }
 
 int mrb_storage_shift(const mrb_level *L, int lattice, double out[27])

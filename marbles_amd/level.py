@@ -294,6 +294,17 @@ class Level:
         flags = (_lib.MRB_PLOT_STREAMING if streaming else 0) | (_lib.MRB_PLOT_DERIVED if derived else 0)
         check(self.lib.mrb_write_plotfile(os.fsencode(path), self.h, float(time), int(step), flags))
 
+    def tag(self, field: str, criterion: str, value: float) -> np.ndarray:
+        """LBM::ErrorEst: uint8 tags (nz, ny, nx) over the domain where the plot
+        variable `field` (e.g. "vel_mag", "vort_mag") meets the criterion
+        ('greater', 'less' or 'adjacent_difference'); the boxes must cover the
+        domain."""
+        crit = {"greater": _lib.MRB_TAG_GREATER, "less": _lib.MRB_TAG_LESS,
+                "adjacent_difference": _lib.MRB_TAG_ADJACENT_DIFFERENCE}[criterion]
+        out = np.zeros(tuple(reversed(self.ncell)), dtype=np.uint8)
+        check(self.lib.mrb_tag_level(self.h, field.encode(), crit, float(value), out.ctypes.data_as(c_void_p)))
+        return out
+
     def tag_cells(self, field, criterion: str, value: float, tags, stream=None) -> None:
         """Refinement tags (uint8 device tensor of ncells) from a device float64
         field: criterion 'greater', 'less' or 'adjacent_difference'."""

@@ -41,3 +41,47 @@ def test_regrid_fill_and_following_steps_match_oracle(oracle, engine, seed):
             err = float(np.abs(a - b).max() / np.abs(b).max())
             assert err <= 1e-12, (name, err)
     H.close()
+
+
+def _tags_reference(f, periodic, criterion, value):
+    """The tagging criteria restated on a domain array f [z, y, x]."""
+    if criterion == "greater":
+        return f > value
+    if criterion == "less":
+        return f < value
+    d = np.zeros_like(f)
+    for ax, per in zip((2, 1, 0), periodic):
+        for s in (-1, 1):
+            diff = np.abs(np.roll(f, -s, axis=ax) - f)
+            if not per:
+                idx = [slice(None)] * 3
+                idx[ax] = -1 if s == 1 else 0
+                diff[tuple(idx)] = 0.0
+            d = np.maximum(d, diff)
+    return d > value
+
+
+@pytest.mark.parametrize("field,criterion", [("vel_mag", "greater"), ("rho", "less"),
+                                             ("vort_mag", "adjacent_difference")])
+def test_tag_level_on_the_coarse_level(oracle, engine, field, criterion):
+    """ErrorEst on the coarse level of a hierarchy after 3 steps (uneven
+    boxes, so the adjacent differences cross box edges) against the
+    criteria restated on the plot fields; then the regrid from those tags."""
+    cfg = small_amr(n=32, nbox=80, seed=4, fraction=0.2)
+    H = engine.Hierarchy(cfg)
+    H.advance(3)
+    names = ["rho", "vel_x", "vel_y", "vel_z", "vel_mag", "two_rho_e", "QCorrX", "QCorrY", "QCorrZ", "pxx", "pyy",
+             "pzz", "pxy", "pxz", "pyz", "qx", "qy", "qz", "temperature", "vort_x", "vort_y", "vort_z", "vort_mag"]
+    P = H.crse.plot_fields(derived=True)
+    f = P[names.index(field)]
+    value = float(np.percentile(f if criterion != "adjacent_difference" else np.abs(np.diff(f, axis=2)),
+                                20 if criterion == "less" else 80))
+    tags = H.crse.tag(field, criterion, value)
+    ref = _tags_reference(f, cfg["periodic"], criterion, value)
+    assert 0 < ref.sum() < ref.size
+    assert np.array_equal(tags.astype(bool), ref)
+    boxes = H.regrid(tags=tags, blocking_factor=8, max_grid_size=16)
+    assert len(boxes) > 0
+    H.advance(1)
+    assert np.isfinite(H.fine.valid("f")).all()
+    H.close()

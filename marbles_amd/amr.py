@@ -88,6 +88,32 @@ class Hierarchy:
         st = (ctypes.c_int * 2)(*[int(x) for x in steps])
         check(_lib.load().mrb_write_plotfile_levels(os.fsencode(path), lv, 2, float(time), st, flags))
 
+    def evolve(self, nsteps: int, regrid_int: int = 0, tagging=(), n_error_buf: int = 1,
+               blocking_factor: int = 8, max_grid_size: int = 32, grid_eff: float = 0.7, stream=None) -> None:
+        """LBM::evolve -> time_step(0) (LBM.cpp:398-512): before a coarse step
+        whose completed-step count is a positive multiple of regrid_int the
+        fine level is regridded (LBM.cpp:455-480) from the refinement
+        indicators `tagging` -- dicts with the reference's keys field_name
+        and one of value_greater, value_less, adjacent_difference_greater
+        (read_tagging_parameters, LBM.cpp:302-396), their tags OR-ed as
+        ErrorEst sets them.  With no tag at all the fine level is kept."""
+        import numpy as np
+
+        crit = {"value_greater": "greater", "value_less": "less", "adjacent_difference_greater": "adjacent_difference"}
+        for _ in range(int(nsteps)):
+            step = getattr(self, "step", 0)
+            if regrid_int > 0 and step > getattr(self, "last_regrid", 0) and step % regrid_int == 0:
+                tags = np.zeros(tuple(reversed(self.crse.ncell)), dtype=np.uint8)
+                for ind in tagging:
+                    (key, value), = [(k, v) for k, v in ind.items() if k in crit]
+                    tags |= self.crse.tag(ind["field_name"], crit[key], float(value))
+                if tags.any():
+                    self.regrid(tags=tags, n_error_buf=n_error_buf, blocking_factor=blocking_factor,
+                                max_grid_size=max_grid_size, grid_eff=grid_eff)
+                self.last_regrid = step
+            self.advance(1, stream=stream)
+            self.step = step + 1
+
     def advance(self, nsteps: int, stream=None, graph: bool = False) -> None:
         """nsteps coarse steps; graph=True replays a captured hipGraph per step."""
         fn = _lib.load().mrb_amr_advance_graph if graph else _lib.load().mrb_amr_advance

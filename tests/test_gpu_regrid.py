@@ -85,3 +85,30 @@ def test_tag_level_on_the_coarse_level(oracle, engine, field, criterion):
     H.advance(1)
     assert np.isfinite(H.fine.valid("f")).all()
     H.close()
+
+
+def test_evolve_regrids_every_regrid_int_steps(engine):
+    """Hierarchy.evolve with regrid_int = 3: the fine boxes change before
+    the fourth coarse step, and the run equals advance / tag / regrid /
+    advance done by hand, bit for bit."""
+    cfg = small_amr(n=32, nbox=80, seed=9, fraction=0.2)
+    tagging = [{"field_name": "vel_mag", "value_greater": 0.0}]
+    A = engine.Hierarchy(cfg)
+    A.advance(3)
+    v = float(np.percentile(A.crse.plot_fields()[4], 80))
+    tagging[0]["value_greater"] = v
+    A.close()
+    H = engine.Hierarchy(cfg)
+    first = sorted(H.fine.boxes)
+    H.evolve(5, regrid_int=3, tagging=tagging, blocking_factor=8, max_grid_size=16)
+    assert sorted(H.fine.boxes) != first
+    M = engine.Hierarchy(cfg)
+    M.advance(3)
+    M.regrid(tags=M.crse.tag("vel_mag", "greater", v), blocking_factor=8, max_grid_size=16)
+    M.advance(2)
+    assert sorted(M.fine.boxes) == sorted(H.fine.boxes)
+    for a, b in ((H.crse, M.crse), (H.fine, M.fine)):
+        for name in ("f", "g"):
+            assert np.array_equal(a.download(name), b.download(name)), name
+    H.close()
+    M.close()

@@ -17,6 +17,9 @@
 //              neighbours, equilibria, relax, write 54 populations to B.
 #include <type_traits>
 
+#include <algorithm>
+#include <cmath>
+
 #include "mrb_internal.h"
 
 namespace mrb {
@@ -99,6 +102,11 @@ struct Half16 {
     const uint32_t *__restrict__ amax_cur;
     uint32_t *__restrict__ amax_next;
     uint32_t *__restrict__ ovf;
+    // the largest |population - shift| the boundary faces bring in (inlet
+    // and pressure-face equilibria), per lattice: a lower bound for the box
+    // maxima that set the next scales, so that a box at rest next to an
+    // inlet keeps room for the inflow
+    float floor_f, floor_g;
 };
 
 // FP16 value of (v - shift) * 2^e: the difference is taken in FP64, then
@@ -485,8 +493,8 @@ __device__ __forceinline__ void collide_body(
     float mf = 0.0f, mg = 0.0f;
     if constexpr (kIsHalf<T>) {
         const int b = int(blockIdx.z) / kScaleSlab;
-        ef = f16_exponent(__uint_as_float(h16.amax_cur[2 * b]));
-        eg = f16_exponent(__uint_as_float(h16.amax_cur[2 * b + 1]));
+        ef = f16_exponent(fmaxf(__uint_as_float(h16.amax_cur[2 * b]), h16.floor_f));
+        eg = f16_exponent(fmaxf(__uint_as_float(h16.amax_cur[2 * b + 1]), h16.floor_g));
         scf = ldexpf(1.0f, ef);
         scg = ldexpf(1.0f, eg);
     }
@@ -647,6 +655,51 @@ static ShiftTab shift_tab(const mrb_level *L)
     return t;
 }
 
+// the boundary floor of the FP16 box maxima: |f_eq - shift| and
+// |g_eq - shift| of the inlet state (for a Channel or Parabolic profile the
+// reference speed bounds the profile) and of the pressure faces' state at
+// rest and at their g velocity
+static void f16_floors(const mrb_level *L, float &ff, float &fg)
+{
+    double mf = 0.0, mg = 0.0;
+    const BCDev &d = L->bcdev;
+    auto take = [&](const double *fv, const double *gv) {
+        for (int q = 0; q < NQ; ++q) {
+            mf = std::max(mf, std::fabs(fv[q] - L->fshift[q]));
+            mg = std::max(mg, std::fabs(gv[q] - L->gshift[q]));
+        }
+    };
+    bool velocity = false;
+    for (int n = 0; n < 6; ++n) velocity = velocity || L->bc.bc_type[n] == MRB_BC_VELOCITY;
+    if (velocity) {
+        if (d.prof.kind) {
+            const VelProfile &P = d.prof;
+            double fv[NQ], gv[NQ];
+            for (int dir = 0; dir < 3; ++dir)
+                for (int sgn = -1; sgn <= 1; sgn += 2) {
+                    double v[3] = {0.0, 0.0, 0.0};
+                    v[dir] = sgn * P.vref;
+                    for (int q = 0; q < NQ; ++q) {
+                        fv[q] = set_equilibrium_value(P.rho, v, P.R * P.T, q);
+                        gv[q] = thermal_g_equilibrium(P.rho, v, P.T, P.R, P.gamma, q);
+                    }
+                    take(fv, gv);
+                }
+        } else {
+            take(d.fval, d.gval);
+        }
+    }
+    if (d.pdir >= 0) {
+        const double zero[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < NQ; ++q) {
+            mf = std::max(mf, std::fabs(set_equilibrium_value(d.prho, zero, d.prt, q) - L->fshift[q]));
+            if (kEV[q][d.pdir] != 0) mg = std::max(mg, std::fabs(d.pgval[q] - L->gshift[q]));
+        }
+    }
+    ff = float(mf);
+    fg = float(mg);
+}
+
 static Half16 half16_of(mrb_level *L)
 {
     Half16 h{};
@@ -656,6 +709,7 @@ static Half16 half16_of(mrb_level *L)
         h.amax_cur = L->d_amax16[L->cur];
         h.amax_next = L->d_amax16[1 - L->cur];
         h.ovf = L->d_ovf16;
+        f16_floors(L, h.floor_f, h.floor_g);
     }
     return h;
 }

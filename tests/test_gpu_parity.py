@@ -222,14 +222,22 @@ def _velocity(E):
     return np.stack([np.tensordot(ev[:, d].astype(float), f, axes=(0, 0)) / rho for d in range(3)]), rho
 
 
-@pytest.mark.parametrize("case", ["tg_tau1.0", "channel_T0.2"])
+def _tracking_case(case):
+    if case == "tg_tau1.0":
+        return configs.taylor_green(32, tau=1.0), 100
+    if case == "channel_profile":
+        return configs.channel_profile((40, 20, 24), kind="channel"), 100
+    if case == "pressure_walled":
+        return configs.pressure_case("walled", (24, 16, 32)), 100
+    return configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 100
+
+
+@pytest.mark.parametrize("case", ["tg_tau1.0", "channel_T0.2", "channel_profile", "pressure_walled"])
 def test_f32_shifted_storage_tracks_f64(oracle, engine, case):
     """FP32 storage of f - f_eq(rho0, 0, T0) with FP64 arithmetic: velocity
-    and density within 1e-6 (relative to max |u|, rho) of the FP64 run."""
-    if case == "tg_tau1.0":
-        cfg, n = configs.taylor_green(32, tau=1.0), 100
-    else:
-        cfg, n = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 100
+    and density within 1e-6 (relative to max |u|, rho) of the FP64 run, also
+    with a Channel-profile inlet and a walled pressure outlet."""
+    cfg, n = _tracking_case(case)
     runs = {}
     for storage in ("f64", "f32s"):
         E = configs.build_level(cfg, layout="compact", storage=storage)
@@ -244,7 +252,7 @@ def test_f32_shifted_storage_tracks_f64(oracle, engine, case):
     assert du <= 1e-6 and dr <= 1e-6, (du, dr)
 
 
-@pytest.mark.parametrize("case", ["tg_tau1.0", "channel"])
+@pytest.mark.parametrize("case", ["tg_tau1.0", "channel", "channel_profile", "pressure_walled"])
 def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
     """FP16 storage of (f - f_eq(rho0, 0, T0)) * 2^e, e per (z-slab box,
     lattice), FP64 arithmetic: velocity and density against the FP64 run
@@ -252,10 +260,7 @@ def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
     that, not by 1e-6; measured 3.6e-4 / 8.7e-4 velocity); no stored value
     saturates; conservation sums in FP64 agree with the FP64 run's; the
     shift cannot change once populations are stored."""
-    if case == "tg_tau1.0":
-        cfg, n = configs.taylor_green(32, tau=1.0), 100
-    else:
-        cfg, n = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 100
+    cfg, n = _tracking_case(case)
     runs, cons = {}, {}
     for storage in ("f64", "f16s"):
         E = configs.build_level(cfg, layout="compact", storage=storage)

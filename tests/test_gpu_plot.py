@@ -193,3 +193,24 @@ def test_two_level_plot_fields_and_plotfile(oracle, engine, tmp_path, case):
     for (lo, hi), fab in zip(H.fine.boxes, fabs):
         np.testing.assert_array_equal(fab, P[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
     H.close()
+
+
+@pytest.mark.parametrize("storage,tol", [("f32s", 1e-6), ("f16s", 5e-3)])
+def test_reduced_storage_plot_fields_track_f64(engine, storage, tol):
+    """Plot macrodata (the pull of the pre-step buffer through the reduced
+    storage) of a channel with an inlet, against the FP64 run: density,
+    velocity and temperature within the storage's tolerance of their range;
+    is_fluid identical."""
+    cfg = configs.channel_sphere((40, 24, 24), radius=5.0, center=(14.0, 12.0, 12.0))
+    P = {}
+    for st in ("f64", storage):
+        E = configs.build_level(cfg, layout="compact", storage=st)
+        E.advance(30)
+        P[st] = E.plot_fields(derived=True)
+        E.close()
+    a, b = P[storage], P["f64"]
+    fluid = b[26] == 1
+    for k in (0, 1, 2, 3, 18):
+        err = np.abs(a[k] - b[k])[fluid].max() / max(np.abs(b[k][fluid]).max(), 1e-300)
+        assert err <= tol, (storage, k, err)
+    assert np.array_equal(a[26], b[26])

@@ -102,3 +102,29 @@ def test_restart_refuses_other_boxes(oracle, engine, tmp_path):
         checkpoint.read_checkpoint(path, [R])
     E.close()
     R.close()
+
+
+@pytest.mark.parametrize("case", ["periodic", "inlet_pressure"])
+def test_two_level_restart_continues_bit_for_bit(engine, tmp_path, case):
+    """Hierarchy (coarse + fine, ratio 2): 4 coarse steps, checkpoint of both
+    levels, 4 more on a fresh hierarchy built on the checkpoint's boxes ==
+    8 uninterrupted coarse steps, on both levels."""
+    from test_gpu_amr import CASES
+
+    cfg = CASES[case]()
+    H = engine.Hierarchy(cfg)
+    H.advance(4)
+    path = str(tmp_path / "chk00004")
+    checkpoint.write_checkpoint(path, [H.crse, H.fine], [4, 8], [1.0, 0.5], [4.0, 4.0])
+    H.advance(4)
+    info = checkpoint.checkpoint_info(path)
+    assert info["nlevels"] == 2 and info["steps"] == [4, 8] and info["dts"] == [1.0, 0.5]
+    assert [tuple(map(tuple, b)) for b in info["boxes"][1]] == [tuple(map(tuple, b)) for b in H.fine.boxes]
+    R = engine.Hierarchy(dict(cfg, fine_boxes=info["boxes"][1]))
+    checkpoint.read_checkpoint(path, [R.crse, R.fine])
+    R.advance(4)
+    for a, b in ((R.crse, H.crse), (R.fine, H.fine)):
+        for name in ("f", "g"):
+            assert np.array_equal(a.valid(name), b.valid(name)), (case, name)
+    H.close()
+    R.close()

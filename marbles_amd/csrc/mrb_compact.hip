@@ -561,6 +561,81 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     collide_body<T, BCX>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
+// Q_corr of an inside fluid cell from its own pull, as k_qpass computes it
+template <class T, int BCX>
+__device__ __forceinline__ Macro pulled_macro(const T *__restrict__ f, const T *__restrict__ g,
+                                              const uint32_t *__restrict__ mask, const CompactArgs &A,
+                                              const BCDev &bc, const ShiftTab &sh, const CollideParams &P, int x,
+                                              int y, int z)
+{
+    const int64_t cell = (int64_t(z) * A.ny + y) * A.nx + x;
+    double fv[NQ], gv[NQ];
+    pull_cell<BCX>(f, g, A, bc, sh, Half16{}, x, y, z, cell, mask[cell], fv, gv);
+    double rho, jx, jy, jz, E;
+    accumulate_q_moments(fv, gv, rho, jx, jy, jz, E);
+    return macro_from(rho, jx, jy, jz, E, P);
+}
+
+// single pass: stream + collide with the neighbours' Q_corr recomputed in
+// place of pass 1.  A wave is one row segment of 64 cells along x
+// (blockDim.x == 64): the x neighbours' Q comes from the lanes beside (the
+// end lanes pull the cells beyond the segment), the y and z neighbours' Q
+// from their own pulls, which the blocks of the neighbouring rows and
+// planes also read, so that they come mostly from L2.  The values equal
+// pass 1's bit for bit (same pull, same sums, same operation order).
+// FP64 and shifted FP32 storage.
+template <class T, int BCX>
+__global__ void __launch_bounds__(256) k_onepass(
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, CompactArgs A, BCDev bc, ShiftTab sh, CollideParams P)
+{
+    int x, y, z;
+    int64_t cell;
+    const bool in = cell_coords(A, x, y, z, cell);
+    const uint32_t m = in ? mask[cell] : 0u;
+    const bool fluid = in && (m & 1u);
+    // gradient(): neighbour inside the domain box and fluid
+    const bool xp = fluid && x < A.nx - 1 && ((m >> 2) & 1u), xm = fluid && x > 0 && ((m >> 1) & 1u);
+    const bool yp = fluid && y < A.ny - 1 && ((m >> 4) & 1u), ym = fluid && y > 0 && ((m >> 3) & 1u);
+    const bool zp = fluid && z < A.nz - 1 && ((m >> 6) & 1u), zm = fluid && z > 0 && ((m >> 5) & 1u);
+    double qyp = 0.0, qym = 0.0, qzp = 0.0, qzm = 0.0, qxe = 0.0;
+    if (yp) qyp = pulled_macro<T, BCX>(f, g, mask, A, bc, sh, P, x, y + 1, z).Qy;
+    if (ym) qym = pulled_macro<T, BCX>(f, g, mask, A, bc, sh, P, x, y - 1, z).Qy;
+    if (zp) qzp = pulled_macro<T, BCX>(f, g, mask, A, bc, sh, P, x, y, z + 1).Qz;
+    if (zm) qzm = pulled_macro<T, BCX>(f, g, mask, A, bc, sh, P, x, y, z - 1).Qz;
+    const int lane = int(threadIdx.x);
+    if ((lane == 0 && xm) || (lane == 63 && xp))
+        qxe = pulled_macro<T, BCX>(f, g, mask, A, bc, sh, P, lane == 0 ? x - 1 : x + 1, y, z).Qx;
+    double fv[NQ], gv[NQ];
+    Moments mo{};
+    Macro M{};
+    if (fluid) {
+        pull_cell<BCX>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
+        accumulate_moments(fv, gv, mo);
+        M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+    }
+    // every lane of the wave takes part in the exchange
+    const double qx_lo = __shfl_up(M.Qx, 1), qx_hi = __shfl_down(M.Qx, 1);
+    const double qxm = lane == 0 ? qxe : qx_lo, qxp = lane == 63 ? qxe : qx_hi;
+    if (in && !fluid) {
+        // solid cell: the stream leaves the -1 sentinel (LBM.cpp:565)
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(-1.0, sh.f(q)),
+                       narrow<T>(-1.0, sh.g(q)));
+        });
+    } else if (fluid) {
+        const double dqx = one_sided_gradient(xp, xm, xp ? qxp : 0.0, M.Qx, xm ? qxm : 0.0, P.inv_dx[0]);
+        const double dqy = one_sided_gradient(yp, ym, qyp, M.Qy, qym, P.inv_dx[1]);
+        const double dqz = one_sided_gradient(zp, zm, qzp, M.Qz, qzm, P.inv_dx[2]);
+        collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
+            constexpr int q = decltype(qc)::value;
+            store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(fnew, sh.f(q)),
+                       narrow<T>(gnew, sh.g(q)));
+        });
+    }
+}
+
 // stream only (parity of the pull with the reference push); FP64 and
 // shifted FP32 storage
 template <class T, int BCX>
@@ -724,6 +799,18 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     const T *g = (const T *)L->pop[1][L->cur];
     T *fo = (T *)L->pop[0][1 - L->cur];
     T *go = (T *)L->pop[1][1 - L->cur];
+    if (variant == kOnePassVariant) {
+        if constexpr (kIsHalf<T>) {
+            return set_error(MRB_E_UNSUPPORTED, "k_onepass: FP64 and shifted FP32 storage only");
+        } else {
+            if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+            k_onepass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, A, L->bcdev, sh,
+                                                                     L->cp);
+            MRB_HIP(hipGetLastError());
+            if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+            return MRB_OK;
+        }
+    }
     if (ev) MRB_HIP(hipEventRecord(ev[0], s));
     k_qpass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
     MRB_HIP(hipGetLastError());
@@ -765,7 +852,7 @@ static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
 }
 
-const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_qpass+k_collide_w3"};
+const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_qpass+k_collide_w3", "k_onepass"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 
@@ -786,7 +873,7 @@ static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
     int rc;
     if (L->hot < kNumTwoPassVariants) {
         rc = two_pass(L, L->hot, s, events ? L->ev : nullptr);
-        nslots = 2;
+        nslots = L->hot == kOnePassVariant ? 1 : 2;
     } else {
         if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
         rc = fused_launch(L, L->hot - kNumTwoPassVariants, s);

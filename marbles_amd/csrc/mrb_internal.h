@@ -316,10 +316,12 @@ const char *compact_kernel_name(const mrb_level *L);
 int fused_launch(mrb_level *L, int variant, hipStream_t s);
 int fused_step(mrb_level *L, int variant, hipStream_t s);
 // compact-layout step variants (mrb_compact.hip): index < kNumTwoPassVariants
-// is a two-pass step (register budgets differ), the rest the fused
+// is a two-pass step (register budgets differ) or, at kOnePassVariant, the
+// single pass that recomputes the neighbours' Q_corr; the rest the fused
 // variants; launch reads the current buffers, writes the spare ones, does
 // not flip
-constexpr int kNumTwoPassVariants = 2;
+constexpr int kNumTwoPassVariants = 3;
+constexpr int kOnePassVariant = 2;
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s);
 int compact_variant_launch(mrb_level *L, int index, hipStream_t s);
 const char *compact_variant_name(int index);

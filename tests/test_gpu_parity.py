@@ -456,3 +456,31 @@ def test_ic_plugin_workloads_match_oracle(oracle, engine, case, layout):
         err = _rel_err(E.valid(name), O.valid(name))
         assert err <= REL_TOL, (case, layout, name, err)
     E.close()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32s"])
+@pytest.mark.parametrize("case", ["channel", "odd", "channel_profile", "pressure_walled", "outlet"])
+def test_onepass_equals_two_pass_bitwise(engine, case, storage):
+    """k_onepass recomputes the neighbours' Q_corr from their pulls instead
+    of reading pass 1's array: the same operations in the same order, so
+    its populations equal the two-pass step's bit for bit (partial tiles,
+    the sphere, inlet profiles, pressure ghosts)."""
+    if case == "channel":
+        cfg, n = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 20
+    elif case == "odd":
+        cfg, n = configs.channel_sphere((37, 29, 23), radius=4.0, center=(12.0, 14.0, 11.0)), 20
+    elif case == "channel_profile":
+        cfg, n = configs.channel_profile((40, 20, 24), kind="channel"), 20
+    elif case == "pressure_walled":
+        cfg, n = configs.pressure_case("walled", (24, 16, 32)), 20
+    else:
+        cfg, n = configs.pressure_case("outlet", (32, 16, 16)), 20
+    out = {}
+    for name in ("k_qpass+k_collide", "k_onepass"):
+        E = configs.build_level(cfg, layout="compact", storage=storage)
+        E.set_hot_kernel(name)
+        E.advance(n)
+        out[name] = (E.valid("f"), E.valid("g"))
+        E.close()
+    for a, b in zip(out["k_onepass"], out["k_qpass+k_collide"]):
+        assert np.array_equal(a, b), (case, storage)

@@ -415,9 +415,10 @@ __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y
 // pass 1: Q_corr of the post-stream state.  With FP16 storage it also
 // clears the box maxima that pass 2 accumulates for the buffer it writes.
 template <class T, int BCX>
-__global__ void __launch_bounds__(256) k_qpass(
+__device__ __forceinline__ void qpass_body(
     const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask,
-    double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, Half16 h16, CollideParams P)
+    double *__restrict__ Q, const CompactArgs &A, const BCDev &bc, const ShiftTab &sh, const Half16 &h16,
+    const CollideParams &P)
 {
     if constexpr (kIsHalf<T>) {
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0 &&
@@ -442,6 +443,20 @@ __global__ void __launch_bounds__(256) k_qpass(
     Q[2 * A.N + cell] = M.Qz;
 }
 
+// pass 1 on blocks of 64 x BY cells (one name per block shape)
+#define MRB_QPASS_KERNEL(NAME, BY)                                                                            \
+    template <class T, int BCX>                                                                               \
+    __global__ void __launch_bounds__(64 * BY) NAME(                                                          \
+        const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask,                  \
+        double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, Half16 h16, CollideParams P)           \
+    {                                                                                                         \
+        qpass_body<T, BCX>(f, g, mask, Q, A, bc, sh, h16, P);                                                 \
+    }
+MRB_QPASS_KERNEL(k_qpass, 4)
+MRB_QPASS_KERNEL(k_qpass_y8, 8)
+MRB_QPASS_KERNEL(k_qpass_y2, 2)
+#undef MRB_QPASS_KERNEL
+
 // FP16 storage: the block's largest |population - shift| per lattice goes
 // to the written buffer's box entries (every block lies in one z-plane, so
 // in one scale box); a block whose largest value exceeded the FP16 range
@@ -449,7 +464,7 @@ __global__ void __launch_bounds__(256) k_qpass(
 // exponent is stored by each of its blocks (the same value)
 __device__ __forceinline__ void half_block_stats(const Half16 &h16, int ef, int eg, float mf, float mg)
 {
-    __shared__ float smf[4], smg[4];
+    __shared__ float smf[16], smg[16];  // one per wave, blocks up to 1024 threads
     for (int o = 32; o > 0; o >>= 1) {
         mf = fmaxf(mf, __shfl_xor(mf, o));
         mg = fmaxf(mg, __shfl_xor(mg, o));
@@ -541,15 +556,21 @@ __device__ __forceinline__ void collide_body(
     if constexpr (kIsHalf<T>) half_block_stats(h16, ef, eg, mf, mg);
 }
 
-// register budget as the compiler chooses (2 waves per SIMD)
-template <class T, int BCX>
-__global__ void __launch_bounds__(256) k_collide(
-    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
-    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
-    Half16 h16, CollideParams P)
-{
-    collide_body<T, BCX>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
-}
+// pass 2 on blocks of 64 x BY cells, register budget as the compiler
+// chooses (2 waves per SIMD); one name per block shape
+#define MRB_COLLIDE_KERNEL(NAME, BY)                                                                          \
+    template <class T, int BCX>                                                                               \
+    __global__ void __launch_bounds__(64 * BY) NAME(                                                          \
+        const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,             \
+        const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, \
+        Half16 h16, CollideParams P)                                                                          \
+    {                                                                                                         \
+        collide_body<T, BCX>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);                                       \
+    }
+MRB_COLLIDE_KERNEL(k_collide, 4)
+MRB_COLLIDE_KERNEL(k_collide_y8, 8)
+MRB_COLLIDE_KERNEL(k_collide_y2, 2)
+#undef MRB_COLLIDE_KERNEL
 
 // at most 168 registers (3 waves per SIMD), some values spilled to scratch
 template <class T, int BCX>
@@ -703,7 +724,7 @@ CompactArgs make_args(const mrb_level *L)
     return A;
 }
 
-dim3 grid_of(const CompactArgs &A) { return dim3((A.nx + 63) / 64, (A.ny + 3) / 4, A.nz); }
+dim3 grid_of(const CompactArgs &A, int by = 4) { return dim3((A.nx + 63) / 64, (A.ny + by - 1) / by, A.nz); }
 
 }  // namespace
 
@@ -811,6 +832,27 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
             return MRB_OK;
         }
     }
+    if (variant == kRows8Variant || variant == kRows2Variant) {
+        // the same two passes on 64x8 or 64x2 blocks
+        const int by = variant == kRows8Variant ? 8 : 2;
+        const dim3 grid = grid_of(A, by), block(64, by, 1);
+        if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+        if (by == 8)
+            k_qpass_y8<T, BCX><<<grid, block, 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+        else
+            k_qpass_y2<T, BCX><<<grid, block, 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+        MRB_HIP(hipGetLastError());
+        if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+        if (by == 8)
+            k_collide_y8<T, BCX><<<grid, block, 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
+                                                         L->cp);
+        else
+            k_collide_y2<T, BCX><<<grid, block, 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
+                                                         L->cp);
+        MRB_HIP(hipGetLastError());
+        if (ev) MRB_HIP(hipEventRecord(ev[2], s));
+        return MRB_OK;
+    }
     if (ev) MRB_HIP(hipEventRecord(ev[0], s));
     k_qpass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
     MRB_HIP(hipGetLastError());
@@ -852,7 +894,8 @@ static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     return set_error(MRB_E_UNSUPPORTED, "storage not supported by this kernel");
 }
 
-const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_qpass+k_collide_w3", "k_onepass"};
+const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_qpass+k_collide_w3", "k_onepass",
+                                                           "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 

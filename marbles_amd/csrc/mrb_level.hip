@@ -1048,6 +1048,10 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
     L->layout = desc->layout;
     L->nlattice = desc->nlattice;
     L->storage = desc->storage;
+    // the fastest block shape of the two-pass step per storage at 256^3
+    // (64x8 rows for FP64, 64x2 for shifted FP32, 64x4 for scaled FP16);
+    // mrb_autotune re-selects on the live state
+    L->hot = L->storage == MRB_STORAGE_F64 ? kRows8Variant : L->storage == MRB_STORAGE_F32_SHIFTED ? kRows2Variant : 0;
     if (L->layout == MRB_LAYOUT_COMPACT) {
         if (desc->nbox != 1 || desc->ng != 0)
             return set_error(MRB_E_ARG, "compact layout is one box without ghost cells");

@@ -458,12 +458,13 @@ def test_ic_plugin_workloads_match_oracle(oracle, engine, case, layout):
     E.close()
 
 
-@pytest.mark.parametrize("storage", ["f64", "f32s"])
+@pytest.mark.parametrize("storage", ["f64", "f32s", "f16s"])
 @pytest.mark.parametrize("case", ["channel", "odd", "channel_profile", "pressure_walled", "outlet"])
-def test_onepass_equals_two_pass_bitwise(engine, case, storage):
-    """k_onepass recomputes the neighbours' Q_corr from their pulls instead
-    of reading pass 1's array: the same operations in the same order, so
-    its populations equal the two-pass step's bit for bit (partial tiles,
+def test_step_variants_equal_two_pass_bitwise(engine, case, storage):
+    """The other block shapes of the two passes (64x8, 64x2) and k_onepass,
+    which recomputes the neighbours' Q_corr from their pulls instead of
+    reading pass 1's array, do the same operations in the same order: their
+    populations equal the 64x4 two-pass step's bit for bit (partial tiles,
     the sphere, inlet profiles, pressure ghosts)."""
     if case == "channel":
         cfg, n = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0)), 20
@@ -475,12 +476,16 @@ def test_onepass_equals_two_pass_bitwise(engine, case, storage):
         cfg, n = configs.pressure_case("walled", (24, 16, 32)), 20
     else:
         cfg, n = configs.pressure_case("outlet", (32, 16, 16)), 20
+    names = ["k_qpass+k_collide", "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2"]
+    if storage != "f16s":
+        names.append("k_onepass")
     out = {}
-    for name in ("k_qpass+k_collide", "k_onepass"):
+    for name in names:
         E = configs.build_level(cfg, layout="compact", storage=storage)
         E.set_hot_kernel(name)
         E.advance(n)
         out[name] = (E.valid("f"), E.valid("g"))
         E.close()
-    for a, b in zip(out["k_onepass"], out["k_qpass+k_collide"]):
-        assert np.array_equal(a, b), (case, storage)
+    for name in names[1:]:
+        for a, b in zip(out[name], out[names[0]]):
+            assert np.array_equal(a, b), (case, storage, name)

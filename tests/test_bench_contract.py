@@ -12,10 +12,13 @@ def test_multi_gpu_request_exits_with_a_message():
     assert r.returncode == 2 and "one GPU" in r.stderr
 
 
-def test_traffic_from_committed_profile_covers_both_storages():
+def test_traffic_from_committed_profile_covers_every_storage():
+    """The summary covers the kernels each storage's level starts on (64x8
+    blocks for FP64, 64x2 for shifted FP32, 64x4 for scaled FP16)."""
     cells = 256 ** 3
-    t64 = bench.measured_traffic("f64", ["k_qpass", "k_collide"])
-    t32 = bench.measured_traffic("f32s", ["k_qpass", "k_collide"])
+    t64 = bench.measured_traffic("f64", ["k_qpass_y8", "k_collide_y8"])
+    t32 = bench.measured_traffic("f32s", ["k_qpass_y2", "k_collide_y2"])
+    t16 = bench.measured_traffic("f16s", ["k_qpass", "k_collide"])
     # at least the algorithmic bytes (read + write of every population once)
-    assert t64 >= 864 * cells and t32 >= 432 * cells
+    assert t64 >= 864 * cells and t32 >= 432 * cells and t16 >= 216 * cells
     assert bench.measured_traffic("f64", ["k_not_profiled"]) is None

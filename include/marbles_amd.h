@@ -182,11 +182,16 @@ int mrb_advance(mrb_level *lvl, int nsteps, void *stream);
  * (slots are named by mrb_hot_kernel_name, separated by '+'), *nslots the
  * number of slots (at most 4).  Synchronous.  Compact layout only. */
 int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_ms, int *nslots);
-/* Kernel variants of the compact-layout step: indices 0 and 1 are the
- * two-pass step (Q-correction pass + collide pass) with the compiler's
- * register budget and with a 3-waves-per-SIMD budget; the following ones the
- * single-pass z-sweep with different tile shapes.  All variants produce
- * identical results. */
+/* Kernel variants of the compact-layout step.
+ *   0  two-pass step (Q-correction pass + collide pass) on 64x4-cell blocks,
+ *      the compiler's register budget;
+ *   1  the same with a 3-waves-per-SIMD budget;
+ *   2  k_onepass: one pass that recomputes the neighbours' Q_corr;
+ *   3  the two-pass step on 64x8-cell blocks;
+ *   4  the two-pass step on 64x2-cell blocks;
+ *   5+ the single-pass z-sweep with different tile shapes.
+ * All variants give the same populations bit for bit.  A level starts on
+ * variant 3 (FP64 storage), 4 (shifted FP32) or 0 (scaled FP16). */
 int mrb_hot_kernel_count(void);
 const char *mrb_hot_kernel_variant_name(int index);
 int mrb_set_hot_kernel(mrb_level *lvl, int index);

@@ -295,8 +295,9 @@ def run_precision(args, torch) -> int:
     """Precision study: periodic n^3 box (default 512^3) with a seeded
     random-phase divergence-free velocity field (E(k) ~ k^-5/3, k <= 32,
     max |u| = 0.04), T0 = 0.2, run for the given steps with FP64,
-    shifted-FP32 and scaled-FP16 population storage (FP64 arithmetic in
-    all); velocity, density and kinetic energy of each compared with the
+    shifted-FP32 and scaled-FP16 population storage (each level's default
+    step: FP64 arithmetic, FP32 in the scaled-FP16 collide); velocity,
+    density and kinetic energy of each compared with the
     FP64 run on the device (relative L2 and max errors).  tau = 0.55: at
     tau = 0.52 the thermal f+g model diverges on this field at 512^3
     between steps 200 and 400 (finite at 200, NaN at 400); tau = 0.55 stays
@@ -327,6 +328,10 @@ def run_precision(args, torch) -> int:
         lvl.init_fields(*fields, adiabatic_exponent=cfg["ic"]["adiabatic_exponent"])
         torch.cuda.synchronize()
         log(f"{st}: level initialised in {time.perf_counter() - t0:.1f}s")
+        # the fastest step variant at this size, timed on the live state
+        # (which it leaves unchanged); the default ones are chosen at 256^3
+        lvl.autotune(reps=3, stream=sh)
+        log(f"{st}: kernel {lvl.hot_kernel_name()}")
         lvl.advance(args.warmup, stream=sh)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -342,11 +347,12 @@ def run_precision(args, torch) -> int:
         lvl.macro_fields(mac, stream=sh)
         cons = lvl.conservation(stream=sh)
         sat = lvl.storage_saturations()
+        kernel = lvl.hot_kernel_name()
         lvl.close()
         rho, u = mac[:N], mac[N:4 * N].view(3, N)
         ke = float(0.5 * (rho * (u * u).sum(0)).sum())
-        leg = {"mlups": round(N * nsteps / secs / 1e6, 1), "seconds": round(secs, 2), "kinetic_energy": ke,
-               "mass": float(cons[0]), "energy_2rhoE": float(cons[4])}
+        leg = {"mlups": round(N * nsteps / secs / 1e6, 1), "seconds": round(secs, 2), "kernel": kernel,
+               "kinetic_energy": ke, "mass": float(cons[0]), "energy_2rhoE": float(cons[4])}
         if st == "f64":
             ref = (rho, u, ke, cons)
         else:
@@ -477,7 +483,7 @@ def main() -> int:
                 "max_rel_velocity_error": du,
                 "max_rel_density_error": dr,
                 "compared_after_steps": args.warmup + args.steps + args.profile_steps,
-                "arithmetic": "f64",
+                "arithmetic": "f32" if any(k.endswith("_h32") for k in rr["per_kernel_ms"]) else "f64",
                 "bytes_per_update": br,
                 "achieved_gbs": round(br * cells / (kms * 1e-3) / 1e9, 1),
                 "traffic": measured_traffic(st, rr["per_kernel_ms"]) if n == 256 else None,

@@ -153,24 +153,34 @@ __host__ __device__ __forceinline__ void accumulate_q_moments(
 }
 
 // Macroscopic fields derived in f_to_macrodata (LBM.cpp:863-901).
-struct Macro {
-    double u, v, w, T, Qx, Qy, Qz;
+// R = the arithmetic type (FP64 everywhere except the FP32-arithmetic
+// collide of the scaled FP16 storage)
+template <class R>
+struct MacroR {
+    R u, v, w, T, Qx, Qy, Qz;
 };
+using Macro = MacroR<double>;
 
-__host__ __device__ __forceinline__ Macro macro_from(
-    double rho, double jx, double jy, double jz, double E, const CollideParams& P)
+template <class R>
+__host__ __device__ __forceinline__ MacroR<R> macro_from_t(R rho, R jx, R jy, R jz, R E, const CollideParams& P)
 {
-    Macro M;
-    M.u = jx * (P.ms / rho);
-    M.v = jy * (P.ms / rho);
-    M.w = jz * (P.ms / rho);
+    MacroR<R> M;
+    M.u = jx * (R(P.ms) / rho);
+    M.v = jy * (R(P.ms) / rho);
+    M.w = jz * (R(P.ms) / rho);
     // get_temperature, scalar overload (Utilities.H:187-196)
-    M.T = (0.50 / P.cv) * ((E / rho) - (M.u * M.u + M.v * M.v + M.w * M.w));
-    const double k = (1.0 - 3.0 * P.R * M.T);
+    M.T = (R(0.50) / R(P.cv)) * ((E / rho) - (M.u * M.u + M.v * M.v + M.w * M.w));
+    const R k = (R(1.0) - R(3.0) * R(P.R) * M.T);
     M.Qx = rho * M.u * (k - M.u * M.u);
     M.Qy = rho * M.v * (k - M.v * M.v);
     M.Qz = rho * M.w * (k - M.w * M.w);
     return M;
+}
+
+__host__ __device__ __forceinline__ Macro macro_from(
+    double rho, double jx, double jy, double jz, double E, const CollideParams& P)
+{
+    return macro_from_t<double>(rho, jx, jy, jz, E, P);
 }
 
 // gradient() of Utilities.H:279-312 given the neighbour availability
@@ -201,39 +211,42 @@ __host__ __device__ __forceinline__ double one_sided_gradient(
 
 // phi of the product-form equilibrium for one lattice-velocity component
 // c in {-1,0,1} (Utilities.H:53-60).
-template <int c>
-__host__ __device__ __forceinline__ double phi_of(double u, double P)
+template <int c, class R>
+__host__ __device__ __forceinline__ R phi_of(R u, R P)
 {
-    return c * 0.5 * u + iabs(c) * (1.50 * P - 1.0) - P + 1.0;
+    return R(c) * R(0.5) * u + R(iabs(c)) * (R(1.50) * P - R(1.0)) - P + R(1.0);
 }
 
 // Per-cell coefficients of the f and g equilibria (LBM.cpp:659-758) and the
 // relaxation rate (LBM.cpp:795-797).
-struct CollideCoeffs {
-    double omega;
-    double rphx[3], phy[3], phz[3];  // rho*phi_x, phi_y, phi_z at c+1
-    double E, a1x, a1y, a1z;
-    double tx[2], ty[2], tz[2];      // (c*c - theta0) * a2_aa for c*c = 0, 1
-    double dxy, dxz, dyz;            // 2 * a2_ab
+template <class R>
+struct CollideCoeffsR {
+    R omega;
+    R rphx[3], phy[3], phz[3];  // rho*phi_x, phi_y, phi_z at c+1
+    R E, a1x, a1y, a1z;
+    R tx[2], ty[2], tz[2];      // (c*c - theta0) * a2_aa for c*c = 0, 1
+    R dxy, dxz, dyz;            // 2 * a2_ab
 };
+using CollideCoeffs = CollideCoeffsR<double>;
 
-__host__ __device__ __forceinline__ CollideCoeffs collide_coeffs(
-    const Moments& m, const Macro& M, double dqx, double dqy, double dqz, const CollideParams& P)
+template <class R>
+__host__ __device__ __forceinline__ CollideCoeffsR<R> collide_coeffs_t(
+    const MomentsR<R>& m, const MacroR<R>& M, R dqx, R dqy, R dqz, const CollideParams& P)
 {
-    CollideCoeffs C;
-    const double R = P.R, dt = P.dt;
-    const double rho = m.rho;
-    const double T = M.T;
-    const double u = M.u, v = M.v, w = M.w;
-    const double omega = 1.0 / (P.nu / (R * T * dt) + 0.5);
-    const double omega_one = 1.0 / (P.alpha / (R * T * dt) + 0.5);
-    const double omega_one_by_omega = omega_one / omega;
-    const double omega_corr = (2.0 - omega) / (2.0 * omega * rho);
+    CollideCoeffsR<R> C;
+    const R Rg = R(P.R), dt = R(P.dt);
+    const R rho = m.rho;
+    const R T = M.T;
+    const R u = M.u, v = M.v, w = M.w;
+    const R omega = R(1.0) / (R(P.nu) / (Rg * T * dt) + R(0.5));
+    const R omega_one = R(1.0) / (R(P.alpha) / (Rg * T * dt) + R(0.5));
+    const R omega_one_by_omega = omega_one / omega;
+    const R omega_corr = (R(2.0) - omega) / (R(2.0) * omega * rho);
     C.omega = omega;
 
-    const double pxx_ext = u * u + R * T + dt * (omega_corr)*dqx;
-    const double pyy_ext = v * v + R * T + dt * (omega_corr)*dqy;
-    const double pzz_ext = w * w + R * T + dt * (omega_corr)*dqz;
+    const R pxx_ext = u * u + Rg * T + dt * (omega_corr)*dqx;
+    const R pyy_ext = v * v + Rg * T + dt * (omega_corr)*dqy;
+    const R pzz_ext = w * w + Rg * T + dt * (omega_corr)*dqz;
     C.rphx[0] = rho * phi_of<-1>(u, pxx_ext);
     C.rphx[1] = rho * phi_of<0>(u, pxx_ext);
     C.rphx[2] = rho * phi_of<1>(u, pxx_ext);
@@ -244,51 +257,57 @@ __host__ __device__ __forceinline__ CollideCoeffs collide_coeffs(
     C.phz[1] = phi_of<0>(w, pzz_ext);
     C.phz[2] = phi_of<1>(w, pzz_ext);
 
-    const double E = m.E;
-    const double energy = E / (2.0 * rho);
-    const double Tv = (0.50 / P.cv) * ((E / rho) - u * u + v * v + w * w);
-    const double p = rho * R * Tv;
-    const double h = energy + (p / rho);
-    const double hp = h + (p / rho);
-    double qx_eq = 2.0 * rho * u * h;
-    double qy_eq = 2.0 * rho * v * h;
-    double qz_eq = 2.0 * rho * w * h;
-    const double rxx = 2.0 * rho * u * u * hp + 2.0 * p * h;
-    const double ryy = 2.0 * rho * v * v * hp + 2.0 * p * h;
-    const double rzz = 2.0 * rho * w * w * hp + 2.0 * p * h;
-    const double rxy = 2.0 * rho * u * v * hp + 0;
-    const double rxz = 2.0 * rho * u * w * hp + 0;
-    const double ryz = 2.0 * rho * v * w * hp + 0;
+    const R E = m.E;
+    const R energy = E / (R(2.0) * rho);
+    const R Tv = (R(0.50) / R(P.cv)) * ((E / rho) - u * u + v * v + w * w);
+    const R p = rho * Rg * Tv;
+    const R h = energy + (p / rho);
+    const R hp = h + (p / rho);
+    R qx_eq = R(2.0) * rho * u * h;
+    R qy_eq = R(2.0) * rho * v * h;
+    R qz_eq = R(2.0) * rho * w * h;
+    const R rxx = R(2.0) * rho * u * u * hp + R(2.0) * p * h;
+    const R ryy = R(2.0) * rho * v * v * hp + R(2.0) * p * h;
+    const R rzz = R(2.0) * rho * w * w * hp + R(2.0) * p * h;
+    const R rxy = R(2.0) * rho * u * v * hp + R(0);
+    const R rxz = R(2.0) * rho * u * w * hp + R(0);
+    const R ryz = R(2.0) * rho * v * w * hp + R(0);
     qx_eq *= omega_one_by_omega;
     qy_eq *= omega_one_by_omega;
     qz_eq *= omega_one_by_omega;
-    qx_eq += (1.0 - omega_one_by_omega) *
-             (m.qx - 2.0 * u * m.pxx - 2.0 * v * m.pxy - 2.0 * w * m.pxz - u * dt * dqx);
-    qy_eq += (1.0 - omega_one_by_omega) *
-             (m.qy - 2.0 * u * m.pxy - 2.0 * v * m.pyy - 2.0 * w * m.pyz - v * dt * dqy);
-    qz_eq += (1.0 - omega_one_by_omega) *
-             (m.qz - 2.0 * u * m.pxz - 2.0 * v * m.pyz - 2.0 * w * m.pzz - w * dt * dqz);
-    const double one_bystheta0 = 1.0 / kTheta0;
+    qx_eq += (R(1.0) - omega_one_by_omega) *
+             (m.qx - R(2.0) * u * m.pxx - R(2.0) * v * m.pxy - R(2.0) * w * m.pxz - u * dt * dqx);
+    qy_eq += (R(1.0) - omega_one_by_omega) *
+             (m.qy - R(2.0) * u * m.pxy - R(2.0) * v * m.pyy - R(2.0) * w * m.pyz - v * dt * dqy);
+    qz_eq += (R(1.0) - omega_one_by_omega) *
+             (m.qz - R(2.0) * u * m.pxz - R(2.0) * v * m.pyz - R(2.0) * w * m.pzz - w * dt * dqz);
+    const R one_bystheta0 = R(1.0 / kTheta0);
     C.E = E;
     C.a1x = qx_eq * one_bystheta0;
     C.a1y = qy_eq * one_bystheta0;
     C.a1z = qz_eq * one_bystheta0;
-    const double a2xx = (rxx - E * 1.0 * kTheta0) * one_bystheta0 * one_bystheta0;
-    const double a2yy = (ryy - E * 1.0 * kTheta0) * one_bystheta0 * one_bystheta0;
-    const double a2zz = (rzz - E * 1.0 * kTheta0) * one_bystheta0 * one_bystheta0;
-    const double a2xy = rxy * one_bystheta0 * one_bystheta0;
-    const double a2xz = rxz * one_bystheta0 * one_bystheta0;
-    const double a2yz = ryz * one_bystheta0 * one_bystheta0;
-    C.tx[0] = (0 - kTheta0) * a2xx;
-    C.tx[1] = (1 - kTheta0) * a2xx;
-    C.ty[0] = (0 - kTheta0) * a2yy;
-    C.ty[1] = (1 - kTheta0) * a2yy;
-    C.tz[0] = (0 - kTheta0) * a2zz;
-    C.tz[1] = (1 - kTheta0) * a2zz;
-    C.dxy = 2.0 * a2xy;
-    C.dxz = 2.0 * a2xz;
-    C.dyz = 2.0 * a2yz;
+    const R a2xx = (rxx - E * R(1.0) * R(kTheta0)) * one_bystheta0 * one_bystheta0;
+    const R a2yy = (ryy - E * R(1.0) * R(kTheta0)) * one_bystheta0 * one_bystheta0;
+    const R a2zz = (rzz - E * R(1.0) * R(kTheta0)) * one_bystheta0 * one_bystheta0;
+    const R a2xy = rxy * one_bystheta0 * one_bystheta0;
+    const R a2xz = rxz * one_bystheta0 * one_bystheta0;
+    const R a2yz = ryz * one_bystheta0 * one_bystheta0;
+    C.tx[0] = R(0 - kTheta0) * a2xx;
+    C.tx[1] = R(1 - kTheta0) * a2xx;
+    C.ty[0] = R(0 - kTheta0) * a2yy;
+    C.ty[1] = R(1 - kTheta0) * a2yy;
+    C.tz[0] = R(0 - kTheta0) * a2zz;
+    C.tz[1] = R(1 - kTheta0) * a2zz;
+    C.dxy = R(2.0) * a2xy;
+    C.dxz = R(2.0) * a2xz;
+    C.dyz = R(2.0) * a2yz;
     return C;
+}
+
+__host__ __device__ __forceinline__ CollideCoeffs collide_coeffs(
+    const Moments& m, const Macro& M, double dqx, double dqy, double dqz, const CollideParams& P)
+{
+    return collide_coeffs_t<double>(m, M, dqx, dqy, dqz, P);
 }
 
 // f equilibrium of direction q (Utilities.H:42-63, product order kept)
@@ -326,20 +345,28 @@ __host__ __device__ __forceinline__ auto geq_of(const Co& C)
 
 // Equilibria of LBM::macrodata_to_equilibrium (LBM.cpp:659-758) hoisted per
 // cell, followed by the BGK relaxation of LBM::relax_f_to_equilibrium
-// (LBM.cpp:793-801).  fo/go may alias f/g.
+// (LBM.cpp:793-801), in arithmetic type R.  fo/go may alias f/g.
+template <class R, class TIn, class FOut>
+__host__ __device__ __forceinline__ void collide_cell_t(
+    const TIn* __restrict__ f, const TIn* __restrict__ g, const MomentsR<R>& m, const MacroR<R>& M, R dqx,
+    R dqy, R dqz, const CollideParams& P, FOut&& out)
+{
+    const CollideCoeffsR<R> C = collide_coeffs_t<R>(m, M, dqx, dqy, dqz, P);
+    const R omega = C.omega;
+    static_for<NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const R fq = R(f[q]);
+        const R gq = R(g[q]);
+        out(qc, fq + omega * (feq_of<q>(C) - fq), gq + omega * (geq_of<q>(C) - gq));
+    });
+}
+
 template <class TIn, class FOut>
 __host__ __device__ __forceinline__ void collide_cell(
     const TIn* __restrict__ f, const TIn* __restrict__ g, const Moments& m, const Macro& M,
     double dqx, double dqy, double dqz, const CollideParams& P, FOut&& out)
 {
-    const CollideCoeffs C = collide_coeffs(m, M, dqx, dqy, dqz, P);
-    const double omega = C.omega;
-    static_for<NQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const double fq = double(f[q]);
-        const double gq = double(g[q]);
-        out(qc, fq + omega * (feq_of<q>(C) - fq), gq + omega * (geq_of<q>(C) - gq));
-    });
+    collide_cell_t<double>(f, g, m, M, dqx, dqy, dqz, P, out);
 }
 
 // Product-form equilibrium without correction (Utilities.H:11-40): the

@@ -119,6 +119,14 @@ __device__ __forceinline__ _Float16 to_half(double v, double shift, float scale,
     return _Float16(__builtin_amdgcn_fmed3f(d * scale, -65504.0f, 65504.0f));
 }
 
+// the same with FP32 arithmetic: v and the shift are FP32 values
+__device__ __forceinline__ _Float16 to_half(float v, float shift, float scale, float &amax)
+{
+    const float d = v - shift;
+    amax = fmaxf(amax, fabsf(d));
+    return _Float16(__builtin_amdgcn_fmed3f(d * scale, -65504.0f, 65504.0f));
+}
+
 // storage <-> FP64 working value (FP64 and shifted FP32 storage)
 template <class T>
 __device__ __forceinline__ double widen(T v, double shift)
@@ -493,7 +501,7 @@ __device__ __forceinline__ void half_block_stats(const Half16 &h16, int ef, int 
 }
 
 // pass 2: stream + collide, writes the B buffers
-template <class T, int BCX>
+template <class T, int BCX, class R = double>
 __device__ __forceinline__ void collide_body(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
@@ -513,11 +521,11 @@ __device__ __forceinline__ void collide_body(
         scf = ldexpf(1.0f, ef);
         scg = ldexpf(1.0f, eg);
     }
-    auto put = [&](auto qc, double fnew, double gnew) {
+    auto put = [&](auto qc, R fnew, R gnew) {
         constexpr int q = decltype(qc)::value;
         const int64_t e = int64_t(q) * A.cs + cell * kCellStride<T>;
         if constexpr (kIsHalf<T>) {
-            store_pair(fo, go, e, to_half(fnew, sh.f(q), scf, mf), to_half(gnew, sh.g(q), scg, mg));
+            store_pair(fo, go, e, to_half(fnew, R(sh.f(q)), scf, mf), to_half(gnew, R(sh.g(q)), scg, mg));
         } else {
             store_pair(fo, go, e, narrow<T>(fnew, sh.f(q)), narrow<T>(gnew, sh.g(q)));
         }
@@ -535,11 +543,11 @@ __device__ __forceinline__ void collide_body(
                 store_pair(fo, go, e, narrow<T>(-1.0, sh.f(q)), narrow<T>(-1.0, sh.g(q)));
         });
     } else if (in) {
-        double fv[NQ], gv[NQ];
+        R fv[NQ], gv[NQ];
         pull_cell<BCX>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
-        Moments mo;
+        MomentsR<R> mo;
         accumulate_moments(fv, gv, mo);
-        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        const MacroR<R> M = macro_from_t<R>(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
         // gradient(): neighbour inside the domain box and fluid
         const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
         const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
@@ -551,7 +559,35 @@ __device__ __forceinline__ void collide_body(
                                               ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
         const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
                                               zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
-        collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) { put(qc, fnew, gnew); });
+        if constexpr (std::is_same<R, double>::value) {
+            collide_cell_t<R>(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, R fnew, R gnew) { put(qc, fnew, gnew); });
+        } else {
+            // FP32 equilibria; their sums would miss rho and E by the FP32
+            // rounding of the coefficients, the same for similar cells
+            // every step.  The rest direction's equilibria are rho and E
+            // less the FP64 sums of the other 26, so that each cell's
+            // equilibria carry its mass and energy; it is relaxed last,
+            // which keeps only the two running sums live.
+            const CollideCoeffsR<R> C = collide_coeffs_t<R>(mo, M, R(dqx), R(dqy), R(dqz), P);
+            const R omega = C.omega;
+            // rho and E as FP64 sums of the pulled values
+            double rho_d = 0.0, e_d = 0.0;
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                rho_d += double(fv[q]);
+                e_d += double(gv[q]);
+            });
+            double sf = 0.0, sg = 0.0;
+            static_for<NQ - 1>([&](auto pc) {
+                constexpr int q = decltype(pc)::value + 1;
+                const R fe = feq_of<q>(C), ge = geq_of<q>(C);
+                sf += double(fe);
+                sg += double(ge);
+                put(std::integral_constant<int, q>{}, fv[q] + omega * (fe - fv[q]), gv[q] + omega * (ge - gv[q]));
+            });
+            const R fe0 = R(rho_d - sf), ge0 = R(e_d - sg);
+            put(std::integral_constant<int, 0>{}, fv[0] + omega * (fe0 - fv[0]), gv[0] + omega * (ge0 - gv[0]));
+        }
     }
     if constexpr (kIsHalf<T>) half_block_stats(h16, ef, eg, mf, mg);
 }
@@ -571,6 +607,18 @@ MRB_COLLIDE_KERNEL(k_collide, 4)
 MRB_COLLIDE_KERNEL(k_collide_y8, 8)
 MRB_COLLIDE_KERNEL(k_collide_y2, 2)
 #undef MRB_COLLIDE_KERNEL
+
+// scaled FP16 storage with FP32 arithmetic in pass 2 (pass 1 and the
+// gradient stay FP64): the FP16 rounding of the stored deviations, about
+// 2.4e-4 of each, dominates the FP32 rounding of the populations
+template <class T, int BCX>
+__global__ void __launch_bounds__(256) k_collide_h32(
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    Half16 h16, CollideParams P)
+{
+    collide_body<T, BCX, float>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+}
 
 // at most 168 registers (3 waves per SIMD), some values spilled to scratch
 template <class T, int BCX>
@@ -853,6 +901,22 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
         if (ev) MRB_HIP(hipEventRecord(ev[2], s));
         return MRB_OK;
     }
+    if (variant == kHalf32Variant) {
+        if constexpr (kIsHalf<T> && (BCX & kBcxPressure) == 0) {
+            if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+            k_qpass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
+                                                                   L->cp);
+            MRB_HIP(hipGetLastError());
+            if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+            k_collide_h32<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A,
+                                                                         L->bcdev, sh, h16, L->cp);
+            MRB_HIP(hipGetLastError());
+            if (ev) MRB_HIP(hipEventRecord(ev[2], s));
+            return MRB_OK;
+        } else {
+            return set_error(MRB_E_UNSUPPORTED, "k_collide_h32: scaled FP16 storage without PRESSURE faces only");
+        }
+    }
     if (ev) MRB_HIP(hipEventRecord(ev[0], s));
     k_qpass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
     MRB_HIP(hipGetLastError());
@@ -895,7 +959,8 @@ static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 }
 
 const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_qpass+k_collide_w3", "k_onepass",
-                                                           "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2"};
+                                                           "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2",
+                                                           "k_qpass+k_collide_h32"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 

@@ -251,6 +251,7 @@ struct mrb_level {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // hot-path kernel variant of the compact layout (compact_variant_name)
     int hot = 0;
+    bool hot_default = false;  // hot is the storage's default, not a caller's choice
     // two-level link, held by the fine level (mrb_amr_link)
     mrb_level *amr_crse = nullptr;
     void *d_coal = nullptr;  // mrb::CoalesceEntry[n_coal]
@@ -320,10 +321,11 @@ int fused_step(mrb_level *L, int variant, hipStream_t s);
 // single pass that recomputes the neighbours' Q_corr; the rest the fused
 // variants; launch reads the current buffers, writes the spare ones, does
 // not flip
-constexpr int kNumTwoPassVariants = 5;
+constexpr int kNumTwoPassVariants = 6;
 constexpr int kOnePassVariant = 2;
 constexpr int kRows8Variant = 3;  // two passes on 64x8 blocks
 constexpr int kRows2Variant = 4;  // two passes on 64x2 blocks
+constexpr int kHalf32Variant = 5; // scaled FP16 storage, FP32 arithmetic in pass 2
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s);
 int compact_variant_launch(mrb_level *L, int index, hipStream_t s);
 const char *compact_variant_name(int index);

@@ -1048,10 +1048,14 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
     L->layout = desc->layout;
     L->nlattice = desc->nlattice;
     L->storage = desc->storage;
-    // the fastest block shape of the two-pass step per storage at 256^3
-    // (64x8 rows for FP64, 64x2 for shifted FP32, 64x4 for scaled FP16);
-    // mrb_autotune re-selects on the live state
-    L->hot = L->storage == MRB_STORAGE_F64 ? kRows8Variant : L->storage == MRB_STORAGE_F32_SHIFTED ? kRows2Variant : 0;
+    // the fastest step variant per storage at 256^3: 64x8-cell blocks for
+    // FP64, 64x2 for shifted FP32, the FP32-arithmetic collide for scaled
+    // FP16 (FP64 arithmetic on levels with PRESSURE faces, mrb_set_bc);
+    // mrb_set_hot_kernel and mrb_autotune select others
+    L->hot = L->storage == MRB_STORAGE_F64            ? kRows8Variant
+             : L->storage == MRB_STORAGE_F32_SHIFTED ? kRows2Variant
+                                                      : kHalf32Variant;
+    L->hot_default = true;
     if (L->layout == MRB_LAYOUT_COMPACT) {
         if (desc->nbox != 1 || desc->ng != 0)
             return set_error(MRB_E_ARG, "compact layout is one box without ghost cells");
@@ -1293,6 +1297,12 @@ int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
         return set_error(MRB_E_ARG, "parabolic velocity bc normal_dir must be 0, 1 or 2");
     L->bc = *bc;
     compute_bc_values(L);
+    // the FP32-arithmetic collide has no PRESSURE faces: such a level takes
+    // the FP64-arithmetic two-pass step (and returns to the FP32 one when
+    // its faces change back)
+    if (L->storage == MRB_STORAGE_F16_SCALED && L->hot == kHalf32Variant && L->bcdev.pdir >= 0) L->hot = 0;
+    else if (L->storage == MRB_STORAGE_F16_SCALED && L->hot == 0 && L->hot_default && L->bcdev.pdir < 0)
+        L->hot = kHalf32Variant;
     if (L->bc.velbc_kind >= 2 && L->layout == MRB_LAYOUT_COMPACT) {
         for (int f = 0; f < 6; ++f)
             if (L->bc.bc_type[f] == MRB_BC_VELOCITY && !L->bcdev.tab[f])
@@ -1883,6 +1893,7 @@ int mrb_set_hot_kernel(mrb_level *L, int index)
     if (index < 0 || index >= mrb_hot_kernel_count()) return set_error(MRB_E_ARG, "no such kernel variant");
     if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
     L->hot = index;
+    L->hot_default = false;
     return MRB_OK;
 }
 
@@ -1969,6 +1980,7 @@ int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best
         }
     }
     L->hot = arg;
+    L->hot_default = false;
     if (best) *best = arg;
     return MRB_OK;
 }

@@ -14,11 +14,12 @@ def test_multi_gpu_request_exits_with_a_message():
 
 def test_traffic_from_committed_profile_covers_every_storage():
     """The summary covers the kernels each storage's level starts on (64x8
-    blocks for FP64, 64x2 for shifted FP32, 64x4 for scaled FP16)."""
+    blocks for FP64, 64x2 for shifted FP32, the FP32-arithmetic collide
+    for scaled FP16)."""
     cells = 256 ** 3
     t64 = bench.measured_traffic("f64", ["k_qpass_y8", "k_collide_y8"])
     t32 = bench.measured_traffic("f32s", ["k_qpass_y2", "k_collide_y2"])
-    t16 = bench.measured_traffic("f16s", ["k_qpass", "k_collide"])
+    t16 = bench.measured_traffic("f16s", ["k_qpass", "k_collide_h32"])
     # at least the algorithmic bytes (read + write of every population once)
     assert t64 >= 864 * cells and t32 >= 432 * cells and t16 >= 216 * cells
     assert bench.measured_traffic("f64", ["k_not_profiled"]) is None

@@ -189,9 +189,14 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *   2  k_onepass: one pass that recomputes the neighbours' Q_corr;
  *   3  the two-pass step on 64x8-cell blocks;
  *   4  the two-pass step on 64x2-cell blocks;
- *   5+ the single-pass z-sweep with different tile shapes.
- * All variants give the same populations bit for bit.  A level starts on
- * variant 3 (FP64 storage), 4 (shifted FP32) or 0 (scaled FP16). */
+ *   5  scaled FP16 storage only, no PRESSURE faces: the collide pass in
+ *      FP32 arithmetic (FP64 sums of rho, E and the equilibria; the mass
+ *      and energy residuals go to the rest direction);
+ *   6+ the single-pass z-sweep with different tile shapes.
+ * Variants other than 5 give the same populations bit for bit.  A level
+ * starts on variant 3 (FP64 storage), 4 (shifted FP32) or 5 (scaled FP16;
+ * 0 while it has PRESSURE faces).  A variant that does not support the
+ * level returns MRB_E_UNSUPPORTED from the step. */
 int mrb_hot_kernel_count(void);
 const char *mrb_hot_kernel_variant_name(int index);
 int mrb_set_hot_kernel(mrb_level *lvl, int index);

@@ -267,6 +267,10 @@ def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
     runs, cons = {}, {}
     for storage in ("f64", "f16s"):
         E = configs.build_level(cfg, layout="compact", storage=storage)
+        if storage == "f16s":
+            # the FP32-arithmetic collide, or FP64 arithmetic with PRESSURE faces
+            pressure = case.startswith("pressure")
+            assert E.hot_kernel_name() == ("k_qpass+k_collide" if pressure else "k_qpass+k_collide_h32")
         E.advance(n)
         runs[storage] = _velocity(E)
         cons[storage] = E.conservation()

@@ -451,18 +451,20 @@ __device__ __forceinline__ void qpass_body(
     Q[2 * A.N + cell] = M.Qz;
 }
 
-// pass 1 on blocks of 64 x BY cells (one name per block shape)
-#define MRB_QPASS_KERNEL(NAME, BY)                                                                            \
+// pass 1 on blocks of BX x BY cells (one name per block shape)
+#define MRB_QPASS_KERNEL(NAME, BX, BY)                                                                        \
     template <class T, int BCX>                                                                               \
-    __global__ void __launch_bounds__(64 * BY) NAME(                                                          \
+    __global__ void __launch_bounds__(BX * BY) NAME(                                                          \
         const T *__restrict__ f, const T *__restrict__ g, const uint32_t *__restrict__ mask,                  \
         double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, Half16 h16, CollideParams P)           \
     {                                                                                                         \
         qpass_body<T, BCX>(f, g, mask, Q, A, bc, sh, h16, P);                                                 \
     }
-MRB_QPASS_KERNEL(k_qpass, 4)
-MRB_QPASS_KERNEL(k_qpass_y8, 8)
-MRB_QPASS_KERNEL(k_qpass_y2, 2)
+MRB_QPASS_KERNEL(k_qpass, 64, 4)
+MRB_QPASS_KERNEL(k_qpass_y8, 64, 8)
+MRB_QPASS_KERNEL(k_qpass_y2, 64, 2)
+MRB_QPASS_KERNEL(k_qpass_x128, 128, 2)
+MRB_QPASS_KERNEL(k_qpass_x256, 256, 1)
 #undef MRB_QPASS_KERNEL
 
 // FP16 storage: the block's largest |population - shift| per lattice goes
@@ -592,20 +594,22 @@ __device__ __forceinline__ void collide_body(
     if constexpr (kIsHalf<T>) half_block_stats(h16, ef, eg, mf, mg);
 }
 
-// pass 2 on blocks of 64 x BY cells, register budget as the compiler
+// pass 2 on blocks of BX x BY cells, register budget as the compiler
 // chooses (2 waves per SIMD); one name per block shape
-#define MRB_COLLIDE_KERNEL(NAME, BY)                                                                          \
+#define MRB_COLLIDE_KERNEL(NAME, BX, BY)                                                                      \
     template <class T, int BCX>                                                                               \
-    __global__ void __launch_bounds__(64 * BY) NAME(                                                          \
+    __global__ void __launch_bounds__(BX * BY) NAME(                                                          \
         const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,             \
         const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh, \
         Half16 h16, CollideParams P)                                                                          \
     {                                                                                                         \
         collide_body<T, BCX>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);                                       \
     }
-MRB_COLLIDE_KERNEL(k_collide, 4)
-MRB_COLLIDE_KERNEL(k_collide_y8, 8)
-MRB_COLLIDE_KERNEL(k_collide_y2, 2)
+MRB_COLLIDE_KERNEL(k_collide, 64, 4)
+MRB_COLLIDE_KERNEL(k_collide_y8, 64, 8)
+MRB_COLLIDE_KERNEL(k_collide_y2, 64, 2)
+MRB_COLLIDE_KERNEL(k_collide_x128, 128, 2)
+MRB_COLLIDE_KERNEL(k_collide_x256, 256, 1)
 #undef MRB_COLLIDE_KERNEL
 
 // scaled FP16 storage with FP32 arithmetic in pass 2 (pass 1 and the
@@ -772,7 +776,10 @@ CompactArgs make_args(const mrb_level *L)
     return A;
 }
 
-dim3 grid_of(const CompactArgs &A, int by = 4) { return dim3((A.nx + 63) / 64, (A.ny + by - 1) / by, A.nz); }
+dim3 grid_of(const CompactArgs &A, int by = 4, int bx = 64)
+{
+    return dim3((A.nx + bx - 1) / bx, (A.ny + by - 1) / by, A.nz);
+}
 
 }  // namespace
 
@@ -880,6 +887,29 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
             return MRB_OK;
         }
     }
+    if (variant == kX128Variant || variant == kX256Variant) {
+        // the same two passes on 128x2 or 256x1 blocks (two or four waves
+        // per row): fewer cache lines shared with the blocks beside in x,
+        // which run on other XCDs
+        const int bx = variant == kX128Variant ? 128 : 256, by = variant == kX128Variant ? 2 : 1;
+        const dim3 grid = grid_of(A, by, bx), block(bx, by, 1);
+        if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+        if (bx == 128)
+            k_qpass_x128<T, BCX><<<grid, block, 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+        else
+            k_qpass_x256<T, BCX><<<grid, block, 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+        MRB_HIP(hipGetLastError());
+        if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+        if (bx == 128)
+            k_collide_x128<T, BCX><<<grid, block, 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
+                                                           L->cp);
+        else
+            k_collide_x256<T, BCX><<<grid, block, 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
+                                                           L->cp);
+        MRB_HIP(hipGetLastError());
+        if (ev) MRB_HIP(hipEventRecord(ev[2], s));
+        return MRB_OK;
+    }
     if (variant == kRows8Variant || variant == kRows2Variant) {
         // the same two passes on 64x8 or 64x2 blocks
         const int by = variant == kRows8Variant ? 8 : 2;
@@ -960,7 +990,9 @@ static int two_pass(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 
 const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_qpass+k_collide_w3", "k_onepass",
                                                            "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2",
-                                                           "k_qpass+k_collide_h32"};
+                                                           "k_qpass+k_collide_h32",
+                                                           "k_qpass_x128+k_collide_x128",
+                                                           "k_qpass_x256+k_collide_x256"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 

@@ -13,11 +13,11 @@ def test_multi_gpu_request_exits_with_a_message():
 
 
 def test_traffic_from_committed_profile_covers_every_storage():
-    """The summary covers the kernels each storage's level starts on (64x8
+    """The summary covers the kernels each storage's level starts on (256x1
     blocks for FP64, 64x2 for shifted FP32, the FP32-arithmetic collide
     for scaled FP16)."""
     cells = 256 ** 3
-    t64 = bench.measured_traffic("f64", ["k_qpass_y8", "k_collide_y8"])
+    t64 = bench.measured_traffic("f64", ["k_qpass_x256", "k_collide_x256"])
     t32 = bench.measured_traffic("f32s", ["k_qpass_y2", "k_collide_y2"])
     t16 = bench.measured_traffic("f16s", ["k_qpass", "k_collide_h32"])
     # at least the algorithmic bytes (read + write of every population once)

@@ -500,9 +500,10 @@ def test_ic_plugin_workloads_match_oracle(oracle, engine, case, layout):
 @pytest.mark.parametrize("storage", ["f64", "f32s", "f16s"])
 @pytest.mark.parametrize("case", ["channel", "odd", "channel_profile", "pressure_walled", "outlet"])
 def test_step_variants_equal_two_pass_bitwise(engine, case, storage):
-    """The other block shapes of the two passes (64x8, 64x2) and k_onepass,
-    which recomputes the neighbours' Q_corr from their pulls instead of
-    reading pass 1's array, do the same operations in the same order: their
+    """The other block shapes of the two passes (64x8, 64x2, 128x2, 256x1)
+    and k_onepass, which recomputes the neighbours' Q_corr
+    from their pulls instead of reading pass 1's array, do the same
+    operations in the same order: their
     populations equal the 64x4 two-pass step's bit for bit (partial tiles,
     the sphere, inlet profiles, pressure ghosts)."""
     if case == "channel":
@@ -515,7 +516,8 @@ def test_step_variants_equal_two_pass_bitwise(engine, case, storage):
         cfg, n = configs.pressure_case("walled", (24, 16, 32)), 20
     else:
         cfg, n = configs.pressure_case("outlet", (32, 16, 16)), 20
-    names = ["k_qpass+k_collide", "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2"]
+    names = ["k_qpass+k_collide", "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2",
+             "k_qpass_x128+k_collide_x128", "k_qpass_x256+k_collide_x256"]
     if storage != "f16s":
         names.append("k_onepass")
     out = {}

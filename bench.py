@@ -392,6 +392,9 @@ def main() -> int:
     ap.add_argument("--storage", default="f64", choices=["f64", "f32s", "f16s"])
     ap.add_argument("--n", type=int, default=256, help="domain edge (256 = the named workload)")
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--kernel", default=None,
+                    help="channel: step variant to run (a name from Level.hot_kernel_variants(); default: the "
+                         "level's own, the fastest for its storage at 256^3)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle timing")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -437,6 +440,8 @@ def main() -> int:
                                                                       128.0 * scale), tau=0.55, max_box=64)
     t0 = time.perf_counter()
     lvl = configs.build_level(cfg, layout="compact", storage=args.storage)
+    if args.kernel:
+        lvl.set_hot_kernel(args.kernel)
     log(f"level built in {time.perf_counter() - t0:.2f}s; kernel: {lvl.hot_kernel_name()}")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream

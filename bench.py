@@ -488,7 +488,7 @@ def main() -> int:
                 "max_rel_velocity_error": du,
                 "max_rel_density_error": dr,
                 "compared_after_steps": args.warmup + args.steps + args.profile_steps,
-                "arithmetic": "f32" if any(k.endswith("_h32") for k in rr["per_kernel_ms"]) else "f64",
+                "arithmetic": "f32" if any("_h32" in k for k in rr["per_kernel_ms"]) else "f64",
                 "bytes_per_update": br,
                 "achieved_gbs": round(br * cells / (kms * 1e-3) / 1e9, 1),
                 "traffic": measured_traffic(st, rr["per_kernel_ms"]) if n == 256 else None,

@@ -194,8 +194,10 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *      and energy residuals go to the rest direction);
  *   6  the two-pass step on 128x2-cell blocks;
  *   7  the two-pass step on 256x1-cell blocks;
- *   8+ the single-pass z-sweep with different tile shapes.
- * Variants other than 5 give the same populations bit for bit.  A level
+ *   8  variant 5 on 256x1-cell blocks;
+ *   9+ the single-pass z-sweep with different tile shapes.
+ * Variants other than 5 and 8 give the same populations bit for bit, and
+ * 5 and 8 give the same as each other.  A level
  * starts on variant 7 (FP64 storage), 4 (shifted FP32) or 5 (scaled FP16;
  * 0 while it has PRESSURE faces).  A variant that does not support the
  * level returns MRB_E_UNSUPPORTED from the step. */

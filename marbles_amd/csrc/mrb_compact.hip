@@ -623,6 +623,15 @@ __global__ void __launch_bounds__(256) k_collide_h32(
 {
     collide_body<T, BCX, float>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
+// the same on 256x1 blocks
+template <class T, int BCX>
+__global__ void __launch_bounds__(256) k_collide_h32_x256(
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    Half16 h16, CollideParams P)
+{
+    collide_body<T, BCX, float>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+}
 
 // at most 168 registers (3 waves per SIMD), some values spilled to scratch
 template <class T, int BCX>
@@ -931,15 +940,23 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
         if (ev) MRB_HIP(hipEventRecord(ev[2], s));
         return MRB_OK;
     }
-    if (variant == kHalf32Variant) {
+    if (variant == kHalf32Variant || variant == kHalf32X256Variant) {
         if constexpr (kIsHalf<T> && (BCX & kBcxPressure) == 0) {
             if (ev) MRB_HIP(hipEventRecord(ev[0], s));
-            k_qpass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
-                                                                   L->cp);
+            if (variant == kHalf32Variant)
+                k_qpass<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh,
+                                                                       h16, L->cp);
+            else
+                k_qpass_x256<T, BCX><<<grid_of(A, 1, 256), dim3(256, 1, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A,
+                                                                                   L->bcdev, sh, h16, L->cp);
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[1], s));
-            k_collide_h32<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A,
-                                                                         L->bcdev, sh, h16, L->cp);
+            if (variant == kHalf32Variant)
+                k_collide_h32<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A,
+                                                                             L->bcdev, sh, h16, L->cp);
+            else
+                k_collide_h32_x256<T, BCX><<<grid_of(A, 1, 256), dim3(256, 1, 1), 0, s>>>(
+                    f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[2], s));
             return MRB_OK;
@@ -992,7 +1009,8 @@ const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_
                                                            "k_qpass_y8+k_collide_y8", "k_qpass_y2+k_collide_y2",
                                                            "k_qpass+k_collide_h32",
                                                            "k_qpass_x128+k_collide_x128",
-                                                           "k_qpass_x256+k_collide_x256"};
+                                                           "k_qpass_x256+k_collide_x256",
+                                                           "k_qpass_x256+k_collide_h32_x256"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 

@@ -321,13 +321,14 @@ int fused_step(mrb_level *L, int variant, hipStream_t s);
 // single pass that recomputes the neighbours' Q_corr; the rest the fused
 // variants; launch reads the current buffers, writes the spare ones, does
 // not flip
-constexpr int kNumTwoPassVariants = 8;
+constexpr int kNumTwoPassVariants = 9;
 constexpr int kOnePassVariant = 2;
 constexpr int kRows8Variant = 3;  // two passes on 64x8 blocks
 constexpr int kRows2Variant = 4;  // two passes on 64x2 blocks
 constexpr int kHalf32Variant = 5; // scaled FP16 storage, FP32 arithmetic in pass 2
 constexpr int kX128Variant = 6;   // two passes on 128x2 blocks
 constexpr int kX256Variant = 7;   // two passes on 256x1 blocks
+constexpr int kHalf32X256Variant = 8; // variant 5 on 256x1 blocks
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s);
 int compact_variant_launch(mrb_level *L, int index, hipStream_t s);
 const char *compact_variant_name(int index);

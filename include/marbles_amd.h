@@ -195,12 +195,22 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *   6  the two-pass step on 128x2-cell blocks;
  *   7  the two-pass step on 256x1-cell blocks;
  *   8  variant 5 on 256x1-cell blocks;
- *   9+ the single-pass z-sweep with different tile shapes.
- * Variants other than 5 and 8 give the same populations bit for bit, and
- * 5 and 8 give the same as each other.  A level
- * starts on variant 7 (FP64 storage), 4 (shifted FP32) or 5 (scaled FP16;
- * 0 while it has PRESSURE faces).  A variant that does not support the
- * level returns MRB_E_UNSUPPORTED from the step. */
+ *   9  shifted FP32 storage only, no PRESSURE faces: the collide pass in
+ *      FP32 arithmetic on the stored deviations from the rest state (FP64
+ *      moments and equilibrium coefficients; mass and energy residuals on
+ *      the rest direction), 64x2-cell blocks;
+ *  10  variant 9 on 256x1-cell blocks;
+ *  11  k_sweep: FP64 storage with rows of at most 256 cells, one pass as a
+ *      z-sweep with one resident block per x-row (cooperative launch;
+ *      MRB_E_UNSUPPORTED when the rows do not fit the GPU at once);
+ *  12+ the single-pass LDS z-sweep with different tile shapes.
+ * The FP64-arithmetic variants give the same populations bit for bit; 5
+ * and 8 give the same as each other, and 9 and 10.  A level
+ * starts on variant 7 (FP64 storage), 4 (shifted FP32) or 5 (scaled FP16).
+ * While a level has a PRESSURE face, variants 5 and 9 are replaced by 0 and
+ * 4, 8 and 10 by 7 (mrb_set_bc), and restored when the faces change back.
+ * A variant that does not support the level returns MRB_E_UNSUPPORTED from
+ * the step. */
 int mrb_hot_kernel_count(void);
 const char *mrb_hot_kernel_variant_name(int index);
 int mrb_set_hot_kernel(mrb_level *lvl, int index);
@@ -208,7 +218,11 @@ int mrb_set_hot_kernel(mrb_level *lvl, int index);
  * current state without advancing it -- each trial reads the current
  * buffers and writes the spare ones -- and selects the fastest.  Compact
  * layout: the kernel variants above (a variant that does not support the
- * level reports -1).  Fab layout: the persistent-block count of the batched
+ * level reports -1; on reduced-precision levels only the variants with
+ * the current variant's collide arithmetic, FP32 for 5, 8, 9 and 10 and
+ * FP64 for the others, are tried, and the FP16 saturation count is left
+ * as it was).
+ * Fab layout: the persistent-block count of the batched
  * two-pass step over the box work-list.  ms_out[i] (may be NULL, else
  * mrb_autotune_count(lvl) entries) receives variant i's mean time and *best
  * its index.  Synchronous. */

@@ -148,8 +148,16 @@ SIGNATURES = {
 _lib = None
 
 
+MRB_E_UNSUPPORTED = -3
+
+
 class MrbError(RuntimeError):
-    pass
+    """An error status of the C ABI; `code` is the MRB_E_* value (None when
+    the library could not be loaded)."""
+
+    def __init__(self, msg: str, code: int | None = None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load() -> ctypes.CDLL:
@@ -172,4 +180,4 @@ def load() -> ctypes.CDLL:
 def check(rc: int) -> None:
     if rc != 0:
         msg = load().mrb_last_error().decode()
-        raise MrbError(f"marbles_amd error {rc}: {msg}")
+        raise MrbError(f"marbles_amd error {rc}: {msg}", rc)

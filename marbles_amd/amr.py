@@ -31,7 +31,7 @@ class Hierarchy:
         for lvl, dt in ((self.crse, 1.0), (self.fine, 0.5)):
             lvl.set_phys(cfg["nu"], cfg.get("alpha"), dt=dt)
             lvl.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
-            lvl.flags_all_regular()
+            configs.set_eb_flags(lvl, cfg.get("eb", {"type": "all_regular"}))
             lvl.init_fields(*configs.level_fields(cfg, lvl.ncell, lvl.boxes, ng), adiabatic_exponent=gamma)
         check(_lib.load().mrb_amr_link(self.crse.h, self.fine.h))
 
@@ -57,7 +57,7 @@ class Hierarchy:
                     layout="fab")
         new.set_phys(cfg["nu"], cfg.get("alpha"), dt=0.5)
         new.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
-        new.flags_all_regular()
+        configs.set_eb_flags(new, cfg.get("eb", {"type": "all_regular"}))
         lib = _lib.load()
         check(lib.mrb_amr_regrid_fill(self.crse.h, old.h, new.h))
         check(lib.mrb_amr_link(self.crse.h, new.h))

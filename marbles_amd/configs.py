@@ -112,6 +112,39 @@ def channel_profile(ncell=(64, 32, 32), kind: str = "channel", tau: float = 0.55
     }
 
 
+def slip_channel(ncell=(24, 40, 16), tau: float = 0.6, u: float = 0.04, temperature: float = 0.2,
+                 sphere: bool = True, max_box: int | None = None) -> dict:
+    """Slip walls (Tests/test_files/single_rotated_box deck: lbm.bc_lo =
+    6 2 0, lbm.bc_hi = 6 5 0): SLIPWALLXNORMAL on both x faces, a velocity
+    inlet along y at y-lo, zero-gradient outflow at y-hi, periodic z; a
+    sphere stands in for the deck's rotated box.  Starts at rest."""
+    gamma = 5.0 / 3.0
+    mach = u / math.sqrt(gamma * temperature)
+    vbc = {"kind": "constant", "dir": 1, "Mach_ref": mach, "initial_density": 1.0,
+           "initial_temperature": temperature, "adiabatic_exponent": gamma, "mean_molecular_mass": AIR_MOLAR_MASS}
+    eb = {"type": "all_regular"}
+    if sphere:
+        eb = {"type": "sphere", "center": (ncell[0] / 2, ncell[1] / 3, ncell[2] / 2), "radius": 4.0,
+              "has_fluid_inside": 0}
+    return {
+        "name": "slip_channel_{}x{}x{}".format(*ncell),
+        "ncell": tuple(ncell),
+        "periodic": (0, 0, 1),
+        "prob_lo": (0.0, 0.0, 0.0),
+        "prob_hi": tuple(float(n) for n in ncell),
+        "bc_lo": (6, 2, 0),
+        "bc_hi": (6, 5, 0),
+        "velocity_bc": vbc,
+        "nu": (tau - 0.5) * temperature,
+        "tau": tau,
+        "T0": temperature,
+        "ic": {"type": "constant", "density": 1.0, "mach": (0.0, 0.0, 0.0),
+               "temperature": temperature, "adiabatic_exponent": gamma},
+        "eb": eb,
+        "max_box": max_box,
+    }
+
+
 def pressure_case(kind: str, ncell=(40, 20, 24), tau: float = 0.6, u: float = 0.04, temperature: float = 0.2,
                   max_box: int | None = None) -> dict:
     """Workloads with PRESSURE faces (pressure_bc_f / pressure_bc_g,
@@ -370,11 +403,14 @@ def refine_boxes(coarse_boxes, fraction: float, tol: float = 0.005, max_box: int
 
 def amr_two_level(ncell=(256, 256, 256), edges=(8, 16, 32, 64), nbox: int | None = None,
                   fine_fraction: float = 0.2, tau: float = 0.6, prandtl: float = 0.71,
-                  temperature: float = 0.2, noise: float = 0.05, seed: int = 3) -> dict:
+                  temperature: float = 0.2, noise: float = 0.05, seed: int = 3, eb: dict | None = None) -> dict:
     """Two levels, ratio 2, periodic: seeded uneven coarse boxes with edges in
     `edges`, a fine level over a seeded 20% of them, thermal f+g with
     nu = (tau - 1/2) R T0 and alpha = nu / Pr, at rest with temperature
     T0 (1 + noise * n(x)), n a seeded smooth field with max |n| <= 1.
+    `eb` (e.g. a sphere in physical coordinates) classifies the cells of
+    both levels, each at its own resolution (LBM::initialize_is_fluid per
+    level).
 
     T0 = 0.2: the reference thermal model is linearly unstable at T0 = 0.3
     with tau = 0.6 (a grid-scale energy mode grows), stable at T0 = 0.2 on
@@ -397,12 +433,23 @@ def amr_two_level(ncell=(256, 256, 256), edges=(8, 16, 32, 64), nbox: int | None
         "T0": temperature,
         "ic": {"type": "thermal_noise", "density": 1.0, "temperature": temperature, "noise": noise,
                "seed": seed + 2, "adiabatic_exponent": 5.0 / 3.0},
-        "eb": {"type": "all_regular"},
+        "eb": dict(eb) if eb else {"type": "all_regular"},
         "coarse_boxes": coarse,
         "fine_boxes": fine,
         "fine_fraction": frac,
         "ratio": 2,
     }
+
+
+def set_eb_flags(lvl, eb: dict) -> None:
+    """Cell classification of a level (engine or oracle) from a workload's
+    `eb` entry."""
+    if eb["type"] == "sphere":
+        lvl.flags_sphere(eb["center"], eb["radius"], eb.get("has_fluid_inside", 0))
+    elif eb["type"] == "all_regular":
+        lvl.flags_all_regular()
+    else:
+        raise ValueError("unsupported eb type " + str(eb["type"]))
 
 
 def smooth_noise(ic: dict, prob_lo, prob_hi, x, y, z):
@@ -503,6 +550,24 @@ def spectral_velocity(n: int, kmax: int, amplitude: float, seed: int):
     u = [np.fft.irfftn(c, s=(n, n, n), axes=(0, 1, 2)) for c in uh]
     scale = amplitude / max(float(np.abs(c).max()) for c in u)
     return [c * scale for c in u]
+
+
+def spectral_fab_fields(cfg: dict, boxes, ng: int):
+    """(rho, u, v, w, T) over every fab cell (ghosts included, box order, x
+    fastest) for the spectral initial condition: the periodic field evaluated
+    at each fab cell's periodic image."""
+    import numpy as np
+
+    ic = cfg["ic"]
+    n = cfg["ncell"][0]
+    vel = spectral_velocity(n, ic["kmax"], ic["amplitude"], ic["seed"])
+    parts = [[], [], []]
+    for lo, hi in boxes:
+        ax = [np.arange(lo[d] - ng, hi[d] + ng + 1) % n for d in range(3)]
+        for c in range(3):
+            parts[c].append(vel[c][np.ix_(ax[2], ax[1], ax[0])].ravel())
+    u, v, w = (np.concatenate(p) for p in parts)
+    return np.full(u.size, ic["density"]), u, v, w, np.full(u.size, ic["temperature"])
 
 
 def compact_fields(cfg: dict):

@@ -93,9 +93,9 @@ struct MomentsR {
 };
 using Moments = MomentsR<double>;
 
-template <class T, class R>
-__host__ __device__ __forceinline__ void accumulate_moments(
-    const T* __restrict__ f, const T* __restrict__ g, MomentsR<R>& m)
+// f, g: anything indexable by q (arrays, pointers, row views of LDS)
+template <class FA, class GA, class R>
+__host__ __device__ __forceinline__ void accumulate_moments(const FA& f, const GA& g, MomentsR<R>& m)
 {
     m.rho = m.jx = m.jy = m.jz = R(0);
     m.pxx = m.pyy = m.pzz = m.pxy = m.pxz = m.pyz = R(0);
@@ -346,9 +346,9 @@ __host__ __device__ __forceinline__ auto geq_of(const Co& C)
 // Equilibria of LBM::macrodata_to_equilibrium (LBM.cpp:659-758) hoisted per
 // cell, followed by the BGK relaxation of LBM::relax_f_to_equilibrium
 // (LBM.cpp:793-801), in arithmetic type R.  fo/go may alias f/g.
-template <class R, class TIn, class FOut>
+template <class R, class FA, class GA, class FOut>
 __host__ __device__ __forceinline__ void collide_cell_t(
-    const TIn* __restrict__ f, const TIn* __restrict__ g, const MomentsR<R>& m, const MacroR<R>& M, R dqx,
+    const FA& f, const GA& g, const MomentsR<R>& m, const MacroR<R>& M, R dqx,
     R dqy, R dqz, const CollideParams& P, FOut&& out)
 {
     const CollideCoeffsR<R> C = collide_coeffs_t<R>(m, M, dqx, dqy, dqz, P);

@@ -52,50 +52,106 @@ __global__ void __launch_bounds__(kAmrBlock) k_coalesce(
     }
 }
 
-// Conservative linear interpolation with monotonised-central limited
-// slopes: child = ((c + s_x o_x) + s_y o_y) + s_z o_z, o = -+1/4.
+// Coarse-to-fine interpolation of FillPatchTwoLevels / InterpFromCoarseLevel
+// with amrex::cell_cons_interp (FillPatchOps.H:124-128,174-177,190): AMReX's
+// conservative linear scheme without linear limiting, ratio 2.  Per coarse
+// parent and component (k_explode_slopes): monotonised-central slopes, the
+// centred difference replaced at an ext_dir face (every non-periodic face,
+// LBM.cpp:108-150) by the one-sided second-order form through the value
+// beyond the face; then one factor alpha <= 1 so that no child leaves the
+// min/max of the 3x3x3 neighbourhood.  The coarse patch AMReX interpolates
+// from holds -1 beyond a non-periodic face: BCFill only writes the boundary
+// value where iv + c_q lies inside the patch shrunk by the level's 3 ghost
+// cells (BC.H:375-380), which a ghost's neighbour never does.  Per child
+// (k_explode): ((c + o_x s_x) + o_y s_y) + o_z s_z with o = -+1/4.
+// Every load is issued unconditionally: the fab's ghost layers (ng >= 2,
+// checked when the map is built) hold P +- 2 in every direction.
+__global__ void __launch_bounds__(kAmrBlock) k_explode_slopes(
+    const double *__restrict__ crse, const BoxDev *__restrict__ cboxes, const ExplodeParent *__restrict__ par,
+    int64_t npar, double *__restrict__ out)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= npar * NQ) return;
+    const int q = int(t / npar);
+    const int64_t p = t - int64_t(q) * npar;
+    const ExplodeParent P = par[p];
+    const BoxDev &CB = cboxes[P.crse_box];
+    const double *c = crse + CB.base * NQ + int64_t(q) * CB.ncell + P.crse_local;
+    const int64_t st[3] = {1, CB.n[0], int64_t(CB.n[0]) * CB.n[1]};
+    const int fl = P.flags;
+    double u[27];
+    for (int oz = -1; oz <= 1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const double v = c[oz * st[2] + oy * st[1] + ox * st[0]];
+                const bool beyond = (ox < 0 && (fl & 1)) || (ox > 0 && (fl & 8)) || (oy < 0 && (fl & 2)) ||
+                                    (oy > 0 && (fl & 16)) || (oz < 0 && (fl & 4)) || (oz > 0 && (fl & 32));
+                u[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] = beyond ? -1.0 : v;
+            }
+    double u2p[3], u2m[3];
+    for (int d = 0; d < 3; ++d) {
+        u2p[d] = c[2 * st[d]];
+        u2m[d] = c[-2 * st[d]];
+    }
+    const double uc = u[13];
+    double s[3];
+    for (int d = 0; d < 3; ++d) {
+        const int sn = d == 0 ? 1 : (d == 1 ? 3 : 9);
+        const double um = u[13 - sn], up = u[13 + sn];
+        double dc = 0.5 * (up - um);
+        if (fl & (1 << d)) {
+            if (fl & (1 << (6 + d)))
+                dc = -16. / 15. * um + 0.5 * uc + 2. / 3. * up - 0.1 * u2p[d];
+            else
+                dc = 0.25 * (up + 5. * uc - 6. * um);
+        }
+        if (fl & (1 << (3 + d))) {
+            if (fl & (1 << (9 + d)))
+                dc = 16. / 15. * up - 0.5 * uc - 2. / 3. * um + 0.1 * u2m[d];
+            else
+                dc = -0.25 * (um + 5. * uc - 6. * up);
+        }
+        const double df = 2.0 * (up - uc);
+        const double db = 2.0 * (uc - um);
+        const double sl = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+        s[d] = copysign(1.0, dc) * fmin(sl, fabs(dc));
+    }
+    double alpha = 1.0;
+    if (s[0] != 0.0 || s[1] != 0.0 || s[2] != 0.0) {
+        const double dumax = fabs(s[0]) * 1.0 / 4.0 + fabs(s[1]) * 1.0 / 4.0 + fabs(s[2]) * 1.0 / 4.0;
+        double umax = uc, umin = uc;
+        for (int n = 0; n < 27; ++n) {
+            umin = fmin(umin, u[n]);
+            umax = fmax(umax, u[n]);
+        }
+        if (dumax * alpha > (umax - uc)) alpha = (umax - uc) / dumax;
+        if (dumax * alpha > (uc - umin)) alpha = (uc - umin) / dumax;
+    }
+    out[(int64_t(0) * NQ + q) * npar + p] = uc;
+    for (int d = 0; d < 3; ++d) out[(int64_t(1 + d) * NQ + q) * npar + p] = s[d] * alpha;
+}
+
 __global__ void __launch_bounds__(kAmrBlock) k_explode(
-    const double *__restrict__ crse, double *__restrict__ fine, const BoxDev *__restrict__ fboxes,
-    const BoxDev *__restrict__ cboxes, const ExplodeEntry *__restrict__ map, int64_t n)
+    const double *__restrict__ slopes, int64_t npar, double *__restrict__ fine, const BoxDev *__restrict__ fboxes,
+    const ExplodeEntry *__restrict__ map, int64_t n)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const ExplodeEntry e = map[t];
     const BoxDev &FB = fboxes[e.fine_box];
-    const BoxDev &CB = cboxes[e.crse_box];
-    const double *c0 = crse + CB.base * NQ + e.crse_local;
     double *fo = fine + FB.base * NQ + e.fine_local;
-    // a direction without both neighbours reads the cell itself (offset 0):
-    // then dl = dr = 0 and the limiter gives the zero slope, with every load
-    // in bounds and issued unconditionally
-    int64_t stride[3] = {1, CB.n[0], int64_t(CB.n[0]) * CB.n[1]};
-    double o[3];
-    for (int d = 0; d < 3; ++d) {
-        o[d] = ((e.bits >> d) & 1) ? 0.25 : -0.25;
-        if (!((e.bits >> (3 + d)) & 1)) stride[d] = 0;
-    }
-    double v[NQ][7];
+    const double ox = (e.bits & 1) ? 0.25 : -0.25;
+    const double oy = (e.bits & 2) ? 0.25 : -0.25;
+    const double oz = (e.bits & 4) ? 0.25 : -0.25;
+    const int64_t p = e.parent;
+    double v[NQ][4];
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const double *cq = c0 + int64_t(q) * CB.ncell;
-        v[q][0] = cq[0];
-        for (int d = 0; d < 3; ++d) {
-            v[q][1 + 2 * d] = cq[-stride[d]];
-            v[q][2 + 2 * d] = cq[stride[d]];
-        }
+        for (int k = 0; k < 4; ++k) v[q][k] = slopes[(int64_t(k) * NQ + q) * npar + p];
     });
     static_for<NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const double c = v[q][0];
-        double s[3];
-        for (int d = 0; d < 3; ++d) {
-            const double cm = v[q][1 + 2 * d], cp = v[q][2 + 2 * d];
-            const double dc = 0.5 * (cp - cm);
-            const double dl = c - cm;
-            const double dr = cp - c;
-            s[d] = dl * dr > 0.0 ? copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc) : 0.0;
-        }
-        fo[int64_t(q) * FB.ncell] = ((c + s[0] * o[0]) + s[1] * o[1]) + s[2] * o[2];
+        fo[int64_t(q) * FB.ncell] = v[q][0] + ox * v[q][1] + oy * v[q][2] + oz * v[q][3];
     });
 }
 
@@ -201,6 +257,57 @@ int check_pair(const mrb_level *crse, const mrb_level *fine)
     return MRB_OK;
 }
 
+// parent list of an interpolation: coarse cell P (its owner box and local
+// index) -> index; flags as ExplodeParent
+struct ParentSet {
+    std::vector<ExplodeParent> list;
+    std::vector<int32_t> index;  // per coarse domain cell, -1 if not a parent
+    int depth = 1;               // coarse FillBoundary depth the slopes read
+};
+
+int32_t parent_of(const mrb_level *crse, const std::vector<int32_t> &cown, const int P[3], ParentSet &ps)
+{
+    if (ps.index.empty()) ps.index.assign(size_t(int64_t(crse->ncell[0]) * crse->ncell[1] * crse->ncell[2]), -1);
+    const int64_t di = dom_index(crse, P);
+    if (ps.index[size_t(di)] >= 0) return ps.index[size_t(di)];
+    ExplodeParent e;
+    e.crse_box = cown[size_t(di)];
+    const BoxDev &CB = crse->boxes[size_t(e.crse_box)];
+    e.crse_local = local_of(CB, P[0], P[1], P[2]);
+    int fl = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (crse->periodic[d]) continue;
+        if (P[d] == 0) fl |= 1 << d;
+        if (P[d] == crse->ncell[d] - 1) fl |= 1 << (3 + d);
+        if (P[d] + 2 <= crse->ncell[d] - 1) fl |= 1 << (6 + d);
+        if (P[d] - 2 >= 0) fl |= 1 << (9 + d);
+        // the 4-point form reads P -+ 2 inward: a box one cell wide there
+        // puts it in the second ghost layer
+        if (((fl >> d) & 1) && ((fl >> (6 + d)) & 1) && P[d] + 2 > CB.vhi[d] + 1) ps.depth = crse->ng;
+        if (((fl >> (3 + d)) & 1) && ((fl >> (9 + d)) & 1) && P[d] - 2 < CB.vlo[d] - 1) ps.depth = crse->ng;
+    }
+    e.flags = fl;
+    ps.index[size_t(di)] = int32_t(ps.list.size());
+    ps.list.push_back(e);
+    return ps.index[size_t(di)];
+}
+
+// the slopes kernel reads P +- 2 along every direction and the 3x3x3 block
+// unconditionally: those cells must lie inside the parent's fab
+int check_parents(const mrb_level *crse, const ParentSet &ps)
+{
+    if (crse->ng < 2) return set_error(MRB_E_UNSUPPORTED, "the interpolation needs 2 coarse ghost layers");
+    for (const ExplodeParent &e : ps.list) {
+        const BoxDev &CB = crse->boxes[size_t(e.crse_box)];
+        const int64_t nxy = int64_t(CB.n[0]) * CB.n[1];
+        const int c[3] = {int(e.crse_local % CB.n[0]), int((e.crse_local % nxy) / CB.n[0]), int(e.crse_local / nxy)};
+        for (int d = 0; d < 3; ++d)
+            if (c[d] - 2 < 0 || c[d] + 2 >= CB.n[d])
+                return set_error(MRB_E_ARG, "interpolation parent too close to its fab edge");
+    }
+    return MRB_OK;
+}
+
 }  // namespace
 
 static void drop_graphs(mrb_level *L)
@@ -219,6 +326,12 @@ void amr_release(mrb_level *L)
     L->cap_stream = nullptr;
     if (L->d_coal) (void)hipFree(L->d_coal);
     if (L->d_expl) (void)hipFree(L->d_expl);
+    if (L->d_expl_par) (void)hipFree(L->d_expl_par);
+    if (L->d_expl_slopes) (void)hipFree(L->d_expl_slopes);
+    L->d_expl_par = nullptr;
+    L->d_expl_slopes = nullptr;
+    L->n_expl_par = 0;
+    L->expl_crse_depth = 1;
     if (L->d_coal_cell) (void)hipFree(L->d_coal_cell);
     if (L->d_coal_vals) (void)hipFree(L->d_coal_vals);
     if (L->d_qring) (void)hipFree(L->d_qring);
@@ -300,6 +413,7 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
     // explosion: fine ghost cells whose periodic image is inside the domain
     // and not covered by a fine box
     std::vector<ExplodeEntry> expl;
+    ParentSet parents;
     for (int b = 0; b < fine->nbox; ++b) {
         const BoxDev &FB = fine->boxes[b];
         for (int k = FB.lo[2]; k < FB.lo[2] + FB.n[2]; ++k)
@@ -319,12 +433,8 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
                     e.fine_local = local_of(FB, i, j, k);
                     e.crse_box = ob;
                     e.crse_local = local_of(crse->boxes[ob], P[0], P[1], P[2]);
-                    for (int d = 0; d < 3; ++d) {
-                        const bool hm = crse->periodic[d] || P[d] - 1 >= 0;
-                        const bool hp = crse->periodic[d] || P[d] + 1 < crse->ncell[d];
-                        e.bits |= (s[d] & 1) << d;
-                        e.bits |= int(hm && hp) << (3 + d);
-                    }
+                    e.parent = parent_of(crse, cown, P, parents);
+                    for (int d = 0; d < 3; ++d) e.bits |= (s[d] & 1) << d;
                     expl.push_back(e);
                 }
     }
@@ -385,14 +495,20 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
     }
     for (const ExplodeEntry &e : expl) {
         const BoxDev &FB = fine->boxes[e.fine_box];
-        const BoxDev &CB = crse->boxes[e.crse_box];
-        const int64_t sz = int64_t(CB.n[0]) * CB.n[1];
-        const int64_t lo = (e.bits & (7 << 3)) ? e.crse_local - sz : e.crse_local;
-        const int64_t hi = (e.bits & (7 << 3)) ? e.crse_local + sz : e.crse_local;
-        if (e.fine_local < 0 || e.fine_local >= FB.ncell || lo < 0 || hi >= CB.ncell)
+        if (e.fine_local < 0 || e.fine_local >= FB.ncell || e.parent < 0 ||
+            e.parent >= int32_t(parents.list.size()))
             return set_error(MRB_E_ARG, "explosion map leaves a fab");
     }
+    if ((rc = check_parents(crse, parents))) return rc;
     amr_release(fine);
+    if (!parents.list.empty()) {
+        MRB_HIP(hipMalloc(&fine->d_expl_par, sizeof(ExplodeParent) * parents.list.size()));
+        MRB_HIP(hipMemcpy(fine->d_expl_par, parents.list.data(), sizeof(ExplodeParent) * parents.list.size(),
+                          hipMemcpyHostToDevice));
+        MRB_HIP(hipMalloc(&fine->d_expl_slopes, sizeof(double) * 4 * NQ * parents.list.size()));
+    }
+    fine->n_expl_par = int64_t(parents.list.size());
+    fine->expl_crse_depth = parents.depth;
     if (!coal.empty()) {
         MRB_HIP(hipMalloc(&fine->d_coal, sizeof(CoalesceEntry) * coal.size()));
         MRB_HIP(hipMemcpy(fine->d_coal, coal.data(), sizeof(CoalesceEntry) * coal.size(), hipMemcpyHostToDevice));
@@ -443,16 +559,27 @@ int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *strea
     if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
     if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
     hipStream_t s = as_stream(stream);
+    // the slopes read the coarse parents' neighbourhoods: FillBoundary of
+    // the coarse level (ghost layers only, so its valid data is unchanged)
+    const int depth = std::max(1, fine->expl_crse_depth);
+    int rc;
+    if (crse->fresh_depth < depth &&
+        (rc = fab_fill_boundary_depth(const_cast<mrb_level *>(crse), depth, s)))
+        return rc;
     // fine-fine ghosts first, then the uncovered ones, then the domain
     // boundary (the three sets of cells are disjoint; physbc reads valid
     // and in-domain ghost cells only)
-    int rc = fab_fill_boundary(fine, lattice, s);
+    rc = fab_fill_boundary(fine, lattice, s);
     if (rc) return rc;
     if (lattice == 1) fine->fresh_depth = fine->ng;
     if (fine->n_expl > 0) {
+        k_explode_slopes<<<blocks_for(fine->n_expl_par * NQ), kAmrBlock, 0, s>>>(
+            (const double *)crse->pop[lattice][crse->cur], crse->d_boxes, (const ExplodeParent *)fine->d_expl_par,
+            fine->n_expl_par, fine->d_expl_slopes);
+        MRB_HIP(hipGetLastError());
         k_explode<<<blocks_for(fine->n_expl), kAmrBlock, 0, s>>>(
-            (const double *)crse->pop[lattice][crse->cur], (double *)fine->pop[lattice][fine->cur],
-            fine->d_boxes, crse->d_boxes, (const ExplodeEntry *)fine->d_expl, fine->n_expl);
+            fine->d_expl_slopes, fine->n_expl_par, (double *)fine->pop[lattice][fine->cur], fine->d_boxes,
+            (const ExplodeEntry *)fine->d_expl, fine->n_expl);
         MRB_HIP(hipGetLastError());
     }
     return mrb_physbc(fine, lattice, stream);
@@ -479,7 +606,8 @@ static int amr_step(mrb_level *crse, mrb_level *fine, void *stream)
 {
     int rc;
     hipStream_t s = as_stream(stream);
-    if (crse->fresh_depth < 1 && (rc = fab_fill_boundary_depth(crse, 1, s))) return rc;
+    const int depth = std::max(1, fine->expl_crse_depth);
+    if (crse->fresh_depth < depth && (rc = fab_fill_boundary_depth(crse, depth, s))) return rc;
     for (int l = 0; l < 2; ++l)
         if ((rc = mrb_physbc(crse, l, stream))) return rc;
     for (int l = 0; l < 2; ++l)
@@ -509,8 +637,6 @@ int mrb_amr_regrid_fill(mrb_level *crse, const mrb_level *old_fine, mrb_level *f
     if (old_fine->layout != MRB_LAYOUT_FAB || old_fine->storage != MRB_STORAGE_F64)
         return set_error(MRB_E_UNSUPPORTED, "regrid needs fab levels with FP64 storage");
     if (!fine->flags_ready) return set_error(MRB_E_STATE, "the new fine level's flags are not set");
-    // the interpolation reads the coarse parents' neighbours
-    if (crse->fresh_depth < 1 && (rc = fab_fill_boundary_depth(crse, 1, nullptr))) return rc;
     const std::vector<int32_t> cown = owner_map(crse);
     const std::vector<int32_t> oown = owner_map(old_fine);
     // FillPatchTwoLevels (FillPatchOps.H:118-129) over the new valid cells:
@@ -518,6 +644,7 @@ int mrb_amr_regrid_fill(mrb_level *crse, const mrb_level *old_fine, mrb_level *f
     // the coarse parent
     std::vector<LevelCopy> copy;
     std::vector<ExplodeEntry> interp;
+    ParentSet parents;
     for (int b = 0; b < fine->nbox; ++b) {
         const BoxDev &FB = fine->boxes[size_t(b)];
         for (int k = FB.vlo[2]; k <= FB.vhi[2]; ++k)
@@ -540,16 +667,23 @@ int mrb_amr_regrid_fill(mrb_level *crse, const mrb_level *old_fine, mrb_level *f
                     e.fine_local = local_of(FB, i, j, k);
                     e.crse_box = cown[size_t(dom_index(crse, P))];
                     e.crse_local = local_of(crse->boxes[size_t(e.crse_box)], P[0], P[1], P[2]);
-                    for (int d = 0; d < 3; ++d) {
-                        const bool hm = crse->periodic[d] || P[d] - 1 >= 0;
-                        const bool hp = crse->periodic[d] || P[d] + 1 < crse->ncell[d];
-                        e.bits |= (iv[d] & 1) << d;
-                        e.bits |= int(hm && hp) << (3 + d);
-                    }
+                    e.parent = parent_of(crse, cown, P, parents);
+                    for (int d = 0; d < 3; ++d) e.bits |= (iv[d] & 1) << d;
                     interp.push_back(e);
                 }
     }
-    void *d_copy = nullptr, *d_interp = nullptr;
+    if ((rc = check_parents(crse, parents))) return rc;
+    // the interpolation reads the coarse parents' neighbourhoods
+    const int depth = std::max(1, parents.depth);
+    if (crse->fresh_depth < depth && (rc = fab_fill_boundary_depth(crse, depth, nullptr))) return rc;
+    const int64_t npar = int64_t(parents.list.size());
+    void *d_copy = nullptr, *d_interp = nullptr, *d_par = nullptr;
+    double *d_slopes = nullptr;
+    if (npar > 0) {
+        MRB_HIP(hipMalloc(&d_par, sizeof(ExplodeParent) * size_t(npar)));
+        MRB_HIP(hipMemcpy(d_par, parents.list.data(), sizeof(ExplodeParent) * size_t(npar), hipMemcpyHostToDevice));
+        MRB_HIP(hipMalloc(&d_slopes, sizeof(double) * 4 * NQ * size_t(npar)));
+    }
     if (!copy.empty()) {
         MRB_HIP(hipMalloc(&d_copy, sizeof(LevelCopy) * copy.size()));
         MRB_HIP(hipMemcpy(d_copy, copy.data(), sizeof(LevelCopy) * copy.size(), hipMemcpyHostToDevice));
@@ -564,10 +698,13 @@ int mrb_amr_regrid_fill(mrb_level *crse, const mrb_level *old_fine, mrb_level *f
             k_copy_levels<<<blocks_for(int64_t(copy.size())), kAmrBlock>>>(
                 (const double *)old_fine->pop[l][old_fine->cur], dst, old_fine->d_boxes, fine->d_boxes,
                 (const LevelCopy *)d_copy, int64_t(copy.size()));
-        if (!interp.empty())
+        if (!interp.empty()) {
+            k_explode_slopes<<<blocks_for(npar * NQ), kAmrBlock>>>((const double *)crse->pop[l][crse->cur],
+                                                                  crse->d_boxes, (const ExplodeParent *)d_par, npar,
+                                                                  d_slopes);
             k_explode<<<blocks_for(int64_t(interp.size())), kAmrBlock>>>(
-                (const double *)crse->pop[l][crse->cur], dst, fine->d_boxes, crse->d_boxes,
-                (const ExplodeEntry *)d_interp, int64_t(interp.size()));
+                d_slopes, npar, dst, fine->d_boxes, (const ExplodeEntry *)d_interp, int64_t(interp.size()));
+        }
         k_zero_solid<<<blocks_for(fine->total_cells), kAmrBlock>>>(dst, fine->d_isf, fine->d_boxes,
                                                                      fine->d_box_base, fine->nbox, fine->total_cells);
     }
@@ -575,6 +712,8 @@ int mrb_amr_regrid_fill(mrb_level *crse, const mrb_level *old_fine, mrb_level *f
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(d_copy);
     (void)hipFree(d_interp);
+    (void)hipFree(d_par);
+    (void)hipFree(d_slopes);
     if (e != hipSuccess) return hip_fail(e, "mrb_amr_regrid_fill");
     fine->fresh_depth = 0;
     fine->pops_set = true;

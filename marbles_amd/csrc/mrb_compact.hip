@@ -68,6 +68,18 @@ struct ShiftTab {
     __device__ __forceinline__ double g(int q) const { return gk[csq(q)]; }
 };
 
+// Rest state of the shifted FP32 storage, for its FP32-arithmetic collide
+// (k_collide_f32): the product-form factors and the Grad coefficients of
+// the subtracted equilibrium f_eq(rho0, 0, T0), g_eq(rho0, 0, T0), and the
+// raw moments of those shifts; FP64.
+struct DevRest {
+    double A0[3];   // rho0 phi0(c), c = -1, 0, 1 (the shift's x factor)
+    double B0[3];   // phi0(c) (its y and z factors)
+    double E0;      // 2 rho0 E of the rest state
+    double t0[2];   // (c^2 - theta0) a2_aa of the rest state, c^2 = 0, 1
+    double mo[14];  // moments of the shifts, in MomentsR order
+};
+
 // Element stride between neighbouring cells of one component: FP64
 // storage keeps the components of a lattice contiguous
 // (pop[(2q + lattice)*N + cell]); shifted FP32 storage keeps f_q and g_q
@@ -108,6 +120,27 @@ struct Half16 {
     // inlet keeps room for the inflow
     float floor_f, floor_g;
 };
+
+// Largest |population - shift| that scale box b's written cells can be
+// pulled from: the maxima of boxes b-1, b and b+1 in the read buffer (a pull
+// reaches one plane into each neighbouring box; the neighbours wrap on a
+// periodic z axis), no smaller than the boundary floor.  A box exactly at
+// the shift state (maximum 0) next to a disturbed one thereby takes the
+// disturbed box's scale instead of the largest exponent.
+__device__ __forceinline__ float f16_pull_max(const uint32_t *__restrict__ amax, int b, int lattice, int nz,
+                                              int periodic_z, float floor)
+{
+    const int nb = (nz + kScaleSlab - 1) / kScaleSlab;
+    float m = fmaxf(__uint_as_float(amax[2 * b + lattice]), floor);
+    int bm = b - 1, bp = b + 1;
+    if (periodic_z) {
+        bm = bm < 0 ? nb - 1 : bm;
+        bp = bp >= nb ? 0 : bp;
+    }
+    if (bm >= 0) m = fmaxf(m, __uint_as_float(amax[2 * bm + lattice]));
+    if (bp < nb) m = fmaxf(m, __uint_as_float(amax[2 * bp + lattice]));
+    return m;
+}
 
 // FP16 value of (v - shift) * 2^e: the difference is taken in FP64, then
 // rounded to FP32, scaled there (exact for a power of two) and clamped to
@@ -262,12 +295,16 @@ __device__ __forceinline__ void pressure_ghosts(
 // W = the working type of the pulled values (double); BCX: the level's
 // boundary features (kBcxProfile: Channel or Parabolic inlet profile, ghost
 // values from the face tables; kBcxPressure: PRESSURE faces)
-template <int BCX, class T, class W>
+// DEV (shifted FP32 storage only): return the stored deviations f - shift
+// themselves (boundary constants less the shift) instead of populations.
+template <int BCX, class T, class W, bool DEV = false>
 __device__ __forceinline__ void pull_cell(
     const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
     const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, W fv[NQ],
     W gv[NQ])
 {
+    static_assert(!DEV || (std::is_same<T, float>::value && (BCX & kBcxPressure) == 0),
+                  "deviation pull: shifted FP32 storage without PRESSURE faces");
     int tx[3], ty[3], tz[3];
     bool gx[3], gy[3], gz[3];
     targets3(x, A.nx, A.per[0], tx, gx);
@@ -275,27 +312,32 @@ __device__ __forceinline__ void pull_cell(
     targets3(z, A.nz, A.per[2], tz, gz);
     const uint32_t c32 = uint32_t(cell);
     T rf[NQ], rg[NQ];
-    uint32_t cmask = 0, dmask = 0;
-    // The boundary handling of the 27 sources as bit sets over q, per cell:
-    // G = sources beyond a non-periodic face, the face being the first of
-    // z, y, x (BCFill's order) in which the source lies outside; V, O =
-    // those whose face is VELOCITY or OUTFLOW.  A source is read directly
-    // (from the wrapped or clamped coordinates) unless it is a solid cell
-    // or a ghost of a NOSLIP or SLIP face.
-    uint32_t G = 0, V = 0, O = 0, Pm = 0;
+    uint32_t cmask = 0, dmask = 0, zmove = 0;
+    // The boundary handling of the 27 sources as bit sets over q, per cell.
+    // BCFill applies the faces a ghost lies beyond in the order x, y, z, the
+    // last one winning (BC.H:365-372); an OUTFLOW face copies the ghost one
+    // step inward, whose own value the remaining faces set (the ghosts are
+    // filled in order of their distance beyond the domain).  So, scanning
+    // z, y, x: G = sources beyond a non-periodic face; Tm = those whose value
+    // a non-OUTFLOW face fixes (the first such face in the scan), with V, Pm
+    // the VELOCITY and PRESSURE ones; O = the rest of G, beyond OUTFLOW
+    // faces only, which read the source clamped into the domain.  A source
+    // is read directly unless it is a solid cell or Tm holds it with a wall.
+    uint32_t G = 0, Tm = 0, V = 0, Pm = 0;
     static_for<3>([&](auto dc) {
         constexpr int d = 2 - decltype(dc)::value;
         const bool *gd = d == 0 ? gx : (d == 1 ? gy : gz);
         const uint32_t Gd = (gd[2] ? qset(d, 1) : 0u) | (gd[0] ? qset(d, -1) : 0u);
-        const uint32_t Fd = Gd & ~G;
+        const uint32_t Ud = Gd & ~Tm;
         G |= Gd;
         const uint32_t tl = (A.bcpack >> (4 * d)) & 15u, th = (A.bcpack >> (4 * (d + 3))) & 15u;
-        const uint32_t lo = Fd & qset(d, 1), hi = Fd & qset(d, -1);
+        const uint32_t lo = Ud & qset(d, 1), hi = Ud & qset(d, -1);
+        Tm |= (tl != MRB_BC_OUTFLOW ? lo : 0u) | (th != MRB_BC_OUTFLOW ? hi : 0u);
         V |= (tl == MRB_BC_VELOCITY ? lo : 0u) | (th == MRB_BC_VELOCITY ? hi : 0u);
-        O |= (tl == MRB_BC_OUTFLOW ? lo : 0u) | (th == MRB_BC_OUTFLOW ? hi : 0u);
         if constexpr ((BCX & kBcxPressure) != 0)
             Pm |= (tl == MRB_BC_PRESSURE ? lo : 0u) | (th == MRB_BC_PRESSURE ? hi : 0u);
     });
+    const uint32_t O = G & ~Tm;
     const uint32_t nondirect = (G & ~(V | O)) | (~G & ~mask & kAllQ);
     if (A.fast_ok && __all(nondirect == 0u)) {
         // every source of every cell of the wave is read directly: one
@@ -338,16 +380,24 @@ __device__ __forceinline__ void pull_cell(
         const bool ghost = hx || hy || hz;
         const uint32_t src = (uint32_t(tz[cz + 1]) * uint32_t(A.ny) + uint32_t(ty[cy + 1])) * uint32_t(A.nx) +
                              uint32_t(tx[cx + 1]);
-        int face = cx > 0 ? 0 : 3;
-        face = hy ? (cy > 0 ? 1 : 4) : face;
-        face = hz ? (cz > 0 ? 2 : 5) : face;
-        const uint32_t t = (A.bcpack >> (4 * face)) & 15u;
+        // the face that fixes a ghost source's value (scan z, y, x past
+        // OUTFLOW faces, as above) and the cell its wall fill reads: the cell
+        // itself moved one step inward across each OUTFLOW face passed
+        const uint32_t tfz = (A.bcpack >> (4 * (cz > 0 ? 2 : 5))) & 15u;
+        const uint32_t tfy = (A.bcpack >> (4 * (cy > 0 ? 1 : 4))) & 15u;
+        const uint32_t tfx = (A.bcpack >> (4 * (cx > 0 ? 0 : 3))) & 15u;
+        const bool pz = hz && tfz == MRB_BC_OUTFLOW;
+        const bool py = hy && tfy == MRB_BC_OUTFLOW && (pz || !hz);
+        const bool px = hx && tfx == MRB_BC_OUTFLOW && (py || !hy) && (pz || !hz);
+        const uint32_t t = (hz && !pz) ? tfz : (hy && !py) ? tfy : (hx && !px) ? tfx : uint32_t(MRB_BC_OUTFLOW);
+        const int32_t roff = (pz ? cz * A.nx * A.ny : 0) + (py ? cy * A.nx : 0) + (px ? cx : 0);
+        zmove |= uint32_t(pz) << q;
         const bool direct = ghost ? t == MRB_BC_OUTFLOW : ((mask >> q) & 1u) != 0;
         int comp = direct ? q : qb;
         comp = (ghost && t == MRB_BC_SLIPX) ? reflect(q, 0) : comp;
         comp = (ghost && t == MRB_BC_SLIPY) ? reflect(q, 1) : comp;
         comp = (ghost && t == MRB_BC_SLIPZ) ? reflect(q, 2) : comp;
-        const int64_t e = int64_t(comp) * A.cs + int64_t(direct ? src : c32) * kCellStride<T>;
+        const int64_t e = int64_t(comp) * A.cs + int64_t(direct ? src : uint32_t(int32_t(c32) + roff)) * kCellStride<T>;
         dmask |= uint32_t(direct) << q;
         if constexpr (kIsHalf<T>) {
             const uint32_t v = *reinterpret_cast<const uint32_t *>(f + e);
@@ -365,6 +415,7 @@ __device__ __forceinline__ void pull_cell(
         cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
     });
     }
+    (void)zmove;
     if constexpr (kIsHalf<T>) {
         // inverse scales of the source planes z - c_z (index c_z + 1); a
         // source that is not the pull's (bounce-back, wall ghosts) is the
@@ -380,7 +431,10 @@ __device__ __forceinline__ void pull_cell(
             constexpr int k = ev(q, 2) + 1;
             const bool use_const = (cmask >> q) & 1u;
             const bool direct = (dmask >> q) & 1u;
-            const float sf = direct ? invf[k] : invf[1], sg = direct ? invg[k] : invg[1];
+            // a wall fill read moved inward across a z OUTFLOW face lies in
+            // plane z + c_z (index 1 - c_z)
+            const int kn = ((zmove >> q) & 1u) ? 1 - ev(q, 2) : 1;
+            const float sf = direct ? invf[k] : invf[kn], sg = direct ? invg[k] : invg[kn];
             // unscaling in FP32 is exact (a power of two on an FP16 value)
             const W vf = W(float(rf[q]) * sf) + W(sh.f(q));
             const W vg = W(float(rg[q]) * sg) + W(sh.g(q));
@@ -398,8 +452,13 @@ __device__ __forceinline__ void pull_cell(
             double cf = bc.fval[q], cg = bc.gval[q];
             if constexpr ((BCX & kBcxProfile) != 0)
                 if (use_const) profile_ghost<q>(bc, gx, gy, gz, x, y, z, cf, cg);
-            fv[q] = use_const ? cf : widen(rf[q], sh.f(q));
-            gv[q] = use_const ? cg : widen(rg[q], sh.g(q));
+            if constexpr (DEV) {
+                fv[q] = use_const ? W(cf - sh.f(q)) : W(rf[q]);
+                gv[q] = use_const ? W(cg - sh.g(q)) : W(rg[q]);
+            } else {
+                fv[q] = use_const ? cf : widen(rf[q], sh.f(q));
+                gv[q] = use_const ? cg : widen(rg[q], sh.g(q));
+            }
         });
     }
     if constexpr ((BCX & kBcxPressure) != 0) {
@@ -518,8 +577,8 @@ __device__ __forceinline__ void collide_body(
     float mf = 0.0f, mg = 0.0f;
     if constexpr (kIsHalf<T>) {
         const int b = int(blockIdx.z) / kScaleSlab;
-        ef = f16_exponent(fmaxf(__uint_as_float(h16.amax_cur[2 * b]), h16.floor_f));
-        eg = f16_exponent(fmaxf(__uint_as_float(h16.amax_cur[2 * b + 1]), h16.floor_g));
+        ef = f16_exponent(f16_pull_max(h16.amax_cur, b, 0, A.nz, A.per[2], h16.floor_f));
+        eg = f16_exponent(f16_pull_max(h16.amax_cur, b, 1, A.nz, A.per[2], h16.floor_g));
         scf = ldexpf(1.0f, ef);
         scg = ldexpf(1.0f, eg);
     }
@@ -592,6 +651,331 @@ __device__ __forceinline__ void collide_body(
         }
     }
     if constexpr (kIsHalf<T>) half_block_stats(h16, ef, eg, mf, mg);
+}
+
+// Pass 2 of the shifted FP32 storage with FP32 arithmetic, in deviation
+// form.  The populations are handled as their stored differences d = f - s
+// from the rest equilibrium s (O(1e-2) of f), so that FP32 rounding applies
+// to the differences, not to the populations:
+//  - moments: FP64 sums of the pulled differences plus the shift's moments;
+//    macroscopic fields, the gradient and the equilibrium coefficients in
+//    FP64, as the FP64 collide computes them;
+//  - f equilibria: with A, B, C the x, y, z factors of the product form and
+//    A0, B0, C0 the shift's, A B C - A0 B0 C0 = (dA B + A0 dB) C + A0 B0 dC,
+//    the factor differences taken in FP64 and the products in FP32;
+//  - g equilibria: w_q (dE + c.a1 + (dS)/2), the differences of the Grad
+//    coefficients from the shift's taken in FP64;
+//  - relaxation d' = d + omega (d_eq - d) in FP32;
+//  - the rest direction's equilibrium difference is the FP64 sum of the
+//    pulled differences less the FP64 sum of the other 26 (for f and g),
+//    so that each cell's equilibria carry its mass and energy exactly.
+template <int BCX>
+__device__ __forceinline__ void collide_f32_body(
+    const float *__restrict__ f, const float *__restrict__ g, float *__restrict__ fo, float *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
+    const ShiftTab &sh, const DevRest &D, const CollideParams &P)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    if (!(m & 1u)) {
+        // solid cell: the stream's -1 sentinel (LBM.cpp:565)
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            store_pair(fo, go, int64_t(q) * A.cs + cell * 2, narrow<float>(-1.0, sh.f(q)),
+                       narrow<float>(-1.0, sh.g(q)));
+        });
+        return;
+    }
+    float fv[NQ], gv[NQ];
+    pull_cell<BCX, float, float, true>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
+    MomentsR<double> mo;
+    accumulate_moments(fv, gv, mo);
+    const double rho_d = mo.rho, e_d = mo.E;  // sums of the differences
+    mo.rho += D.mo[0];
+    mo.jx += D.mo[1];
+    mo.jy += D.mo[2];
+    mo.jz += D.mo[3];
+    mo.pxx += D.mo[4];
+    mo.pyy += D.mo[5];
+    mo.pzz += D.mo[6];
+    mo.pxy += D.mo[7];
+    mo.pxz += D.mo[8];
+    mo.pyz += D.mo[9];
+    mo.E += D.mo[10];
+    mo.qx += D.mo[11];
+    mo.qy += D.mo[12];
+    mo.qz += D.mo[13];
+    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
+    const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
+    const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
+    const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0, P.inv_dx[0]);
+    const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
+                                          ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
+    const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
+                                          zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
+    const CollideCoeffs C = collide_coeffs(mo, M, dqx, dqy, dqz, P);
+    // factor and coefficient differences from the rest state (FP64, then FP32)
+    float dA[3], Bf[3], dB[3], Cf[3], dC[3], A0f[3], B0f[3];
+    for (int k = 0; k < 3; ++k) {
+        dA[k] = float(C.rphx[k] - D.A0[k]);
+        Bf[k] = float(C.phy[k]);
+        dB[k] = float(C.phy[k] - D.B0[k]);
+        Cf[k] = float(C.phz[k]);
+        dC[k] = float(C.phz[k] - D.B0[k]);
+        A0f[k] = float(D.A0[k]);
+        B0f[k] = float(D.B0[k]);
+    }
+    const float dE = float(C.E - D.E0);
+    const float a1x = float(C.a1x), a1y = float(C.a1y), a1z = float(C.a1z);
+    float dtx[2], dty[2], dtz[2];
+    for (int k = 0; k < 2; ++k) {
+        dtx[k] = float(C.tx[k] - D.t0[k]);
+        dty[k] = float(C.ty[k] - D.t0[k]);
+        dtz[k] = float(C.tz[k] - D.t0[k]);
+    }
+    const float dxy = float(C.dxy), dxz = float(C.dxz), dyz = float(C.dyz);
+    const float omega = float(C.omega);
+    auto put = [&](auto qc, float fn, float gn) {
+        constexpr int q = decltype(qc)::value;
+        store_pair(fo, go, int64_t(q) * A.cs + cell * 2, fn, gn);
+    };
+    double sf = 0.0, sg = 0.0;
+    static_for<NQ - 1>([&](auto pc) {
+        constexpr int q = decltype(pc)::value + 1;
+        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+        const float fe = (dA[cx + 1] * Bf[cy + 1] + A0f[cx + 1] * dB[cy + 1]) * Cf[cz + 1] +
+                         (A0f[cx + 1] * B0f[cy + 1]) * dC[cz + 1];
+        float ge = dE;
+        if constexpr (cx == 1) ge = ge + a1x;
+        if constexpr (cx == -1) ge = ge - a1x;
+        if constexpr (cy == 1) ge = ge + a1y;
+        if constexpr (cy == -1) ge = ge - a1y;
+        if constexpr (cz == 1) ge = ge + a1z;
+        if constexpr (cz == -1) ge = ge - a1z;
+        float S = dtx[cx * cx] + dty[cy * cy];
+        if constexpr (cx * cy == 1) S = S + dxy;
+        if constexpr (cx * cy == -1) S = S - dxy;
+        S = S + dtz[cz * cz];
+        if constexpr (cx * cz == 1) S = S + dxz;
+        if constexpr (cx * cz == -1) S = S - dxz;
+        if constexpr (cy * cz == 1) S = S + dyz;
+        if constexpr (cy * cz == -1) S = S - dyz;
+        ge = (ge + 0.5f * S) * float(weight(q));
+        sf += double(fe);
+        sg += double(ge);
+        put(std::integral_constant<int, q>{}, fv[q] + omega * (fe - fv[q]), gv[q] + omega * (ge - gv[q]));
+    });
+    const float fe0 = float(rho_d - sf), ge0 = float(e_d - sg);
+    put(std::integral_constant<int, 0>{}, fv[0] + omega * (fe0 - fv[0]), gv[0] + omega * (ge0 - gv[0]));
+}
+
+// the FP32-arithmetic pass 2 of the shifted FP32 storage on 64x2 and 256x1
+// blocks
+template <int BCX>
+__global__ void __launch_bounds__(128) k_collide_f32(
+    const float *__restrict__ f, const float *__restrict__ g, float *__restrict__ fo, float *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    DevRest D, CollideParams P)
+{
+    collide_f32_body<BCX>(f, g, fo, go, mask, Q, A, bc, sh, D, P);
+}
+template <int BCX>
+__global__ void __launch_bounds__(256) k_collide_f32_x256(
+    const float *__restrict__ f, const float *__restrict__ g, float *__restrict__ fo, float *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    DevRest D, CollideParams P)
+{
+    collide_f32_body<BCX>(f, g, fo, go, mask, Q, A, bc, sh, D, P);
+}
+
+// ---------------------------------------------------------------------------
+// Single pass as a z-sweep (FP64 storage, rows of at most 256 cells): one
+// block per x-row y, all rows resident at once (cooperative launch).  The
+// block walks z; at plane z it pulls plane z+1 into registers and forms
+// its Q_corr, publishes Q_y of that row for the rows y -+ 1, waits until
+// those rows have published plane z, then collides plane z from the pull
+// it holds in LDS, and moves plane z+1 into LDS.  Each population is read
+// from HBM once and written once; the neighbours' Q_corr travel through
+// LDS (x), registers (z) and a ring of 4 planes of Q_y in global memory (y),
+// read and written past the non-coherent L2 (agent-scope accesses), with a
+// per-row count of published planes.  A row can run at most two planes
+// ahead of a neighbour (it waits for the neighbour's previous plane before
+// publishing the next), so 4 ring planes are never overwritten while read.
+// The cell arithmetic is the two-pass step's (same functions, same order).
+// ---------------------------------------------------------------------------
+constexpr int kSweepRing = 4;
+constexpr int kSweepRow = 256;
+constexpr uint32_t kSweepSpinLimit = 1u << 22;
+
+// f_q (or g_q) of the block's row, cell x, held in LDS at [2q + lattice][x]
+struct LdsRow {
+    const double *p;
+    __device__ __forceinline__ double operator[](int q) const { return p[q * 2 * kSweepRow]; }
+};
+
+__device__ __forceinline__ void store_agent(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// every memory access of the wave issued so far has completed
+__device__ __forceinline__ void drain_wave()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+template <int BCX>
+__global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo, double *__restrict__ go,
+    const uint32_t *__restrict__ mask, double *qring, uint32_t *flags, uint32_t *fault, CompactArgs A, BCDev bc,
+    ShiftTab sh, CollideParams P)
+{
+    __shared__ double hold[2 * NQ * kSweepRow];
+    __shared__ double qxrow[kSweepRow];
+    __shared__ int gave_up;  // a wait for a neighbour row hit the spin limit
+    const int x = int(threadIdx.x), y = int(blockIdx.x);
+    if (x == 0) gave_up = 0;
+    const bool in = x < A.nx;
+    const int64_t nxy = int64_t(A.nx) * A.ny;
+    const int64_t row = int64_t(y) * A.nx + x;
+    // two planes in registers: the pull of plane p (rf, rg: its Q_corr is
+    // known) and the pull of plane p+1 in flight (nf, ng) while plane p-1,
+    // held in LDS, is collided
+    double rf[NQ], rg[NQ], nf[NQ], ng[NQ];
+    auto pull_into = [&](int zp, uint32_t &m, double (&pf)[NQ], double (&pg)[NQ]) {
+        m = 0u;
+        if (!in) return;
+        const int64_t cell = int64_t(zp) * nxy + row;
+        m = mask[cell];
+        if (!(m & 1u)) return;
+        pull_cell<BCX>(f, g, A, bc, sh, Half16{}, x, y, zp, cell, m, pf, pg);
+    };
+    auto q_of = [&](uint32_t m, const double (&pf)[NQ], const double (&pg)[NQ], double &qx, double &qy,
+                    double &qz) {
+        qx = qy = qz = 0.0;
+        if (!in || !(m & 1u)) return;
+        double rho, jx, jy, jz, E;
+        accumulate_q_moments(pf, pg, rho, jx, jy, jz, E);
+        const Macro M = macro_from(rho, jx, jy, jz, E, P);
+        qx = M.Qx;
+        qy = M.Qy;
+        qz = M.Qz;
+    };
+    auto publish = [&](int zp, double qy) {
+        if (in) store_agent(&qring[int64_t(zp % kSweepRing) * nxy + row], qy);
+        drain_wave();
+        __syncthreads();
+        if (x == 0) __hip_atomic_store(&flags[y], uint32_t(zp + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // bounded wait: past the spin limit the block counts a fault and writes
+    // NaN from then on instead of waiting longer
+    auto wait_row = [&](int yy, uint32_t target) {
+        uint32_t it = 0;
+        while (__hip_atomic_load(&flags[yy], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++it > kSweepSpinLimit) {
+                atomicAdd(fault, 1u);
+                gave_up = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    };
+    // plane 0 into registers, its Q_corr published
+    uint32_t m1 = 0u, m0 = 0u;
+    double qx1, qy1, qz1;
+    pull_into(0, m1, rf, rg);
+    q_of(m1, rf, rg, qx1, qy1, qz1);
+    publish(0, qy1);
+    double qz0 = 0.0, qzm = 0.0;  // Q_z of planes p-1 and p-2
+    for (int z = 0; z <= A.nz; ++z) {
+        // plane z+1: loads issued now, used after the collide
+        uint32_t m2 = 0u;
+        if (z + 1 < A.nz) pull_into(z + 1, m2, nf, ng);
+        if (z >= 1) {
+            // collide plane z-1 (held in LDS) once the rows beside have
+            // published it
+            const int zc = z - 1;
+            if (x == 0 && !gave_up) {
+                if (y > 0) wait_row(y - 1, uint32_t(zc + 1));
+                if (y + 1 < A.ny) wait_row(y + 1, uint32_t(zc + 1));
+            }
+            __syncthreads();
+            const bool poisoned = gave_up != 0;
+            if (in) {
+                const int64_t cell = int64_t(zc) * nxy + row;
+                const uint32_t m = m0;
+                if (!(m & 1u)) {
+                    static_for<NQ>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        store_pair(fo, go, int64_t(q) * A.cs + cell, -1.0, -1.0);
+                    });
+                } else {
+                    const LdsRow F{&hold[x]}, G{&hold[kSweepRow + x]};
+                    MomentsR<double> mo;
+                    accumulate_moments(F, G, mo);
+                    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+                    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+                    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
+                    const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
+                    double *ring = qring + int64_t(zc % kSweepRing) * nxy + row;
+                    const double dqx = one_sided_gradient(xp, xm, xp ? qxrow[x + 1] : 0.0, M.Qx,
+                                                          xm ? qxrow[x - 1] : 0.0, P.inv_dx[0]);
+                    const double dqy = one_sided_gradient(yp, ym, yp ? load_agent(ring + A.nx) : 0.0, M.Qy,
+                                                          ym ? load_agent(ring - A.nx) : 0.0, P.inv_dx[1]);
+                    const double dqz = one_sided_gradient(zp, zm, zp ? qz1 : 0.0, M.Qz, zm ? qzm : 0.0,
+                                                          P.inv_dx[2]);
+                    const double nan = __builtin_nan("");
+                    collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P,
+                                           [&](auto qc, double fnew, double gnew) {
+                                               constexpr int q = decltype(qc)::value;
+                                               store_pair(fo, go, int64_t(q) * A.cs + cell, poisoned ? nan : fnew,
+                                                          poisoned ? nan : gnew);
+                                           });
+                }
+            }
+            __syncthreads();
+        }
+        if (z == A.nz) break;
+        // plane z becomes the held one
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            hold[(2 * q) * kSweepRow + x] = rf[q];
+            hold[(2 * q + 1) * kSweepRow + x] = rg[q];
+        });
+        qxrow[x] = qx1;
+        qzm = qz0;
+        qz0 = qz1;
+        m0 = m1;
+        // plane z+1: its Q_corr, published
+        if (z + 1 < A.nz) {
+            q_of(m2, nf, ng, qx1, qy1, qz1);
+            publish(z + 1, qy1);
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                rf[q] = nf[q];
+                rg[q] = ng[q];
+            });
+            m1 = m2;
+        } else {
+            qz1 = 0.0;
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *__restrict__ a, int n)
+{
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) a[i] = 0u;
 }
 
 // pass 2 on blocks of BX x BY cells, register budget as the compiler
@@ -860,6 +1244,41 @@ static void f16_floors(const mrb_level *L, float &ff, float &fg)
     fg = float(mg);
 }
 
+static DevRest dev_rest(const mrb_level *L)
+{
+    DevRest d{};
+    if (L->storage != MRB_STORAGE_F32_SHIFTED) return d;
+    const double R = kAirMolarMass / (L->phys_set ? L->phys.m_bar : kAirMolarMass);
+    const double gamma = L->phys_set ? L->phys.adiabatic_exponent : 5.0 / 3.0;
+    const double rho0 = L->shift_rho0, T0 = L->shift_T0, rt = R * T0;
+    // the factors of set_equilibrium_value at u = 0 (the f shift is
+    // ((rho0 phi_x) phi_y) phi_z of these)
+    for (int c = -1; c <= 1; ++c) {
+        const double pxx = 0.0 * 0.0 + rt;
+        const double phi = c * 0.5 * 0.0 + iabs(c) * (1.50 * pxx - 1.0) - pxx + 1.0;
+        d.B0[c + 1] = phi;
+        d.A0[c + 1] = rho0 * phi;
+    }
+    // the Grad coefficients of thermal_g_equilibrium at u = 0
+    const double cv = R / (gamma - 1.0);
+    const double E = rho0 * (2.0 * cv * T0 + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+    const double energy = E / (2.0 * rho0);
+    const double Tv = (0.50 / cv) * ((E / rho0) - 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+    const double p = rho0 * R * Tv;
+    const double h = energy + (p / rho0);
+    const double rxx = 2.0 * rho0 * 0.0 * 0.0 * (h + p / rho0) + 2.0 * p * h;
+    const double inv = 1.0 / kTheta0;
+    const double a2xx = (rxx - E * 1.0 * kTheta0) * inv * inv;
+    d.E0 = E;
+    d.t0[0] = (0 - kTheta0) * a2xx;
+    d.t0[1] = (1 - kTheta0) * a2xx;
+    MomentsR<double> m;
+    accumulate_moments(L->fshift, L->gshift, m);
+    const double mo[14] = {m.rho, m.jx, m.jy, m.jz, m.pxx, m.pyy, m.pzz, m.pxy, m.pxz, m.pyz, m.E, m.qx, m.qy, m.qz};
+    for (int k = 0; k < 14; ++k) d.mo[k] = mo[k];
+    return d;
+}
+
 static Half16 half16_of(mrb_level *L)
 {
     Half16 h{};
@@ -884,6 +1303,34 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     const T *g = (const T *)L->pop[1][L->cur];
     T *fo = (T *)L->pop[0][1 - L->cur];
     T *go = (T *)L->pop[1][1 - L->cur];
+    if (variant == kSweepVariant) {
+        if constexpr (std::is_same<T, double>::value) {
+            if (!L->d_sweep_q) return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 rows of at most 256 cells only");
+            k_zero_u32<<<(A.ny + 255) / 256, 256, 0, s>>>(L->d_sweep_flags, A.ny);
+            MRB_HIP(hipGetLastError());
+            if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+            double *qr = L->d_sweep_q;
+            uint32_t *fl = L->d_sweep_flags, *fa = L->d_sweep_fault;
+            const uint32_t *mk = L->d_mask;
+            CompactArgs Ac = A;
+            BCDev bcc = L->bcdev;
+            ShiftTab shc = sh;
+            CollideParams Pc = L->cp;
+            void *args[] = {(void *)&f, (void *)&g, (void *)&fo, (void *)&go, (void *)&mk, (void *)&qr,
+                            (void *)&fl, (void *)&fa, (void *)&Ac, (void *)&bcc, (void *)&shc, (void *)&Pc};
+            const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_sweep<BCX>),
+                                                            dim3(A.ny), dim3(kSweepRow), args, 0, s);
+            if (e == hipErrorCooperativeLaunchTooLarge) {
+                (void)hipGetLastError();
+                return set_error(MRB_E_UNSUPPORTED, "k_sweep: more rows than the GPU holds at once");
+            }
+            if (e != hipSuccess) return hip_fail(e, "k_sweep launch");
+            if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+            return MRB_OK;
+        } else {
+            return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 storage only");
+        }
+    }
     if (variant == kOnePassVariant) {
         if constexpr (kIsHalf<T>) {
             return set_error(MRB_E_UNSUPPORTED, "k_onepass: FP64 and shifted FP32 storage only");
@@ -939,6 +1386,34 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
         MRB_HIP(hipGetLastError());
         if (ev) MRB_HIP(hipEventRecord(ev[2], s));
         return MRB_OK;
+    }
+    if (variant == kF32Variant || variant == kF32X256Variant) {
+        if constexpr (std::is_same<T, float>::value && (BCX & kBcxPressure) == 0) {
+            const DevRest D = dev_rest(L);
+            const bool wide = variant == kF32X256Variant;
+            const dim3 grid = wide ? grid_of(A, 1, 256) : grid_of(A, 2);
+            if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+            if (wide)
+                k_qpass_x256<T, BCX><<<grid, dim3(256, 1, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh,
+                                                                     h16, L->cp);
+            else
+                k_qpass_y2<T, BCX><<<grid, dim3(64, 2, 1), 0, s>>>(f, g, L->d_mask, L->d_q, A, L->bcdev, sh, h16,
+                                                                  L->cp);
+            MRB_HIP(hipGetLastError());
+            if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+            if (wide)
+                k_collide_f32_x256<BCX><<<grid, dim3(256, 1, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A,
+                                                                        L->bcdev, sh, D, L->cp);
+            else
+                k_collide_f32<BCX><<<grid, dim3(64, 2, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev,
+                                                                  sh, D, L->cp);
+            MRB_HIP(hipGetLastError());
+            if (ev) MRB_HIP(hipEventRecord(ev[2], s));
+            return MRB_OK;
+        } else {
+            return set_error(MRB_E_UNSUPPORTED,
+                             "k_collide_f32: shifted FP32 storage without PRESSURE faces only");
+        }
     }
     if (variant == kHalf32Variant || variant == kHalf32X256Variant) {
         if constexpr (kIsHalf<T> && (BCX & kBcxPressure) == 0) {
@@ -1010,7 +1485,10 @@ const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_
                                                            "k_qpass+k_collide_h32",
                                                            "k_qpass_x128+k_collide_x128",
                                                            "k_qpass_x256+k_collide_x256",
-                                                           "k_qpass_x256+k_collide_h32_x256"};
+                                                           "k_qpass_x256+k_collide_h32_x256",
+                                                           "k_qpass_y2+k_collide_f32",
+                                                           "k_qpass_x256+k_collide_f32_x256",
+                                                           "k_sweep"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 
@@ -1031,7 +1509,7 @@ static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
     int rc;
     if (L->hot < kNumTwoPassVariants) {
         rc = two_pass(L, L->hot, s, events ? L->ev : nullptr);
-        nslots = L->hot == kOnePassVariant ? 1 : 2;
+        nslots = (L->hot == kOnePassVariant || L->hot == kSweepVariant) ? 1 : 2;
     } else {
         if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
         rc = fused_launch(L, L->hot - kNumTwoPassVariants, s);

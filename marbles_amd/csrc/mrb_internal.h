@@ -113,8 +113,20 @@ struct ExplodeEntry {
     int64_t crse_local;  // the parent
     int32_t fine_box;
     int32_t crse_box;
-    int32_t bits;  // bit d: high child in direction d; bit 3+d: both neighbours in d inside the domain
-    int32_t pad;
+    int32_t bits;    // bit d: high child in direction d
+    int32_t parent;  // index of the parent in the level's ExplodeParent list
+};
+
+// A coarse parent of interpolated fine cells: its slopes are computed once
+// (k_explode_slopes) and read by its children.  flags (as the oracle's
+// orc_amr_slopes): bit d / 3+d the parent lies at the low / high face of a
+// non-periodic direction d (ext_dir: neighbours beyond it read -1, the slope
+// is one-sided); bit 6+d / 9+d the cell P + 2e_d / P - 2e_d lies inside the
+// domain (the 4-point one-sided form, else the 3-point one).
+struct ExplodeParent {
+    int64_t crse_local;
+    int32_t crse_box;
+    int32_t flags;
 };
 
 // Single-pass z-sweep kernel shapes: rows per tile and z planes per chunk.
@@ -138,11 +150,16 @@ extern const int kNumFusedVariants;
 // are clamped to the FP16 range, and each (block, lattice) write that
 // clamped is counted.  Solid cells hold 0 (downloads return the stream's
 // -1 sentinel there).
+// The largest |population - shift| a box is taken to hold is at least
+// 2^-20: the FP32-arithmetic collide writes rounding noise of about
+// 2^-24 of a population (up to ~6e-8) into a box exactly at the shift
+// state, which a larger exponent would push past the FP16 range.
 constexpr int kScaleSlab = 8;
 constexpr int kF16Target = 12;
+constexpr float kF16MinMax = 9.5367431640625e-07f;  // 2^-20
 __host__ __device__ inline int f16_exponent(float m)
 {
-    if (!(m > 0.0f)) return 40;
+    if (!(m > kF16MinMax)) m = kF16MinMax;
     int k;
     (void)frexpf(m, &k);  // m < 2^k
     const int e = kF16Target - k;
@@ -218,6 +235,10 @@ struct mrb_level {
     int64_t n_gmap1 = 0;
     int64_t *d_bcells = nullptr;        // fab cells outside non-periodic faces
     int64_t n_bcells = 0;
+    // d_bcells is ordered by the distance beyond the domain (sum over the
+    // non-periodic directions of how many cells out); cells of distance
+    // p are [bcell_off[p-1], bcell_off[p])
+    std::vector<int64_t> bcell_off;
     // physics
     mrb::CollideParams cp{};
     mrb_phys phys{};
@@ -252,12 +273,25 @@ struct mrb_level {
     // hot-path kernel variant of the compact layout (compact_variant_name)
     int hot = 0;
     bool hot_default = false;  // hot is the storage's default, not a caller's choice
+    // FP32-arithmetic variant set aside while the level has a PRESSURE face
+    // (those collides have none); restored when the faces change back
+    int hot_h32 = -1;
+    // single-pass z-sweep (compact FP64 levels with rows of at most 256
+    // cells): ring of 4 planes of Q_y, per-row count of published planes,
+    // count of waits that gave up
+    double *d_sweep_q = nullptr;
+    uint32_t *d_sweep_flags = nullptr;
+    uint32_t *d_sweep_fault = nullptr;
     // two-level link, held by the fine level (mrb_amr_link)
     mrb_level *amr_crse = nullptr;
     void *d_coal = nullptr;  // mrb::CoalesceEntry[n_coal]
     int64_t n_coal = 0;
     void *d_expl = nullptr;  // mrb::ExplodeEntry[n_expl]
     int64_t n_expl = 0;
+    void *d_expl_par = nullptr;  // mrb::ExplodeParent[n_expl_par]
+    int64_t n_expl_par = 0;
+    double *d_expl_slopes = nullptr;  // [4 * NQ][parent]: value and limited slopes of one lattice
+    int expl_crse_depth = 1;          // coarse FillBoundary depth the interpolation reads
     int32_t *d_coal_cell = nullptr;  // per coarse fab cell: coalescence entry or -1
     double *d_coal_vals = nullptr;   // [2q + lattice][entry]: masked means of the children
     uint64_t *d_coal_has = nullptr;  // per entry: bit 2q + lattice set if some child is not -1
@@ -321,7 +355,7 @@ int fused_step(mrb_level *L, int variant, hipStream_t s);
 // single pass that recomputes the neighbours' Q_corr; the rest the fused
 // variants; launch reads the current buffers, writes the spare ones, does
 // not flip
-constexpr int kNumTwoPassVariants = 9;
+constexpr int kNumTwoPassVariants = 12;
 constexpr int kOnePassVariant = 2;
 constexpr int kRows8Variant = 3;  // two passes on 64x8 blocks
 constexpr int kRows2Variant = 4;  // two passes on 64x2 blocks
@@ -329,6 +363,26 @@ constexpr int kHalf32Variant = 5; // scaled FP16 storage, FP32 arithmetic in pas
 constexpr int kX128Variant = 6;   // two passes on 128x2 blocks
 constexpr int kX256Variant = 7;   // two passes on 256x1 blocks
 constexpr int kHalf32X256Variant = 8; // variant 5 on 256x1 blocks
+constexpr int kF32Variant = 9;         // shifted FP32 storage, FP32 arithmetic in pass 2 (64x2 blocks)
+constexpr int kF32X256Variant = 10;    // variant 9 on 256x1 blocks
+constexpr int kSweepVariant = 11;      // single-pass z-sweep, one block per x-row (FP64, rows <= 256 cells)
+// the variants whose pass 2 computes in FP32 (their results differ from the
+// FP64-arithmetic variants'), and the FP64-arithmetic variant with the same
+// block shape that replaces one on a level with PRESSURE faces
+inline bool fp32_collide_variant(int v)
+{
+    return v == kHalf32Variant || v == kHalf32X256Variant || v == kF32Variant || v == kF32X256Variant;
+}
+inline int fp64_counterpart(int v)
+{
+    return v == kHalf32Variant ? 0 : v == kF32Variant ? kRows2Variant : kX256Variant;
+}
+// conservation sums (mrb_level.hip): block partial sums (5 per block, at
+// most kConservationBlocks blocks, grid-strided over any level size) into
+// `part`; the caller adds them in block order
+constexpr unsigned kConservationBlocks = 4096;
+unsigned conservation_blocks(const mrb_level *L);
+int conservation_partials(const mrb_level *L, double *part, hipStream_t s);
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s);
 int compact_variant_launch(mrb_level *L, int index, hipStream_t s);
 const char *compact_variant_name(int index);

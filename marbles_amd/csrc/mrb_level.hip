@@ -1056,6 +1056,7 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
              : L->storage == MRB_STORAGE_F32_SHIFTED ? kRows2Variant
                                                       : kHalf32Variant;
     L->hot_default = true;
+    L->hot_h32 = -1;
     if (L->layout == MRB_LAYOUT_COMPACT) {
         if (desc->nbox != 1 || desc->ng != 0)
             return set_error(MRB_E_ARG, "compact layout is one box without ghost cells");
@@ -1112,7 +1113,7 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
     }
     MRB_HIP(hipMalloc(&L->d_isf, sizeof(int32_t) * 2 * T));
     MRB_HIP(hipMemset(L->d_isf, 0, sizeof(int32_t) * 2 * T));
-    MRB_HIP(hipMalloc(&L->d_red, sizeof(double) * 5 * 4096));
+    MRB_HIP(hipMalloc(&L->d_red, sizeof(double) * 5 * kConservationBlocks));
     MRB_HIP(hipMalloc(&L->d_small, sizeof(double) * 6 * NQ));
     for (auto &e : L->ev) MRB_HIP(hipEventCreate(&e));
     MRB_HIP(hipMemset(L->d_small, 0, sizeof(double) * 6 * NQ));
@@ -1153,18 +1154,30 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
         MRB_HIP(hipMalloc(&L->d_tiles_c, sizeof(FabTile) * tiles.size()));
         MRB_HIP(hipMemcpy(L->d_tiles_c, tiles.data(), sizeof(FabTile) * tiles.size(), hipMemcpyHostToDevice));
         // fab cells outside a non-periodic domain face
-        std::vector<int64_t> cells;
+        // by distance beyond the domain: an OUTFLOW ghost copies the cell one
+        // step closer, which a launch per distance has filled before
+        const int maxdist = 3 * L->ng;
+        std::vector<std::vector<int64_t>> bydist(size_t(maxdist + 1));
         for (int b = 0; b < L->nbox; ++b) {
             const BoxDev &B = L->boxes[b];
             for (int k = 0; k < B.n[2]; ++k)
                 for (int j = 0; j < B.n[1]; ++j)
                     for (int i = 0; i < B.n[0]; ++i) {
                         const int iv[3] = {B.lo[0] + i, B.lo[1] + j, B.lo[2] + k};
-                        bool outside = false;
-                        for (int d = 0; d < 3; ++d)
-                            if (!L->periodic[d] && (iv[d] < 0 || iv[d] >= L->ncell[d])) outside = true;
-                        if (outside) cells.push_back(B.base + (int64_t(k) * B.n[1] + j) * B.n[0] + i);
+                        int out = 0;
+                        for (int d = 0; d < 3; ++d) {
+                            if (L->periodic[d]) continue;
+                            if (iv[d] < 0) out += -iv[d];
+                            if (iv[d] >= L->ncell[d]) out += iv[d] - (L->ncell[d] - 1);
+                        }
+                        if (out > 0) bydist[size_t(out)].push_back(B.base + (int64_t(k) * B.n[1] + j) * B.n[0] + i);
                     }
+        }
+        std::vector<int64_t> cells;
+        L->bcell_off.assign(size_t(maxdist + 1), 0);
+        for (int p = 1; p <= maxdist; ++p) {
+            cells.insert(cells.end(), bydist[size_t(p)].begin(), bydist[size_t(p)].end());
+            L->bcell_off[size_t(p)] = int64_t(cells.size());
         }
         L->n_bcells = int64_t(cells.size());
         if (!cells.empty()) {
@@ -1176,6 +1189,15 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
         MRB_HIP(hipMemset(L->d_mask, 0, sizeof(uint32_t) * T));
         MRB_HIP(hipMalloc(&L->d_q, sizeof(double) * 3 * T));
         MRB_HIP(hipMemset(L->d_q, 0, sizeof(double) * 3 * T));
+        if (L->storage == MRB_STORAGE_F64 && L->ncell[0] <= 256) {
+            const size_t plane = size_t(L->ncell[0]) * size_t(L->ncell[1]);
+            MRB_HIP(hipMalloc(&L->d_sweep_q, sizeof(double) * 4 * plane));
+            MRB_HIP(hipMemset(L->d_sweep_q, 0, sizeof(double) * 4 * plane));
+            MRB_HIP(hipMalloc(&L->d_sweep_flags, sizeof(uint32_t) * size_t(L->ncell[1])));
+            MRB_HIP(hipMemset(L->d_sweep_flags, 0, sizeof(uint32_t) * size_t(L->ncell[1])));
+            MRB_HIP(hipMalloc(&L->d_sweep_fault, sizeof(uint32_t)));
+            MRB_HIP(hipMemset(L->d_sweep_fault, 0, sizeof(uint32_t)));
+        }
     }
     // defaults (LBM.H:272-318)
     mrb_phys p{};
@@ -1229,6 +1251,9 @@ int mrb_level_destroy(mrb_level *L)
     (void)hipFree(L->d_gmap);
     (void)hipFree(L->d_gmap1);
     (void)hipFree(L->d_bcells);
+    (void)hipFree(L->d_sweep_q);
+    (void)hipFree(L->d_sweep_flags);
+    (void)hipFree(L->d_sweep_fault);
     (void)hipFree(L->d_tiles_c);
     amr_release(L);
     delete L;
@@ -1291,6 +1316,15 @@ int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
         }
     }
     if (bc->velbc_kind < 0 || bc->velbc_kind > 3) return set_error(MRB_E_ARG, "unknown velocity bc kind");
+    // compact layout: a Channel or Parabolic inlet face meeting an OUTFLOW
+    // face that BCFill applies after it (a higher direction) would read the
+    // profile at an edge ghost the OUTFLOW face copies; not supported
+    if (L->layout == MRB_LAYOUT_COMPACT && bc->velbc_kind >= 2)
+        for (int f = 0; f < 6; ++f)
+            for (int g = 0; g < 6; ++g)
+                if (bc->bc_type[f] == MRB_BC_VELOCITY && bc->bc_type[g] == MRB_BC_OUTFLOW && g % 3 > f % 3)
+                    return set_error(MRB_E_UNSUPPORTED,
+                                     "velocity profile face next to a later OUTFLOW face (compact layout)");
     if ((bc->velbc_kind == 1 || bc->velbc_kind == 3) && (bc->vel_dir < 0 || bc->vel_dir > 2))
         return set_error(MRB_E_ARG, "velocity bc dir must be 0, 1 or 2");
     if (bc->velbc_kind == 3 && (bc->normal_dir < 0 || bc->normal_dir > 2))
@@ -1300,9 +1334,15 @@ int mrb_set_bc(mrb_level *L, const mrb_bc *bc)
     // the FP32-arithmetic collide has no PRESSURE faces: such a level takes
     // the FP64-arithmetic two-pass step (and returns to the FP32 one when
     // its faces change back)
-    if (L->storage == MRB_STORAGE_F16_SCALED && L->hot == kHalf32Variant && L->bcdev.pdir >= 0) L->hot = 0;
-    else if (L->storage == MRB_STORAGE_F16_SCALED && L->hot == 0 && L->hot_default && L->bcdev.pdir < 0)
-        L->hot = kHalf32Variant;
+    if (L->storage != MRB_STORAGE_F64) {
+        if (L->bcdev.pdir >= 0 && fp32_collide_variant(L->hot)) {
+            L->hot_h32 = L->hot;
+            L->hot = fp64_counterpart(L->hot);
+        } else if (L->bcdev.pdir < 0 && L->hot_h32 >= 0 && L->hot == fp64_counterpart(L->hot_h32)) {
+            L->hot = L->hot_h32;
+        }
+        if (L->bcdev.pdir < 0) L->hot_h32 = -1;
+    }
     if (L->bc.velbc_kind >= 2 && L->layout == MRB_LAYOUT_COMPACT) {
         for (int f = 0; f < 6; ++f)
             if (L->bc.bc_type[f] == MRB_BC_VELOCITY && !L->bcdev.tab[f])
@@ -1751,10 +1791,15 @@ int mrb_physbc(mrb_level *L, int lattice, void *stream)
     if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
     if (L->n_bcells == 0) return MRB_OK;
     hipStream_t s = as_stream(stream);
-    k_physbc_fab<<<nblocks(L->n_bcells), kBlock, 0, s>>>(
-        (double *)L->pop[lattice][L->cur], L->d_boxes, L->d_box_base, L->nbox, L->d_bcells, L->n_bcells, 0,
-        0, 0, L->ncell[0] - 1, L->ncell[1] - 1, L->ncell[2] - 1, L->bcdev, lattice);
-    MRB_HIP(hipGetLastError());
+    for (size_t p = 1; p < L->bcell_off.size(); ++p) {
+        const int64_t n = L->bcell_off[p] - L->bcell_off[p - 1];
+        if (n == 0) continue;
+        k_physbc_fab<<<nblocks(n), kBlock, 0, s>>>((double *)L->pop[lattice][L->cur], L->d_boxes, L->d_box_base,
+                                                   L->nbox, L->d_bcells + L->bcell_off[p - 1], n, 0, 0, 0,
+                                                   L->ncell[0] - 1, L->ncell[1] - 1, L->ncell[2] - 1, L->bcdev,
+                                                   lattice);
+        MRB_HIP(hipGetLastError());
+    }
     return MRB_OK;
 }
 
@@ -1894,6 +1939,7 @@ int mrb_set_hot_kernel(mrb_level *L, int index)
     if (L->layout != MRB_LAYOUT_COMPACT) return set_error(MRB_E_STATE, "needs the compact layout");
     L->hot = index;
     L->hot_default = false;
+    L->hot_h32 = -1;
     return MRB_OK;
 }
 
@@ -1954,10 +2000,23 @@ int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best
     if (reps < 1) return set_error(MRB_E_ARG, "reps must be positive");
     hipStream_t s = as_stream(stream);
     const int n = mrb_hot_kernel_count();
-    int arg = 0;
+    int arg = L->hot;
     double best_ms = 1e300;
+    // reduced-precision levels: the candidates keep the arithmetic of the
+    // current step (the FP32- and FP64-arithmetic collides give different
+    // results); the FP16 saturation counter is restored afterwards, so that
+    // the timing launches do not add to it
+    const bool f16 = L->storage == MRB_STORAGE_F16_SCALED;
+    const bool reduced = L->storage != MRB_STORAGE_F64;
+    uint32_t ovf_saved = 0;
+    if (f16) MRB_HIP(hipMemcpyAsync(&ovf_saved, L->d_ovf16, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (f16) MRB_HIP(hipStreamSynchronize(s));
     for (int v = 0; v < n; ++v) {
         auto launch = [&]() { return compact_variant_launch(L, v, s); };
+        if (reduced && fp32_collide_variant(v) != fp32_collide_variant(L->hot)) {
+            if (ms_out) ms_out[v] = -1.0;
+            continue;
+        }
         int rc = launch();  // warm-up
         if (rc == MRB_E_UNSUPPORTED) {
             // variant not available for this level (storage, boundary types)
@@ -1979,8 +2038,13 @@ int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best
             arg = v;
         }
     }
+    if (f16) {
+        MRB_HIP(hipMemcpyAsync(L->d_ovf16, &ovf_saved, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        MRB_HIP(hipStreamSynchronize(s));
+    }
     L->hot = arg;
     L->hot_default = false;
+    L->hot_h32 = -1;
     if (best) *best = arg;
     return MRB_OK;
 }
@@ -1993,31 +2057,49 @@ int mrb_stream_compact(mrb_level *L, void *stream)
     return compact_stream_only(L, as_stream(stream));
 }
 
-int mrb_reduce_conservation(const mrb_level *L, double out[5], void *stream)
+}  // extern "C"
+
+namespace mrb {
+
+unsigned conservation_blocks(const mrb_level *L) { return std::min<unsigned>(kConservationBlocks, nblocks(L->total_cells)); }
+
+int conservation_partials(const mrb_level *L, double *part, hipStream_t s)
 {
-    if (!L || !out) return set_error(MRB_E_ARG, "null argument");
-    hipStream_t s = as_stream(stream);
-    const unsigned nb = std::min<unsigned>(4096, nblocks(L->total_cells));
+    const unsigned nb = conservation_blocks(L);
     // shift tables live at d_small[0..2*NQ) (zero for FP64 storage)
     const double *shifts = L->d_small;
     const int64_t nxy = int64_t(L->ncell[0]) * L->ncell[1];
     if (L->storage == MRB_STORAGE_F64) {
         k_conservation<double><<<nb, kBlock, 0, s>>>(
             (const double *)L->pop[0][L->cur], (const double *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L),
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, part, comp_stride(L), cell_stride(L),
             nullptr, nxy);
     } else if (L->storage == MRB_STORAGE_F32_SHIFTED) {
         k_conservation<float><<<nb, kBlock, 0, s>>>(
             (const float *)L->pop[0][L->cur], (const float *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L),
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, part, comp_stride(L), cell_stride(L),
             nullptr, nxy);
     } else {
         k_conservation<_Float16><<<nb, kBlock, 0, s>>>(
             (const _Float16 *)L->pop[0][L->cur], (const _Float16 *)L->pop[1][L->cur], L->d_isf, L->d_boxes,
-            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, L->d_red, comp_stride(L), cell_stride(L),
+            L->d_box_base, L->nbox, L->total_cells, shifts, shifts + NQ, part, comp_stride(L), cell_stride(L),
             L->d_e16[L->cur], nxy);
     }
     MRB_HIP(hipGetLastError());
+    return MRB_OK;
+}
+
+}  // namespace mrb
+
+extern "C" {
+
+int mrb_reduce_conservation(const mrb_level *L, double out[5], void *stream)
+{
+    if (!L || !out) return set_error(MRB_E_ARG, "null argument");
+    hipStream_t s = as_stream(stream);
+    const unsigned nb = conservation_blocks(L);
+    int rc = conservation_partials(L, L->d_red, s);
+    if (rc) return rc;
     std::vector<double> part(size_t(5 * nb));
     MRB_HIP(hipMemcpyAsync(part.data(), L->d_red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
     MRB_HIP(hipStreamSynchronize(s));

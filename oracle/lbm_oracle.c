@@ -755,23 +755,35 @@ static void bcfill_cell(orc_level *L, int b, double *data, const int iv[3], int 
     }
 }
 
+/* PhysBCFunct's application of BCFill (FillPatchOps.H:113,153) to every
+ * fab cell beyond a non-periodic face.  The reference runs the functor on
+ * all those cells at once (a GPU kernel: an OUTFLOW ghost that copies
+ * another ghost races with that ghost's own fill); here the cells are
+ * taken in order of their distance beyond the domain (the sum over
+ * non-periodic directions of how many cells out they lie), so that the
+ * cell an OUTFLOW ghost copies (one step closer) is final before it is
+ * read: an edge ghost takes the value its face neighbour's boundary fill
+ * gives.  Within one distance the fills are independent. */
 void orc_physbc(orc_level *L, int lattice)
 {
     double **arr = lattice == 0 ? L->f : L->g;
-    for (int b = 0; b < L->nbox; b++) {
-        const fabdim *fd = &L->fdg[b];
-        for (int k = fd->lo[2]; k <= fd->hi[2]; k++)
-            for (int j = fd->lo[1]; j <= fd->hi[1]; j++)
-                for (int i = fd->lo[0]; i <= fd->hi[0]; i++) {
-                    const int iv[3] = {i, j, k};
-                    int outside = 0;
-                    for (int d = 0; d < 3; d++)
-                        if (!L->periodic[d] &&
-                            (iv[d] < L->dom_lo[d] || iv[d] > L->dom_hi[d]))
-                            outside = 1;
-                    if (outside) bcfill_cell(L, b, arr[b], iv, lattice == 1);
-                }
-    }
+    const int maxdist = 3 * L->ng;
+    for (int dist = 1; dist <= maxdist; dist++)
+        for (int b = 0; b < L->nbox; b++) {
+            const fabdim *fd = &L->fdg[b];
+            for (int k = fd->lo[2]; k <= fd->hi[2]; k++)
+                for (int j = fd->lo[1]; j <= fd->hi[1]; j++)
+                    for (int i = fd->lo[0]; i <= fd->hi[0]; i++) {
+                        const int iv[3] = {i, j, k};
+                        int out = 0;
+                        for (int d = 0; d < 3; d++) {
+                            if (L->periodic[d]) continue;
+                            if (iv[d] < L->dom_lo[d]) out += L->dom_lo[d] - iv[d];
+                            if (iv[d] > L->dom_hi[d]) out += iv[d] - L->dom_hi[d];
+                        }
+                        if (out == dist) bcfill_cell(L, b, arr[b], iv, lattice == 1);
+                    }
+        }
 }
 
 /* FillPatchOps::fillpatch for level 0 (FillPatchOps.H:75-132): the bounce
@@ -1629,24 +1641,121 @@ void orc_amr_coalesce(const orc_level *fine, orc_level *crse, int lattice)
  * s = sign(dc) min(|dc|, 2|dl|, 2|dr|) with dc = (c+ - c-)/2, dl = c - c-,
  * dr = c+ - c, and 0 where dl and dr differ in sign (local extremum) or a
  * neighbour is outside the domain. */
-double orc_amr_interp(
-    double c, const double cm[3], const double cp[3], const int has_m[3],
-    const int has_p[3], const int hi_child[3])
+/* Coarse-to-fine interpolation of FillPatchTwoLevels / InterpFromCoarseLevel
+ * with m_mapper = &amrex::cell_cons_interp (FillPatchOps.H:124-128,174-177,
+ * 190): AMReX's CellConservativeLinear with do_linear_limiting = false, whose
+ * published steps (AMReX MFInterp: mf_compute_slopes_{x,y,z},
+ * mf_cell_cons_lin_interp_mcslope, mf_cell_cons_lin_interp) are restated
+ * here for refinement ratio 2.  AMReX itself is the un-vendored submodule
+ * Submodules/AMReX (commit not recorded), so the steps follow its published
+ * source as described, not a copy of it:
+ *  - per direction, the centred difference dc, replaced at an ext_dir face
+ *    (every non-periodic face, LBM.cpp:108-150) by the one-sided second-order
+ *    form through the ghost value (4-point form, or a 3-point fallback when
+ *    the cell two further in is not available);
+ *  - MC limiting: s = sign(dc) min(|dc|, min(|df|, |db|)) with df, db twice
+ *    the forward and backward differences, 0 when they differ in sign;
+ *  - one factor alpha <= 1 per coarse cell so that no child leaves the
+ *    min/max of the 3x3x3 coarse neighbourhood: the largest child deviation
+ *    dumax = sum_d |s_d| (r-1)/(2r) is held within umax - u and u - umin;
+ *  - child = u + xoff s_x alpha + yoff s_y alpha + zoff s_z alpha with
+ *    offsets -+1/4.
+ * u[27]: the neighbourhood, u[(oz+1)*9 + (oy+1)*3 + (ox+1)]; u2[2d] the
+ * value at P + 2 e_d (read at a low face), u2[2d+1] at P - 2 e_d (high
+ * face); flags: bit d low ext_dir face in d, bit 3+d high face, bit 6+d
+ * u2[2d] available, bit 9+d u2[2d+1] available.  slope[d] receives
+ * s_d alpha. */
+void orc_amr_slopes(const double u[27], const double u2[6], int flags, double slope[3])
 {
+    const double uc = u[13];
     double s[3];
     for (int d = 0; d < 3; d++) {
-        s[d] = 0.0;
-        if (!has_m[d] || !has_p[d]) continue;
-        const double dc = 0.5 * (cp[d] - cm[d]);
-        const double dl = c - cm[d];
-        const double dr = cp[d] - c;
-        if (dl * dr > 0.0)
-            s[d] = copysign(fmin(fabs(dc), fmin(2.0 * fabs(dl), 2.0 * fabs(dr))), dc);
+        const int st = d == 0 ? 1 : (d == 1 ? 3 : 9);
+        const double um = u[13 - st], up = u[13 + st];
+        double dc = 0.5 * (up - um);
+        if (flags & (1 << d)) {
+            if (flags & (1 << (6 + d)))
+                dc = -16. / 15. * um + 0.5 * uc + 2. / 3. * up - 0.1 * u2[2 * d];
+            else
+                dc = 0.25 * (up + 5. * uc - 6. * um);
+        }
+        if (flags & (1 << (3 + d))) {
+            if (flags & (1 << (9 + d)))
+                dc = 16. / 15. * up - 0.5 * uc - 2. / 3. * um + 0.1 * u2[2 * d + 1];
+            else
+                dc = -0.25 * (um + 5. * uc - 6. * up);
+        }
+        const double df = 2.0 * (up - uc);
+        const double db = 2.0 * (uc - um);
+        double sl = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+        s[d] = copysign(1.0, dc) * fmin(sl, fabs(dc));
     }
+    double alpha = 1.0;
+    if (s[0] != 0.0 || s[1] != 0.0 || s[2] != 0.0) {
+        const double dumax = fabs(s[0]) * 1.0 / 4.0 + fabs(s[1]) * 1.0 / 4.0 + fabs(s[2]) * 1.0 / 4.0;
+        double umax = uc, umin = uc;
+        for (int n = 0; n < 27; n++) {
+            umin = fmin(umin, u[n]);
+            umax = fmax(umax, u[n]);
+        }
+        if (dumax * alpha > (umax - uc)) alpha = (umax - uc) / dumax;
+        if (dumax * alpha > (uc - umin)) alpha = (uc - umin) / dumax;
+    }
+    for (int d = 0; d < 3; d++) slope[d] = s[d] * alpha;
+}
+
+double orc_amr_interp(const double u[27], const double u2[6], int flags, const int hi_child[3])
+{
+    double sl[3];
+    orc_amr_slopes(u, u2, flags, sl);
     const double ox = hi_child[0] ? 0.25 : -0.25;
     const double oy = hi_child[1] ? 0.25 : -0.25;
     const double oz = hi_child[2] ? 0.25 : -0.25;
-    return ((c + s[0] * ox) + s[1] * oy) + s[2] * oz;
+    return u[13] + ox * sl[0] + oy * sl[1] + oz * sl[2];
+}
+
+/* Coarse value of component q at cell c as the interpolation's coarse
+ * patch holds it: the valid value of the cell or of its periodic image;
+ * beyond a non-periodic face, -1.  There the patch (the coarsened fine
+ * region grown by one cell) is filled by BCFill, whose `inside` box is the
+ * patch shrunk by the level's 3 ghost cells (BC.H:375-380, m_ng =
+ * m_f[0].nGrowVect(), LBM.cpp:1394); a ghost cell's neighbour iv + c_q lies
+ * at most one cell inside the domain, never inside that box, so every
+ * component gets the -1 dummy (BC.H:463-466). */
+static double crse_patch_value(const orc_level *crse, double *const *ca, int q, const int c[3])
+{
+    int s[3];
+    if (!periodic_image(crse, c, s)) return -1.0;
+    const int ob = owner_of(crse, s);
+    return ca[ob][fidx(&crse->fdg[ob], q, s[0], s[1], s[2])];
+}
+
+/* the neighbourhood, face flags and extra values of coarse parent P */
+static void crse_neighbourhood(const orc_level *crse, double *const *ca, int q, const int P[3], double u[27],
+                               double u2[6], int *flags)
+{
+    int fl = 0;
+    for (int d = 0; d < 3; d++) {
+        if (crse->periodic[d]) continue;
+        if (P[d] == crse->dom_lo[d]) fl |= 1 << d;
+        if (P[d] == crse->dom_hi[d]) fl |= 1 << (3 + d);
+        if (P[d] + 2 <= crse->dom_hi[d]) fl |= 1 << (6 + d);
+        if (P[d] - 2 >= crse->dom_lo[d]) fl |= 1 << (9 + d);
+    }
+    for (int oz = -1; oz <= 1; oz++)
+        for (int oy = -1; oy <= 1; oy++)
+            for (int ox = -1; ox <= 1; ox++) {
+                const int c[3] = {P[0] + ox, P[1] + oy, P[2] + oz};
+                u[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] = crse_patch_value(crse, ca, q, c);
+            }
+    for (int d = 0; d < 3; d++) {
+        int a[3] = {P[0], P[1], P[2]}, b[3] = {P[0], P[1], P[2]};
+        a[d] += 2;
+        b[d] -= 2;
+        u2[2 * d] = (fl & (1 << d)) && (fl & (1 << (6 + d))) ? crse_patch_value(crse, ca, q, a) : 0.0;
+        u2[2 * d + 1] = (fl & (1 << (3 + d))) && (fl & (1 << (9 + d))) ? crse_patch_value(crse, ca, q, b) : 0.0;
+    }
+    *flags = fl;
 }
 
 void orc_amr_explode(const orc_level *crse, orc_level *fine, int lattice)
@@ -1668,31 +1777,16 @@ void orc_amr_explode(const orc_level *crse, orc_level *fine, int lattice)
                     if (!periodic_image(fine, iv, s)) continue; /* physbc */
                     if (owner_of(fine, s) >= 0) continue;       /* FillBoundary */
                     const int P[3] = {s[0] / 2, s[1] / 2, s[2] / 2};
-                    const int ob = owner_of(crse, P);
-                    if (ob < 0) {
+                    if (owner_of(crse, P) < 0) {
                         fprintf(stderr, "oracle: coarse level does not cover its domain\n");
                         abort();
                     }
-                    const fabdim *cd = &crse->fdg[ob];
-                    int has_m[3], has_p[3], hi_child[3];
-                    for (int d = 0; d < 3; d++) {
-                        has_m[d] = crse->periodic[d] || P[d] - 1 >= crse->dom_lo[d];
-                        has_p[d] = crse->periodic[d] || P[d] + 1 <= crse->dom_hi[d];
-                        hi_child[d] = s[d] & 1;
-                    }
+                    const int hi_child[3] = {s[0] & 1, s[1] & 1, s[2] & 1};
                     for (int q = 0; q < ORC_NQ; q++) {
-                        const double *cq = ca[ob];
-                        const double c = cq[fidx(cd, q, P[0], P[1], P[2])];
-                        double cm[3], cp[3];
-                        for (int d = 0; d < 3; d++) {
-                            int m[3] = {P[0], P[1], P[2]}, p[3] = {P[0], P[1], P[2]};
-                            m[d] -= 1;
-                            p[d] += 1;
-                            cm[d] = has_m[d] ? cq[fidx(cd, q, m[0], m[1], m[2])] : 0.0;
-                            cp[d] = has_p[d] ? cq[fidx(cd, q, p[0], p[1], p[2])] : 0.0;
-                        }
-                        fa[b][fidx(fd, q, i, j, k)] =
-                            orc_amr_interp(c, cm, cp, has_m, has_p, hi_child);
+                        double u[27], u2[6];
+                        int fl;
+                        crse_neighbourhood(crse, ca, q, P, u, u2, &fl);
+                        fa[b][fidx(fd, q, i, j, k)] = orc_amr_interp(u, u2, fl, hi_child);
                     }
                 }
     }
@@ -1725,26 +1819,12 @@ void orc_amr_regrid_fill(const orc_level *crse, const orc_level *old_fine, orc_l
                             continue;
                         }
                         const int P[3] = {i / 2, j / 2, k / 2};
-                        const int cb = owner_of(crse, P);
-                        const fabdim *cd = &crse->fdg[cb];
-                        int has_m[3], has_p[3], hi_child[3];
-                        for (int d = 0; d < 3; d++) {
-                            has_m[d] = crse->periodic[d] || P[d] - 1 >= crse->dom_lo[d];
-                            has_p[d] = crse->periodic[d] || P[d] + 1 <= crse->dom_hi[d];
-                            hi_child[d] = s[d] & 1;
-                        }
+                        const int hi_child[3] = {s[0] & 1, s[1] & 1, s[2] & 1};
                         for (int q = 0; q < ORC_NQ; q++) {
-                            const double *cq = ca[cb];
-                            const double c = cq[fidx(cd, q, P[0], P[1], P[2])];
-                            double cm[3], cp[3];
-                            for (int d = 0; d < 3; d++) {
-                                int m[3] = {P[0], P[1], P[2]}, p[3] = {P[0], P[1], P[2]};
-                                m[d] -= 1;
-                                p[d] += 1;
-                                cm[d] = has_m[d] ? cq[fidx(cd, q, m[0], m[1], m[2])] : 0.0;
-                                cp[d] = has_p[d] ? cq[fidx(cd, q, p[0], p[1], p[2])] : 0.0;
-                            }
-                            fa[b][fidx(fd, q, i, j, k)] = orc_amr_interp(c, cm, cp, has_m, has_p, hi_child);
+                            double u[27], u2[6];
+                            int fl;
+                            crse_neighbourhood(crse, ca, q, P, u, u2, &fl);
+                            fa[b][fidx(fd, q, i, j, k)] = orc_amr_interp(u, u2, fl, hi_child);
                         }
                     }
             const int32_t *isf = fine->isf[b];

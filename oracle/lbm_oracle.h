@@ -152,12 +152,15 @@ void orc_amr_explode(const orc_level *crse, orc_level *fine, int lattice);
  * fine level where it has them, else the coarse interpolation; 0 in solid
  * cells); the new level's flags must be set. */
 void orc_amr_regrid_fill(const orc_level *crse, const orc_level *old_fine, orc_level *fine);
-/* the interpolation formula alone for one fine child: coarse value c, its
- * -/+ neighbours per direction (has_m/has_p = 0 drops that direction's
- * slope), child offset sign per direction (0 = low child, 1 = high) */
-double orc_amr_interp(
-    double c, const double cm[3], const double cp[3], const int has_m[3],
-    const int has_p[3], const int hi_child[3]);
+/* the interpolation alone (AMReX cell_cons_interp, ratio 2) for one coarse
+ * parent: u[27] its 3x3x3 neighbourhood (u[(oz+1)*9+(oy+1)*3+(ox+1)]; a
+ * cell beyond a non-periodic face holds -1), u2[2d] / u2[2d+1] the values
+ * at P + 2e_d / P - 2e_d, flags bit d / 3+d: P at the low / high ext_dir
+ * face in d, bit 6+d / 9+d: u2[2d] / u2[2d+1] available.  orc_amr_slopes
+ * gives the limited slopes (MC slopes times the min/max factor alpha);
+ * orc_amr_interp the child (hi_child[d] = 0 low, 1 high). */
+void orc_amr_slopes(const double u[27], const double u2[6], int flags, double slope[3]);
+double orc_amr_interp(const double u[27], const double u2[6], int flags, const int hi_child[3]);
 /* nsteps coarse steps of LBM::evolve/time_step/advance (LBM.cpp:398-544)
  * for a two-level hierarchy; the fine level advances two substeps per
  * coarse step with its own phys (dt halved). */

@@ -96,7 +96,8 @@ def load() -> ctypes.CDLL:
                                  c_double, c_double]),
         "orc_amr_coalesce": (None, [c_void_p, c_void_p, c_int]),
         "orc_amr_explode": (None, [c_void_p, c_void_p, c_int]),
-        "orc_amr_interp": (c_double, [c_double, D3, D3, I3, I3, I3]),
+        "orc_amr_interp": (c_double, [c_void_p, c_void_p, c_int, I3]),
+        "orc_amr_slopes": (None, [c_void_p, c_void_p, c_int, c_void_p]),
         "orc_amr_evolve": (None, [c_void_p, c_void_p, c_int]),
         "orc_set_threads": (None, [c_int]),
         "orc_get_threads": (c_int, []),
@@ -343,8 +344,21 @@ def amr_evolve(crse: OracleLevel, fine: OracleLevel, nsteps: int) -> None:
     load().orc_amr_evolve(crse.h, fine.h, int(nsteps))
 
 
-def amr_interp(c, cm, cp, has_m, has_p, hi_child) -> float:
-    return load().orc_amr_interp(float(c), d3(cm), d3(cp), i3(has_m), i3(has_p), i3(hi_child))
+def amr_interp(u, hi_child, u2=None, flags=0) -> float:
+    """AMReX cell_cons_interp child of the coarse neighbourhood u (3x3x3,
+    indexed [oz+1, oy+1, ox+1]); see orc_amr_slopes for u2 and flags."""
+    a = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(27))
+    b = np.ascontiguousarray(np.zeros(6) if u2 is None else np.asarray(u2, dtype=np.float64).reshape(6))
+    return load().orc_amr_interp(a.ctypes.data_as(c_void_p), b.ctypes.data_as(c_void_p), int(flags), i3(hi_child))
+
+
+def amr_slopes(u, u2=None, flags=0) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(27))
+    b = np.ascontiguousarray(np.zeros(6) if u2 is None else np.asarray(u2, dtype=np.float64).reshape(6))
+    out = np.zeros(3)
+    load().orc_amr_slopes(a.ctypes.data_as(c_void_p), b.ctypes.data_as(c_void_p), int(flags),
+                          out.ctypes.data_as(c_void_p))
+    return out
 
 
 def set_threads(n: int) -> None:
@@ -383,6 +397,10 @@ def build_oracle_level(cfg: dict, boxes=None, ng: int = 3) -> OracleLevel:
 
         L.ic_fields(*fab_fields(ic["kind"], ic["params"], cfg["ncell"], cfg["prob_lo"], cfg["prob_hi"],
                                 L.boxes, ng), adiabatic_exponent=ic["adiabatic_exponent"], m_bar=ic["m_bar"])
+    elif ic["type"] == "spectral":
+        from marbles_amd.configs import spectral_fab_fields
+
+        L.ic_fields(*spectral_fab_fields(cfg, L.boxes, ng), adiabatic_exponent=ic["adiabatic_exponent"])
     else:
         L.ic_constant(ic["density"], ic["mach"], ic["temperature"], ic["adiabatic_exponent"])
     L.init_finish()

@@ -12,6 +12,17 @@ def small_amr(n: int = 16, seed: int = 5, fraction: float = 0.3, nbox: int = 24,
                                  seed=seed, **kw)
 
 
+def with_sphere_across_fine_edge(cfg: dict, radius: float = 3.5) -> dict:
+    """The workload with an EB sphere centred on the high-x face of the first
+    fine box's coarse footprint: solid cells under the footprint, in the
+    coalescence ring and in the explosion shell, on both levels."""
+    cfg = dict(cfg)
+    lo, hi = cfg["fine_boxes"][0]
+    centre = ((hi[0] + 1) / 2.0, (lo[1] + hi[1] + 1) / 4.0, (lo[2] + hi[2] + 1) / 4.0)
+    cfg["eb"] = {"type": "sphere", "center": centre, "radius": radius, "has_fluid_inside": 0}
+    return cfg
+
+
 def fine_ncell(cfg):
     return tuple(2 * n for n in cfg["ncell"])
 
@@ -28,7 +39,7 @@ def oracle_hierarchy(cfg: dict, ng: int = 3, fine_boxes=None):
     for L, dt in ((C, 1.0), (F, 0.5)):
         L.set_phys(cfg["nu"], cfg.get("alpha"), dt=dt)
         L.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
-        L.flags_all_regular()
+        configs.set_eb_flags(L, cfg.get("eb", {"type": "all_regular"}))
         L.ic_fields(*configs.level_fields(cfg, L.ncell, L.boxes, ng), adiabatic_exponent=gamma)
         L.init_finish()
     return C, F

@@ -3,7 +3,7 @@ through the C ABI (bit-exact expected: FP64, same operation order)."""
 import numpy as np
 import pytest
 
-from amr_cases import oracle_hierarchy, small_amr
+from amr_cases import oracle_hierarchy, small_amr, with_sphere_across_fine_edge
 
 pytestmark = pytest.mark.gpu
 
@@ -42,6 +42,7 @@ CASES = {
     "outflow_x": lambda: _nonperiodic(small_amr(n=32, nbox=80, seed=9, fraction=0.25)),
     "uneven_32": lambda: small_amr(n=32, nbox=80, seed=4, fraction=0.2),
     "inlet_pressure": lambda: _inlet_pressure(small_amr(n=32, nbox=80, seed=9, fraction=0.25)),
+    "eb_sphere": lambda: with_sphere_across_fine_edge(small_amr(n=32, nbox=80, seed=4, fraction=0.2)),
 }
 
 

@@ -203,7 +203,10 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *  11  k_sweep: FP64 storage with rows of at most 256 cells, one pass as a
  *      z-sweep with one resident block per x-row (cooperative launch;
  *      MRB_E_UNSUPPORTED when the rows do not fit the GPU at once);
- *  12+ the single-pass LDS z-sweep with different tile shapes.
+ *  12  k_sweep_x64: the same z-sweep on single-wave blocks of 64 cells of
+ *      a row (any row length; MRB_E_UNSUPPORTED when the blocks do not fit
+ *      the GPU at once);
+ *  13+ the single-pass LDS z-sweep with different tile shapes.
  * The FP64-arithmetic variants give the same populations bit for bit; 5
  * and 8 give the same as each other, and 9 and 10.  A level
  * starts on variant 7 (FP64 storage), 4 (shifted FP32) or 5 (scaled FP16).

@@ -168,6 +168,14 @@ def load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise MrbError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    # torch ships its own libamdhip64.so.7; in one process the HIP runtime
+    # loaded first serves every library that needs that soname, and torch
+    # finds no GPU on any runtime but its own.  Loading torch first lets the
+    # engine and torch tensors share torch's runtime in any call order.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

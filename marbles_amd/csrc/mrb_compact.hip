@@ -795,26 +795,30 @@ __global__ void __launch_bounds__(256) k_collide_f32_x256(
 // ---------------------------------------------------------------------------
 // Single pass as a z-sweep (FP64 storage, rows of at most 256 cells): one
 // block per x-row y, all rows resident at once (cooperative launch).  The
-// block walks z; at plane z it pulls plane z+1 into registers and forms
-// its Q_corr, publishes Q_y of that row for the rows y -+ 1, waits until
-// those rows have published plane z, then collides plane z from the pull
-// it holds in LDS, and moves plane z+1 into LDS.  Each population is read
+// block walks z; at step z it issues the loads of plane z+1, collides
+// plane z-1 from the pull held in LDS once the rows y -+ 1 have published
+// plane z-1 (the loads are in flight meanwhile), moves plane z from
+// registers into LDS, then forms plane z+1's Q_corr and publishes its Q_y
+// for the rows beside.  Each population is read
 // from HBM once and written once; the neighbours' Q_corr travel through
-// LDS (x), registers (z) and a ring of 4 planes of Q_y in global memory (y),
+// LDS (x), registers (z) and a ring of 8 planes of Q_y in global memory (y),
 // read and written past the non-coherent L2 (agent-scope accesses), with a
-// per-row count of published planes.  A row can run at most two planes
-// ahead of a neighbour (it waits for the neighbour's previous plane before
-// publishing the next), so 4 ring planes are never overwritten while read.
+// per-row count of published planes.  A row publishes plane k+1 only
+// after it waited for its neighbours' plane k-1, so a neighbour still reads
+// plane k-2 or later: the ring's 8 planes (4 would do) are never
+// overwritten while read.
 // The cell arithmetic is the two-pass step's (same functions, same order).
 // ---------------------------------------------------------------------------
-constexpr int kSweepRing = 4;
+constexpr int kSweepRing = kSweepRingPlanes;
 constexpr int kSweepRow = 256;
 constexpr uint32_t kSweepSpinLimit = 1u << 22;
 
 // f_q (or g_q) of the block's row, cell x, held in LDS at [2q + lattice][x]
+// (stride: 2 x the block's row width)
 struct LdsRow {
     const double *p;
-    __device__ __forceinline__ double operator[](int q) const { return p[q * 2 * kSweepRow]; }
+    int stride = 2 * kSweepRow;
+    __device__ __forceinline__ double operator[](int q) const { return p[q * stride]; }
 };
 
 __device__ __forceinline__ void store_agent(double *p, double v)
@@ -968,6 +972,151 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
         } else {
             qz1 = 0.0;
             __syncthreads();
+        }
+    }
+}
+
+// The z-sweep on single-wave blocks of 64 cells of an x-row: four blocks,
+// independent of each other, share a CU (the 256-wide block runs its four
+// waves in lockstep).  A block exchanges Q_y with the blocks of rows y -+ 1
+// and the Q_x of its two end cells with the blocks beside in x, through
+// the same ring (Q_x of the end cells in a second ring), and waits for all
+// four; inside the block Q_x moves between lanes.
+constexpr int kSweepW = 64;
+
+template <int BCX>
+__global__ void __launch_bounds__(kSweepW, 1) k_sweep_x64(
+    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo, double *__restrict__ go,
+    const uint32_t *__restrict__ mask, double *qring, double *qxring, uint32_t *flags, uint32_t *fault,
+    CompactArgs A, BCDev bc, ShiftTab sh, CollideParams P)
+{
+    __shared__ double hold[2 * NQ * kSweepW];
+    const int lane = int(threadIdx.x), xb = int(blockIdx.x), y = int(blockIdx.y), nxb = int(gridDim.x);
+    const int x = xb * kSweepW + lane;
+    const bool in = x < A.nx;
+    const int last = min(kSweepW, A.nx - xb * kSweepW) - 1;  // the block's last lane inside the domain
+    const int64_t nxy = int64_t(A.nx) * A.ny;
+    const int64_t row = int64_t(y) * A.nx + x;
+    const int64_t edge_plane = int64_t(A.ny) * nxb * 2;  // Q_x ring entries per plane
+    uint32_t *const own_flag = &flags[y * nxb + xb];
+    double rf[NQ], rg[NQ], nf[NQ], ng[NQ];
+    auto pull_into = [&](int zp, uint32_t &m, double (&pf)[NQ], double (&pg)[NQ]) {
+        m = 0u;
+        if (!in) return;
+        const int64_t cell = int64_t(zp) * nxy + row;
+        m = mask[cell];
+        if (!(m & 1u)) return;
+        pull_cell<BCX>(f, g, A, bc, sh, Half16{}, x, y, zp, cell, m, pf, pg);
+    };
+    auto q_of = [&](uint32_t m, const double (&pf)[NQ], const double (&pg)[NQ], double &qx, double &qy,
+                    double &qz) {
+        qx = qy = qz = 0.0;
+        if (!in || !(m & 1u)) return;
+        double rho, jx, jy, jz, E;
+        accumulate_q_moments(pf, pg, rho, jx, jy, jz, E);
+        const Macro M = macro_from(rho, jx, jy, jz, E, P);
+        qx = M.Qx;
+        qy = M.Qy;
+        qz = M.Qz;
+    };
+    auto publish = [&](int zp, double qx, double qy) {
+        const int slot = zp % kSweepRing;
+        if (in) store_agent(&qring[int64_t(slot) * nxy + row], qy);
+        if (lane == 0 || lane == last)
+            store_agent(&qxring[int64_t(slot) * edge_plane + (int64_t(y) * nxb + xb) * 2 + (lane == 0 ? 0 : 1)], qx);
+        drain_wave();
+        if (lane == 0) __hip_atomic_store(own_flag, uint32_t(zp + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    bool gave_up = false;  // wave-uniform
+    auto wait_block = [&](int bx, int by, uint32_t target) {
+        uint32_t it = 0;
+        while (__hip_atomic_load(&flags[by * nxb + bx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++it > kSweepSpinLimit) {
+                if (lane == 0) atomicAdd(fault, 1u);
+                gave_up = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    };
+    uint32_t m1 = 0u, m0 = 0u;
+    double qx1, qy1, qz1;
+    pull_into(0, m1, rf, rg);
+    q_of(m1, rf, rg, qx1, qy1, qz1);
+    publish(0, qx1, qy1);
+    double qx0 = 0.0, qz0 = 0.0, qzm = 0.0;  // Q_x of the held plane, Q_z of planes p-1, p-2
+    for (int z = 0; z <= A.nz; ++z) {
+        uint32_t m2 = 0u;
+        if (z + 1 < A.nz) pull_into(z + 1, m2, nf, ng);
+        if (z >= 1) {
+            const int zc = z - 1;
+            if (!gave_up) {
+                if (y > 0) wait_block(xb, y - 1, uint32_t(zc + 1));
+                if (y + 1 < A.ny) wait_block(xb, y + 1, uint32_t(zc + 1));
+                if (xb > 0) wait_block(xb - 1, y, uint32_t(zc + 1));
+                if (xb + 1 < nxb) wait_block(xb + 1, y, uint32_t(zc + 1));
+            }
+            const int slot = zc % kSweepRing;
+            // Q_x of x -+ 1: the lanes beside, or the end cells of the blocks beside
+            double qxp = __shfl_down(qx0, 1, kSweepW), qxm = __shfl_up(qx0, 1, kSweepW);
+            const double *er = qxring + int64_t(slot) * edge_plane + int64_t(y) * nxb * 2;
+            if (lane == last && xb + 1 < nxb) qxp = load_agent(const_cast<double *>(&er[(xb + 1) * 2 + 0]));
+            if (lane == 0 && xb > 0) qxm = load_agent(const_cast<double *>(&er[(xb - 1) * 2 + 1]));
+            if (in) {
+                const int64_t cell = int64_t(zc) * nxy + row;
+                const uint32_t m = m0;
+                if (!(m & 1u)) {
+                    static_for<NQ>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        store_pair(fo, go, int64_t(q) * A.cs + cell, -1.0, -1.0);
+                    });
+                } else {
+                    const LdsRow F{&hold[lane], 2 * kSweepW}, G{&hold[kSweepW + lane], 2 * kSweepW};
+                    MomentsR<double> mo;
+                    accumulate_moments(F, G, mo);
+                    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+                    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+                    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
+                    const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
+                    double *ring = qring + int64_t(slot) * nxy + row;
+                    const double dqx = one_sided_gradient(xp, xm, xp ? qxp : 0.0, M.Qx, xm ? qxm : 0.0, P.inv_dx[0]);
+                    const double dqy = one_sided_gradient(yp, ym, yp ? load_agent(ring + A.nx) : 0.0, M.Qy,
+                                                          ym ? load_agent(ring - A.nx) : 0.0, P.inv_dx[1]);
+                    const double dqz = one_sided_gradient(zp, zm, zp ? qz1 : 0.0, M.Qz, zm ? qzm : 0.0,
+                                                          P.inv_dx[2]);
+                    const double nan = __builtin_nan("");
+                    const bool poisoned = gave_up;
+                    collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P,
+                                           [&](auto qc, double fnew, double gnew) {
+                                               constexpr int q = decltype(qc)::value;
+                                               store_pair(fo, go, int64_t(q) * A.cs + cell, poisoned ? nan : fnew,
+                                                          poisoned ? nan : gnew);
+                                           });
+                }
+            }
+        }
+        if (z == A.nz) break;
+        // plane z becomes the held one (each lane its own column: no barrier)
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            hold[(2 * q) * kSweepW + lane] = rf[q];
+            hold[(2 * q + 1) * kSweepW + lane] = rg[q];
+        });
+        qx0 = qx1;
+        qzm = qz0;
+        qz0 = qz1;
+        m0 = m1;
+        if (z + 1 < A.nz) {
+            q_of(m2, nf, ng, qx1, qy1, qz1);
+            publish(z + 1, qx1, qy1);
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                rf[q] = nf[q];
+                rg[q] = ng[q];
+            });
+            m1 = m2;
+        } else {
+            qz1 = 0.0;
         }
     }
 }
@@ -1303,9 +1452,39 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     const T *g = (const T *)L->pop[1][L->cur];
     T *fo = (T *)L->pop[0][1 - L->cur];
     T *go = (T *)L->pop[1][1 - L->cur];
+    if (variant == kSweepX64Variant) {
+        if constexpr (std::is_same<T, double>::value) {
+            if (!L->d_sweep_q) return set_error(MRB_E_UNSUPPORTED, "k_sweep_x64: FP64 storage only");
+            const int nxb = (A.nx + kSweepW - 1) / kSweepW;
+            k_zero_u32<<<(nxb * A.ny + 255) / 256, 256, 0, s>>>(L->d_sweep_flags, nxb * A.ny);
+            MRB_HIP(hipGetLastError());
+            if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+            double *qr = L->d_sweep_q, *qxr = L->d_sweep_qx;
+            uint32_t *fl = L->d_sweep_flags, *fa = L->d_sweep_fault;
+            const uint32_t *mk = L->d_mask;
+            CompactArgs Ac = A;
+            BCDev bcc = L->bcdev;
+            ShiftTab shc = sh;
+            CollideParams Pc = L->cp;
+            void *args[] = {(void *)&f,  (void *)&g,  (void *)&fo, (void *)&go,  (void *)&mk,  (void *)&qr, (void *)&qxr,
+                            (void *)&fl, (void *)&fa, (void *)&Ac, (void *)&bcc, (void *)&shc, (void *)&Pc};
+            const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_sweep_x64<BCX>),
+                                                            dim3(nxb, A.ny), dim3(kSweepW), args, 0, s);
+            if (e == hipErrorCooperativeLaunchTooLarge) {
+                (void)hipGetLastError();
+                return set_error(MRB_E_UNSUPPORTED, "k_sweep_x64: more blocks than the GPU holds at once");
+            }
+            if (e != hipSuccess) return hip_fail(e, "k_sweep_x64 launch");
+            if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+            return MRB_OK;
+        } else {
+            return set_error(MRB_E_UNSUPPORTED, "k_sweep_x64: FP64 storage only");
+        }
+    }
     if (variant == kSweepVariant) {
         if constexpr (std::is_same<T, double>::value) {
-            if (!L->d_sweep_q) return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 rows of at most 256 cells only");
+            if (!L->d_sweep_q || A.nx > kSweepRow)
+                return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 rows of at most 256 cells only");
             k_zero_u32<<<(A.ny + 255) / 256, 256, 0, s>>>(L->d_sweep_flags, A.ny);
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[0], s));
@@ -1488,7 +1667,7 @@ const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_
                                                            "k_qpass_x256+k_collide_h32_x256",
                                                            "k_qpass_y2+k_collide_f32",
                                                            "k_qpass_x256+k_collide_f32_x256",
-                                                           "k_sweep"};
+                                                           "k_sweep", "k_sweep_x64"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 
@@ -1509,7 +1688,7 @@ static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
     int rc;
     if (L->hot < kNumTwoPassVariants) {
         rc = two_pass(L, L->hot, s, events ? L->ev : nullptr);
-        nslots = (L->hot == kOnePassVariant || L->hot == kSweepVariant) ? 1 : 2;
+        nslots = (L->hot == kOnePassVariant || L->hot == kSweepVariant || L->hot == kSweepX64Variant) ? 1 : 2;
     } else {
         if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
         rc = fused_launch(L, L->hot - kNumTwoPassVariants, s);

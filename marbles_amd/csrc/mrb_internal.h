@@ -277,9 +277,10 @@ struct mrb_level {
     // (those collides have none); restored when the faces change back
     int hot_h32 = -1;
     // single-pass z-sweep (compact FP64 levels with rows of at most 256
-    // cells): ring of 4 planes of Q_y, per-row count of published planes,
-    // count of waits that gave up
+    // cells): ring of kSweepRingPlanes planes of Q_y, per-row count of
+    // published planes, count of waits that gave up
     double *d_sweep_q = nullptr;
+    double *d_sweep_qx = nullptr;  // Q_x of the end cells of 64-cell blocks (k_sweep_x64)
     uint32_t *d_sweep_flags = nullptr;
     uint32_t *d_sweep_fault = nullptr;
     // two-level link, held by the fine level (mrb_amr_link)
@@ -355,7 +356,7 @@ int fused_step(mrb_level *L, int variant, hipStream_t s);
 // single pass that recomputes the neighbours' Q_corr; the rest the fused
 // variants; launch reads the current buffers, writes the spare ones, does
 // not flip
-constexpr int kNumTwoPassVariants = 12;
+constexpr int kNumTwoPassVariants = 13;
 constexpr int kOnePassVariant = 2;
 constexpr int kRows8Variant = 3;  // two passes on 64x8 blocks
 constexpr int kRows2Variant = 4;  // two passes on 64x2 blocks
@@ -366,6 +367,8 @@ constexpr int kHalf32X256Variant = 8; // variant 5 on 256x1 blocks
 constexpr int kF32Variant = 9;         // shifted FP32 storage, FP32 arithmetic in pass 2 (64x2 blocks)
 constexpr int kF32X256Variant = 10;    // variant 9 on 256x1 blocks
 constexpr int kSweepVariant = 11;      // single-pass z-sweep, one block per x-row (FP64, rows <= 256 cells)
+constexpr int kSweepRingPlanes = 8;    // planes of Q_y the z-sweep keeps for the rows beside
+constexpr int kSweepX64Variant = 12;   // the z-sweep on single-wave blocks of 64 cells of a row (FP64)
 // the variants whose pass 2 computes in FP32 (their results differ from the
 // FP64-arithmetic variants'), and the FP64-arithmetic variant with the same
 // block shape that replaces one on a level with PRESSURE faces

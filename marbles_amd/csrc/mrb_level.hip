@@ -1189,12 +1189,15 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
         MRB_HIP(hipMemset(L->d_mask, 0, sizeof(uint32_t) * T));
         MRB_HIP(hipMalloc(&L->d_q, sizeof(double) * 3 * T));
         MRB_HIP(hipMemset(L->d_q, 0, sizeof(double) * 3 * T));
-        if (L->storage == MRB_STORAGE_F64 && L->ncell[0] <= 256) {
+        if (L->storage == MRB_STORAGE_F64) {
             const size_t plane = size_t(L->ncell[0]) * size_t(L->ncell[1]);
-            MRB_HIP(hipMalloc(&L->d_sweep_q, sizeof(double) * 4 * plane));
-            MRB_HIP(hipMemset(L->d_sweep_q, 0, sizeof(double) * 4 * plane));
-            MRB_HIP(hipMalloc(&L->d_sweep_flags, sizeof(uint32_t) * size_t(L->ncell[1])));
-            MRB_HIP(hipMemset(L->d_sweep_flags, 0, sizeof(uint32_t) * size_t(L->ncell[1])));
+            const size_t nblk = size_t((L->ncell[0] + 63) / 64) * size_t(L->ncell[1]);
+            MRB_HIP(hipMalloc(&L->d_sweep_q, sizeof(double) * kSweepRingPlanes * plane));
+            MRB_HIP(hipMemset(L->d_sweep_q, 0, sizeof(double) * kSweepRingPlanes * plane));
+            MRB_HIP(hipMalloc(&L->d_sweep_qx, sizeof(double) * kSweepRingPlanes * 2 * nblk));
+            MRB_HIP(hipMemset(L->d_sweep_qx, 0, sizeof(double) * kSweepRingPlanes * 2 * nblk));
+            MRB_HIP(hipMalloc(&L->d_sweep_flags, sizeof(uint32_t) * nblk));
+            MRB_HIP(hipMemset(L->d_sweep_flags, 0, sizeof(uint32_t) * nblk));
             MRB_HIP(hipMalloc(&L->d_sweep_fault, sizeof(uint32_t)));
             MRB_HIP(hipMemset(L->d_sweep_fault, 0, sizeof(uint32_t)));
         }
@@ -1252,6 +1255,7 @@ int mrb_level_destroy(mrb_level *L)
     (void)hipFree(L->d_gmap1);
     (void)hipFree(L->d_bcells);
     (void)hipFree(L->d_sweep_q);
+    (void)hipFree(L->d_sweep_qx);
     (void)hipFree(L->d_sweep_flags);
     (void)hipFree(L->d_sweep_fault);
     (void)hipFree(L->d_tiles_c);

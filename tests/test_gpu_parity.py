@@ -202,7 +202,7 @@ def test_every_hot_kernel_variant_matches_oracle(oracle, engine, case):
             assert err <= REL_TOL, (case, vname, name, err)
         E.close()
         checked.append(vname)
-    assert "k_sweep" in checked and "k_onepass" in checked
+    assert "k_sweep" in checked and "k_sweep_x64" in checked and "k_onepass" in checked
 
 
 def test_autotune_does_not_advance_the_state(oracle, engine):
@@ -640,8 +640,9 @@ def test_ic_plugin_workloads_match_oracle(oracle, engine, case, layout):
 def test_step_variants_equal_two_pass_bitwise(engine, case, storage):
     """The other block shapes of the two passes (64x8, 64x2, 128x2, 256x1),
     k_onepass, which recomputes the neighbours' Q_corr from their pulls
-    instead of reading pass 1's array, and the single-pass z-sweep k_sweep
-    (FP64), which passes Q_corr between rows, do the same operations in the
+    instead of reading pass 1's array, and the single-pass z-sweeps k_sweep
+    and k_sweep_x64 (FP64), which pass Q_corr between rows, do the same
+    operations in the
     same order: their populations equal the 64x4 two-pass step's bit for bit
     (partial tiles, the sphere, inlet profiles, pressure ghosts, slip
     walls)."""
@@ -662,7 +663,7 @@ def test_step_variants_equal_two_pass_bitwise(engine, case, storage):
     if storage != "f16s":
         names.append("k_onepass")
     if storage == "f64":
-        names.append("k_sweep")
+        names += ["k_sweep", "k_sweep_x64"]
     out = {}
     for name in names:
         E = configs.build_level(cfg, layout="compact", storage=storage)

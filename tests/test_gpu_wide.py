@@ -73,7 +73,7 @@ def test_wide_rows_every_variant_matches_oracle(oracle, engine):
             assert err <= REL_TOL, (vname, name, err)
         E.close()
         checked.append(vname)
-    assert "k_qpass_x256+k_collide_x256" in checked and "k_onepass" in checked
+    assert "k_qpass_x256+k_collide_x256" in checked and "k_onepass" in checked and "k_sweep_x64" in checked
     print("wide rows: " + ", ".join(checked))
 
 

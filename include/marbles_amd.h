@@ -240,6 +240,18 @@ int mrb_stream_compact(mrb_level *lvl, void *stream);
  * FP64 sums over valid fluid cells: mass, x/y/z momentum, 2*rho*E.
  * Synchronous (waits for `stream`). */
 int mrb_reduce_conservation(const mrb_level *lvl, double out[5], void *stream);
+/* The same five sums without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  The per-block partial sums go to a
+ * static device array of the library (5 x 4096 doubles, one copy per GPU,
+ * held until the process ends; the conservation kernel strides over any
+ * level size with at most 4096 blocks, so no level exceeds it) and are
+ * added on the device in the order mrb_reduce_conservation adds them, so
+ * the sums are bit-identical.  out_device: 5 doubles of device memory on
+ * the current device (the level's).  Asynchronous on `stream`; nothing is
+ * allocated, freed or waited for.  Calls that overlap on one GPU share the
+ * array and race: order them on one stream. */
+int mrb_reduce_conservation_noalloc(const mrb_level *lvl, double *out_device, void *stream);
 /* LBM::compute_eb_forces (Source/LBM.cpp:994-1044) on one level (no finer
  * level masking): momentum-exchange force on the embedded boundary,
  * F = sum over solid cells with a fluid face neighbour and over q of

@@ -28,6 +28,7 @@ SOURCES = [
     "mrb_batched.hip",
     "mrb_adjoint.hip",
     "mrb_io.hip",
+    "mrb_noalloc.hip",
 ]
 
 COMMON_FLAGS = [

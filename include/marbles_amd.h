@@ -201,8 +201,8 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *      the rest direction), 64x2-cell blocks;
  *  10  variant 9 on 256x1-cell blocks;
  *  11  k_sweep: FP64 storage with rows of at most 256 cells, one pass as a
- *      z-sweep with one resident block per x-row (cooperative launch;
- *      MRB_E_UNSUPPORTED when the rows do not fit the GPU at once);
+ *      z-sweep with one resident block per x-row (MRB_E_UNSUPPORTED when
+ *      the kernel's occupancy does not let every row be resident at once);
  *  12  k_sweep_x64: the same z-sweep on single-wave blocks of 64 cells of
  *      a row (any row length; MRB_E_UNSUPPORTED when the blocks do not fit
  *      the GPU at once);

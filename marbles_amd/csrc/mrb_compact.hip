@@ -20,6 +20,9 @@
 #include <algorithm>
 #include <cmath>
 
+#include <map>
+#include <mutex>
+
 #include "mrb_internal.h"
 
 namespace mrb {
@@ -794,7 +797,8 @@ __global__ void __launch_bounds__(256) k_collide_f32_x256(
 
 // ---------------------------------------------------------------------------
 // Single pass as a z-sweep (FP64 storage, rows of at most 256 cells): one
-// block per x-row y, all rows resident at once (cooperative launch).  The
+// block per x-row y, all rows resident at once (checked on the host from
+// the kernel's occupancy).  The
 // block walks z; at step z it issues the loads of plane z+1, collides
 // plane z-1 from the pull held in LDS once the rows y -+ 1 have published
 // plane z-1 (the loads are in flight meanwhile), moves plane z from
@@ -901,6 +905,13 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
     publish(0, qy1);
     double qz0 = 0.0, qzm = 0.0;  // Q_z of planes p-1 and p-2
     for (int z = 0; z <= A.nz; ++z) {
+        // the counts of the rows beside, loaded before this step's pull so
+        // that their latency hides behind it
+        uint32_t cnt_m = 0xffffffffu, cnt_p = 0xffffffffu;
+        if (z >= 1 && x == 0) {
+            if (y > 0) cnt_m = __hip_atomic_load(&flags[y - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (y + 1 < A.ny) cnt_p = __hip_atomic_load(&flags[y + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         // plane z+1: loads issued now, used after the collide
         uint32_t m2 = 0u;
         if (z + 1 < A.nz) pull_into(z + 1, m2, nf, ng);
@@ -909,8 +920,8 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
             // published it
             const int zc = z - 1;
             if (x == 0 && !gave_up) {
-                if (y > 0) wait_row(y - 1, uint32_t(zc + 1));
-                if (y + 1 < A.ny) wait_row(y + 1, uint32_t(zc + 1));
+                if (cnt_m < uint32_t(zc + 1)) wait_row(y - 1, uint32_t(zc + 1));
+                if (cnt_p < uint32_t(zc + 1)) wait_row(y + 1, uint32_t(zc + 1));
             }
             __syncthreads();
             const bool poisoned = gave_up != 0;
@@ -923,18 +934,20 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
                         store_pair(fo, go, int64_t(q) * A.cs + cell, -1.0, -1.0);
                     });
                 } else {
+                    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+                    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
+                    const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
+                    // the rows beside's Q_y, loaded before the moments
+                    double *ring = qring + int64_t(zc % kSweepRing) * nxy + row;
+                    const double qyp = yp ? load_agent(ring + A.nx) : 0.0;
+                    const double qym = ym ? load_agent(ring - A.nx) : 0.0;
                     const LdsRow F{&hold[x]}, G{&hold[kSweepRow + x]};
                     MomentsR<double> mo;
                     accumulate_moments(F, G, mo);
                     const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
-                    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
-                    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
-                    const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
-                    double *ring = qring + int64_t(zc % kSweepRing) * nxy + row;
                     const double dqx = one_sided_gradient(xp, xm, xp ? qxrow[x + 1] : 0.0, M.Qx,
                                                           xm ? qxrow[x - 1] : 0.0, P.inv_dx[0]);
-                    const double dqy = one_sided_gradient(yp, ym, yp ? load_agent(ring + A.nx) : 0.0, M.Qy,
-                                                          ym ? load_agent(ring - A.nx) : 0.0, P.inv_dx[1]);
+                    const double dqy = one_sided_gradient(yp, ym, qyp, M.Qy, qym, P.inv_dx[1]);
                     const double dqz = one_sided_gradient(zp, zm, zp ? qz1 : 0.0, M.Qz, zm ? qzm : 0.0,
                                                           P.inv_dx[2]);
                     const double nan = __builtin_nan("");
@@ -1442,9 +1455,36 @@ static Half16 half16_of(mrb_level *L)
     return h;
 }
 
+// blocks of a kernel that the GPU holds at once (occupancy x CUs): the
+// z-sweeps wait on each other and need every block resident; they are
+// launched only when the occupancy calculation says so (their waits are
+// bounded, so a block that is not resident after all turns into NaN
+// results and a fault count, not a hang)
+static int resident_blocks(const void *kernel, int threads, int &fit)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, int> cache;  // (device, kernel) -> blocks
+    int dev = 0;
+    MRB_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(dev, kernel);
+    const auto it = cache.find(key);
+    if (it != cache.end()) {
+        fit = it->second;
+        return MRB_OK;
+    }
+    int per_cu = 0, cus = 0;
+    MRB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    MRB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    fit = per_cu * cus;
+    cache[key] = fit;
+    return MRB_OK;
+}
+
 template <class T, int BCX>
 static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
 {
+    int rc = MRB_OK;
     const CompactArgs A = make_args(L);
     const ShiftTab sh = shift_tab(L);
     const Half16 h16 = half16_of(L);
@@ -1466,15 +1506,13 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
             BCDev bcc = L->bcdev;
             ShiftTab shc = sh;
             CollideParams Pc = L->cp;
-            void *args[] = {(void *)&f,  (void *)&g,  (void *)&fo, (void *)&go,  (void *)&mk,  (void *)&qr, (void *)&qxr,
-                            (void *)&fl, (void *)&fa, (void *)&Ac, (void *)&bcc, (void *)&shc, (void *)&Pc};
-            const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_sweep_x64<BCX>),
-                                                            dim3(nxb, A.ny), dim3(kSweepW), args, 0, s);
-            if (e == hipErrorCooperativeLaunchTooLarge) {
-                (void)hipGetLastError();
+            int fit = 0;
+            if ((rc = resident_blocks(reinterpret_cast<const void *>(&k_sweep_x64<BCX>), kSweepW, fit))) return rc;
+            if (int64_t(nxb) * A.ny > fit)
                 return set_error(MRB_E_UNSUPPORTED, "k_sweep_x64: more blocks than the GPU holds at once");
-            }
-            if (e != hipSuccess) return hip_fail(e, "k_sweep_x64 launch");
+            k_sweep_x64<BCX><<<dim3(nxb, A.ny), dim3(kSweepW), 0, s>>>(f, g, fo, go, mk, qr, qxr, fl, fa, Ac, bcc, shc,
+                                                                      Pc);
+            MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[1], s));
             return MRB_OK;
         } else {
@@ -1495,15 +1533,11 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
             BCDev bcc = L->bcdev;
             ShiftTab shc = sh;
             CollideParams Pc = L->cp;
-            void *args[] = {(void *)&f, (void *)&g, (void *)&fo, (void *)&go, (void *)&mk, (void *)&qr,
-                            (void *)&fl, (void *)&fa, (void *)&Ac, (void *)&bcc, (void *)&shc, (void *)&Pc};
-            const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_sweep<BCX>),
-                                                            dim3(A.ny), dim3(kSweepRow), args, 0, s);
-            if (e == hipErrorCooperativeLaunchTooLarge) {
-                (void)hipGetLastError();
-                return set_error(MRB_E_UNSUPPORTED, "k_sweep: more rows than the GPU holds at once");
-            }
-            if (e != hipSuccess) return hip_fail(e, "k_sweep launch");
+            int fit = 0;
+            if ((rc = resident_blocks(reinterpret_cast<const void *>(&k_sweep<BCX>), kSweepRow, fit))) return rc;
+            if (A.ny > fit) return set_error(MRB_E_UNSUPPORTED, "k_sweep: more rows than the GPU holds at once");
+            k_sweep<BCX><<<dim3(A.ny), dim3(kSweepRow), 0, s>>>(f, g, fo, go, mk, qr, fl, fa, Ac, bcc, shc, Pc);
+            MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[1], s));
             return MRB_OK;
         } else {

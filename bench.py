@@ -177,6 +177,44 @@ def run_amr(args, torch) -> int:
     return 0
 
 
+VANDV = {"cd": 3.096, "cl": 0.8396, "st": 0.2986}  # Docs/sphinx/VandV.rst:14
+
+
+def run_cylinder(args, torch) -> int:
+    """The reference's flow-past-a-cylinder validation: a cylinder deck run
+    from its initial state, force coefficients and Strouhal number over the
+    last 30 % of the run, beside the published values."""
+    from marbles_amd import validation
+
+    res, t, forces = validation.run_cylinder(args.deck, steps=args.steps, sample_every=25, reynolds=args.reynolds,
+                                             log=log)
+    out_dir = os.path.join(ROOT, "gpurun_out")
+    if os.path.isdir(out_dir):
+        import numpy as np
+
+        np.savetxt(os.path.join(out_dir, f"forces_{res['cfg']}.txt"), np.column_stack([t, forces]),
+                   header="step fx fy fz")
+    log(json.dumps(res))
+    rel = {}
+    if res["strouhal"] is not None:
+        rel = {"cd_max": res["cd_max"] / VANDV["cd"] - 1.0, "cl_max": res["cl_max"] / VANDV["cl"] - 1.0,
+               "strouhal": res["strouhal"] / VANDV["st"] - 1.0}
+    print(json.dumps({
+        "metric": "EB cylinder drag / lift coefficients and Strouhal number",
+        "value": res["cd_max"], "unit": "C_d (max over the window)", "n_gpus": 1, "steps": args.steps,
+        "warmup": 0, "ms_per_step": round(res["seconds"] / args.steps * 1e3, 5), "higher_is_better": None,
+        "vs_baseline": None, "dtype": "f64", "data": "deck",
+        "config": {"workload": res["cfg"], "deck": args.deck, "kernel": res["kernel"],
+                   "reynolds_umax": res["reynolds_umax"], "reynolds_umean": res["reynolds_umean"],
+                   "nu": res["nu"]},
+        "coefficients": {k: res[k] for k in ("cd_mean", "cd_max", "cd_min", "cl_max", "cl_min", "cl_amplitude",
+                                             "strouhal", "periods_in_window", "window_steps")},
+        "published": VANDV, "relative_to_published": rel, "mlups": round(res["mlups"], 1),
+        "mass_change": res["mass_change"], "finite": res["finite"],
+    }), flush=True)
+    return 0 if res["finite"] else 1
+
+
 def run_adjoint(args, torch) -> int:
     """Gradient of the flow objective (outlet-plane mean u_x + 1e-3 KE) with
     respect to n_s, tau and u0 on an n^3 box with an n_s field of seeded
@@ -399,10 +437,16 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the shifted-FP32 and scaled-FP16 comparison runs")
-    ap.add_argument("--workload", default="channel", choices=["channel", "amr", "adjoint", "precision"],
+    ap.add_argument("--workload", default="channel",
+                    choices=["channel", "amr", "adjoint", "precision", "cylinder"],
                     help="channel: the single-level 256^3 channel (default); amr: the two-level workload; "
                          "adjoint: gradient cost and finite-difference check at 128^3; precision: FP64, "
-                         "shifted FP32 and scaled FP16 storage on a 512^3 spectral field, errors vs FP64")
+                         "shifted FP32 and scaled FP16 storage on a 512^3 spectral field, errors vs FP64; "
+                         "cylinder: force coefficients of an EB-cylinder deck against the published values")
+    ap.add_argument("--deck", default="turek_2d2", choices=["turek_2d2", "single_cylinder"],
+                    help="cylinder: the deck to run")
+    ap.add_argument("--reynolds", type=float, default=None,
+                    help="cylinder: Re on the mean inflow velocity (sets nu; default: the deck's nu)")
     ap.add_argument("--graph", action="store_true", help="amr: replay captured hipGraphs")
     args = ap.parse_args()
 
@@ -424,6 +468,10 @@ def main() -> int:
         if "--warmup" not in sys.argv:
             args.warmup = 0
         return run_precision(args, torch)
+    if args.workload == "cylinder":
+        if "--steps" not in sys.argv:
+            args.steps = 200000
+        return run_cylinder(args, torch)
     if args.workload == "adjoint":
         if "--steps" not in sys.argv:
             args.steps = 200

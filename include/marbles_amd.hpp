@@ -144,6 +144,11 @@ class Level {
     {
         check(mrb_compute_flags_sphere(h_, center.data(), radius, fluid_inside ? 1 : 0));
     }
+    void set_cylinder(const std::array<double, 3> &center, double radius, double height, int direction,
+                      bool fluid_inside)
+    {
+        check(mrb_compute_flags_cylinder(h_, center.data(), radius, height, direction, fluid_inside ? 1 : 0));
+    }
 
     /* populations of MRB_ARR_F / MRB_ARR_G (FP64, AMReX FAB order) */
     std::int64_t elements(int arr) const

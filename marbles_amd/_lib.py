@@ -99,6 +99,7 @@ SIGNATURES = {
     "mrb_plot_fields": (c_int, [c_void_p, c_void_p, c_int]),
     "mrb_download": (c_int, [c_void_p, c_int, c_void_p]),
     "mrb_compute_flags_sphere": (c_int, [c_void_p, POINTER(c_double), c_double, c_int]),
+    "mrb_compute_flags_cylinder": (c_int, [c_void_p, POINTER(c_double), c_double, c_double, c_int, c_int]),
     "mrb_finalize_flags": (c_int, [c_void_p]),
     "mrb_init_taylorgreen": (c_int, [c_void_p, c_double, c_double, POINTER(c_double)]),
     "mrb_init_constant": (

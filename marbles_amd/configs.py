@@ -203,6 +203,73 @@ def pressure_case(kind: str, ncell=(40, 20, 24), tau: float = 0.6, u: float = 0.
     }
 
 
+def cylinder_deck(kind: str = "turek_2d2", scale: int = 1, reynolds: float | None = None) -> dict:
+    """The reference's EB-cylinder decks, optionally coarsened by `scale`
+    (cells, radius and centre divided by it; same Mach numbers and
+    temperature, viscosity divided by it to keep the Reynolds number).
+
+    "turek_2d2" (Tests/test_files/cylinder_turek_2d2/cylinder_turek_2d2.inp):
+    the Schaefer-Turek 2D-2 channel, 660 x 123 x 4 cells (z periodic),
+    parabolic inflow at x-lo (Mach 0.1 at the centreline, T = 0.03333,
+    gamma = 1.4), no-slip walls in y, outflow at x-hi, cylinder r = 15 along
+    z at (60, 60), nu = alpha = 0.00648 (Re = u_max D / nu = 100); starts
+    from uniform flow at Mach 0.1.
+    "single_cylinder" (Tests/test_files/single_cylinder/single_cylinder.inp):
+    246 x 576 x 4 cells, parabolic inflow at y-lo (Mach 0.015, T = 0.03,
+    gamma = 5/3), no-slip walls in x, outflow at y-hi, cylinder r = 30 along
+    z at (120, 120), nu = 0.006; starts at rest.
+    `reynolds` (optional) replaces the deck's viscosity (and diffusivity)
+    by u_mean D / Re, the Schaefer-Turek definition of Re on the mean inflow
+    velocity."""
+    if kind == "turek_2d2":
+        n, gamma, T, mach, r, c = (660, 123, 4), 1.4, 0.03333, 0.1, 15.0, (60.0, 60.0)
+        nu, bc_lo, bc_hi, normal, tangential = 0.00648, (2, 1, 0), (5, 1, 0), 1, 0
+        ic_mach = (0.1, 0.0, 0.0)
+    elif kind == "single_cylinder":
+        n, gamma, T, mach, r, c = (246, 576, 4), 5.0 / 3.0, 0.03, 0.015, 30.0, (120.0, 120.0)
+        nu, bc_lo, bc_hi, normal, tangential = 0.006, (1, 2, 0), (1, 5, 0), 0, 1
+        ic_mach = (0.0, 0.0, 0.0)
+    else:
+        raise ValueError("unknown cylinder deck " + kind)
+    if scale != 1:
+        n = (n[0] // scale, n[1] // scale, n[2])
+        r, c, nu = r / scale, (c[0] / scale, c[1] / scale), nu / scale
+    vbc = {"kind": "parabolic", "Mach_ref": mach, "normal_dir": normal, "tangential_dir": tangential,
+           "initial_density": 1.0, "initial_temperature": T, "adiabatic_exponent": gamma,
+           "mean_molecular_mass": AIR_MOLAR_MASS}
+    u_max = mach * math.sqrt(gamma * T)
+    name = f"cylinder_{kind}_{n[0]}x{n[1]}x{n[2]}"
+    if reynolds is not None:
+        nu = (2.0 * u_max / 3.0) * (2.0 * r) / reynolds
+        name += f"_Re{reynolds:g}"
+    return {
+        "name": name,
+        "ncell": n,
+        "periodic": (0, 0, 1),
+        "prob_lo": (0.0, 0.0, -n[2] / 2.0),
+        "prob_hi": (float(n[0]), float(n[1]), n[2] / 2.0),
+        "bc_lo": bc_lo,
+        "bc_hi": bc_hi,
+        "velocity_bc": vbc,
+        "nu": nu,
+        "alpha": nu,
+        "adiabatic_exponent": gamma,
+        "T0": T,
+        "ic": {"type": "constant", "density": 1.0, "mach": ic_mach, "temperature": T, "adiabatic_exponent": gamma},
+        "eb": {"type": "cylinder", "center": (c[0], c[1], 0.0), "radius": r, "height": 1000.0, "direction": 2,
+               "has_fluid_inside": 0},
+        # force coefficients: 2 F / (rho U^2 D L_z) with U the mean inflow
+        # velocity, 2/3 of the parabola's maximum (the Schaefer-Turek
+        # normalisation), St = f D / U
+        "u_max": u_max,
+        "u_mean": 2.0 * u_max / 3.0,
+        "diameter": 2.0 * r,
+        "flow_dir": tangential,
+        "lift_dir": normal,
+        "max_box": 32,
+    }
+
+
 def plugin_case(kind: str, ncell, params: dict, nu: float, alpha: float | None = None,
                 periodic=(1, 1, 1), bc_lo=(0, 0, 0), bc_hi=(0, 0, 0), prob_lo=(0.0, 0.0, 0.0),
                 prob_hi=None, max_box: int | None = None) -> dict:
@@ -281,11 +348,7 @@ def build_level(cfg: dict, layout: str = "compact", storage: str = "f64", ng: in
     if storage != "f64":
         # the shifted storages subtract the rest equilibrium of this state
         lvl.set_shift_state(*reference_state(cfg))
-    eb = cfg["eb"]
-    if eb["type"] == "sphere":
-        lvl.flags_sphere(eb["center"], eb["radius"], eb.get("has_fluid_inside", 0))
-    else:
-        lvl.flags_all_regular()
+    set_eb_flags(lvl, cfg["eb"])
     ic = cfg["ic"]
     if ic["type"] == "taylorgreen":
         lvl.init_taylorgreen(ic["rho0"], ic["v0"], ic["omega"])
@@ -446,6 +509,8 @@ def set_eb_flags(lvl, eb: dict) -> None:
     `eb` entry."""
     if eb["type"] == "sphere":
         lvl.flags_sphere(eb["center"], eb["radius"], eb.get("has_fluid_inside", 0))
+    elif eb["type"] == "cylinder":
+        lvl.flags_cylinder(eb["center"], eb["radius"], eb["height"], eb["direction"], eb.get("has_fluid_inside", 0))
     elif eb["type"] == "all_regular":
         lvl.flags_all_regular()
     else:

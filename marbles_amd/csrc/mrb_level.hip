@@ -446,32 +446,46 @@ __global__ void k_eqrelax_fab(
 // ---------------------------------------------------------------------------
 // flags: EB sphere classification, eb_boundary ring (LBM.cpp:1213-1262)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool sphere_covered(
-    double x0, double y0, double z0, double dx, double dy, double dz, double cx, double cy,
-    double cz, double r, int fluid_inside)
+// A cell is covered when all 8 of its corner nodes lie in the body; a node
+// on the surface counts as body.  Sphere: |x - c| <= r.  Cylinder along
+// `axis` (EB2 "cylinder"): the distance from the axis <= r and
+// |x_axis - c_axis| <= height / 2.  has_fluid_inside takes the complement.
+struct BodyArgs {
+    double plo[3], dx[3], c[3], r;
+    double half_height;  // cylinder only
+    int axis;            // -1: sphere, else the cylinder's direction
+    int fluid_inside;
+};
+
+__device__ __forceinline__ bool body_covered(const BodyArgs &S, double x0, double y0, double z0)
 {
-    const double r2 = r * r;
+    const double r2 = S.r * S.r;
     for (int a = 0; a < 2; ++a)
         for (int c = 0; c < 2; ++c)
             for (int e = 0; e < 2; ++e) {
-                const double x = x0 + a * dx - cx;
-                const double y = y0 + c * dy - cy;
-                const double z = z0 + e * dz - cz;
-                const double d2 = x * x + y * y + z * z;
-                const bool in_body = fluid_inside ? (d2 >= r2) : (d2 <= r2);
-                if (!in_body) return false;
+                const double x = x0 + a * S.dx[0] - S.c[0];
+                const double y = y0 + c * S.dx[1] - S.c[1];
+                const double z = z0 + e * S.dx[2] - S.c[2];
+                bool inside;
+                if (S.axis < 0) {
+                    const double d2 = x * x + y * y + z * z;
+                    inside = S.fluid_inside ? (d2 >= r2) : (d2 <= r2);
+                } else {
+                    const double p[3] = {x, y, z};
+                    const int u = S.axis == 0 ? 1 : 0, v = S.axis == 2 ? 1 : 2;
+                    const double d2 = p[u] * p[u] + p[v] * p[v];
+                    const double h = fabs(p[S.axis]);
+                    inside = S.fluid_inside ? (d2 >= r2 || h >= S.half_height)
+                                            : (d2 <= r2 && h <= S.half_height);
+                }
+                if (!inside) return false;
             }
     return true;
 }
 
-struct SphereArgs {
-    double plo[3], dx[3], c[3], r;
-    int fluid_inside;
-};
-
-__global__ void k_flags_sphere(
+__global__ void k_flags_body(
     int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes, const int64_t *__restrict__ base,
-    int nbox, int64_t total, SphereArgs S)
+    int nbox, int64_t total, BodyArgs S)
 {
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= total) return;
@@ -480,9 +494,7 @@ __global__ void k_flags_sphere(
     const int64_t loc = c - B.base;
     int i, j, k;
     local_to_ijk(B, loc, i, j, k);
-    const bool cov = sphere_covered(
-        S.plo[0] + i * S.dx[0], S.plo[1] + j * S.dx[1], S.plo[2] + k * S.dx[2], S.dx[0], S.dx[1],
-        S.dx[2], S.c[0], S.c[1], S.c[2], S.r, S.fluid_inside);
+    const bool cov = body_covered(S, S.plo[0] + i * S.dx[0], S.plo[1] + j * S.dx[1], S.plo[2] + k * S.dx[2]);
     isf[B.base * 2 + loc] = cov ? 0 : 1;
     isf[B.base * 2 + B.ncell + loc] = 0;
 }
@@ -1189,7 +1201,7 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
         MRB_HIP(hipMemset(L->d_mask, 0, sizeof(uint32_t) * T));
         MRB_HIP(hipMalloc(&L->d_q, sizeof(double) * 3 * T));
         MRB_HIP(hipMemset(L->d_q, 0, sizeof(double) * 3 * T));
-        if (L->storage == MRB_STORAGE_F64) {
+        if (L->storage == MRB_STORAGE_F64 || L->storage == MRB_STORAGE_F32_SHIFTED) {
             const size_t plane = size_t(L->ncell[0]) * size_t(L->ncell[1]);
             const size_t nblk = size_t((L->ncell[0] + 63) / 64) * size_t(L->ncell[1]);
             MRB_HIP(hipMalloc(&L->d_sweep_q, sizeof(double) * kSweepRingPlanes * plane));
@@ -1488,19 +1500,11 @@ int mrb_download(const mrb_level *L, int arr, void *host)
     return rc;
 }
 
-int mrb_compute_flags_sphere(mrb_level *L, const double center[3], double radius, int has_fluid_inside)
+// comp0 from the body, then the eb_boundary ring (LBM.cpp:1213-1262)
+static int flags_of_body(mrb_level *L, const BodyArgs &S)
 {
-    if (!L || !center) return set_error(MRB_E_ARG, "null argument");
-    SphereArgs S{};
-    for (int d = 0; d < 3; ++d) {
-        S.plo[d] = L->prob_lo[d];
-        S.dx[d] = L->dx[d];
-        S.c[d] = center[d];
-    }
-    S.r = radius;
-    S.fluid_inside = has_fluid_inside;
-    k_flags_sphere<<<nblocks(L->total_cells), kBlock>>>(L->d_isf, L->d_boxes, L->d_box_base, L->nbox,
-                                                        L->total_cells, S);
+    k_flags_body<<<nblocks(L->total_cells), kBlock>>>(L->d_isf, L->d_boxes, L->d_box_base, L->nbox,
+                                                      L->total_cells, S);
     MRB_HIP(hipGetLastError());
     if (L->layout == MRB_LAYOUT_FAB) {
         int rc = fill_boundary(L, L->d_isf, 2, true, false, nullptr);
@@ -1510,25 +1514,55 @@ int mrb_compute_flags_sphere(mrb_level *L, const double center[3], double radius
         MRB_HIP(hipGetLastError());
         rc = fill_boundary(L, L->d_isf, 2, true, false, nullptr);
         if (rc) return rc;
-    } else {
+    } else if (!S.fluid_inside) {
         // ghost cells beyond non-periodic faces must be fluid for the
-        // compact layout's inline boundary handling
+        // compact layout's inline boundary handling: the body may not reach
+        // within two cells of such a face (a sufficient check on the
+        // centre's distance, along the axis on the cylinder's half height)
         for (int d = 0; d < 3; ++d) {
             if (L->periodic[d]) continue;
+            const double extent = d == S.axis ? S.half_height : S.r;
             for (int side = 0; side < 2; ++side) {
-                const double node = side == 0 ? L->prob_lo[d] - L->dx[d] : L->prob_hi[d];
-                // the sphere covers a ghost cell only if it reaches beyond
-                // the face; a sufficient check is the distance of the centre
-                const double dist = side == 0 ? center[d] - L->prob_lo[d] : L->prob_hi[d] - center[d];
-                if (!has_fluid_inside && dist - radius <= 2.0 * L->dx[d])
-                    return set_error(MRB_E_UNSUPPORTED,
-                                     "compact layout: EB body reaches a non-periodic face");
-                (void)node;
+                const double dist = side == 0 ? S.c[d] - L->prob_lo[d] : L->prob_hi[d] - S.c[d];
+                if (dist - extent <= 2.0 * L->dx[d])
+                    return set_error(MRB_E_UNSUPPORTED, "compact layout: EB body reaches a non-periodic face");
             }
         }
     }
     MRB_HIP(hipDeviceSynchronize());
     return mrb_finalize_flags(L);
+}
+
+static BodyArgs body_args(const mrb_level *L, const double center[3], double radius, int has_fluid_inside)
+{
+    BodyArgs S{};
+    for (int d = 0; d < 3; ++d) {
+        S.plo[d] = L->prob_lo[d];
+        S.dx[d] = L->dx[d];
+        S.c[d] = center[d];
+    }
+    S.r = radius;
+    S.axis = -1;
+    S.fluid_inside = has_fluid_inside;
+    return S;
+}
+
+int mrb_compute_flags_sphere(mrb_level *L, const double center[3], double radius, int has_fluid_inside)
+{
+    if (!L || !center) return set_error(MRB_E_ARG, "null argument");
+    return flags_of_body(L, body_args(L, center, radius, has_fluid_inside));
+}
+
+int mrb_compute_flags_cylinder(
+    mrb_level *L, const double center[3], double radius, double height, int direction, int has_fluid_inside)
+{
+    if (!L || !center) return set_error(MRB_E_ARG, "null argument");
+    if (direction < 0 || direction > 2) return set_error(MRB_E_ARG, "cylinder direction must be 0, 1 or 2");
+    if (!(radius > 0.0) || !(height > 0.0)) return set_error(MRB_E_ARG, "cylinder radius and height must be > 0");
+    BodyArgs S = body_args(L, center, radius, has_fluid_inside);
+    S.axis = direction;
+    S.half_height = 0.5 * height;
+    return flags_of_body(L, S);
 }
 
 int mrb_stencil(int evs[NQ * 3], double weights[NQ], int bounce_dirs[NQ])

@@ -117,6 +117,11 @@ class Level:
     def flags_sphere(self, center, radius, has_fluid_inside=False) -> None:
         check(self.lib.mrb_compute_flags_sphere(self.h, _d3(center), float(radius), int(has_fluid_inside)))
 
+    def flags_cylinder(self, center, radius, height, direction, has_fluid_inside=False) -> None:
+        """EB2 "cylinder" along axis `direction` (eb2.cylinder_* inputs)."""
+        check(self.lib.mrb_compute_flags_cylinder(self.h, _d3(center), float(radius), float(height),
+                                                  int(direction), int(has_fluid_inside)))
+
     def flags_all_regular(self) -> None:
         isf = np.zeros(self.fab_elems("is_fluid"), dtype=np.int32)
         off = 0

@@ -1281,6 +1281,35 @@ int orc_sphere_covered(
     return 1;
 }
 
+/* EB cylinder classification (AMReX EB2 CylinderIF, geom_type "cylinder"
+ * via EB.cpp:28-37; inputs eb2.cylinder_radius, _height, _direction,
+ * _center, _has_fluid_inside as in single_cylinder.inp:38-43).  The body is
+ * the set of points whose distance from the axis through `center` along
+ * `direction` is at most `radius` and whose distance from `center` along
+ * the axis is at most height / 2; a cell is covered when all 8 corner nodes
+ * lie in it (nodes on the surface count as body).  Ties: parity unpinned,
+ * as for the sphere. */
+int orc_cylinder_covered(
+    const double node_lo[3], const double dx[3], const double center[3], double radius,
+    double height, int direction, int has_fluid_inside)
+{
+    const double r2 = radius * radius, hh = 0.5 * height;
+    const int u = direction == 0 ? 1 : 0, v = direction == 2 ? 1 : 2;
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 2; c++)
+            for (int e = 0; e < 2; e++) {
+                const double p[3] = {node_lo[0] + a * dx[0] - center[0],
+                                     node_lo[1] + c * dx[1] - center[1],
+                                     node_lo[2] + e * dx[2] - center[2]};
+                const double d2 = p[u] * p[u] + p[v] * p[v];
+                const double h = fabs(p[direction]);
+                const int inside = d2 <= r2 && h <= hh;
+                const int in_body = has_fluid_inside ? (d2 >= r2 || h >= hh) : inside;
+                if (!in_body) return 0;
+            }
+    return 1;
+}
+
 /* LBM::initialize_is_fluid (LBM.cpp:1213-1262) given comp0 for every fab
  * cell: FillBoundary, EB boundary ring (comp1), FillBoundary. */
 static void finish_is_fluid(orc_level *L)
@@ -1336,6 +1365,30 @@ void orc_init_flags_sphere(
                                             L->prob_lo[2] + k * L->dx[2]};
                     L->isf[b][fidx(fd, 0, i, j, k)] =
                         orc_sphere_covered(node, L->dx, center, radius, has_fluid_inside)
+                            ? 0
+                            : 1;
+                }
+    }
+    finish_is_fluid(L);
+}
+
+void orc_init_flags_cylinder(
+    orc_level *L, const double center[3], double radius, double height, int direction,
+    int has_fluid_inside)
+{
+    for (int b = 0; b < L->nbox; b++) {
+        const fabdim *fd = &L->fdg[b];
+        for (int64_t n = 0; n < 2 * fd->ncell; n++) L->isf[b][n] = 0;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int k = fd->lo[2]; k <= fd->hi[2]; k++)
+            for (int j = fd->lo[1]; j <= fd->hi[1]; j++)
+                for (int i = fd->lo[0]; i <= fd->hi[0]; i++) {
+                    const double node[3] = {L->prob_lo[0] + i * L->dx[0],
+                                            L->prob_lo[1] + j * L->dx[1],
+                                            L->prob_lo[2] + k * L->dx[2]};
+                    L->isf[b][fidx(fd, 0, i, j, k)] =
+                        orc_cylinder_covered(node, L->dx, center, radius, height, direction,
+                                             has_fluid_inside)
                             ? 0
                             : 1;
                 }

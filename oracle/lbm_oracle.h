@@ -91,6 +91,9 @@ void orc_set_fab(orc_level *L, int kind, int box, const void *in);
 void orc_init_flags_all_regular(orc_level *L);
 void orc_init_flags_sphere(
     orc_level *L, const double center[3], double radius, int has_fluid_inside);
+void orc_init_flags_cylinder(
+    orc_level *L, const double center[3], double radius, double height, int direction,
+    int has_fluid_inside);
 void orc_init_flags_from_mask(orc_level *L, const uint8_t *solid_mask);
 void orc_ic_taylorgreen(orc_level *L, double rho0, double v0, const double omega[3]);
 void orc_ic_constant(
@@ -186,6 +189,9 @@ double orc_masked_avgdown8(const double v[8]);
 int orc_sphere_covered(
     const double node_lo[3], const double dx[3], const double center[3],
     double radius, int has_fluid_inside);
+int orc_cylinder_covered(
+    const double node_lo[3], const double dx[3], const double center[3],
+    double radius, double height, int direction, int has_fluid_inside);
 
 /* OpenMP threads used by the oracle (1 = serial). */
 void orc_set_threads(int n);

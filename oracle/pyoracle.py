@@ -60,6 +60,7 @@ def load() -> ctypes.CDLL:
         "orc_set_fab": (None, [c_void_p, c_int, c_int, c_void_p]),
         "orc_init_flags_all_regular": (None, [c_void_p]),
         "orc_init_flags_sphere": (None, [c_void_p, D3, c_double, c_int]),
+        "orc_init_flags_cylinder": (None, [c_void_p, D3, c_double, c_double, c_int, c_int]),
         "orc_init_flags_from_mask": (None, [c_void_p, c_void_p]),
         "orc_ic_taylorgreen": (None, [c_void_p, c_double, c_double, D3]),
         "orc_ic_constant": (None, [c_void_p, c_double, D3, c_double, c_double, c_double]),
@@ -88,6 +89,7 @@ def load() -> ctypes.CDLL:
         "orc_get_temperature_vec": (c_double, [c_double, c_double, D3, c_double]),
         "orc_masked_avgdown8": (c_double, [D3]),
         "orc_sphere_covered": (c_int, [D3, D3, D3, c_double, c_int]),
+        "orc_cylinder_covered": (c_int, [D3, D3, D3, c_double, c_double, c_int, c_int]),
         "orc_eb_force": (None, [c_void_p, D3]),
         "orc_vorticity": (None, [c_void_p, c_void_p]),
         "orc_plot_vars": (None, [c_void_p, c_int, c_int, c_void_p]),
@@ -234,6 +236,10 @@ class OracleLevel:
 
     def flags_sphere(self, center, radius, has_fluid_inside=0):
         self.lib.orc_init_flags_sphere(self.h, d3(center), float(radius), int(has_fluid_inside))
+
+    def flags_cylinder(self, center, radius, height, direction, has_fluid_inside=0):
+        self.lib.orc_init_flags_cylinder(self.h, d3(center), float(radius), float(height), int(direction),
+                                         int(has_fluid_inside))
 
     def flags_mask(self, solid):
         m = np.ascontiguousarray(solid, dtype=np.uint8)
@@ -385,6 +391,8 @@ def build_oracle_level(cfg: dict, boxes=None, ng: int = 3) -> OracleLevel:
     eb = cfg["eb"]
     if eb["type"] == "sphere":
         L.flags_sphere(eb["center"], eb["radius"], eb.get("has_fluid_inside", 0))
+    elif eb["type"] == "cylinder":
+        L.flags_cylinder(eb["center"], eb["radius"], eb["height"], eb["direction"], eb.get("has_fluid_inside", 0))
     elif eb["type"] == "mask":
         L.flags_mask(eb["solid"])
     else:

@@ -131,10 +131,20 @@ int mrb_upload(mrb_level *lvl, int arr, const void *host);
 int mrb_download(const mrb_level *lvl, int arr, void *host);
 
 /* ---- flags (Source/LBM.cpp:1213-1262; EB sphere through Source/EB.cpp:5-38)
- * After mrb_upload(MRB_ARR_ISFLUID) or mrb_compute_flags_sphere the level
- * derives its stream masks.  Synchronous. */
+ * After mrb_upload(MRB_ARR_ISFLUID) or mrb_compute_flags_sphere /
+ * mrb_compute_flags_cylinder the level derives its stream masks.  A cell is
+ * solid when all 8 of its corner nodes lie in the body (a node on the
+ * surface counts as body).  Synchronous. */
 int mrb_compute_flags_sphere(
     mrb_level *lvl, const double center[3], double radius, int has_fluid_inside);
+/* EB2 geom_type "cylinder" (Source/EB.cpp:28-37, the eb2.cylinder_radius,
+ * _height, _direction, _center, _has_fluid_inside inputs of
+ * Tests/test_files/single_cylinder/single_cylinder.inp:38-43): a cylinder
+ * along axis `direction` (0, 1, 2) of the given radius, centred at
+ * `center` and `height` long. */
+int mrb_compute_flags_cylinder(
+    mrb_level *lvl, const double center[3], double radius, double height, int direction,
+    int has_fluid_inside);
 int mrb_finalize_flags(mrb_level *lvl);
 
 /* ---- initial state (Source/IC.H:460-525, IC.cpp:4-45; LBM.cpp:1278-1298)
@@ -200,9 +210,10 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *      moments and equilibrium coefficients; mass and energy residuals on
  *      the rest direction), 64x2-cell blocks;
  *  10  variant 9 on 256x1-cell blocks;
- *  11  k_sweep: FP64 storage with rows of at most 256 cells, one pass as a
- *      z-sweep with one resident block per x-row (MRB_E_UNSUPPORTED when
- *      the kernel's occupancy does not let every row be resident at once);
+ *  11  k_sweep: FP64 or shifted FP32 storage with rows of at most 256
+ *      cells, one pass as a z-sweep with one resident block per x-row
+ *      (MRB_E_UNSUPPORTED when the kernel's occupancy does not let every
+ *      row be resident at once);
  *  12  k_sweep_x64: the same z-sweep on single-wave blocks of 64 cells of
  *      a row (any row length; MRB_E_UNSUPPORTED when the blocks do not fit
  *      the GPU at once);

@@ -79,25 +79,43 @@ def cpu_baseline(cfg: dict, budget_s: float, threads: int) -> dict:
     }
 
 
-def copy_ceiling(torch, nbytes: int = 2 << 30, iters: int = 20) -> float:
-    """Device-to-device copy bandwidth (read + write bytes / s) of torch's copy
-    kernel on `nbytes` buffers: the measured HBM ceiling of this box, and the
-    known-size kernel that calibrates the rocprof byte counters."""
+def copy_ceiling(torch, nbytes: int = 4 << 30, iters: int = 10) -> tuple[float, dict]:
+    """Read + write bandwidth of the library's hand-written streaming copy
+    kernel (mrb_stream_copy) on `nbytes` buffers, best over a few grid
+    sizes: the measured HBM ceiling of this box.  Returns (GB/s, detail
+    with every grid size's rate and torch's own copy for comparison)."""
+    from marbles_amd import _lib
+    from marbles_amd._lib import check
+
+    lib = _lib.load()
     a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     a.fill_(1.0)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(iters):
-        b.copy_(a)
-    e.record()
-    torch.cuda.synchronize()
-    secs = s.elapsed_time(e) * 1e-3
+
+    def rate(fn) -> float:
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        s.record(stream)
+        for _ in range(iters):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return 2.0 * nbytes * iters / (s.elapsed_time(e) * 1e-3) / 1e9
+
+    detail = {}
+    for blocks in (512, 1024, 2048, 4096):
+        detail[f"blocks_{blocks}"] = round(rate(lambda: check(lib.mrb_stream_copy(b.data_ptr(), a.data_ptr(), nbytes,
+                                                                                   blocks, sh))), 1)
+    if not torch.equal(a, b):
+        raise RuntimeError("mrb_stream_copy: copy differs from its source")
+    detail["torch_copy"] = round(rate(lambda: b.copy_(a)), 1)
     del a, b
-    return 2.0 * nbytes * iters / secs / 1e9
+    best = max(v for k, v in detail.items() if k.startswith("blocks_"))
+    return best, detail
 
 
 def measured_traffic(storage: str, kernels) -> float | None:
@@ -437,6 +455,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the shifted-FP32 and scaled-FP16 comparison runs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep each level's default kernel instead of timing every variant on the live state")
     ap.add_argument("--workload", default="channel",
                     choices=["channel", "amr", "adjoint", "precision", "cylinder"],
                     help="channel: the single-level 256^3 channel (default); amr: the two-level workload; "
@@ -488,11 +508,15 @@ def main() -> int:
                                                                       128.0 * scale), tau=0.55, max_box=64)
     t0 = time.perf_counter()
     lvl = configs.build_level(cfg, layout="compact", storage=args.storage)
-    if args.kernel:
-        lvl.set_hot_kernel(args.kernel)
-    log(f"level built in {time.perf_counter() - t0:.2f}s; kernel: {lvl.hot_kernel_name()}")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
+    tuned = {}
+    if args.kernel:
+        lvl.set_hot_kernel(args.kernel)
+    elif not args.no_autotune:
+        tuned = lvl.autotune(reps=5, stream=sh)
+    log(f"level built in {time.perf_counter() - t0:.2f}s; kernel: {lvl.hot_kernel_name()}"
+        + (" (autotuned: " + ", ".join(f"{k}={v:.3f}" for k, v in tuned.items() if v > 0) + " ms)" if tuned else ""))
 
     cells = n ** 3
     run = time_level(lvl, args.steps, args.warmup, args.profile_steps, sh, torch)
@@ -520,6 +544,9 @@ def main() -> int:
         umax = max(float(np.abs(c[fluid]).max()) for c in u64)
         for st in legs:
             lr = configs.build_level(cfg, layout="compact", storage=st)
+            if not args.no_autotune:
+                lr.autotune(reps=5, stream=sh)
+            log(f"{st}: kernel {lr.hot_kernel_name()}")
             rr = time_level(lr, args.steps, args.warmup, args.profile_steps, sh, torch)
             (ur, rhor) = velocity_density(lr, cfg)
             sat = lr.storage_saturations()
@@ -547,8 +574,8 @@ def main() -> int:
             log(f"{st}: {mr:.1f} MLUPS ({mr / mlups:.2f}x), max rel velocity error {du:.2e}, density {dr:.2e}")
     else:
         lvl.close()
-    copy_gbs = copy_ceiling(torch)
-    log(f"copy-kernel ceiling: {copy_gbs:.1f} GB/s")
+    copy_gbs, copy_detail = copy_ceiling(torch)
+    log(f"streaming-copy ceiling: {copy_gbs:.1f} GB/s ({copy_detail})")
 
     result = {
         "metric": METRIC,
@@ -581,6 +608,7 @@ def main() -> int:
             "bytes_per_update": bpu,
             "kernel_ms_per_step": {k: round(v, 5) for k, v in per_kernel_ms.items()},
             "copy_ceiling_gbs": round(copy_gbs, 1),
+            "copy_ceiling_detail": copy_detail,
             "frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
         },
         "cpu_baseline": None,

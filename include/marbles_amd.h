@@ -487,6 +487,11 @@ int mrb_storage_saturations(const mrb_level *lvl, int64_t *count);
  * direction (zero for FP64), for callers that read the raw buffers of
  * mrb_population_ptr.  No reference counterpart. */
 int mrb_storage_shift(const mrb_level *lvl, int lattice, double out[27]);
+/* Streaming device-to-device copy of `bytes` (a multiple of 16, 16-byte
+ * aligned buffers) by a hand-written read + write kernel on `blocks` blocks
+ * of 256 threads (0: 1024), enqueued on `stream`: the measured HBM ceiling
+ * a step kernel's bandwidth is compared with.  No reference counterpart. */
+int mrb_stream_copy(void *dst, const void *src, int64_t bytes, int blocks, void *stream);
 
 #ifdef __cplusplus
 }

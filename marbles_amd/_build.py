@@ -29,6 +29,7 @@ SOURCES = [
     "mrb_adjoint.hip",
     "mrb_io.hip",
     "mrb_noalloc.hip",
+    "mrb_copy.hip",
 ]
 
 COMMON_FLAGS = [

@@ -118,6 +118,7 @@ SIGNATURES = {
     "mrb_autotune": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_double), POINTER(c_int)]),
     "mrb_reduce_conservation": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
     "mrb_reduce_conservation_noalloc": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "mrb_stream_copy": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int, c_void_p]),
     "mrb_last_error": (c_char_p, []),
     "mrb_hot_kernel_name": (c_char_p, [c_void_p]),
     "mrb_population_ptr": (c_int, [c_void_p, c_int, POINTER(c_void_p), POINTER(c_int64)]),

@@ -841,9 +841,9 @@ __device__ __forceinline__ void drain_wave()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-template <int BCX>
+template <class T, int BCX>
 __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
-    const double *__restrict__ f, const double *__restrict__ g, double *__restrict__ fo, double *__restrict__ go,
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, double *qring, uint32_t *flags, uint32_t *fault, CompactArgs A, BCDev bc,
     ShiftTab sh, CollideParams P)
 {
@@ -931,7 +931,8 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
                 if (!(m & 1u)) {
                     static_for<NQ>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        store_pair(fo, go, int64_t(q) * A.cs + cell, -1.0, -1.0);
+                        store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(-1.0, sh.f(q)),
+                                   narrow<T>(-1.0, sh.g(q)));
                     });
                 } else {
                     const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
@@ -954,8 +955,9 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
                     collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P,
                                            [&](auto qc, double fnew, double gnew) {
                                                constexpr int q = decltype(qc)::value;
-                                               store_pair(fo, go, int64_t(q) * A.cs + cell, poisoned ? nan : fnew,
-                                                          poisoned ? nan : gnew);
+                                               store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>,
+                                                          narrow<T>(poisoned ? nan : fnew, sh.f(q)),
+                                                          narrow<T>(poisoned ? nan : gnew, sh.g(q)));
                                            });
                 }
             }
@@ -1520,9 +1522,9 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
         }
     }
     if (variant == kSweepVariant) {
-        if constexpr (std::is_same<T, double>::value) {
+        if constexpr (!kIsHalf<T>) {
             if (!L->d_sweep_q || A.nx > kSweepRow)
-                return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 rows of at most 256 cells only");
+                return set_error(MRB_E_UNSUPPORTED, "k_sweep: rows of at most 256 cells only");
             k_zero_u32<<<(A.ny + 255) / 256, 256, 0, s>>>(L->d_sweep_flags, A.ny);
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[0], s));
@@ -1534,14 +1536,14 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
             ShiftTab shc = sh;
             CollideParams Pc = L->cp;
             int fit = 0;
-            if ((rc = resident_blocks(reinterpret_cast<const void *>(&k_sweep<BCX>), kSweepRow, fit))) return rc;
+            if ((rc = resident_blocks(reinterpret_cast<const void *>(&k_sweep<T, BCX>), kSweepRow, fit))) return rc;
             if (A.ny > fit) return set_error(MRB_E_UNSUPPORTED, "k_sweep: more rows than the GPU holds at once");
-            k_sweep<BCX><<<dim3(A.ny), dim3(kSweepRow), 0, s>>>(f, g, fo, go, mk, qr, fl, fa, Ac, bcc, shc, Pc);
+            k_sweep<T, BCX><<<dim3(A.ny), dim3(kSweepRow), 0, s>>>(f, g, fo, go, mk, qr, fl, fa, Ac, bcc, shc, Pc);
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[1], s));
             return MRB_OK;
         } else {
-            return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 storage only");
+            return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 and shifted FP32 storage only");
         }
     }
     if (variant == kOnePassVariant) {

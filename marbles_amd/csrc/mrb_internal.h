@@ -366,7 +366,7 @@ constexpr int kX256Variant = 7;   // two passes on 256x1 blocks
 constexpr int kHalf32X256Variant = 8; // variant 5 on 256x1 blocks
 constexpr int kF32Variant = 9;         // shifted FP32 storage, FP32 arithmetic in pass 2 (64x2 blocks)
 constexpr int kF32X256Variant = 10;    // variant 9 on 256x1 blocks
-constexpr int kSweepVariant = 11;      // single-pass z-sweep, one block per x-row (FP64, rows <= 256 cells)
+constexpr int kSweepVariant = 11;      // single-pass z-sweep, one block per x-row (FP64 / FP32, rows <= 256 cells)
 constexpr int kSweepRingPlanes = 8;    // planes of Q_y the z-sweep keeps for the rows beside
 constexpr int kSweepX64Variant = 12;   // the z-sweep on single-wave blocks of 64 cells of a row (FP64)
 // the variants whose pass 2 computes in FP32 (their results differ from the

@@ -2049,30 +2049,37 @@ int mrb_autotune(mrb_level *L, int reps, void *stream, double *ms_out, int *best
     uint32_t ovf_saved = 0;
     if (f16) MRB_HIP(hipMemcpyAsync(&ovf_saved, L->d_ovf16, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     if (f16) MRB_HIP(hipStreamSynchronize(s));
+    // every candidate is timed in two rounds and keeps its faster mean, so
+    // that a clock ramp or a transient on the device does not decide; the
+    // current variant runs first as a warm-up of the device
+    std::vector<double> mean(size_t(n), -1.0);
+    for (int r = 0; r < reps; ++r) {
+        const int rc = compact_variant_launch(L, L->hot, s);
+        if (rc && rc != MRB_E_UNSUPPORTED) return rc;
+    }
+    for (int round = 0; round < 2; ++round) {
+        for (int v = 0; v < n; ++v) {
+            auto launch = [&]() { return compact_variant_launch(L, v, s); };
+            if (reduced && fp32_collide_variant(v) != fp32_collide_variant(L->hot)) continue;
+            if (round == 1 && mean[size_t(v)] < 0.0) continue;  // not available for this level
+            int rc = launch();  // warm-up
+            if (rc == MRB_E_UNSUPPORTED) continue;  // storage, boundary types, occupancy
+            if (rc) return rc;
+            MRB_HIP(hipEventRecord(L->ev[3], s));
+            for (int r = 0; r < reps; ++r)
+                if ((rc = launch())) return rc;
+            MRB_HIP(hipEventRecord(L->ev[4], s));
+            MRB_HIP(hipEventSynchronize(L->ev[4]));
+            float ms = 0.f;
+            MRB_HIP(hipEventElapsedTime(&ms, L->ev[3], L->ev[4]));
+            const double m = double(ms) / reps;
+            if (mean[size_t(v)] < 0.0 || m < mean[size_t(v)]) mean[size_t(v)] = m;
+        }
+    }
     for (int v = 0; v < n; ++v) {
-        auto launch = [&]() { return compact_variant_launch(L, v, s); };
-        if (reduced && fp32_collide_variant(v) != fp32_collide_variant(L->hot)) {
-            if (ms_out) ms_out[v] = -1.0;
-            continue;
-        }
-        int rc = launch();  // warm-up
-        if (rc == MRB_E_UNSUPPORTED) {
-            // variant not available for this level (storage, boundary types)
-            if (ms_out) ms_out[v] = -1.0;
-            continue;
-        }
-        if (rc) return rc;
-        MRB_HIP(hipEventRecord(L->ev[3], s));
-        for (int r = 0; r < reps; ++r)
-            if ((rc = launch())) return rc;
-        MRB_HIP(hipEventRecord(L->ev[4], s));
-        MRB_HIP(hipEventSynchronize(L->ev[4]));
-        float ms = 0.f;
-        MRB_HIP(hipEventElapsedTime(&ms, L->ev[3], L->ev[4]));
-        const double mean = double(ms) / reps;
-        if (ms_out) ms_out[v] = mean;
-        if (mean < best_ms) {
-            best_ms = mean;
+        if (ms_out) ms_out[v] = mean[size_t(v)];
+        if (mean[size_t(v)] >= 0.0 && mean[size_t(v)] < best_ms) {
+            best_ms = mean[size_t(v)];
             arg = v;
         }
     }

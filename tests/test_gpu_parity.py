@@ -641,8 +641,8 @@ def test_step_variants_equal_two_pass_bitwise(engine, case, storage):
     """The other block shapes of the two passes (64x8, 64x2, 128x2, 256x1),
     k_onepass, which recomputes the neighbours' Q_corr from their pulls
     instead of reading pass 1's array, and the single-pass z-sweeps k_sweep
-    and k_sweep_x64 (FP64), which pass Q_corr between rows, do the same
-    operations in the
+    (FP64 and FP32 storage) and k_sweep_x64 (FP64), which pass Q_corr
+    between rows, do the same operations in the
     same order: their populations equal the 64x4 two-pass step's bit for bit
     (partial tiles, the sphere, inlet profiles, pressure ghosts, slip
     walls)."""
@@ -664,6 +664,8 @@ def test_step_variants_equal_two_pass_bitwise(engine, case, storage):
         names.append("k_onepass")
     if storage == "f64":
         names += ["k_sweep", "k_sweep_x64"]
+    elif storage == "f32s":
+        names.append("k_sweep")
     out = {}
     for name in names:
         E = configs.build_level(cfg, layout="compact", storage=storage)

@@ -120,17 +120,22 @@ def copy_ceiling(torch, nbytes: int = 4 << 30, iters: int = 10) -> tuple[float, 
 
 def measured_traffic(storage: str, kernels) -> float | None:
     """HBM bytes per step of the given kernels from the committed rocprofv3
-    summary (calibrated FETCH_SIZE + WRITE_SIZE per dispatch), or None when
-    the summary does not cover them."""
+    summaries (calibrated FETCH_SIZE + WRITE_SIZE per dispatch, the `json`
+    blocks of profiles/*.md), or None when no summary covers them."""
+    import glob
     import re
 
-    path = os.path.join(ROOT, "profiles", "k_qpass_k_collide_channel256.md")
-    try:
-        text = open(path).read()
-        block = re.search(r"```json\n(.*?)\n```", text, re.S).group(1)
-        data = json.loads(block)[storage]["kernels"]
-    except (OSError, AttributeError, KeyError, ValueError):
-        return None
+    data = {}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        try:
+            text = open(path).read()
+        except OSError:
+            continue
+        for block in re.findall(r"```json\n(.*?)\n```", text, re.S):
+            try:
+                data.update(json.loads(block).get(storage, {}).get("kernels", {}))
+            except (ValueError, AttributeError):
+                continue
     elem = {"f64": "double", "f32s": "float", "f16s": "_Float16"}[storage]
     total = 0.0
     for k in kernels:

@@ -18,7 +18,8 @@ import numpy as np
 from . import configs
 
 
-def coefficients(cfg: dict, t: np.ndarray, forces: np.ndarray, window: float = 0.3) -> dict:
+def coefficients(cfg: dict, t: np.ndarray, forces: np.ndarray, window: float = 0.3,
+                 min_lift_amplitude: float = 0.02) -> dict:
     """Drag / lift coefficients and Strouhal number from a force history
     (t: step numbers; forces: (n, 3)), over the last `window` of it."""
     rho = cfg["ic"]["density"]
@@ -30,15 +31,25 @@ def coefficients(cfg: dict, t: np.ndarray, forces: np.ndarray, window: float = 0
     k0 = int(len(t) * (1.0 - window))
     tw, cdw, clw = t[k0:], cd[k0:], cl[k0:]
     # shedding frequency from the upward zero crossings of the lift about
-    # its mean (linear interpolation between samples)
+    # its mean (linear interpolation between samples), counted only after
+    # the lift has been below -h since the last one (h: a quarter of the
+    # amplitude), so that noise on a steady lift is not taken for shedding;
+    # none when the lift swings by less than min_lift_amplitude
     c = clw - clw.mean()
-    up = np.nonzero((c[:-1] < 0.0) & (c[1:] >= 0.0))[0]
+    amp = 0.5 * float(clw.max() - clw.min())
     st, periods = None, 0
-    if len(up) >= 3:
-        tc = tw[up] + (tw[up + 1] - tw[up]) * (-c[up]) / (c[up + 1] - c[up])
-        period = float(np.mean(np.diff(tc)))
-        periods = len(up) - 1
-        st = d / (period * u)
+    if amp >= min_lift_amplitude:
+        h, armed, tc = 0.25 * amp, False, []
+        for k in range(len(c) - 1):
+            if c[k] < -h:
+                armed = True
+            if armed and c[k] < 0.0 <= c[k + 1]:
+                tc.append(tw[k] + (tw[k + 1] - tw[k]) * (-c[k]) / (c[k + 1] - c[k]))
+                armed = False
+        if len(tc) >= 3:
+            period = float(np.mean(np.diff(tc)))
+            periods = len(tc) - 1
+            st = d / (period * u)
     return {
         "cd_mean": float(cdw.mean()), "cd_max": float(cdw.max()), "cd_min": float(cdw.min()),
         "cl_max": float(clw.max()), "cl_min": float(clw.min()), "cl_amplitude": float(0.5 * (clw.max() - clw.min())),

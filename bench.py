@@ -187,6 +187,11 @@ def run_amr(args, torch) -> int:
     mlups = ups * args.steps / secs / 1e6
     log(f"timed {args.steps} coarse steps: {secs:.4f}s -> {mlups:.1f} MLUPS")
     H.close()
+    # algorithmic bytes: every population of every update read once and
+    # written once (864 B), as for the single-level step
+    bpu = configs.bytes_per_update("f64", 2)
+    achieved = bpu * ups * args.steps / secs / 1e9
+    traffic = amr_traffic(n)
     print(json.dumps({
         "metric": "two-level D3Q27 MLUPS (N_coarse + 2 N_fine updates per coarse step)",
         "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -196,8 +201,26 @@ def run_amr(args, torch) -> int:
                                f"{len(cfg['fine_boxes'])} fine boxes over {cfg['fine_fraction']:.4f} of the "
                                f"volume, thermal f+g, T0={cfg['T0']}, tau={cfg['tau']}, Pr=0.71",
                    "updates_per_step": ups, "graph": bool(args.graph)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_update": bpu, "note": "whole coarse step (all kernels), wall clock"},
     }), flush=True)
     return 0
+
+
+def amr_traffic(n: int) -> float | None:
+    """HBM bytes per coarse step of the two-level workload from the committed
+    rocprofv3 summary (calibrated FETCH_SIZE + WRITE_SIZE summed over the
+    step's kernels), or None when the summary does not cover this size."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "amr_two_level_256.md")
+    try:
+        block = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S).group(1)
+        rec = json.loads(block)["amr_f64"]
+    except (OSError, AttributeError, KeyError, ValueError):
+        return None
+    return rec["traffic_per_coarse_step"] if rec.get("n") == n else None
 
 
 VANDV = {"cd": 3.096, "cl": 0.8396, "st": 0.2986}  # Docs/sphinx/VandV.rst:14
@@ -335,13 +358,15 @@ def run_adjoint(args, torch) -> int:
         f"FD max rel err {err:.2e}")
     print(json.dumps({
         "metric": "adjoint gradient cost / forward cost (isothermal D3Q27, partial bounce-back; value = "
-                  "recorded forward + reverse sweep)",
-        "value": round(t_grad / t_fwd, 3), "unit": "x", "n_gpus": 1, "steps": nsteps, "warmup": 0,
+                  "recorded forward + reverse sweep with ceil(sqrt(N)) checkpoints and recomputation; "
+                  "every state kept: memory_aware_ratio)",
+        "value": round(sq["ratio"], 3), "unit": "x", "memory_aware_ratio": round(t_grad / t_fwd, 3),
+        "n_gpus": 1, "steps": nsteps, "warmup": 0,
         "ms_per_step": round(t_fwd / nsteps * 1e3, 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n}^3 box, n_s blobs (15 % mean), inflow 0.04, outflow, tau {tau}, "
-                               f"{nsteps} steps, checkpoint interval chosen by device memory "
-                               f"(every {kept['every']})",
+                               f"{nsteps} steps, checkpoints every {sq['every']} steps (memory-aware "
+                               f"interval: every {kept['every']})",
                    "J": J, "forward_s": t_fwd, "gradient_s": t_grad,
                    "recorded_forward_s": t_rec, "reverse_s": t_rev,
                    "reverse_over_forward": t_rev / t_fwd,

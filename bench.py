@@ -153,8 +153,13 @@ def velocity_density(lvl, cfg) -> tuple:
     return vd(lvl.valid("f").reshape(27, -1))
 
 
-def time_level(lvl, steps: int, warmup: int, profile_steps: int, sh, torch) -> dict:
+def time_level(lvl, steps: int, warmup: int, profile_steps: int, sh, torch, autotune: bool = False) -> dict:
+    """Warm-up steps, then (optionally) the autotuner on the warmed-up state,
+    then the timed steps and the per-kernel profile steps."""
     lvl.advance(warmup, stream=sh)
+    tuned = lvl.autotune(reps=10, stream=sh) if autotune else {}
+    if tuned:
+        log(f"autotuned ({lvl.hot_kernel_name()}): " + ", ".join(f"{k}={v:.3f}" for k, v in tuned.items() if v > 0))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lvl.advance(steps, stream=sh)
@@ -540,16 +545,14 @@ def main() -> int:
     lvl = configs.build_level(cfg, layout="compact", storage=args.storage)
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
-    tuned = {}
     if args.kernel:
         lvl.set_hot_kernel(args.kernel)
-    elif not args.no_autotune:
-        tuned = lvl.autotune(reps=5, stream=sh)
+    tune = not args.kernel and not args.no_autotune
     log(f"level built in {time.perf_counter() - t0:.2f}s; kernel: {lvl.hot_kernel_name()}"
-        + (" (autotuned: " + ", ".join(f"{k}={v:.3f}" for k, v in tuned.items() if v > 0) + " ms)" if tuned else ""))
+        + (" (autotuned after the warm-up steps)" if tune else ""))
 
     cells = n ** 3
-    run = time_level(lvl, args.steps, args.warmup, args.profile_steps, sh, torch)
+    run = time_level(lvl, args.steps, args.warmup, args.profile_steps, sh, torch, autotune=tune)
     secs = run["secs"]
     mlups = cells * args.steps / secs / 1e6
     log(f"timed {args.steps} steps: {secs:.4f}s -> {mlups:.1f} MLUPS")
@@ -574,10 +577,8 @@ def main() -> int:
         umax = max(float(np.abs(c[fluid]).max()) for c in u64)
         for st in legs:
             lr = configs.build_level(cfg, layout="compact", storage=st)
-            if not args.no_autotune:
-                lr.autotune(reps=5, stream=sh)
+            rr = time_level(lr, args.steps, args.warmup, args.profile_steps, sh, torch, autotune=not args.no_autotune)
             log(f"{st}: kernel {lr.hot_kernel_name()}")
-            rr = time_level(lr, args.steps, args.warmup, args.profile_steps, sh, torch)
             (ur, rhor) = velocity_density(lr, cfg)
             sat = lr.storage_saturations()
             lr.close()

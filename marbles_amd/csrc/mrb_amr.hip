@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kAmrBlock) k_explode_slopes(
     const double *__restrict__ crse, const BoxDev *__restrict__ cboxes, const ExplodeParent *__restrict__ par,
     int64_t npar, double *__restrict__ out)
 {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= npar * NQ) return;
     const int q = int(t / npar);
     const int64_t p = t - int64_t(q) * npar;
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kAmrBlock) k_explode(
     const double *__restrict__ slopes, int64_t npar, double *__restrict__ fine, const BoxDev *__restrict__ fboxes,
     const ExplodeEntry *__restrict__ map, int64_t n)
 {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const ExplodeEntry e = map[t];
     const BoxDev &FB = fboxes[e.fine_box];
@@ -336,6 +336,9 @@ void amr_release(mrb_level *L)
     if (L->d_coal_vals) (void)hipFree(L->d_coal_vals);
     if (L->d_qring) (void)hipFree(L->d_qring);
     L->d_qring = nullptr;
+    if (L->d_gmap_step) (void)hipFree(L->d_gmap_step);
+    L->d_gmap_step = nullptr;
+    L->n_gmap_step = 0;
     L->n_expl_le2 = 0;
     if (L->d_coal_has) (void)hipFree(L->d_coal_has);
     L->d_coal = L->d_expl = nullptr;
@@ -472,6 +475,49 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
         if (ring[o] == 1) qring.push_back(expl[o]);
     }
     expl.swap(sorted);
+    // The fine level's FillBoundary inside the two-level step: every first-
+    // layer ghost (the pulls of valid cells), and deeper ghosts only where
+    // a kernel of the step reads them: next to an uncovered ghost (the
+    // listed ghost stream and Q pass pull its 26 neighbours) or within the
+    // ghost depth of a non-periodic domain face (the boundary functor).
+    // The other deeper ghosts keep the values of the last full fill.
+    std::vector<GhostPair> step_map;
+    {
+        std::vector<uint8_t> need(size_t(fine->total_cells), 0);
+        auto fab_ijk = [](const BoxDev &B, int64_t loc, int c[3]) {
+            const int64_t nxy = int64_t(B.n[0]) * B.n[1];
+            c[0] = int(loc % B.n[0]);
+            c[1] = int((loc % nxy) / B.n[0]);
+            c[2] = int(loc / nxy);
+        };
+        for (const ExplodeEntry &e : expl) {
+            const BoxDev &FB = fine->boxes[e.fine_box];
+            int c[3];
+            fab_ijk(FB, e.fine_local, c);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int a = c[0] + dx, b = c[1] + dy, k = c[2] + dz;
+                        if (a < 0 || b < 0 || k < 0 || a >= FB.n[0] || b >= FB.n[1] || k >= FB.n[2]) continue;
+                        need[size_t(FB.base + (int64_t(k) * FB.n[1] + b) * FB.n[0] + a)] = 1;
+                    }
+        }
+        std::vector<GhostPair> full;
+        build_ghost_map(fine, fine->ng, full);
+        for (const GhostPair &p : full) {
+            const BoxDev &FB = fine->boxes[p.dst_box];
+            int c[3];
+            fab_ijk(FB, p.dst_local, c);
+            int layer = 0;
+            bool near_face = false;
+            for (int d = 0; d < 3; ++d) {
+                const int g = FB.lo[d] + c[d];  // domain index
+                layer = std::max(layer, std::max(FB.vlo[d] - g, g - FB.vhi[d]));
+                if (!fine->periodic[d] && (g < fine->ng || g >= fine->ncell[d] - fine->ng)) near_face = true;
+            }
+            if (layer <= 1 || near_face || need[size_t(FB.base + p.dst_local)]) step_map.push_back(p);
+        }
+    }
     // per coarse fab cell: the coalescence entry of valid cells under the
     // fine footprint or its ring
     std::vector<int32_t> coal_cell(size_t(crse->total_cells), -1);
@@ -517,6 +563,12 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine)
         MRB_HIP(hipMalloc(&fine->d_expl, sizeof(ExplodeEntry) * expl.size()));
         MRB_HIP(hipMemcpy(fine->d_expl, expl.data(), sizeof(ExplodeEntry) * expl.size(), hipMemcpyHostToDevice));
     }
+    if (!step_map.empty()) {
+        MRB_HIP(hipMalloc(&fine->d_gmap_step, sizeof(GhostPair) * step_map.size()));
+        MRB_HIP(hipMemcpy(fine->d_gmap_step, step_map.data(), sizeof(GhostPair) * step_map.size(),
+                          hipMemcpyHostToDevice));
+    }
+    fine->n_gmap_step = int64_t(step_map.size());
     if (!qring.empty()) {
         MRB_HIP(hipMalloc(&fine->d_qring, sizeof(ExplodeEntry) * qring.size()));
         MRB_HIP(hipMemcpy(fine->d_qring, qring.data(), sizeof(ExplodeEntry) * qring.size(), hipMemcpyHostToDevice));
@@ -553,12 +605,24 @@ int mrb_coalesce(const mrb_level *fine, mrb_level *crse, void *stream)
     return MRB_OK;
 }
 
+namespace mrb {
+// the two-level fillpatch of one lattice; `step_map`: the fine FillBoundary
+// covers what the two-level step reads (the linked pair's step map) rather
+// than every ghost layer
+static int explode_impl(const mrb_level *crse, mrb_level *fine, int lattice, hipStream_t s, bool step_map);
+}  // namespace mrb
+
 int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *stream)
 {
     if (!fine || !crse) return set_error(MRB_E_ARG, "null level");
     if (fine->amr_crse != crse) return set_error(MRB_E_STATE, "levels are not linked (mrb_amr_link)");
     if (lattice < 0 || lattice > 1) return set_error(MRB_E_ARG, "lattice must be 0 or 1");
-    hipStream_t s = as_stream(stream);
+    return explode_impl(crse, fine, lattice, as_stream(stream), false);
+}
+
+namespace mrb {
+static int explode_impl(const mrb_level *crse, mrb_level *fine, int lattice, hipStream_t s, bool step_map)
+{
     // the slopes read the coarse parents' neighbourhoods: FillBoundary of
     // the coarse level (ghost layers only, so its valid data is unchanged)
     const int depth = std::max(1, fine->expl_crse_depth);
@@ -569,9 +633,11 @@ int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *strea
     // fine-fine ghosts first, then the uncovered ones, then the domain
     // boundary (the three sets of cells are disjoint; physbc reads valid
     // and in-domain ghost cells only)
-    rc = fab_fill_boundary(fine, lattice, s);
+    const bool part = step_map && fine->d_gmap_step;
+    rc = part ? fab_fill_boundary_map(fine, lattice, fine->d_gmap_step, fine->n_gmap_step, s)
+              : fab_fill_boundary(fine, lattice, s);
     if (rc) return rc;
-    if (lattice == 1) fine->fresh_depth = fine->ng;
+    if (lattice == 1) fine->fresh_depth = part ? 1 : fine->ng;
     if (fine->n_expl > 0) {
         k_explode_slopes<<<blocks_for(fine->n_expl_par * NQ), kAmrBlock, 0, s>>>(
             (const double *)crse->pop[lattice][crse->cur], crse->d_boxes, (const ExplodeParent *)fine->d_expl_par,
@@ -582,8 +648,9 @@ int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *strea
             (const ExplodeEntry *)fine->d_expl, fine->n_expl);
         MRB_HIP(hipGetLastError());
     }
-    return mrb_physbc(fine, lattice, stream);
+    return mrb_physbc(fine, lattice, s);
 }
+}  // namespace mrb
 
 static int check_advance(mrb_level *crse, mrb_level *fine)
 {
@@ -611,16 +678,25 @@ static int amr_step(mrb_level *crse, mrb_level *fine, void *stream)
     for (int l = 0; l < 2; ++l)
         if ((rc = mrb_physbc(crse, l, stream))) return rc;
     for (int l = 0; l < 2; ++l)
-        if ((rc = mrb_explode(crse, fine, l, stream))) return rc;
+        if ((rc = explode_impl(crse, fine, l, s, true))) return rc;
     for (int sub = 0; sub < 2; ++sub) {
         for (int l = 0; l < 2; ++l)
             if ((rc = mrb_physbc(fine, l, stream))) return rc;
         // time_step(1) -> advance(1) (LBM.cpp:523-544): stream f, g and
         // collide as one batched step
         fine->stream_rings_le2 = sub == 1;
-        rc = fab_batched_step(fine, nullptr, sub == 0 ? fine->ng : 0, s);
+        rc = fab_batched_step(fine, nullptr, 0, s);
         fine->stream_rings_le2 = false;
         if (rc) return rc;
+        if (sub == 0) {
+            // the ghosts the second substep reads
+            for (int l = 0; l < 2; ++l) {
+                rc = fine->d_gmap_step ? fab_fill_boundary_map(fine, l, fine->d_gmap_step, fine->n_gmap_step, s)
+                                       : fab_fill_boundary(fine, l, s);
+                if (rc) return rc;
+            }
+            fine->fresh_depth = fine->d_gmap_step ? 1 : fine->ng;
+        }
     }
     // advance(0): stream, average_down_to, collide, with the coalescence
     // fused into the coarse passes

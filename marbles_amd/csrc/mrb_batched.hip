@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) k_coalesce_vals(
     const double *__restrict__ ff, const double *__restrict__ fg, const BoxDev *__restrict__ fboxes,
     const CoalesceEntry *__restrict__ map, int64_t n, double *__restrict__ vals, uint64_t *__restrict__ has)
 {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const CoalesceEntry e = map[t];
     const BoxDev &FB = fboxes[e.fine_box];
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256) k_fab_qpass_listed(
     const BoxDev *__restrict__ boxes, const uint32_t *__restrict__ mask, const ExplodeEntry *__restrict__ list,
     int64_t n, CollideParams P)
 {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const ExplodeEntry e = list[t];
     const BoxDev &B = boxes[e.fine_box];
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) k_stream_listed(
     double *__restrict__ go, const uint32_t *__restrict__ mask, const BoxDev *__restrict__ boxes,
     const ExplodeEntry *__restrict__ list, int64_t n)
 {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const ExplodeEntry e = list[t];
     const BoxDev &B = boxes[e.fine_box];

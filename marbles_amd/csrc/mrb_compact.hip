@@ -1181,6 +1181,27 @@ __global__ void __launch_bounds__(256) k_collide_h32_x256(
     collide_body<T, BCX, float>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
 }
 
+// The two on levels without inlet profiles, held to 128 registers (four
+// waves per SIMD; they need 118 then, and 129 without the bound, which
+// leaves three).  With a profile inlet the bound would spill, so those
+// levels use the kernels above (193 registers).
+template <class T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_collide_h32_w4(
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    Half16 h16, CollideParams P)
+{
+    collide_body<T, 0, float>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+}
+template <class T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_collide_h32_x256_w4(
+    const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, CompactArgs A, BCDev bc, ShiftTab sh,
+    Half16 h16, CollideParams P)
+{
+    collide_body<T, 0, float>(f, g, fo, go, mask, Q, A, bc, sh, h16, P);
+}
+
 // at most 168 registers (3 waves per SIMD), some values spilled to scratch
 template <class T, int BCX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_collide_w3(
@@ -1641,12 +1662,21 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
                                                                                    L->bcdev, sh, h16, L->cp);
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[1], s));
-            if (variant == kHalf32Variant)
-                k_collide_h32<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A,
-                                                                             L->bcdev, sh, h16, L->cp);
-            else
-                k_collide_h32_x256<T, BCX><<<grid_of(A, 1, 256), dim3(256, 1, 1), 0, s>>>(
-                    f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+            if (variant == kHalf32Variant) {
+                if constexpr (BCX == 0)
+                    k_collide_h32_w4<T><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q, A,
+                                                                               L->bcdev, sh, h16, L->cp);
+                else
+                    k_collide_h32<T, BCX><<<grid_of(A), dim3(64, 4, 1), 0, s>>>(f, g, fo, go, L->d_mask, L->d_q,
+                                                                                 A, L->bcdev, sh, h16, L->cp);
+            } else {
+                if constexpr (BCX == 0)
+                    k_collide_h32_x256_w4<T><<<grid_of(A, 1, 256), dim3(256, 1, 1), 0, s>>>(
+                        f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+                else
+                    k_collide_h32_x256<T, BCX><<<grid_of(A, 1, 256), dim3(256, 1, 1), 0, s>>>(
+                        f, g, fo, go, L->d_mask, L->d_q, A, L->bcdev, sh, h16, L->cp);
+            }
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[2], s));
             return MRB_OK;

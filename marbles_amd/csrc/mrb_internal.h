@@ -233,6 +233,11 @@ struct mrb_level {
     int64_t n_gmap = 0;
     mrb::GhostPair *d_gmap1 = nullptr;  // FillBoundary map, 1 layer (macrodata)
     int64_t n_gmap1 = 0;
+    // fine level of a linked pair: the FillBoundary entries the two-level
+    // step reads (the first layer, and deeper ghosts next to uncovered or
+    // out-of-domain ghosts); built by mrb_amr_link
+    mrb::GhostPair *d_gmap_step = nullptr;
+    int64_t n_gmap_step = 0;
     int64_t *d_bcells = nullptr;        // fab cells outside non-periodic faces
     int64_t n_bcells = 0;
     // d_bcells is ordered by the distance beyond the domain (sum over the
@@ -339,6 +344,18 @@ int hip_fail(hipError_t e, const char *what);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// The block index with the grid renumbered so that each XCD takes one
+// contiguous run of indices (the hardware hands block b to XCD b mod 8):
+// list kernels whose neighbouring entries gather neighbouring data then
+// share that XCD's L2 instead of spreading each neighbourhood over eight.
+// A bijection on [0, gridDim.x).
+__device__ __forceinline__ int64_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb / 8u, r = nb % 8u, x = b % 8u, k = b / 8u;
+    return int64_t(x * q + (x < r ? x : r) + k);
+}
+
 // compact-layout kernels (mrb_compact.hip)
 int compact_stream_collide(mrb_level *L, hipStream_t s);
 // one step with an event recorded before kernel slot k (ev[k]) and after the
@@ -391,6 +408,10 @@ int compact_variant_launch(mrb_level *L, int index, hipStream_t s);
 const char *compact_variant_name(int index);
 // FillBoundary of one lattice's current fab buffer (mrb_level.hip)
 int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s);
+// FillBoundary of one lattice's current fab buffer over the given map
+int fab_fill_boundary_map(mrb_level *L, int lattice, const GhostPair *map, int64_t n, hipStream_t s);
+// every ghost cell (within `depth` layers) that FillBoundary writes, with its owner
+void build_ghost_map(const mrb_level *L, int depth, std::vector<GhostPair> &map);
 // FillBoundary of both lattices, `depth` ghost layers (1 or ng); sets fresh_depth
 int fab_fill_boundary_depth(mrb_level *L, int depth, hipStream_t s);
 // FillBoundary of the batched step's Q array (3 comps in d_dq), one layer

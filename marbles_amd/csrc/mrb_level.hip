@@ -91,7 +91,7 @@ template <class T, int NC>
 __global__ void k_fill_map(
     T *__restrict__ arr, const BoxDev *__restrict__ boxes, const GhostPair *__restrict__ map, int64_t nmap)
 {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= nmap) return;
     const GhostPair p = map[t];
     const int64_t dn = boxes[p.dst_box].ncell, sn = boxes[p.src_box].ncell;
@@ -742,7 +742,7 @@ static hipError_t copy_components(
 // Fills `map` with every ghost cell of every box (within `depth` layers) that
 // AMReX FillBoundary would overwrite: its periodic image lies inside the
 // domain and in some box's valid region.
-static void build_ghost_map(const mrb_level *L, int depth, std::vector<GhostPair> &map)
+void build_ghost_map(const mrb_level *L, int depth, std::vector<GhostPair> &map)
 {
     const int64_t nd0 = L->ncell[0], nd1 = L->ncell[1], nd2 = L->ncell[2];
     std::vector<int32_t> owner(size_t(nd0 * nd1 * nd2), -1);
@@ -812,6 +812,14 @@ static int fill_boundary(mrb_level *L, void *arr, int ncomp, bool is_int, bool o
 int fab_fill_boundary(mrb_level *L, int lattice, hipStream_t s)
 {
     return fill_boundary(L, L->pop[lattice][L->cur], NQ, false, false, s);
+}
+
+int fab_fill_boundary_map(mrb_level *L, int lattice, const GhostPair *map, int64_t n, hipStream_t s)
+{
+    if (n == 0) return MRB_OK;
+    k_fill_map<double, NQ><<<nblocks(n), kBlock, 0, s>>>((double *)L->pop[lattice][L->cur], L->d_boxes, map, n);
+    MRB_HIP(hipGetLastError());
+    return MRB_OK;
 }
 
 int fab_fill_boundary_q(mrb_level *L, hipStream_t s) { return fill_boundary(L, L->d_dq, 3, false, true, s); }

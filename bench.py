@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "D3Q27 MLUPS & % HBM roofline, FP64 vs FP32-shifted at fixed error"
+# the FP64-arithmetic collide of the shifted FP32 storage, run beside the
+# storage's default FP32-arithmetic one (the autotuner then chooses among
+# the block shapes of that arithmetic)
+F32_FP64_KERNEL = "k_qpass_y2+k_collide_y2"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
@@ -386,8 +390,8 @@ def run_precision(args, torch) -> int:
     """Precision study: periodic n^3 box (default 512^3) with a seeded
     random-phase divergence-free velocity field (E(k) ~ k^-5/3, k <= 32,
     max |u| = 0.04), T0 = 0.2, run for the given steps with FP64,
-    shifted-FP32 and scaled-FP16 population storage (each level's default
-    step: FP64 arithmetic, FP32 in the scaled-FP16 collide); velocity,
+    shifted-FP32 (FP32 arithmetic, the default, and FP64 arithmetic) and
+    scaled-FP16 population storage (FP32 in its collide); velocity,
     density and kinetic energy of each compared with the
     FP64 run on the device (relative L2 and max errors).  tau = 0.55: at
     tau = 0.52 the thermal f+g model diverges on this field at 512^3
@@ -408,7 +412,8 @@ def run_precision(args, torch) -> int:
     ref = None
     legs = {}
     chunk = 100
-    for st in ("f64", "f32s", "f16s"):
+    for leg_name in ("f64", "f32s", "f32s_fp64", "f16s"):
+        st = leg_name.split("_")[0]
         t0 = time.perf_counter()
         lvl = Level(cfg["ncell"], cfg["periodic"], cfg["prob_lo"], cfg["prob_hi"], layout="compact", storage=st)
         lvl.set_phys(cfg["nu"], cfg.get("alpha"))
@@ -418,11 +423,13 @@ def run_precision(args, torch) -> int:
         lvl.flags_all_regular()
         lvl.init_fields(*fields, adiabatic_exponent=cfg["ic"]["adiabatic_exponent"])
         torch.cuda.synchronize()
-        log(f"{st}: level initialised in {time.perf_counter() - t0:.1f}s")
-        # the fastest step variant at this size, timed on the live state
-        # (which it leaves unchanged); the default ones are chosen at 256^3
+        log(f"{leg_name}: level initialised in {time.perf_counter() - t0:.1f}s")
+        if leg_name == "f32s_fp64":  # the FP64-arithmetic collide of the same storage
+            lvl.set_hot_kernel(F32_FP64_KERNEL)
+        # the fastest step variant at this size among those of the level's
+        # arithmetic, timed on the live state (which it leaves unchanged)
         lvl.autotune(reps=3, stream=sh)
-        log(f"{st}: kernel {lvl.hot_kernel_name()}")
+        log(f"{leg_name}: kernel {lvl.hot_kernel_name()}")
         lvl.advance(args.warmup, stream=sh)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -432,7 +439,7 @@ def run_precision(args, torch) -> int:
             lvl.advance(k, stream=sh)
             torch.cuda.synchronize()
             done += k
-            log(f"{st}: {done}/{nsteps} steps, {time.perf_counter() - t0:.1f}s")
+            log(f"{leg_name}: {done}/{nsteps} steps, {time.perf_counter() - t0:.1f}s")
         secs = time.perf_counter() - t0
         mac = torch.empty(5 * N, dtype=torch.float64, device="cuda")
         lvl.macro_fields(mac, stream=sh)
@@ -458,8 +465,9 @@ def run_precision(args, torch) -> int:
                 "mass_rel": abs(float(cons[0]) - float(c0[0])) / abs(float(c0[0])),
                 "saturated_values": sat,
             })
-        legs[st] = leg
-        log(f"{st}: " + ", ".join(f"{k}={v:.3g}" if isinstance(v, float) else f"{k}={v}" for k, v in leg.items()))
+        legs[leg_name] = leg
+        log(f"{leg_name}: " + ", ".join(f"{k}={v:.3g}" if isinstance(v, float) else f"{k}={v}"
+                                        for k, v in leg.items()))
         del mac
         torch.cuda.empty_cache()
     print(json.dumps({
@@ -567,7 +575,7 @@ def main() -> int:
 
     # the same run with shifted FP32 and with scaled FP16 populations,
     # compared at equal step count
-    legs = {"f32s": None, "f16s": None}
+    legs = {"f32s": None, "f32s_fp64": None, "f16s": None}
     if args.storage == "f64" and not args.no_f32:
         import numpy as np
 
@@ -575,10 +583,13 @@ def main() -> int:
         lvl.close()
         fluid = rho64 > 0
         umax = max(float(np.abs(c[fluid]).max()) for c in u64)
-        for st in legs:
+        for leg_name in legs:
+            st = leg_name.split("_")[0]
             lr = configs.build_level(cfg, layout="compact", storage=st)
+            if leg_name == "f32s_fp64":
+                lr.set_hot_kernel(F32_FP64_KERNEL)
             rr = time_level(lr, args.steps, args.warmup, args.profile_steps, sh, torch, autotune=not args.no_autotune)
-            log(f"{st}: kernel {lr.hot_kernel_name()}")
+            log(f"{leg_name}: kernel {lr.hot_kernel_name()}")
             (ur, rhor) = velocity_density(lr, cfg)
             sat = lr.storage_saturations()
             lr.close()
@@ -587,22 +598,23 @@ def main() -> int:
             kms = sum(rr["per_kernel_ms"].values())
             mr = cells * args.steps / rr["secs"] / 1e6
             br = configs.bytes_per_update(st, 2)
-            legs[st] = {
+            legs[leg_name] = {
                 "value": round(mr, 2),
                 "unit": "MLUPS",
                 "speedup_vs_f64": round(mr / mlups, 3),
                 "max_rel_velocity_error": du,
                 "max_rel_density_error": dr,
                 "compared_after_steps": args.warmup + args.steps + args.profile_steps,
-                "arithmetic": "f32" if any("_h32" in k for k in rr["per_kernel_ms"]) else "f64",
+                "arithmetic": "f32" if any(("_h32" in k or "_f32" in k) for k in rr["per_kernel_ms"]) else "f64",
                 "bytes_per_update": br,
                 "achieved_gbs": round(br * cells / (kms * 1e-3) / 1e9, 1),
                 "traffic": measured_traffic(st, rr["per_kernel_ms"]) if n == 256 else None,
                 "kernel_ms_per_step": {k: round(v, 5) for k, v in rr["per_kernel_ms"].items()},
             }
             if st == "f16s":
-                legs[st]["saturated_values"] = sat
-            log(f"{st}: {mr:.1f} MLUPS ({mr / mlups:.2f}x), max rel velocity error {du:.2e}, density {dr:.2e}")
+                legs[leg_name]["saturated_values"] = sat
+            log(f"{leg_name}: {mr:.1f} MLUPS ({mr / mlups:.2f}x), max rel velocity error {du:.2e}, "
+                f"density {dr:.2e}")
     else:
         lvl.close()
     copy_gbs, copy_detail = copy_ceiling(torch)
@@ -644,6 +656,7 @@ def main() -> int:
         },
         "cpu_baseline": None,
         "f32_shifted": legs["f32s"],
+        "f32_shifted_fp64_arithmetic": legs["f32s_fp64"],
         "f16_scaled": legs["f16s"],
     }
     if not args.no_cpu_baseline:

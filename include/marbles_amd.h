@@ -208,7 +208,8 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *   9  shifted FP32 storage only, no PRESSURE faces: the collide pass in
  *      FP32 arithmetic on the stored deviations from the rest state (FP64
  *      moments and equilibrium coefficients; mass and energy residuals on
- *      the rest direction), 64x2-cell blocks;
+ *      the rest direction, momentum residuals on the axis pairs),
+ *      64x2-cell blocks;
  *  10  variant 9 on 256x1-cell blocks;
  *  11  k_sweep: FP64 or shifted FP32 storage with rows of at most 256
  *      cells, one pass as a z-sweep with one resident block per x-row
@@ -220,7 +221,7 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *  13+ the single-pass LDS z-sweep with different tile shapes.
  * The FP64-arithmetic variants give the same populations bit for bit; 5
  * and 8 give the same as each other, and 9 and 10.  A level
- * starts on variant 7 (FP64 storage), 4 (shifted FP32) or 5 (scaled FP16).
+ * starts on variant 7 (FP64 storage), 9 (shifted FP32) or 5 (scaled FP16).
  * While a level has a PRESSURE face, variants 5 and 9 are replaced by 0 and
  * 4, 8 and 10 by 7 (mrb_set_bc), and restored when the faces change back.
  * A variant that does not support the level returns MRB_E_UNSUPPORTED from
@@ -288,16 +289,24 @@ int mrb_amr_link(mrb_level *crse, mrb_level *fine);
  * list wins.  Both lattices; asynchronous on `stream`. */
 int mrb_coalesce(const mrb_level *fine, mrb_level *crse, void *stream);
 /* FillPatchOps::fillpatch on a fine level (Source/FillPatchOps.H:66-130,
- * FillPatchTwoLevels branch): FillBoundary among fine boxes, fine ghost
- * cells over uncovered domain interpolated from the coarse level
- * (conservative linear, monotonised-central limited slopes; replaces AMReX
- * cell_cons_interp), then the physical boundary functor.  Asynchronous. */
+ * FillPatchTwoLevels branch): FillBoundary among fine boxes (every ghost
+ * layer), fine ghost cells over uncovered domain interpolated from the
+ * coarse level (AMReX cell_cons_interp, FillPatchOps.H:190, restated:
+ * monotonised-central slopes, one-sided at the non-periodic faces the
+ * reference marks ext_dir, one factor per coarse cell keeping the children
+ * within the 3x3x3 neighbourhood's min/max), then the physical boundary
+ * functor.  Asynchronous. */
 int mrb_explode(const mrb_level *crse, mrb_level *fine, int lattice, void *stream);
 /* nsteps coarse steps of LBM::evolve -> time_step -> advance
  * (Source/LBM.cpp:398-544) on a linked pair: coarse ghost fill; explode
  * f, g; twice (fine physbc, fine stream f, g, fine collide); coarse
  * stream f, g; coalesce; coarse collide.  The fine level's dt must be half
- * the coarse one (compute_dt, LBM.cpp:1047-1077).  Asynchronous. */
+ * the coarse one (compute_dt, LBM.cpp:1047-1077).  Inside the step the fine
+ * FillBoundary refreshes the first ghost layer and the deeper ghosts that
+ * a step kernel reads (next to uncovered ghosts or non-periodic faces);
+ * other deeper fine ghosts keep older values until mrb_fill_ghosts or
+ * mrb_explode fills every layer.  Valid cells are unaffected.
+ * Asynchronous. */
 int mrb_amr_advance(mrb_level *crse, mrb_level *fine, int nsteps, void *stream);
 /* Regrid (AmrCore::regrid -> LBM::RemakeLevel, Source/LBM.cpp:1302-1365):
  * mrb_cluster_tags turns refinement tags on the coarse domain (one byte per

@@ -47,6 +47,14 @@ constexpr int reflect(int q, int d)
     return -1;
 }
 
+// Index of the direction with lattice velocity (cx, cy, cz).
+constexpr int qof(int cx, int cy, int cz)
+{
+    for (int p = 0; p < NQ; ++p)
+        if (kEV[p][0] == cx && kEV[p][1] == cy && kEV[p][2] == cz) return p;
+    return -1;
+}
+
 // Stencil.H:81-85: WO is computed from the other three weights.
 constexpr double kWSC1 = 2.0 / 27.0;
 constexpr double kWFCC1 = 1.0 / 54.0;

@@ -349,12 +349,12 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // list kernels whose neighbouring entries gather neighbouring data then
 // share that XCD's L2 instead of spreading each neighbourhood over eight.
 // A bijection on [0, gridDim.x).
-__device__ __forceinline__ int64_t xcd_block()
+__device__ __forceinline__ uint32_t xcd_index(uint32_t b, uint32_t nb)
 {
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t q = nb / 8u, r = nb % 8u, x = b % 8u, k = b / 8u;
-    return int64_t(x * q + (x < r ? x : r) + k);
+    return x * q + (x < r ? x : r) + k;
 }
+__device__ __forceinline__ int64_t xcd_block() { return int64_t(xcd_index(blockIdx.x, gridDim.x)); }
 
 // compact-layout kernels (mrb_compact.hip)
 int compact_stream_collide(mrb_level *L, hipStream_t s);

@@ -1068,12 +1068,13 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
     L->layout = desc->layout;
     L->nlattice = desc->nlattice;
     L->storage = desc->storage;
-    // the fastest step variant per storage at 256^3: 256x1-cell blocks for
-    // FP64, 64x2 for shifted FP32, the FP32-arithmetic collide for scaled
-    // FP16 (FP64 arithmetic on levels with PRESSURE faces, mrb_set_bc);
+    // the default step variant per storage: two passes on 256x1-cell blocks
+    // for FP64; for the reduced storages the FP32-arithmetic collide (64x2
+    // blocks for shifted FP32, 64x4 for scaled FP16), which levels with
+    // PRESSURE faces replace by FP64 arithmetic (mrb_set_bc);
     // mrb_set_hot_kernel and mrb_autotune select others
     L->hot = L->storage == MRB_STORAGE_F64            ? kX256Variant
-             : L->storage == MRB_STORAGE_F32_SHIFTED ? kRows2Variant
+             : L->storage == MRB_STORAGE_F32_SHIFTED ? kF32Variant
                                                       : kHalf32Variant;
     L->hot_default = true;
     L->hot_h32 = -1;

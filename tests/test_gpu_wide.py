@@ -117,8 +117,8 @@ def test_wide_rows_reduced_storages(oracle, engine, storage):
 
     u64 = vel(f64)
     for name in names:
-        # FP32 storage: 1e-6 with FP64 arithmetic, 5e-6 with the FP32 collide
-        bar = (5e-6 if "_f32" in name else 1e-6) if storage == "f32s" else 5e-3
+        # FP32 storage: 1e-6 with FP64 or FP32 arithmetic
+        bar = 1e-6 if storage == "f32s" else 5e-3
         du = float(np.abs(vel(out[name][0]) - u64).max() / np.abs(u64).max())
         assert du <= bar, (storage, name, du)
 

@@ -485,7 +485,10 @@ int mrb_stencil(int evs[81], double weights[27], int bounce_dirs[27]);
 /* Name of the kernel variant used by mrb_stream_collide. */
 const char *mrb_hot_kernel_name(const mrb_level *lvl);
 /* Device pointers of the current population buffers (for zero-copy use by
- * the Python/torch wrappers).  lattice 0 = f, 1 = g. */
+ * the Python/torch wrappers).  lattice 0 = f, 1 = g.  Fab layout: each
+ * box's FAB, components contiguous.  Compact layout: f_q and g_q of a cell
+ * side by side, element (q*N + cell)*2 + lattice of the f pointer (the g
+ * pointer is the f pointer plus one element). */
 int mrb_population_ptr(const mrb_level *lvl, int lattice, void **dev_ptr, int64_t *bytes);
 /* FP16 storage: number of block writes (256 cells, one lattice) since the
  * level was created in which some value exceeded the FP16 range at its

@@ -2,11 +2,10 @@
 // pull-stream with halfway bounce-back, periodic wrap and physical-boundary
 // ghost values resolved inside the kernel, followed by the thermal collide.
 //
-// HBM layout: populations SoA, cell = (z*ny + y)*nx + x.  FP64: f and g
-// interleaved by component, pop[(2q + lattice)*N + cell], so f_q =
-// f[q*cs + cell], g_q = g[q*cs + cell] with g = f + N and cs = 2N.  Shifted
-// FP32 and scaled FP16: pop[(q*N + cell)*2 + lattice], f_q = f[q*cs +
-// 2*cell] with g = f + 1 (one 8- or 4-byte access moves f_q and g_q).
+// HBM layout: populations SoA by direction, cell = (z*ny + y)*nx + x, with
+// f_q and g_q of a cell side by side: pop[(q*N + cell)*2 + lattice], so
+// f_q = f[q*cs + 2*cell] with g = f + 1 and cs = 2N; one 16-byte (FP64),
+// 8-byte (shifted FP32) or 4-byte (scaled FP16) access moves both.
 // Two buffers (A/B); a 32-bit stream mask per cell (bit q = the pull
 // source x - c_q is fluid).
 //
@@ -83,13 +82,11 @@ struct DevRest {
     double mo[14];  // moments of the shifts, in MomentsR order
 };
 
-// Element stride between neighbouring cells of one component: FP64
-// storage keeps the components of a lattice contiguous
-// (pop[(2q + lattice)*N + cell]); shifted FP32 storage keeps f_q and g_q
-// of a cell side by side (pop[(q*N + cell)*2 + lattice]) so that one
-// 8-byte load or store moves both.
+// Element stride between neighbouring cells of one component: every
+// storage keeps f_q and g_q of a cell side by side
+// (pop[(q*N + cell)*2 + lattice]) so that one load or store moves both.
 template <class T>
-constexpr int kCellStride = std::is_same<T, double>::value ? 1 : 2;
+constexpr int kCellStride = 2;
 template <class T>
 constexpr bool kIsHalf = std::is_same<T, _Float16>::value;
 
@@ -100,11 +97,12 @@ __device__ __forceinline__ void store_pair(T *__restrict__ fo, T *__restrict__ g
     if constexpr (kIsHalf<T>) {
         const uint32_t lo = __builtin_bit_cast(uint16_t, fv), hi = __builtin_bit_cast(uint16_t, gv);
         *reinterpret_cast<uint32_t *>(fo + e) = lo | (hi << 16);
-    } else if constexpr (kCellStride<T> == 2) {
-        *reinterpret_cast<float2 *>(fo + e) = make_float2(fv, gv);
+    } else if constexpr (std::is_same<T, double>::value) {
+        (void)go;
+        *reinterpret_cast<double2 *>(fo + e) = make_double2(fv, gv);
     } else {
-        fo[e] = fv;
-        go[e] = gv;
+        (void)go;
+        *reinterpret_cast<float2 *>(fo + e) = make_float2(fv, gv);
     }
 }
 
@@ -362,14 +360,14 @@ __device__ __forceinline__ void pull_cell(
                 const uint32_t v = *reinterpret_cast<const uint32_t *>(bf + off);
                 rf[q] = __builtin_bit_cast(_Float16, uint16_t(v & 0xffffu));
                 rg[q] = __builtin_bit_cast(_Float16, uint16_t(v >> 16));
-            } else if constexpr (kCellStride<T> == 2) {
-                const float2 v = *reinterpret_cast<const float2 *>(bf + off);
+            } else if constexpr (std::is_same<T, double>::value) {
+                const double2 v = *reinterpret_cast<const double2 *>(bf + off);
                 rf[q] = v.x;
                 rg[q] = v.y;
             } else {
-                const char *bg = reinterpret_cast<const char *>(g) + int64_t(q) * A.cs * int64_t(sizeof(T));
-                rf[q] = *reinterpret_cast<const T *>(bf + off);
-                rg[q] = *reinterpret_cast<const T *>(bg + off);
+                const float2 v = *reinterpret_cast<const float2 *>(bf + off);
+                rf[q] = v.x;
+                rg[q] = v.y;
             }
         });
         cmask = V;
@@ -406,14 +404,16 @@ __device__ __forceinline__ void pull_cell(
             const uint32_t v = *reinterpret_cast<const uint32_t *>(f + e);
             rf[q] = __builtin_bit_cast(_Float16, uint16_t(v & 0xffffu));
             rg[q] = __builtin_bit_cast(_Float16, uint16_t(v >> 16));
-        } else if constexpr (kCellStride<T> == 2) {
-            // f_q and g_q side by side: one 8-byte load (g == f + 1)
-            const float2 v = *reinterpret_cast<const float2 *>(f + e);
+        } else if constexpr (std::is_same<T, double>::value) {
+            // f_q and g_q side by side: one 16-byte load (g == f + 1)
+            const double2 v = *reinterpret_cast<const double2 *>(f + e);
             rf[q] = v.x;
             rg[q] = v.y;
         } else {
-            rf[q] = f[e];
-            rg[q] = g[e];
+            // one 8-byte load
+            const float2 v = *reinterpret_cast<const float2 *>(f + e);
+            rf[q] = v.x;
+            rg[q] = v.y;
         }
         cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
     });
@@ -1113,7 +1113,7 @@ __global__ void __launch_bounds__(kSweepW, 1) k_sweep_x64(
                 if (!(m & 1u)) {
                     static_for<NQ>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        store_pair(fo, go, int64_t(q) * A.cs + cell, -1.0, -1.0);
+                        store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<double>, -1.0, -1.0);
                     });
                 } else {
                     const LdsRow F{&hold[lane], 2 * kSweepW}, G{&hold[kSweepW + lane], 2 * kSweepW};
@@ -1134,7 +1134,8 @@ __global__ void __launch_bounds__(kSweepW, 1) k_sweep_x64(
                     collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P,
                                            [&](auto qc, double fnew, double gnew) {
                                                constexpr int q = decltype(qc)::value;
-                                               store_pair(fo, go, int64_t(q) * A.cs + cell, poisoned ? nan : fnew,
+                                               store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<double>,
+                                                          poisoned ? nan : fnew,
                                                           poisoned ? nan : gnew);
                                            });
                 }
@@ -1379,7 +1380,8 @@ CompactArgs make_args(const mrb_level *L)
     for (int n = 0; n < 6; ++n) A.bc[n] = L->bc.bc_type[n];
     A.bcpack = 0;
     for (int n = 0; n < 6; ++n) A.bcpack |= uint32_t(A.bc[n] & 15) << (4 * n);
-    const int64_t cell_bytes = L->storage == MRB_STORAGE_F64 ? 8 : (L->storage == MRB_STORAGE_F32_SHIFTED ? 8 : 4);
+    // bytes of one cell's (f_q, g_q) pair
+    const int64_t cell_bytes = L->storage == MRB_STORAGE_F64 ? 16 : (L->storage == MRB_STORAGE_F32_SHIFTED ? 8 : 4);
     A.fast_ok = A.N * cell_bytes < (int64_t(1) << 32) ? 1 : 0;
     return A;
 }

@@ -81,7 +81,8 @@ __device__ __forceinline__ DimSrc dim_src(int x, int n, int periodic, int bc_lo,
 }
 
 // Pull the 27 populations of the lane's lattice for cell (x,y,z).
-// lat_off = 0 (f) or N (g).  Cells away from the domain faces take a
+// lat_off = 0 (f) or 1 (g): f_q and g_q of a cell are neighbours,
+// element (q*N + cell)*2 + lattice.  Cells away from the domain faces take a
 // branch-free path with uniform offsets; face cells resolve periodic wraps
 // and physical-boundary ghosts with selects.  The face applied to a ghost is
 // the last non-periodic face in BCFill's loop order (BC.H:365-372); at a
@@ -103,8 +104,8 @@ __device__ __forceinline__ void pull_lattice(
             // base: component qlo; direct source at +(q-qlo)*cs, bounce-back
             // (own cell, opposite direction) at +(qb-qlo)*cs
             const uint32_t back = uint32_t(cx) + uint32_t(cy) * dy + uint32_t(cz) * dz;
-            const uint32_t e_direct = uint32_t(q - qlo) * cs + cell - back;
-            const uint32_t e_bounce = uint32_t(qb - qlo) * cs + cell;
+            const uint32_t e_direct = uint32_t(q - qlo) * cs + 2u * (cell - back);
+            const uint32_t e_bounce = uint32_t(qb - qlo) * cs + 2u * cell;
             const uint32_t e = ((mask >> q) & 1u) ? e_direct : e_bounce;
             v[q] = load_at(f + int64_t(qlo) * A.cs, (e + lat_off) * 8u);
         });
@@ -122,8 +123,8 @@ __device__ __forceinline__ void pull_lattice(
         const int kind = kz ? kz : (ky ? ky : kx);
         const uint32_t src = uint32_t(Z.s[cz + 1]) * dz + uint32_t(Y.s[cy + 1]) * dy + uint32_t(X.s[cx + 1]);
         const bool bounce = kind == MRB_BC_NOSLIP || (kind == 0 && !((mask >> q) & 1u));
-        const uint32_t e_direct = uint32_t(q - qlo) * cs + src;
-        const uint32_t e_bounce = uint32_t(qb - qlo) * cs + cell;
+        const uint32_t e_direct = uint32_t(q - qlo) * cs + 2u * src;
+        const uint32_t e_bounce = uint32_t(qb - qlo) * cs + 2u * cell;
         const uint32_t e = bounce ? e_bounce : e_direct;
         const double val = load_at(f + int64_t(qlo) * A.cs, (e + lat_off) * 8u);
         const double inlet = lat ? bcv[NQ + q] : bcv[q];
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__(64 * TY) k_fused(
     const bool interior = in_dom && c >= 1 && c <= kInner && row >= 1 && row <= TY - 2;
     const int z0 = tz * A.zc;
     const int z1 = min(z0 + A.zc, A.nz);
-    const uint32_t lat_off = lat ? uint32_t(A.N) : 0u;
+    const uint32_t lat_off = lat ? 1u : 0u;
     const int64_t plane = int64_t(A.nx) * A.ny;
     const int64_t col = int64_t(y) * A.nx + x;
 
@@ -255,7 +256,7 @@ __global__ void __launch_bounds__(64 * TY) k_fused(
         __syncthreads();
         if (interior) {
             const int64_t cell = int64_t(z) * plane + col;
-            const uint32_t soff = (uint32_t(cell) + lat_off) * 8u;
+            const uint32_t soff = (2u * uint32_t(cell) + lat_off) * 8u;
             if (mc & 1u) {
                 if constexpr (!HOLD) pull_lattice(f, bcv, lat, lat_off, A, x, y, z, uint32_t(cell), mc, Pc);
                 double S[10];
@@ -341,8 +342,8 @@ template <int TY, bool HOLD>
 int launch_fused(mrb_level *L, int zc, hipStream_t s)
 {
     const FusedArgs A = make_fused_args(L, TY, zc);
-    // byte offsets of a component are 32-bit: (2N + N) * 8 < 2^32
-    if (3 * A.N * 8 >= (int64_t(1) << 32))
+    // byte offsets from a component pair's base are 32-bit: (2N + 2N) * 8 < 2^32
+    if (4 * A.N * 8 >= (int64_t(1) << 32))
         return set_error(MRB_E_UNSUPPORTED, "fused kernel: domain too large for 32-bit offsets");
     const unsigned nb = unsigned((A.ntiles + 7) / 8 * 8);
     const double *f = (const double *)L->pop[0][L->cur];

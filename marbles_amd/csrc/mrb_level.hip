@@ -594,7 +594,8 @@ __global__ void k_fill_const(double *__restrict__ a, int64_t n, double v)
 // cs = component stride (0: the box's own fab size, the reference layout)
 __global__ void k_init_uniform(
     double *__restrict__ pop, const int32_t *__restrict__ isf, const BoxDev *__restrict__ boxes,
-    const int64_t *__restrict__ base, int nbox, int64_t total, const double *__restrict__ val, int64_t cs)
+    const int64_t *__restrict__ base, int nbox, int64_t total, const double *__restrict__ val, int64_t cs,
+    int cst)
 {
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= total) return;
@@ -603,7 +604,7 @@ __global__ void k_init_uniform(
     const int64_t loc = c - B.base;
     const bool solid = isf[B.base * 2 + loc] == 0;
     const int64_t stride = cs ? cs : B.ncell;
-    for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * stride + loc] = solid ? 0.0 : val[q];
+    for (int q = 0; q < NQ; ++q) pop[B.base * NQ + q * stride + loc * cst] = solid ? 0.0 : val[q];
 }
 
 // LBM::compute_eb_forces (LBM.cpp:994-1044) on one level: over valid cells
@@ -715,21 +716,18 @@ __global__ void k_init_fields(
 // host-side helpers
 // ---------------------------------------------------------------------------
 
-// Component stride of the population arrays: the compact layout interleaves
-// f and g components (stride 2N); the reference layout uses each box's fab
-// size (0 = per box).
+// Component stride of the population arrays: the compact layout keeps f_q
+// and g_q of a cell side by side (component stride 2N); the reference
+// layout uses each box's fab size (0 = per box).
 static int64_t comp_stride(const mrb_level *L)
 {
     return L->layout == MRB_LAYOUT_COMPACT ? 2 * L->total_cells : 0;
 }
 
-// Element stride between neighbouring cells: the reduced-precision storages
-// on the compact layout keep f_q and g_q of a cell side by side
-// (pop[(q*N + cell)*2 + lattice]) so that one load fetches both.
-static int cell_stride(const mrb_level *L)
-{
-    return (L->layout == MRB_LAYOUT_COMPACT && L->storage != MRB_STORAGE_F64) ? 2 : 1;
-}
+// Element stride between neighbouring cells: the compact layout keeps f_q
+// and g_q of a cell side by side (pop[(q*N + cell)*2 + lattice]) so that
+// one load fetches both.
+static int cell_stride(const mrb_level *L) { return L->layout == MRB_LAYOUT_COMPACT ? 2 : 1; }
 
 // Host <-> device copy of one lattice's 27 components (host: contiguous
 // components of n elements; device: component stride dst/src_stride).
@@ -1118,8 +1116,8 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
     MRB_HIP(hipMalloc(&L->d_box_base, sizeof(int64_t) * (L->nbox + 1)));
     MRB_HIP(hipMemcpy(L->d_box_base, bases.data(), sizeof(int64_t) * (L->nbox + 1), hipMemcpyHostToDevice));
     if (L->layout == MRB_LAYOUT_COMPACT) {
-        // f and g interleaved by component in one allocation per buffer:
-        // f_q at component 2q, g_q at 2q+1 (component stride 2N, g = f + N)
+        // f and g in one allocation per buffer, f_q and g_q of a cell side
+        // by side: element (q*N + cell)*2 + lattice (g = f + 1)
         for (int ab = 0; ab < 2; ++ab) {
             MRB_HIP(hipMalloc(&L->pop[0][ab], es * 2 * NQ * T));
             MRB_HIP(hipMemset(L->pop[0][ab], 0, es * 2 * NQ * T));
@@ -1410,14 +1408,21 @@ static int copy_lattice(const mrb_level *L, int arr, void *host, bool to_device)
         MRB_HIP(hipMalloc(&tmp, es * NQ * size_t(N)));
         hipError_t e = hipSuccess;
         const int64_t n = NQ * N;
+        const bool f64 = es == sizeof(double);
         if (to_device) {
             e = hipMemcpy(tmp, host, es * size_t(n), hipMemcpyHostToDevice);
             if (e == hipSuccess) {
-                k_strided_copy<float><<<nblocks(n), kBlock>>>((const float *)tmp, 1, (float *)dev, 2, n);
+                if (f64)
+                    k_strided_copy<double><<<nblocks(n), kBlock>>>((const double *)tmp, 1, (double *)dev, 2, n);
+                else
+                    k_strided_copy<float><<<nblocks(n), kBlock>>>((const float *)tmp, 1, (float *)dev, 2, n);
                 e = hipGetLastError();
             }
         } else {
-            k_strided_copy<float><<<nblocks(n), kBlock>>>((const float *)dev, 2, (float *)tmp, 1, n);
+            if (f64)
+                k_strided_copy<double><<<nblocks(n), kBlock>>>((const double *)dev, 2, (double *)tmp, 1, n);
+            else
+                k_strided_copy<float><<<nblocks(n), kBlock>>>((const float *)dev, 2, (float *)tmp, 1, n);
             e = hipGetLastError();
             if (e == hipSuccess) e = hipMemcpy(host, tmp, es * size_t(n), hipMemcpyDeviceToHost);
         }
@@ -1753,7 +1758,7 @@ int mrb_init_constant(
         MRB_HIP(hipMemcpy(dval, l == 0 ? fv : gv, sizeof(double) * NQ, hipMemcpyHostToDevice));
         k_init_uniform<<<nblocks(L->total_cells), kBlock>>>((double *)L->pop[l][L->cur], L->d_isf, L->d_boxes,
                                                             L->d_box_base, L->nbox, L->total_cells, dval,
-                                                            comp_stride(L));
+                                                            comp_stride(L), cell_stride(L));
         MRB_HIP(hipGetLastError());
     }
     MRB_HIP(hipDeviceSynchronize());

@@ -826,21 +826,31 @@ __global__ void __launch_bounds__(256) k_collide_f32_x256(
 }
 
 // ---------------------------------------------------------------------------
-// Single pass as a z-sweep (FP64 storage, rows of at most 256 cells): one
-// block per x-row y, all rows resident at once (checked on the host from
-// the kernel's occupancy).  The
-// block walks z; at step z it issues the loads of plane z+1, collides
-// plane z-1 from the pull held in LDS once the rows y -+ 1 have published
-// plane z-1 (the loads are in flight meanwhile), moves plane z from
-// registers into LDS, then forms plane z+1's Q_corr and publishes its Q_y
-// for the rows beside.  Each population is read
-// from HBM once and written once; the neighbours' Q_corr travel through
-// LDS (x), registers (z) and a ring of 8 planes of Q_y in global memory (y),
-// read and written past the non-coherent L2 (agent-scope accesses), with a
-// per-row count of published planes.  A row publishes plane k+1 only
-// after it waited for its neighbours' plane k-1, so a neighbour still reads
-// plane k-2 or later: the ring's 8 planes (4 would do) are never
-// overwritten while read.
+// Single pass as a z-sweep (FP64 and shifted FP32 storage, rows of at most
+// 256 cells): one block per x-row y, all rows resident at once (checked on
+// the host from the kernel's occupancy).  The block walks z; at step z it
+// issues the loads of plane z+1, collides plane z-1 from the pull held in
+// LDS once the rows y -+ 1 have published plane z-1's Q_y (the loads are in
+// flight meanwhile), moves plane z from registers into LDS, then forms
+// plane z+1's Q_corr and publishes its Q_y for the rows beside.  Each
+// population is read from HBM once and written once; the neighbours'
+// Q_corr travel through LDS (x), registers (z) and a ring of 8 planes of
+// Q_y in global memory (y), read and written past the non-coherent L2
+// (agent-scope accesses).
+//
+// A ring entry is two 64-bit words, each holding the plane's tag in its
+// high half and one half of Q_y's bits in its low half; each word is
+// written and read atomically, so a reader that sees the expected tag in
+// both words holds that plane's Q_y, and the writer needs no fence (its
+// population stores keep flowing while the next plane's loads issue).
+// Tags: epoch * (nz + 1) + plane + 1, where the epoch (a counter a one-thread
+// kernel advances before every launch) makes the tags of one launch differ
+// from every earlier launch's: a stale entry of the previous launch carries
+// a tag (nz + 1) + plane - plane' away from the expected one, never equal.
+// Every lane of a row waits for both rows beside, fluid or not, so a row
+// publishes plane k+1 only after it saw its neighbours' plane k-1, which
+// they published after they had read plane k-3 of this row: the ring's 8
+// planes (4 would do) are never overwritten while read.
 // The cell arithmetic is the two-pass step's (same functions, same order).
 // ---------------------------------------------------------------------------
 constexpr int kSweepRing = kSweepRingPlanes;
@@ -871,11 +881,37 @@ __device__ __forceinline__ void drain_wave()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// tagged ring entry: word 0 = tag : high 32 bits of v, word 1 = tag : low
+__device__ __forceinline__ void put_tagged(uint64_t *p, uint32_t tag, double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v), t = uint64_t(tag) << 32;
+    __hip_atomic_store(p, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, t | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+struct Tagged {
+    uint64_t w0 = 0, w1 = 0;
+    __device__ __forceinline__ void load(uint64_t *p)
+    {
+        w0 = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ bool has(uint32_t tag) const
+    {
+        return uint32_t(w0 >> 32) == tag && uint32_t(w1 >> 32) == tag;
+    }
+    __device__ __forceinline__ double value() const
+    {
+        return __builtin_bit_cast(double, (w0 << 32) | (w1 & 0xffffffffull));
+    }
+};
+
+__global__ void k_bump_u32(uint32_t *a) { a[0] += 1u; }
+
 template <class T, int BCX>
 __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
-    const uint32_t *__restrict__ mask, double *qring, uint32_t *flags, uint32_t *fault, CompactArgs A, BCDev bc,
-    ShiftTab sh, CollideParams P)
+    const uint32_t *__restrict__ mask, uint64_t *qring, const uint32_t *epoch, uint32_t *fault, CompactArgs A,
+    BCDev bc, ShiftTab sh, CollideParams P)
 {
     __shared__ double hold[2 * NQ * kSweepRow];
     __shared__ double qxrow[kSweepRow];
@@ -885,6 +921,10 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
     const bool in = x < A.nx;
     const int64_t nxy = int64_t(A.nx) * A.ny;
     const int64_t row = int64_t(y) * A.nx + x;
+    const uint32_t tag0 =
+        __hip_atomic_load(epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * uint32_t(A.nz + 1) + 1u;
+    // the lanes that wait for the row before (y-1) and the row after (y+1)
+    const bool wm = in && y > 0, wp = in && y + 1 < A.ny;
     // two planes in registers: the pull of plane p (rf, rg: its Q_corr is
     // known) and the pull of plane p+1 in flight (nf, ng) while plane p-1,
     // held in LDS, is collided
@@ -908,24 +948,9 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
         qy = M.Qy;
         qz = M.Qz;
     };
+    auto entry = [&](int zp, int64_t r) { return qring + 2 * (int64_t(zp % kSweepRing) * nxy + r); };
     auto publish = [&](int zp, double qy) {
-        if (in) store_agent(&qring[int64_t(zp % kSweepRing) * nxy + row], qy);
-        drain_wave();
-        __syncthreads();
-        if (x == 0) __hip_atomic_store(&flags[y], uint32_t(zp + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    };
-    // bounded wait: past the spin limit the block counts a fault and writes
-    // NaN from then on instead of waiting longer
-    auto wait_row = [&](int yy, uint32_t target) {
-        uint32_t it = 0;
-        while (__hip_atomic_load(&flags[yy], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++it > kSweepSpinLimit) {
-                atomicAdd(fault, 1u);
-                gave_up = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
+        if (in) put_tagged(entry(zp, row), tag0 + uint32_t(zp), qy);
     };
     // plane 0 into registers, its Q_corr published
     uint32_t m1 = 0u, m0 = 0u;
@@ -933,25 +958,45 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
     pull_into(0, m1, rf, rg);
     q_of(m1, rf, rg, qx1, qy1, qz1);
     publish(0, qy1);
+    __syncthreads();  // gave_up
     double qz0 = 0.0, qzm = 0.0;  // Q_z of planes p-1 and p-2
     for (int z = 0; z <= A.nz; ++z) {
-        // the counts of the rows beside, loaded before this step's pull so
-        // that their latency hides behind it
-        uint32_t cnt_m = 0xffffffffu, cnt_p = 0xffffffffu;
-        if (z >= 1 && x == 0) {
-            if (y > 0) cnt_m = __hip_atomic_load(&flags[y - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (y + 1 < A.ny) cnt_p = __hip_atomic_load(&flags[y + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the rows beside's Q_y of plane z-1, loaded before this step's
+        // pull so that their latency hides behind it
+        Tagged em, ep;
+        if (z >= 1) {
+            if (wm) em.load(entry(z - 1, row - A.nx));
+            if (wp) ep.load(entry(z - 1, row + A.nx));
         }
         // plane z+1: loads issued now, used after the collide
         uint32_t m2 = 0u;
         if (z + 1 < A.nz) pull_into(z + 1, m2, nf, ng);
         if (z >= 1) {
             // collide plane z-1 (held in LDS) once the rows beside have
-            // published it
+            // published it; bounded wait: past the spin limit the block
+            // counts a fault and writes NaN from then on
             const int zc = z - 1;
-            if (x == 0 && !gave_up) {
-                if (cnt_m < uint32_t(zc + 1)) wait_row(y - 1, uint32_t(zc + 1));
-                if (cnt_p < uint32_t(zc + 1)) wait_row(y + 1, uint32_t(zc + 1));
+            const uint32_t tag = tag0 + uint32_t(zc);
+            bool need_m = wm && !em.has(tag), need_p = wp && !ep.has(tag);
+            if (need_m || need_p) {
+                uint32_t it = 0;
+                while (!gave_up) {
+                    if (++it > kSweepSpinLimit) {
+                        atomicAdd(fault, 1u);
+                        gave_up = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                    if (need_m) {
+                        em.load(entry(zc, row - A.nx));
+                        need_m = !em.has(tag);
+                    }
+                    if (need_p) {
+                        ep.load(entry(zc, row + A.nx));
+                        need_p = !ep.has(tag);
+                    }
+                    if (!need_m && !need_p) break;
+                }
             }
             __syncthreads();
             const bool poisoned = gave_up != 0;
@@ -968,10 +1013,8 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
                     const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
                     const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
                     const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
-                    // the rows beside's Q_y, loaded before the moments
-                    double *ring = qring + int64_t(zc % kSweepRing) * nxy + row;
-                    const double qyp = yp ? load_agent(ring + A.nx) : 0.0;
-                    const double qym = ym ? load_agent(ring - A.nx) : 0.0;
+                    const double qyp = yp ? ep.value() : 0.0;
+                    const double qym = ym ? em.value() : 0.0;
                     const LdsRow F{&hold[x]}, G{&hold[kSweepRow + x]};
                     MomentsR<double> mo;
                     accumulate_moments(F, G, mo);
@@ -1016,7 +1059,6 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
             m1 = m2;
         } else {
             qz1 = 0.0;
-            __syncthreads();
         }
     }
 }
@@ -1576,13 +1618,13 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
     }
     if (variant == kSweepVariant) {
         if constexpr (!kIsHalf<T>) {
-            if (!L->d_sweep_q || A.nx > kSweepRow)
+            if (!L->d_sweep_qt || A.nx > kSweepRow)
                 return set_error(MRB_E_UNSUPPORTED, "k_sweep: rows of at most 256 cells only");
-            k_zero_u32<<<(A.ny + 255) / 256, 256, 0, s>>>(L->d_sweep_flags, A.ny);
+            k_bump_u32<<<1, 1, 0, s>>>(L->d_sweep_epoch);
             MRB_HIP(hipGetLastError());
             if (ev) MRB_HIP(hipEventRecord(ev[0], s));
-            double *qr = L->d_sweep_q;
-            uint32_t *fl = L->d_sweep_flags, *fa = L->d_sweep_fault;
+            uint64_t *qr = L->d_sweep_qt;
+            uint32_t *fl = L->d_sweep_epoch, *fa = L->d_sweep_fault;
             const uint32_t *mk = L->d_mask;
             CompactArgs Ac = A;
             BCDev bcc = L->bcdev;

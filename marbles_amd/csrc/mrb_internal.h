@@ -281,13 +281,17 @@ struct mrb_level {
     // FP32-arithmetic variant set aside while the level has a PRESSURE face
     // (those collides have none); restored when the faces change back
     int hot_h32 = -1;
-    // single-pass z-sweep (compact FP64 levels with rows of at most 256
-    // cells): ring of kSweepRingPlanes planes of Q_y, per-row count of
-    // published planes, count of waits that gave up
+    // single-pass z-sweep on 64-cell blocks (k_sweep_x64, compact FP64
+    // levels): ring of kSweepRingPlanes planes of Q_y, per-block count of
+    // published planes; count of waits that gave up (both z-sweeps)
     double *d_sweep_q = nullptr;
     double *d_sweep_qx = nullptr;  // Q_x of the end cells of 64-cell blocks (k_sweep_x64)
     uint32_t *d_sweep_flags = nullptr;
     uint32_t *d_sweep_fault = nullptr;
+    // k_sweep's ring of tagged Q_y (two 64-bit words per cell and plane) and
+    // the launch count its tags start from
+    uint64_t *d_sweep_qt = nullptr;
+    uint32_t *d_sweep_epoch = nullptr;
     // two-level link, held by the fine level (mrb_amr_link)
     mrb_level *amr_crse = nullptr;
     void *d_coal = nullptr;  // mrb::CoalesceEntry[n_coal]

@@ -1219,6 +1219,10 @@ int mrb_level_create(const mrb_level_desc *desc, mrb_level **out)
             MRB_HIP(hipMemset(L->d_sweep_flags, 0, sizeof(uint32_t) * nblk));
             MRB_HIP(hipMalloc(&L->d_sweep_fault, sizeof(uint32_t)));
             MRB_HIP(hipMemset(L->d_sweep_fault, 0, sizeof(uint32_t)));
+            MRB_HIP(hipMalloc(&L->d_sweep_qt, sizeof(uint64_t) * 2 * kSweepRingPlanes * plane));
+            MRB_HIP(hipMemset(L->d_sweep_qt, 0, sizeof(uint64_t) * 2 * kSweepRingPlanes * plane));
+            MRB_HIP(hipMalloc(&L->d_sweep_epoch, sizeof(uint32_t)));
+            MRB_HIP(hipMemset(L->d_sweep_epoch, 0, sizeof(uint32_t)));
         }
     }
     // defaults (LBM.H:272-318)
@@ -1277,6 +1281,8 @@ int mrb_level_destroy(mrb_level *L)
     (void)hipFree(L->d_sweep_qx);
     (void)hipFree(L->d_sweep_flags);
     (void)hipFree(L->d_sweep_fault);
+    (void)hipFree(L->d_sweep_qt);
+    (void)hipFree(L->d_sweep_epoch);
     (void)hipFree(L->d_tiles_c);
     amr_release(L);
     delete L;

@@ -298,21 +298,27 @@ __device__ __forceinline__ void pressure_ghosts(
 // values from the face tables; kBcxPressure: PRESSURE faces)
 // DEV (shifted FP32 storage only): return the stored deviations f - shift
 // themselves (boundary constants less the shift) instead of populations.
-template <int BCX, class T, class W, bool DEV = false>
-__device__ __forceinline__ void pull_cell(
-    const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
-    const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, W fv[NQ],
-    W gv[NQ])
+// The pull in two phases: pull_raw issues the loads of the stored values
+// (rf, rg) and returns which sources take a boundary constant (cmask), are
+// read directly (dmask), were moved inward across a z OUTFLOW face (zmove)
+// or lie beyond a PRESSURE face (Pm); pull_finish turns them into
+// populations.  A caller with other work to do (the z-sweep) places that
+// work between the two, so that the loads stay in flight meanwhile.
+struct PullBits {
+    uint32_t cmask, dmask, zmove, Pm;
+};
+
+template <int BCX, class T, bool GENERAL_ONLY = false>
+__device__ __forceinline__ PullBits pull_raw(
+    const T *__restrict__ f, const CompactArgs &A, int x, int y, int z, int64_t cell, uint32_t mask, T rf[NQ],
+    T rg[NQ])
 {
-    static_assert(!DEV || (std::is_same<T, float>::value && (BCX & kBcxPressure) == 0),
-                  "deviation pull: shifted FP32 storage without PRESSURE faces");
     int tx[3], ty[3], tz[3];
     bool gx[3], gy[3], gz[3];
     targets3(x, A.nx, A.per[0], tx, gx);
     targets3(y, A.ny, A.per[1], ty, gy);
     targets3(z, A.nz, A.per[2], tz, gz);
     const uint32_t c32 = uint32_t(cell);
-    T rf[NQ], rg[NQ];
     uint32_t cmask = 0, dmask = 0, zmove = 0;
     // The boundary handling of the 27 sources as bit sets over q, per cell.
     // BCFill applies the faces a ghost lies beyond in the order x, y, z, the
@@ -340,7 +346,7 @@ __device__ __forceinline__ void pull_cell(
     });
     const uint32_t O = G & ~Tm;
     const uint32_t nondirect = (G & ~(V | O)) | (~G & ~mask & kAllQ);
-    if (A.fast_ok && __all(nondirect == 0u)) {
+    if (!GENERAL_ONLY && A.fast_ok && __all(nondirect == 0u)) {
         // every source of every cell of the wave is read directly: one
         // wave-uniform base per direction and a 32-bit byte offset per lane
         constexpr uint32_t bytes = kCellStride<T> * sizeof(T);
@@ -418,7 +424,25 @@ __device__ __forceinline__ void pull_cell(
         cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
     });
     }
+    return PullBits{cmask, dmask, zmove, Pm};
+}
+
+template <int BCX, class T, class W, bool DEV = false>
+__device__ __forceinline__ void pull_finish(
+    const T *__restrict__ f, const CompactArgs &A, const BCDev &bc, const ShiftTab &sh, const Half16 &h16,
+    int x, int y, int z, uint32_t mask, const PullBits &pb, const T rf[NQ], const T rg[NQ], W fv[NQ], W gv[NQ])
+{
+    static_assert(!DEV || (std::is_same<T, float>::value && (BCX & kBcxPressure) == 0),
+                  "deviation pull: shifted FP32 storage without PRESSURE faces");
+    int tx[3], ty[3], tz[3];
+    bool gx[3], gy[3], gz[3];
+    targets3(x, A.nx, A.per[0], tx, gx);
+    targets3(y, A.ny, A.per[1], ty, gy);
+    targets3(z, A.nz, A.per[2], tz, gz);
+    const uint32_t cmask = pb.cmask, dmask = pb.dmask, zmove = pb.zmove, Pm = pb.Pm;
     (void)zmove;
+    (void)tx;
+    (void)ty;
     if constexpr (kIsHalf<T>) {
         // inverse scales of the source planes z - c_z (index c_z + 1); a
         // source that is not the pull's (bounce-back, wall ghosts) is the
@@ -470,6 +494,18 @@ __device__ __forceinline__ void pull_cell(
     } else {
         (void)Pm;
     }
+}
+
+template <int BCX, class T, class W, bool DEV = false>
+__device__ __forceinline__ void pull_cell(
+    const T *__restrict__ f, const T *__restrict__ g, const CompactArgs &A, const BCDev &bc,
+    const ShiftTab &sh, const Half16 &h16, int x, int y, int z, int64_t cell, uint32_t mask, W fv[NQ],
+    W gv[NQ])
+{
+    (void)g;  // g == f + 1 (f_q and g_q of a cell side by side)
+    T rf[NQ], rg[NQ];
+    const PullBits pb = pull_raw<BCX>(f, A, x, y, z, cell, mask, rf, rg);
+    pull_finish<BCX, T, W, DEV>(f, A, bc, sh, h16, x, y, z, mask, pb, rf, rg, fv, gv);
 }
 
 __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y, int &z, int64_t &cell)
@@ -916,128 +952,59 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
     __shared__ double hold[2 * NQ * kSweepRow];
     __shared__ double qxrow[kSweepRow];
     __shared__ int gave_up;  // a wait for a neighbour row hit the spin limit
-    const int x = int(threadIdx.x), y = int(blockIdx.x);
-    if (x == 0) gave_up = 0;
-    const bool in = x < A.nx;
+    if (threadIdx.x == 0) gave_up = 0;
+    // No branch of the sweep depends on the lane, so that the compiler
+    // counts the memory operations in flight exactly and the collide runs
+    // while the next plane's loads are outstanding: lanes past the row's
+    // end repeat the last cell's work (the same values to the same
+    // addresses), and solid cells pull and collide like fluid ones, their
+    // results replaced by the solid marker when stored.
+    const int x = min(int(threadIdx.x), A.nx - 1), y = int(blockIdx.x);
     const int64_t nxy = int64_t(A.nx) * A.ny;
     const int64_t row = int64_t(y) * A.nx + x;
+    // the rows beside (this row itself at the y faces, read but not used)
+    const bool wm = y > 0, wp = y + 1 < A.ny;
+    const int64_t rowm = wm ? row - A.nx : row, rowp = wp ? row + A.nx : row;
     const uint32_t tag0 =
         __hip_atomic_load(epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * uint32_t(A.nz + 1) + 1u;
-    // the lanes that wait for the row before (y-1) and the row after (y+1)
-    const bool wm = in && y > 0, wp = in && y + 1 < A.ny;
     // two planes in registers: the pull of plane p (rf, rg: its Q_corr is
-    // known) and the pull of plane p+1 in flight (nf, ng) while plane p-1,
-    // held in LDS, is collided
-    double rf[NQ], rg[NQ], nf[NQ], ng[NQ];
-    auto pull_into = [&](int zp, uint32_t &m, double (&pf)[NQ], double (&pg)[NQ]) {
-        m = 0u;
-        if (!in) return;
-        const int64_t cell = int64_t(zp) * nxy + row;
-        m = mask[cell];
-        if (!(m & 1u)) return;
-        pull_cell<BCX>(f, g, A, bc, sh, Half16{}, x, y, zp, cell, m, pf, pg);
+    // known) and the stored values of plane p+1 in flight (nf, ng) while
+    // plane p-1, held in LDS, is collided; they become populations after
+    // the collide (pull_finish), so the collide does not wait for them
+    double rf[NQ], rg[NQ];
+    T nf[NQ], ng[NQ];
+    PullBits pb{0u, 0u, 0u, 0u};
+    // planes past the last one are replaced by the last (their loads and
+    // results go unused), and a plane's stream mask is loaded one plane
+    // ahead of its pull, so that the pull's loads do not wait for it
+    auto mask_of = [&](int zp) { return mask[int64_t(min(zp, A.nz - 1)) * nxy + row]; };
+    // (a solid cell pulls as if all its sources were fluid, so that it does
+    // not keep its wave off the direct-load path)
+    auto pull_issue = [&](int zp, uint32_t m) {
+        zp = min(zp, A.nz - 1);
+        pb = pull_raw<BCX, T, true>(f, A, x, y, zp, int64_t(zp) * nxy + row, (m & 1u) ? m : (m | kAllQ), nf, ng);
     };
-    auto q_of = [&](uint32_t m, const double (&pf)[NQ], const double (&pg)[NQ], double &qx, double &qy,
-                    double &qz) {
-        qx = qy = qz = 0.0;
-        if (!in || !(m & 1u)) return;
+    auto pull_done = [&](int zp, uint32_t m) {
+        zp = min(zp, A.nz - 1);
+        pull_finish<BCX, T, double>(f, A, bc, sh, Half16{}, x, y, zp, m, pb, nf, ng, rf, rg);
+    };
+    auto q_of = [&](uint32_t m, double &qx, double &qy, double &qz) {
         double rho, jx, jy, jz, E;
-        accumulate_q_moments(pf, pg, rho, jx, jy, jz, E);
+        accumulate_q_moments(rf, rg, rho, jx, jy, jz, E);
         const Macro M = macro_from(rho, jx, jy, jz, E, P);
-        qx = M.Qx;
-        qy = M.Qy;
-        qz = M.Qz;
+        const bool fluid = (m & 1u) != 0;
+        qx = fluid ? M.Qx : 0.0;
+        qy = fluid ? M.Qy : 0.0;
+        qz = fluid ? M.Qz : 0.0;
     };
     auto entry = [&](int zp, int64_t r) { return qring + 2 * (int64_t(zp % kSweepRing) * nxy + r); };
-    auto publish = [&](int zp, double qy) {
-        if (in) put_tagged(entry(zp, row), tag0 + uint32_t(zp), qy);
-    };
-    // plane 0 into registers, its Q_corr published
-    uint32_t m1 = 0u, m0 = 0u;
-    double qx1, qy1, qz1;
-    pull_into(0, m1, rf, rg);
-    q_of(m1, rf, rg, qx1, qy1, qz1);
-    publish(0, qy1);
-    __syncthreads();  // gave_up
+    auto publish = [&](int zp, double qy) { put_tagged(entry(zp, row), tag0 + uint32_t(zp), qy); };
+    uint32_t m0 = 0u, m1 = mask_of(0), mnext = mask_of(1);
+    double qx1, qy1, qz1;      // Q_corr of plane p (pulled, in rf/rg)
     double qz0 = 0.0, qzm = 0.0;  // Q_z of planes p-1 and p-2
-    for (int z = 0; z <= A.nz; ++z) {
-        // the rows beside's Q_y of plane z-1, loaded before this step's
-        // pull so that their latency hides behind it
-        Tagged em, ep;
-        if (z >= 1) {
-            if (wm) em.load(entry(z - 1, row - A.nx));
-            if (wp) ep.load(entry(z - 1, row + A.nx));
-        }
-        // plane z+1: loads issued now, used after the collide
-        uint32_t m2 = 0u;
-        if (z + 1 < A.nz) pull_into(z + 1, m2, nf, ng);
-        if (z >= 1) {
-            // collide plane z-1 (held in LDS) once the rows beside have
-            // published it; bounded wait: past the spin limit the block
-            // counts a fault and writes NaN from then on
-            const int zc = z - 1;
-            const uint32_t tag = tag0 + uint32_t(zc);
-            bool need_m = wm && !em.has(tag), need_p = wp && !ep.has(tag);
-            if (need_m || need_p) {
-                uint32_t it = 0;
-                while (!gave_up) {
-                    if (++it > kSweepSpinLimit) {
-                        atomicAdd(fault, 1u);
-                        gave_up = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                    if (need_m) {
-                        em.load(entry(zc, row - A.nx));
-                        need_m = !em.has(tag);
-                    }
-                    if (need_p) {
-                        ep.load(entry(zc, row + A.nx));
-                        need_p = !ep.has(tag);
-                    }
-                    if (!need_m && !need_p) break;
-                }
-            }
-            __syncthreads();
-            const bool poisoned = gave_up != 0;
-            if (in) {
-                const int64_t cell = int64_t(zc) * nxy + row;
-                const uint32_t m = m0;
-                if (!(m & 1u)) {
-                    static_for<NQ>([&](auto qc) {
-                        constexpr int q = decltype(qc)::value;
-                        store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(-1.0, sh.f(q)),
-                                   narrow<T>(-1.0, sh.g(q)));
-                    });
-                } else {
-                    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
-                    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
-                    const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
-                    const double qyp = yp ? ep.value() : 0.0;
-                    const double qym = ym ? em.value() : 0.0;
-                    const LdsRow F{&hold[x]}, G{&hold[kSweepRow + x]};
-                    MomentsR<double> mo;
-                    accumulate_moments(F, G, mo);
-                    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
-                    const double dqx = one_sided_gradient(xp, xm, xp ? qxrow[x + 1] : 0.0, M.Qx,
-                                                          xm ? qxrow[x - 1] : 0.0, P.inv_dx[0]);
-                    const double dqy = one_sided_gradient(yp, ym, qyp, M.Qy, qym, P.inv_dx[1]);
-                    const double dqz = one_sided_gradient(zp, zm, zp ? qz1 : 0.0, M.Qz, zm ? qzm : 0.0,
-                                                          P.inv_dx[2]);
-                    const double nan = __builtin_nan("");
-                    collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P,
-                                           [&](auto qc, double fnew, double gnew) {
-                                               constexpr int q = decltype(qc)::value;
-                                               store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>,
-                                                          narrow<T>(poisoned ? nan : fnew, sh.f(q)),
-                                                          narrow<T>(poisoned ? nan : gnew, sh.g(q)));
-                                           });
-                }
-            }
-            __syncthreads();
-        }
-        if (z == A.nz) break;
-        // plane z becomes the held one
+    // plane p becomes the held one, plane p+1 (in flight) is finished,
+    // its Q_corr formed and published
+    auto advance = [&](int p, uint32_t m2) {
         static_for<NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             hold[(2 * q) * kSweepRow + x] = rf[q];
@@ -1047,20 +1014,90 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
         qzm = qz0;
         qz0 = qz1;
         m0 = m1;
-        // plane z+1: its Q_corr, published
-        if (z + 1 < A.nz) {
-            q_of(m2, nf, ng, qx1, qy1, qz1);
-            publish(z + 1, qy1);
-            static_for<NQ>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                rf[q] = nf[q];
-                rg[q] = ng[q];
-            });
-            m1 = m2;
-        } else {
-            qz1 = 0.0;
+        pull_done(p + 1, m2);
+        q_of(m2, qx1, qy1, qz1);
+        publish(p + 1, qy1);
+        m1 = m2;
+    };
+    // collide plane zc (held in LDS) once the rows beside have published
+    // its Q_y (em, ep: their entries, loaded before); bounded wait: past the
+    // spin limit the block counts a fault and writes NaN from then on
+    auto collide = [&](int zc, Tagged &em, Tagged &ep) {
+        const uint32_t tag = tag0 + uint32_t(zc);
+        bool need_m = wm && !em.has(tag), need_p = wp && !ep.has(tag);
+        if (need_m || need_p) {
+            uint32_t it = 0;
+            while (!gave_up) {
+                if (++it > kSweepSpinLimit) {
+                    atomicAdd(fault, 1u);
+                    gave_up = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                if (need_m) {
+                    em.load(entry(zc, rowm));
+                    need_m = !em.has(tag);
+                }
+                if (need_p) {
+                    ep.load(entry(zc, rowp));
+                    need_p = !ep.has(tag);
+                }
+                if (!need_m && !need_p) break;
+            }
         }
+        __syncthreads();
+        const bool poisoned = gave_up != 0;
+        const int64_t cell = int64_t(zc) * nxy + row;
+        const uint32_t m = m0;
+        const bool fluid = (m & 1u) != 0;
+        const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+        const bool yp = wp && ((m >> 4) & 1u), ym = wm && ((m >> 3) & 1u);
+        const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
+        const double qyp = yp ? ep.value() : 0.0;
+        const double qym = ym ? em.value() : 0.0;
+        const LdsRow F{&hold[x]}, G{&hold[kSweepRow + x]};
+        MomentsR<double> mo;
+        accumulate_moments(F, G, mo);
+        const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+        const double dqx =
+            one_sided_gradient(xp, xm, xp ? qxrow[x + 1] : 0.0, M.Qx, xm ? qxrow[x - 1] : 0.0, P.inv_dx[0]);
+        const double dqy = one_sided_gradient(yp, ym, qyp, M.Qy, qym, P.inv_dx[1]);
+        const double dqz = one_sided_gradient(zp, zm, zp ? qz1 : 0.0, M.Qz, zm ? qzm : 0.0, P.inv_dx[2]);
+        const double nan = __builtin_nan("");
+        collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
+            constexpr int q = decltype(qc)::value;
+            const double vf = fluid ? (poisoned ? nan : fnew) : -1.0;
+            const double vg = fluid ? (poisoned ? nan : gnew) : -1.0;
+            store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(vf, sh.f(q)),
+                       narrow<T>(vg, sh.g(q)));
+        });
+        __syncthreads();
+    };
+    // plane 0 pulled, its Q_corr published; plane 1 in flight
+    pull_issue(0, m1);
+    pull_done(0, m1);
+    q_of(m1, qx1, qy1, qz1);
+    publish(0, qy1);
+    uint32_t m2 = mnext;
+    mnext = mask_of(2);
+    pull_issue(1, m2);
+    __syncthreads();  // gave_up
+    advance(0, m2);
+    // step z: plane z+1's loads issued, plane z-1 collided, plane z held
+    for (int z = 1; z < A.nz; ++z) {
+        Tagged em, ep;
+        em.load(entry(z - 1, rowm));
+        ep.load(entry(z - 1, rowp));
+        m2 = mnext;
+        mnext = mask_of(z + 2);
+        pull_issue(z + 1, m2);
+        collide(z - 1, em, ep);
+        advance(z, m2);
     }
+    Tagged em, ep;
+    em.load(entry(A.nz - 1, rowm));
+    ep.load(entry(A.nz - 1, rowp));
+    collide(A.nz - 1, em, ep);
 }
 
 // The z-sweep on single-wave blocks of 64 cells of an x-row: four blocks,

@@ -308,7 +308,7 @@ struct PullBits {
     uint32_t cmask, dmask, zmove, Pm;
 };
 
-template <int BCX, class T, bool GENERAL_ONLY = false>
+template <int BCX, class T, bool UNIFORM_LOADS = false>
 __device__ __forceinline__ PullBits pull_raw(
     const T *__restrict__ f, const CompactArgs &A, int x, int y, int z, int64_t cell, uint32_t mask, T rf[NQ],
     T rg[NQ])
@@ -346,7 +346,79 @@ __device__ __forceinline__ PullBits pull_raw(
     });
     const uint32_t O = G & ~Tm;
     const uint32_t nondirect = (G & ~(V | O)) | (~G & ~mask & kAllQ);
-    if (!GENERAL_ONLY && A.fast_ok && __all(nondirect == 0u)) {
+    // one (f_q, g_q) pair from element e of the f buffer (g == f + 1)
+    auto load_pair = [&](int q, int64_t e) {
+        if constexpr (kIsHalf<T>) {
+            const uint32_t v = *reinterpret_cast<const uint32_t *>(f + e);
+            rf[q] = __builtin_bit_cast(_Float16, uint16_t(v & 0xffffu));
+            rg[q] = __builtin_bit_cast(_Float16, uint16_t(v >> 16));
+        } else if constexpr (std::is_same<T, double>::value) {
+            const double2 v = *reinterpret_cast<const double2 *>(f + e);  // one 16-byte load
+            rf[q] = v.x;
+            rg[q] = v.y;
+        } else {
+            const float2 v = *reinterpret_cast<const float2 *>(f + e);  // one 8-byte load
+            rf[q] = v.x;
+            rg[q] = v.y;
+        }
+    };
+    // the element of direction q's source on the general path: the pull's
+    // source, its bounce-back, a reflection or a cell moved inward
+    auto general_e = [&](auto qc) -> int64_t {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+        constexpr int qb = kBounce[q];
+        const bool hx = gx[cx + 1], hy = gy[cy + 1], hz = gz[cz + 1];
+        const bool ghost = hx || hy || hz;
+        const uint32_t src = (uint32_t(tz[cz + 1]) * uint32_t(A.ny) + uint32_t(ty[cy + 1])) * uint32_t(A.nx) +
+                             uint32_t(tx[cx + 1]);
+        // the face that fixes a ghost source's value (scan z, y, x past
+        // OUTFLOW faces, as above) and the cell its wall fill reads: the cell
+        // itself moved one step inward across each OUTFLOW face passed
+        const uint32_t tfz = (A.bcpack >> (4 * (cz > 0 ? 2 : 5))) & 15u;
+        const uint32_t tfy = (A.bcpack >> (4 * (cy > 0 ? 1 : 4))) & 15u;
+        const uint32_t tfx = (A.bcpack >> (4 * (cx > 0 ? 0 : 3))) & 15u;
+        const bool pz = hz && tfz == MRB_BC_OUTFLOW;
+        const bool py = hy && tfy == MRB_BC_OUTFLOW && (pz || !hz);
+        const bool px = hx && tfx == MRB_BC_OUTFLOW && (py || !hy) && (pz || !hz);
+        const uint32_t t = (hz && !pz) ? tfz : (hy && !py) ? tfy : (hx && !px) ? tfx : uint32_t(MRB_BC_OUTFLOW);
+        const int32_t roff = (pz ? cz * A.nx * A.ny : 0) + (py ? cy * A.nx : 0) + (px ? cx : 0);
+        zmove |= uint32_t(pz) << q;
+        const bool direct = ghost ? t == MRB_BC_OUTFLOW : ((mask >> q) & 1u) != 0;
+        int comp = direct ? q : qb;
+        comp = (ghost && t == MRB_BC_SLIPX) ? reflect(q, 0) : comp;
+        comp = (ghost && t == MRB_BC_SLIPY) ? reflect(q, 1) : comp;
+        comp = (ghost && t == MRB_BC_SLIPZ) ? reflect(q, 2) : comp;
+        dmask |= uint32_t(direct) << q;
+        cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
+        return int64_t(comp) * A.cs + int64_t(direct ? src : uint32_t(int32_t(c32) + roff)) * kCellStride<T>;
+    };
+    const bool all_direct = __all(nondirect == 0u);
+    if constexpr (UNIFORM_LOADS) {
+        // the 27 source elements on the direct path or the general one (no
+        // memory access on either), then the loads in one fixed sequence:
+        // the number of loads in flight is then the same on every path
+        int64_t e[NQ];
+        if (all_direct) {
+            const uint32_t nxy = uint32_t(A.nx) * uint32_t(A.ny);
+            uint32_t xi[3], yi[3], zi[3];
+            for (int k = 0; k < 3; ++k) {
+                xi[k] = uint32_t(tx[k]);
+                yi[k] = uint32_t(ty[k]) * uint32_t(A.nx);
+                zi[k] = uint32_t(tz[k]) * nxy;
+            }
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
+                e[q] = int64_t(q) * A.cs + int64_t(zi[cz + 1] + yi[cy + 1] + xi[cx + 1]) * kCellStride<T>;
+            });
+            cmask = V;
+            dmask = kAllQ;
+        } else {
+            static_for<NQ>([&](auto qc) { e[decltype(qc)::value] = general_e(qc); });
+        }
+        static_for<NQ>([&](auto qc) { load_pair(decltype(qc)::value, e[decltype(qc)::value]); });
+    } else if (A.fast_ok && all_direct) {
         // every source of every cell of the wave is read directly: one
         // wave-uniform base per direction and a 32-bit byte offset per lane
         constexpr uint32_t bytes = kCellStride<T> * sizeof(T);
@@ -379,50 +451,7 @@ __device__ __forceinline__ PullBits pull_raw(
         cmask = V;
         dmask = kAllQ;
     } else {
-    static_for<NQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = ev(q, 0), cy = ev(q, 1), cz = ev(q, 2);
-        constexpr int qb = kBounce[q];
-        const bool hx = gx[cx + 1], hy = gy[cy + 1], hz = gz[cz + 1];
-        const bool ghost = hx || hy || hz;
-        const uint32_t src = (uint32_t(tz[cz + 1]) * uint32_t(A.ny) + uint32_t(ty[cy + 1])) * uint32_t(A.nx) +
-                             uint32_t(tx[cx + 1]);
-        // the face that fixes a ghost source's value (scan z, y, x past
-        // OUTFLOW faces, as above) and the cell its wall fill reads: the cell
-        // itself moved one step inward across each OUTFLOW face passed
-        const uint32_t tfz = (A.bcpack >> (4 * (cz > 0 ? 2 : 5))) & 15u;
-        const uint32_t tfy = (A.bcpack >> (4 * (cy > 0 ? 1 : 4))) & 15u;
-        const uint32_t tfx = (A.bcpack >> (4 * (cx > 0 ? 0 : 3))) & 15u;
-        const bool pz = hz && tfz == MRB_BC_OUTFLOW;
-        const bool py = hy && tfy == MRB_BC_OUTFLOW && (pz || !hz);
-        const bool px = hx && tfx == MRB_BC_OUTFLOW && (py || !hy) && (pz || !hz);
-        const uint32_t t = (hz && !pz) ? tfz : (hy && !py) ? tfy : (hx && !px) ? tfx : uint32_t(MRB_BC_OUTFLOW);
-        const int32_t roff = (pz ? cz * A.nx * A.ny : 0) + (py ? cy * A.nx : 0) + (px ? cx : 0);
-        zmove |= uint32_t(pz) << q;
-        const bool direct = ghost ? t == MRB_BC_OUTFLOW : ((mask >> q) & 1u) != 0;
-        int comp = direct ? q : qb;
-        comp = (ghost && t == MRB_BC_SLIPX) ? reflect(q, 0) : comp;
-        comp = (ghost && t == MRB_BC_SLIPY) ? reflect(q, 1) : comp;
-        comp = (ghost && t == MRB_BC_SLIPZ) ? reflect(q, 2) : comp;
-        const int64_t e = int64_t(comp) * A.cs + int64_t(direct ? src : uint32_t(int32_t(c32) + roff)) * kCellStride<T>;
-        dmask |= uint32_t(direct) << q;
-        if constexpr (kIsHalf<T>) {
-            const uint32_t v = *reinterpret_cast<const uint32_t *>(f + e);
-            rf[q] = __builtin_bit_cast(_Float16, uint16_t(v & 0xffffu));
-            rg[q] = __builtin_bit_cast(_Float16, uint16_t(v >> 16));
-        } else if constexpr (std::is_same<T, double>::value) {
-            // f_q and g_q side by side: one 16-byte load (g == f + 1)
-            const double2 v = *reinterpret_cast<const double2 *>(f + e);
-            rf[q] = v.x;
-            rg[q] = v.y;
-        } else {
-            // one 8-byte load
-            const float2 v = *reinterpret_cast<const float2 *>(f + e);
-            rf[q] = v.x;
-            rg[q] = v.y;
-        }
-        cmask |= uint32_t(ghost && t == MRB_BC_VELOCITY) << q;
-    });
+        static_for<NQ>([&](auto qc) { load_pair(decltype(qc)::value, general_e(qc)); });
     }
     return PullBits{cmask, dmask, zmove, Pm};
 }
@@ -713,31 +742,22 @@ __device__ __forceinline__ void collide_body(
 //    they carry the pulled momentum too (without it the rounding, alike in
 //    neighbouring cells, drifts the velocity: 2e-6 after 100 steps against
 //    4e-7 with it).
-template <int BCX>
-__device__ __forceinline__ void collide_f32_body(
-    const float *__restrict__ f, const float *__restrict__ g, float *__restrict__ fo, float *__restrict__ go,
-    const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
-    const ShiftTab &sh, const DevRest &D, const CollideParams &P)
+// the FP64 moments of the pulled differences of a cell (fv, gv: arrays or
+// LDS rows of FP32 values) plus the shift's; their mass, energy and
+// momentum sums kept apart for the residuals of the FP32 equilibria
+struct F32Sums {
+    double rho_d, e_d, jd[3];
+};
+template <class SF, class SG>
+__device__ __forceinline__ void f32_moments(const SF &fv, const SG &gv, const DevRest &D, MomentsR<double> &mo,
+                                            F32Sums &ds)
 {
-    int x, y, z;
-    int64_t cell;
-    if (!cell_coords(A, x, y, z, cell)) return;
-    const uint32_t m = mask[cell];
-    if (!(m & 1u)) {
-        // solid cell: the stream's -1 sentinel (LBM.cpp:565)
-        static_for<NQ>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            store_pair(fo, go, int64_t(q) * A.cs + cell * 2, narrow<float>(-1.0, sh.f(q)),
-                       narrow<float>(-1.0, sh.g(q)));
-        });
-        return;
-    }
-    float fv[NQ], gv[NQ];
-    pull_cell<BCX, float, float, true>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
-    MomentsR<double> mo;
     accumulate_moments(fv, gv, mo);
-    const double rho_d = mo.rho, e_d = mo.E;  // sums of the differences
-    const double jd[3] = {mo.jx, mo.jy, mo.jz};
+    ds.rho_d = mo.rho;  // sums of the differences
+    ds.e_d = mo.E;
+    ds.jd[0] = mo.jx;
+    ds.jd[1] = mo.jy;
+    ds.jd[2] = mo.jz;
     mo.rho += D.mo[0];
     mo.jx += D.mo[1];
     mo.jy += D.mo[2];
@@ -752,16 +772,17 @@ __device__ __forceinline__ void collide_f32_body(
     mo.qx += D.mo[11];
     mo.qy += D.mo[12];
     mo.qz += D.mo[13];
-    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
-    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
-    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
-    const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
-    const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
-    const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0, P.inv_dx[0]);
-    const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
-                                          ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
-    const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
-                                          zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
+}
+
+// the equilibria in FP32 from the FP64 coefficients and the relaxation of
+// the differences; put(q, f', g') receives the post-collision differences
+template <class SF, class SG, class Put>
+__device__ __forceinline__ void f32_relax(const SF &fv, const SG &gv, const MomentsR<double> &mo, const Macro &M,
+                                          const F32Sums &ds, double dqx, double dqy, double dqz, const DevRest &D,
+                                          const CollideParams &P, Put &&put)
+{
+    const double rho_d = ds.rho_d, e_d = ds.e_d;
+    const double *jd = ds.jd;
     const CollideCoeffs C = collide_coeffs(mo, M, dqx, dqy, dqz, P);
     // factor and coefficient differences from the rest state (FP64, then FP32)
     float dA[3], Bf[3], dB[3], Cf[3], dC[3], A0f[3], B0f[3];
@@ -784,10 +805,6 @@ __device__ __forceinline__ void collide_f32_body(
     }
     const float dxy = float(C.dxy), dxz = float(C.dxz), dyz = float(C.dyz);
     const float omega = float(C.omega);
-    auto put = [&](auto qc, float fn, float gn) {
-        constexpr int q = decltype(qc)::value;
-        store_pair(fo, go, int64_t(q) * A.cs + cell * 2, fn, gn);
-    };
     // the 26 moving directions' equilibrium differences, with FP64 sums of
     // their momentum
     float fe[NQ], ge[NQ];
@@ -840,6 +857,46 @@ __device__ __forceinline__ void collide_f32_body(
     });
     const float fe0 = float(rho_d - sf), ge0 = float(e_d - sg);
     put(std::integral_constant<int, 0>{}, fv[0] + omega * (fe0 - fv[0]), gv[0] + omega * (ge0 - gv[0]));
+}
+
+template <int BCX>
+__device__ __forceinline__ void collide_f32_body(
+    const float *__restrict__ f, const float *__restrict__ g, float *__restrict__ fo, float *__restrict__ go,
+    const uint32_t *__restrict__ mask, const double *__restrict__ Q, const CompactArgs &A, const BCDev &bc,
+    const ShiftTab &sh, const DevRest &D, const CollideParams &P)
+{
+    int x, y, z;
+    int64_t cell;
+    if (!cell_coords(A, x, y, z, cell)) return;
+    const uint32_t m = mask[cell];
+    if (!(m & 1u)) {
+        // solid cell: the stream's -1 sentinel (LBM.cpp:565)
+        static_for<NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            store_pair(fo, go, int64_t(q) * A.cs + cell * 2, narrow<float>(-1.0, sh.f(q)),
+                       narrow<float>(-1.0, sh.g(q)));
+        });
+        return;
+    }
+    float fv[NQ], gv[NQ];
+    pull_cell<BCX, float, float, true>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
+    MomentsR<double> mo;
+    F32Sums ds;
+    f32_moments(fv, gv, D, mo, ds);
+    const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
+    const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
+    const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
+    const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
+    const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
+    const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0, P.inv_dx[0]);
+    const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
+                                          ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
+    const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
+                                          zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
+    f32_relax(fv, gv, mo, M, ds, dqx, dqy, dqz, D, P, [&](auto qc, float fn, float gn) {
+        constexpr int q = decltype(qc)::value;
+        store_pair(fo, go, int64_t(q) * A.cs + cell * 2, fn, gn);
+    });
 }
 
 // the FP32-arithmetic pass 2 of the shifted FP32 storage on 64x2 and 256x1
@@ -895,11 +952,13 @@ constexpr uint32_t kSweepSpinLimit = 1u << 22;
 
 // f_q (or g_q) of the block's row, cell x, held in LDS at [2q + lattice][x]
 // (stride: 2 x the block's row width)
-struct LdsRow {
-    const double *p;
+template <class H>
+struct LdsRowT {
+    const H *p;
     int stride = 2 * kSweepRow;
-    __device__ __forceinline__ double operator[](int q) const { return p[q * stride]; }
+    __device__ __forceinline__ H operator[](int q) const { return p[q * stride]; }
 };
+using LdsRow = LdsRowT<double>;
 
 __device__ __forceinline__ void store_agent(double *p, double v)
 {
@@ -943,13 +1002,20 @@ struct Tagged {
 
 __global__ void k_bump_u32(uint32_t *a) { a[0] += 1u; }
 
-template <class T, int BCX>
+// F32A (shifted FP32 storage): the collide of the two-pass step's
+// FP32-arithmetic pass 2 (k_collide_f32, deviation form), the held plane
+// kept as the stored FP32 differences; the neighbours' Q_corr as pass 1
+// forms it (FP64, from the populations).
+template <class T, int BCX, bool F32A = false>
 __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
     const T *__restrict__ f, const T *__restrict__ g, T *__restrict__ fo, T *__restrict__ go,
     const uint32_t *__restrict__ mask, uint64_t *qring, const uint32_t *epoch, uint32_t *fault, CompactArgs A,
-    BCDev bc, ShiftTab sh, CollideParams P)
+    BCDev bc, ShiftTab sh, CollideParams P, DevRest D)
 {
-    __shared__ double hold[2 * NQ * kSweepRow];
+    static_assert(!F32A || (std::is_same<T, float>::value && (BCX & kBcxPressure) == 0),
+                  "FP32-arithmetic sweep: shifted FP32 storage without PRESSURE faces");
+    using H = std::conditional_t<F32A, float, double>;  // held values
+    __shared__ H hold[2 * NQ * kSweepRow];
     __shared__ double qxrow[kSweepRow];
     __shared__ int gave_up;  // a wait for a neighbour row hit the spin limit
     if (threadIdx.x == 0) gave_up = 0;
@@ -969,9 +1035,10 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
         __hip_atomic_load(epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * uint32_t(A.nz + 1) + 1u;
     // two planes in registers: the pull of plane p (rf, rg: its Q_corr is
     // known) and the stored values of plane p+1 in flight (nf, ng) while
-    // plane p-1, held in LDS, is collided; they become populations after
-    // the collide (pull_finish), so the collide does not wait for them
-    double rf[NQ], rg[NQ];
+    // plane p-1, held in LDS, is collided; they become populations (or FP32
+    // differences) after the collide (pull_finish), so the collide does not
+    // wait for them
+    H rf[NQ], rg[NQ];
     T nf[NQ], ng[NQ];
     PullBits pb{0u, 0u, 0u, 0u};
     // planes past the last one are replaced by the last (their loads and
@@ -986,11 +1053,21 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
     };
     auto pull_done = [&](int zp, uint32_t m) {
         zp = min(zp, A.nz - 1);
-        pull_finish<BCX, T, double>(f, A, bc, sh, Half16{}, x, y, zp, m, pb, nf, ng, rf, rg);
+        pull_finish<BCX, T, H, F32A>(f, A, bc, sh, Half16{}, x, y, zp, m, pb, nf, ng, rf, rg);
     };
-    auto q_of = [&](uint32_t m, double &qx, double &qy, double &qz) {
+    // Q_corr of plane zp from its populations (with F32A formed from the
+    // stored values still in nf, ng)
+    auto q_of = [&](int zp, uint32_t m, double &qx, double &qy, double &qz) {
         double rho, jx, jy, jz, E;
-        accumulate_q_moments(rf, rg, rho, jx, jy, jz, E);
+        if constexpr (F32A) {
+            zp = min(zp, A.nz - 1);
+            double pf[NQ], pg[NQ];
+            pull_finish<BCX, T, double>(f, A, bc, sh, Half16{}, x, y, zp, m, pb, nf, ng, pf, pg);
+            accumulate_q_moments(pf, pg, rho, jx, jy, jz, E);
+        } else {
+            (void)zp;
+            accumulate_q_moments(rf, rg, rho, jx, jy, jz, E);
+        }
         const Macro M = macro_from(rho, jx, jy, jz, E, P);
         const bool fluid = (m & 1u) != 0;
         qx = fluid ? M.Qx : 0.0;
@@ -1015,7 +1092,7 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
         qz0 = qz1;
         m0 = m1;
         pull_done(p + 1, m2);
-        q_of(m2, qx1, qy1, qz1);
+        q_of(p + 1, m2, qx1, qy1, qz1);
         publish(p + 1, qy1);
         m1 = m2;
     };
@@ -1055,28 +1132,43 @@ __global__ void __launch_bounds__(kSweepRow, 1) k_sweep(
         const bool zp = zc < A.nz - 1 && ((m >> 6) & 1u), zm = zc > 0 && ((m >> 5) & 1u);
         const double qyp = yp ? ep.value() : 0.0;
         const double qym = ym ? em.value() : 0.0;
-        const LdsRow F{&hold[x]}, G{&hold[kSweepRow + x]};
+        const LdsRowT<H> F{&hold[x]}, G{&hold[kSweepRow + x]};
         MomentsR<double> mo;
-        accumulate_moments(F, G, mo);
+        F32Sums ds;
+        if constexpr (F32A)
+            f32_moments(F, G, D, mo, ds);
+        else
+            accumulate_moments(F, G, mo);
         const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
         const double dqx =
             one_sided_gradient(xp, xm, xp ? qxrow[x + 1] : 0.0, M.Qx, xm ? qxrow[x - 1] : 0.0, P.inv_dx[0]);
         const double dqy = one_sided_gradient(yp, ym, qyp, M.Qy, qym, P.inv_dx[1]);
         const double dqz = one_sided_gradient(zp, zm, zp ? qz1 : 0.0, M.Qz, zm ? qzm : 0.0, P.inv_dx[2]);
         const double nan = __builtin_nan("");
-        collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
-            constexpr int q = decltype(qc)::value;
-            const double vf = fluid ? (poisoned ? nan : fnew) : -1.0;
-            const double vg = fluid ? (poisoned ? nan : gnew) : -1.0;
-            store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(vf, sh.f(q)),
-                       narrow<T>(vg, sh.g(q)));
-        });
+        if constexpr (F32A) {
+            (void)ds;
+            f32_relax(F, G, mo, M, ds, dqx, dqy, dqz, D, P, [&](auto qc, float fn, float gn) {
+                constexpr int q = decltype(qc)::value;
+                const float vf = fluid ? (poisoned ? float(nan) : fn) : narrow<float>(-1.0, sh.f(q));
+                const float vg = fluid ? (poisoned ? float(nan) : gn) : narrow<float>(-1.0, sh.g(q));
+                store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, vf, vg);
+            });
+        } else {
+            (void)D;
+            collide_cell_t<double>(F, G, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
+                constexpr int q = decltype(qc)::value;
+                const double vf = fluid ? (poisoned ? nan : fnew) : -1.0;
+                const double vg = fluid ? (poisoned ? nan : gnew) : -1.0;
+                store_pair(fo, go, int64_t(q) * A.cs + cell * kCellStride<T>, narrow<T>(vf, sh.f(q)),
+                           narrow<T>(vg, sh.g(q)));
+            });
+        }
         __syncthreads();
     };
     // plane 0 pulled, its Q_corr published; plane 1 in flight
     pull_issue(0, m1);
     pull_done(0, m1);
-    q_of(m1, qx1, qy1, qz1);
+    q_of(0, m1, qx1, qy1, qz1);
     publish(0, qy1);
     uint32_t m2 = mnext;
     mnext = mask_of(2);
@@ -1653,27 +1745,38 @@ static int two_pass_tp(mrb_level *L, int variant, hipStream_t s, hipEvent_t *ev)
             return set_error(MRB_E_UNSUPPORTED, "k_sweep_x64: FP64 storage only");
         }
     }
-    if (variant == kSweepVariant) {
+    if (variant == kSweepVariant || variant == kSweepF32Variant) {
+        const bool f32a = variant == kSweepF32Variant;
+        if (f32a && !std::is_same<T, float>::value)
+            return set_error(MRB_E_UNSUPPORTED, "k_sweep_f32: shifted FP32 storage only");
         if constexpr (!kIsHalf<T>) {
             if (!L->d_sweep_qt || A.nx > kSweepRow)
                 return set_error(MRB_E_UNSUPPORTED, "k_sweep: rows of at most 256 cells only");
-            k_bump_u32<<<1, 1, 0, s>>>(L->d_sweep_epoch);
-            MRB_HIP(hipGetLastError());
-            if (ev) MRB_HIP(hipEventRecord(ev[0], s));
-            uint64_t *qr = L->d_sweep_qt;
-            uint32_t *fl = L->d_sweep_epoch, *fa = L->d_sweep_fault;
-            const uint32_t *mk = L->d_mask;
-            CompactArgs Ac = A;
-            BCDev bcc = L->bcdev;
-            ShiftTab shc = sh;
-            CollideParams Pc = L->cp;
-            int fit = 0;
-            if ((rc = resident_blocks(reinterpret_cast<const void *>(&k_sweep<T, BCX>), kSweepRow, fit))) return rc;
-            if (A.ny > fit) return set_error(MRB_E_UNSUPPORTED, "k_sweep: more rows than the GPU holds at once");
-            k_sweep<T, BCX><<<dim3(A.ny), dim3(kSweepRow), 0, s>>>(f, g, fo, go, mk, qr, fl, fa, Ac, bcc, shc, Pc);
-            MRB_HIP(hipGetLastError());
-            if (ev) MRB_HIP(hipEventRecord(ev[1], s));
-            return MRB_OK;
+            auto go_sweep = [&](auto kernel) -> int {
+                int fit = 0;
+                int rc2 = resident_blocks(reinterpret_cast<const void *>(kernel), kSweepRow, fit);
+                if (rc2) return rc2;
+                if (A.ny > fit) return set_error(MRB_E_UNSUPPORTED, "k_sweep: more rows than the GPU holds at once");
+                k_bump_u32<<<1, 1, 0, s>>>(L->d_sweep_epoch);
+                MRB_HIP(hipGetLastError());
+                if (ev) MRB_HIP(hipEventRecord(ev[0], s));
+                uint64_t *qr = L->d_sweep_qt;
+                uint32_t *fl = L->d_sweep_epoch, *fa = L->d_sweep_fault;
+                const uint32_t *mk = L->d_mask;
+                CompactArgs Ac = A;
+                BCDev bcc = L->bcdev;
+                ShiftTab shc = sh;
+                CollideParams Pc = L->cp;
+                DevRest Dc = dev_rest(L);
+                kernel<<<dim3(A.ny), dim3(kSweepRow), 0, s>>>(f, g, fo, go, mk, qr, fl, fa, Ac, bcc, shc, Pc, Dc);
+                MRB_HIP(hipGetLastError());
+                if (ev) MRB_HIP(hipEventRecord(ev[1], s));
+                return MRB_OK;
+            };
+            if (!f32a) return go_sweep(&k_sweep<T, BCX, false>);
+            if constexpr (std::is_same<T, float>::value && (BCX & kBcxPressure) == 0)
+                return go_sweep(&k_sweep<T, BCX, true>);
+            return set_error(MRB_E_UNSUPPORTED, "k_sweep_f32: no PRESSURE faces");
         } else {
             return set_error(MRB_E_UNSUPPORTED, "k_sweep: FP64 and shifted FP32 storage only");
         }
@@ -1844,7 +1947,7 @@ const char *const kTwoPassNames[kNumTwoPassVariants] = {"k_qpass+k_collide", "k_
                                                            "k_qpass_x256+k_collide_h32_x256",
                                                            "k_qpass_y2+k_collide_f32",
                                                            "k_qpass_x256+k_collide_f32_x256",
-                                                           "k_sweep", "k_sweep_x64"};
+                                                           "k_sweep", "k_sweep_x64", "k_sweep_f32"};
 
 int compact_two_pass_launch(mrb_level *L, int variant, hipStream_t s) { return two_pass(L, variant, s, nullptr); }
 
@@ -1865,7 +1968,10 @@ static int step_impl(mrb_level *L, hipStream_t s, bool events, int &nslots)
     int rc;
     if (L->hot < kNumTwoPassVariants) {
         rc = two_pass(L, L->hot, s, events ? L->ev : nullptr);
-        nslots = (L->hot == kOnePassVariant || L->hot == kSweepVariant || L->hot == kSweepX64Variant) ? 1 : 2;
+        nslots = (L->hot == kOnePassVariant || L->hot == kSweepVariant || L->hot == kSweepX64Variant ||
+                  L->hot == kSweepF32Variant)
+                     ? 1
+                     : 2;
     } else {
         if (events) MRB_HIP(hipEventRecord(L->ev[0], s));
         rc = fused_launch(L, L->hot - kNumTwoPassVariants, s);

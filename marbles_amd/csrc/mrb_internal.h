@@ -377,7 +377,7 @@ int fused_step(mrb_level *L, int variant, hipStream_t s);
 // single pass that recomputes the neighbours' Q_corr; the rest the fused
 // variants; launch reads the current buffers, writes the spare ones, does
 // not flip
-constexpr int kNumTwoPassVariants = 13;
+constexpr int kNumTwoPassVariants = 14;
 constexpr int kOnePassVariant = 2;
 constexpr int kRows8Variant = 3;  // two passes on 64x8 blocks
 constexpr int kRows2Variant = 4;  // two passes on 64x2 blocks
@@ -390,16 +390,21 @@ constexpr int kF32X256Variant = 10;    // variant 9 on 256x1 blocks
 constexpr int kSweepVariant = 11;      // single-pass z-sweep, one block per x-row (FP64 / FP32, rows <= 256 cells)
 constexpr int kSweepRingPlanes = 8;    // planes of Q_y the z-sweep keeps for the rows beside
 constexpr int kSweepX64Variant = 12;   // the z-sweep on single-wave blocks of 64 cells of a row (FP64)
+constexpr int kSweepF32Variant = 13;   // the z-sweep with the FP32-arithmetic collide (shifted FP32)
 // the variants whose pass 2 computes in FP32 (their results differ from the
 // FP64-arithmetic variants'), and the FP64-arithmetic variant with the same
 // block shape that replaces one on a level with PRESSURE faces
 inline bool fp32_collide_variant(int v)
 {
-    return v == kHalf32Variant || v == kHalf32X256Variant || v == kF32Variant || v == kF32X256Variant;
+    return v == kHalf32Variant || v == kHalf32X256Variant || v == kF32Variant || v == kF32X256Variant ||
+           v == kSweepF32Variant;
 }
 inline int fp64_counterpart(int v)
 {
-    return v == kHalf32Variant ? 0 : v == kF32Variant ? kRows2Variant : kX256Variant;
+    return v == kHalf32Variant     ? 0
+           : v == kF32Variant      ? kRows2Variant
+           : v == kSweepF32Variant ? kSweepVariant
+                                   : kX256Variant;
 }
 // conservation sums (mrb_level.hip): block partial sums (5 per block, at
 // most kConservationBlocks blocks, grid-strided over any level size) into

@@ -312,6 +312,52 @@ def test_f32_arithmetic_variant_survives_a_pressure_face(engine):
     E.close()
 
 
+@pytest.mark.parametrize("case", ["tg_tau1.0", "channel", "odd", "channel_profile"])
+def test_fp32_arithmetic_sweep_bitwise(engine, case):
+    """The z-sweep with the FP32-arithmetic collide (k_sweep_f32: stored
+    FP32 differences held in LDS, neighbours' Q_corr formed in FP64 from the
+    populations as pass 1 forms them) equals the two-pass FP32-arithmetic
+    step bit for bit, over 20 steps: periodic, an EB sphere, sizes that
+    leave partial planes, and a Channel-profile inlet."""
+    if case == "tg_tau1.0":
+        cfg = configs.taylor_green(32, tau=1.0)
+    elif case == "channel":
+        cfg = configs.channel_sphere((48, 24, 24), radius=5.0, center=(16.0, 12.0, 12.0))
+    elif case == "odd":
+        cfg = configs.channel_sphere((37, 29, 23), radius=4.0, center=(12.0, 14.0, 11.0))
+    else:
+        cfg = configs.channel_profile((40, 20, 24), kind="channel")
+    out = {}
+    for name in ("k_qpass_y2+k_collide_f32", "k_sweep_f32"):
+        E = configs.build_level(cfg, layout="compact", storage="f32s")
+        E.set_hot_kernel(name)
+        assert E.hot_kernel_name() == name
+        E.advance(20)
+        out[name] = (E.valid("f"), E.valid("g"))
+        E.close()
+    (fa, ga), (fb, gb) = out.values()
+    assert np.isfinite(fa).all() and np.isfinite(ga).all()
+    assert np.array_equal(fa, fb) and np.array_equal(ga, gb), case
+
+
+def test_fp32_arithmetic_sweep_yields_to_a_pressure_face(engine):
+    """k_sweep_f32 has no PRESSURE faces: a level on it that gets one steps
+    on the FP64-arithmetic sweep of the same storage, and returns to the
+    FP32 one when the faces change back."""
+    cfg = configs.pressure_case("walled", (24, 16, 32))
+    E = configs.build_level(cfg, layout="compact", storage="f32s")
+    E.set_bc((1, 0, 1), (1, 0, 1))
+    E.set_hot_kernel("k_sweep_f32")
+    E.set_bc(cfg["bc_lo"], cfg["bc_hi"], cfg.get("velocity_bc"))
+    assert E.hot_kernel_name() == "k_sweep"
+    E.advance(5)
+    E.set_bc((1, 0, 1), (1, 0, 1))
+    assert E.hot_kernel_name() == "k_sweep_f32"
+    E.advance(5)
+    assert np.isfinite(E.valid("f")).all()
+    E.close()
+
+
 @pytest.mark.parametrize("case", ["tg_tau1.0", "channel", "channel_profile", "pressure_walled"])
 def test_f16_scaled_storage_tracks_f64(oracle, engine, case):
     """FP16 storage of (f - f_eq(rho0, 0, T0)) * 2^e, e per (z-slab box,

@@ -218,12 +218,15 @@ int mrb_advance_timed(mrb_level *lvl, int nsteps, void *stream, double *kernel_m
  *  12  k_sweep_x64: the same z-sweep on single-wave blocks of 64 cells of
  *      a row (any row length; MRB_E_UNSUPPORTED when the blocks do not fit
  *      the GPU at once);
- *  13+ the single-pass LDS z-sweep with different tile shapes.
+ *  13  k_sweep_f32: shifted FP32 storage only, no PRESSURE faces: the
+ *      z-sweep of variant 11 with the collide of variant 9;
+ *  14+ the single-pass LDS z-sweep with different tile shapes.
  * The FP64-arithmetic variants give the same populations bit for bit; 5
- * and 8 give the same as each other, and 9 and 10.  A level
+ * and 8 give the same as each other, and 9, 10 and 13.  A level
  * starts on variant 7 (FP64 storage), 9 (shifted FP32) or 5 (scaled FP16).
  * While a level has a PRESSURE face, variants 5 and 9 are replaced by 0 and
- * 4, 8 and 10 by 7 (mrb_set_bc), and restored when the faces change back.
+ * 4, 8 and 10 by 7, 13 by 11 (mrb_set_bc), and restored when the faces
+ * change back.
  * A variant that does not support the level returns MRB_E_UNSUPPORTED from
  * the step. */
 int mrb_hot_kernel_count(void);

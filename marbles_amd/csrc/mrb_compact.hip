@@ -98,11 +98,22 @@ __device__ __forceinline__ void store_pair(T *__restrict__ fo, T *__restrict__ g
         const uint32_t lo = __builtin_bit_cast(uint16_t, fv), hi = __builtin_bit_cast(uint16_t, gv);
         *reinterpret_cast<uint32_t *>(fo + e) = lo | (hi << 16);
     } else if constexpr (std::is_same<T, double>::value) {
+        // non-temporal: the next step reads the buffer back from HBM
+        // (measured at 256^3: z-sweep 3.039 -> 3.024 ms FP64, 2.34 -> 2.30 ms
+        // shifted FP32)
         (void)go;
-        *reinterpret_cast<double2 *>(fo + e) = make_double2(fv, gv);
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        d2 v;
+        v.x = fv;
+        v.y = gv;
+        __builtin_nontemporal_store(v, reinterpret_cast<d2 *>(fo + e));
     } else {
         (void)go;
-        *reinterpret_cast<float2 *>(fo + e) = make_float2(fv, gv);
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        f2 v;
+        v.x = fv;
+        v.y = gv;
+        __builtin_nontemporal_store(v, reinterpret_cast<f2 *>(fo + e));
     }
 }
 

@@ -312,6 +312,46 @@ def test_f32_arithmetic_variant_survives_a_pressure_face(engine):
     E.close()
 
 
+@pytest.mark.parametrize("case", ["nz1", "nz2", "nz3", "ny1", "row256", "mostly_solid"])
+def test_sweep_edge_shapes_bitwise(engine, case):
+    """The z-sweeps at the shapes their prologue, epilogue and lane
+    handling treat apart: one, two and three planes (the planes past the
+    last one are replaced by the last), one row, rows exactly 256 cells
+    wide (no lane past the row's end), and a sphere filling most of the
+    domain (waves of solid cells only, which pull and collide like fluid
+    ones and store the solid marker).  FP64 and shifted FP32 storage, both
+    arithmetics, against the two-pass step bit for bit over 6 steps."""
+    if case == "nz1":
+        cfg = configs.channel_sphere((24, 12, 1), radius=3.0, center=(8.0, 6.0, 0.0))
+    elif case == "nz2":
+        cfg = configs.channel_sphere((24, 16, 2), radius=3.0, center=(8.0, 8.0, 1.0))
+    elif case == "nz3":
+        cfg = configs.channel_sphere((24, 16, 3), radius=3.0, center=(8.0, 8.0, 1.0))
+    elif case == "ny1":
+        cfg = configs.channel_sphere((24, 1, 16), radius=3.0, center=(8.0, 0.0, 8.0))
+    elif case == "row256":
+        cfg = configs.channel_sphere((256, 6, 8), radius=3.0, center=(40.0, 3.0, 4.0))
+    else:
+        cfg = configs.channel_sphere((32, 16, 16), radius=11.0, center=(16.0, 8.0, 8.0))
+    runs = (("f64", ["k_qpass+k_collide", "k_sweep", "k_sweep_x64"]),
+            ("f32s", ["k_qpass+k_collide", "k_sweep"]),
+            ("f32s", ["k_qpass_y2+k_collide_f32", "k_sweep_f32"]))
+    for storage, names in runs:
+        out = {}
+        for name in names:
+            E = configs.build_level(cfg, layout="compact", storage=storage)
+            E.set_hot_kernel(name)
+            assert E.hot_kernel_name() == name
+            E.advance(6)
+            out[name] = (E.valid("f"), E.valid("g"))
+            E.close()
+        ref = out[names[0]]
+        assert np.isfinite(ref[0]).all() and np.isfinite(ref[1]).all(), (case, storage)
+        for name in names[1:]:
+            for a, b in zip(out[name], ref):
+                assert np.array_equal(a, b), (case, storage, name)
+
+
 @pytest.mark.parametrize("case", ["tg_tau1.0", "channel", "odd", "channel_profile"])
 def test_fp32_arithmetic_sweep_bitwise(engine, case):
     """The z-sweep with the FP32-arithmetic collide (k_sweep_f32: stored

@@ -50,7 +50,7 @@ struct mrb_iso {
     double *dqbar = nullptr;              // 3 N
     double *uo = nullptr;                 // 3 N: u of the pulled state
     double *bu = nullptr;                 // 4 N: shared lambda terms of one adjoint pass
-    double *ab = nullptr;                 // 4 N: per-cell gather coefficients alpha, beta
+    double *ab = nullptr;                 // 4 N: per-cell gather coefficients (alpha, beta), one 32-byte record per cell
     double *part = nullptr;               // one partial sum per block of the cell grid
     int64_t nparts = 0;
     // checkpoints (one per segment), the states and Q arrays of one segment;
@@ -255,16 +255,19 @@ __global__ void __launch_bounds__(256) k_iso_seed(const double *__restrict__ f, 
 
 // lambda of one post-collision population read through the pull by the cell
 // t: the pulled populations' lambda there plus the Q-path term
-// alpha(t) + beta(t) . c_q (ab holds alpha, beta_x, beta_y, beta_z)
+// alpha(t) + beta(t) . c_q (ab holds the record alpha, beta_x, beta_y,
+// beta_z of each cell: the coefficients a gather needs from one neighbour
+// share one cache line)
 template <int q>
 __device__ __forceinline__ double lam_at(const double *__restrict__ lams, const double *__restrict__ ab, int64_t N,
                                          int64_t t)
 {
-    double v = lams[int64_t(q) * N + t] + ab[t];
+    const double *r = ab + 4 * t;
+    double v = lams[int64_t(q) * N + t] + r[0];
     static_for<3>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        if constexpr (ev(q, a) == 1) v += ab[(a + 1) * N + t];
-        if constexpr (ev(q, a) == -1) v -= ab[(a + 1) * N + t];
+        if constexpr (ev(q, a) == 1) v += r[a + 1];
+        if constexpr (ev(q, a) == -1) v -= r[a + 1];
     });
     return v;
 }
@@ -427,6 +430,7 @@ __global__ void __launch_bounds__(256) k_iso_adj_q(const double *__restrict__ dq
     const int pos[3] = {x, y, z}, n[3] = {A.nx, A.ny, A.nz};
     const int64_t st[3] = {1, A.nx, int64_t(A.nx) * A.ny};
     double alpha = bu[cell];
+    double beta[3];
     for (int a = 0; a < 3; ++a) {
         const double *g = dqbar + a * A.N;
         const bool p_ok = pos[a] + 1 < n[a], m_ok = pos[a] - 1 >= 0;
@@ -440,9 +444,11 @@ __global__ void __launch_bounds__(256) k_iso_adj_q(const double *__restrict__ dq
         if (p_ok) v -= (pos[a] + 2 < n[a] ? 0.5 : 1.0) * gp;
         const double ua = uo[a * A.N + cell];
         alpha += v * (2.0 * ua * ua * ua);
-        ab[(a + 1) * A.N + cell] = bu[(a + 1) * A.N + cell] + v * ((A.k - 3.0 * ua * ua) * A.ms);
+        beta[a] = bu[(a + 1) * A.N + cell] + v * ((A.k - 3.0 * ua * ua) * A.ms);
     }
-    ab[cell] = alpha;
+    double2 *r = reinterpret_cast<double2 *>(ab + 4 * cell);
+    r[0] = make_double2(alpha, beta[0]);
+    r[1] = make_double2(beta[1], beta[2]);
 }
 
 // dJ/du0 from lambda_0 (gathered) through the initial equilibrium

@@ -239,6 +239,15 @@ __global__ void __launch_bounds__(kFabTile) k_fab_collide(
         }
         double fv[NQ], gv[NQ];
         gather_cell<COAL>(f, g, B, loc, m, C, fv, gv);
+        // the six neighbours' Q_corr, loaded with the pull (a valid cell's
+        // neighbours along each axis lie in the fab, at most in its first
+        // ghost layer; a value the gradient does not use is loaded anyway),
+        // so that the gradient does not wait on loads issued after the
+        // moments
+        const double *q = Q + B.base * 3 + loc;
+        const int64_t sy = B.n[0], sz = int64_t(B.n[0]) * B.n[1];
+        const double qxp = q[1], qxm = q[-1], qyp = q[s + sy], qym = q[s - sy], qzp = q[2 * s + sz],
+                     qzm = q[2 * s - sz];
         Moments mo;
         accumulate_moments(fv, gv, mo);
         const Macro M = macro_from(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
@@ -247,13 +256,9 @@ __global__ void __launch_bounds__(kFabTile) k_fab_collide(
         const bool xp = i + 1 < A.dom[0] && ((m >> 2) & 1u), xm = i - 1 >= 0 && ((m >> 1) & 1u);
         const bool yp = j + 1 < A.dom[1] && ((m >> 4) & 1u), ym = j - 1 >= 0 && ((m >> 3) & 1u);
         const bool zp = k + 1 < A.dom[2] && ((m >> 6) & 1u), zm = k - 1 >= 0 && ((m >> 5) & 1u);
-        const double *q = Q + B.base * 3 + loc;
-        const int64_t sy = B.n[0], sz = int64_t(B.n[0]) * B.n[1];
-        const double dqx = one_sided_gradient(xp, xm, xp ? q[1] : 0.0, M.Qx, xm ? q[-1] : 0.0, P.inv_dx[0]);
-        const double dqy = one_sided_gradient(yp, ym, yp ? q[s + sy] : 0.0, M.Qy, ym ? q[s - sy] : 0.0,
-                                              P.inv_dx[1]);
-        const double dqz = one_sided_gradient(zp, zm, zp ? q[2 * s + sz] : 0.0, M.Qz,
-                                              zm ? q[2 * s - sz] : 0.0, P.inv_dx[2]);
+        const double dqx = one_sided_gradient(xp, xm, xp ? qxp : 0.0, M.Qx, xm ? qxm : 0.0, P.inv_dx[0]);
+        const double dqy = one_sided_gradient(yp, ym, yp ? qyp : 0.0, M.Qy, ym ? qym : 0.0, P.inv_dx[1]);
+        const double dqz = one_sided_gradient(zp, zm, zp ? qzp : 0.0, M.Qz, zm ? qzm : 0.0, P.inv_dx[2]);
         collide_cell(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, double fnew, double gnew) {
             constexpr int qq = decltype(qc)::value;
             __builtin_nontemporal_store(fnew, fob + qq * s);

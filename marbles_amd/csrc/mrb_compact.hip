@@ -548,6 +548,27 @@ __device__ __forceinline__ void pull_cell(
     pull_finish<BCX, T, W, DEV>(f, A, bc, sh, h16, x, y, z, mask, pb, rf, rg, fv, gv);
 }
 
+// The six neighbours' Q_corr of a cell, loaded unconditionally with the
+// pull (indices clamped into the domain: a value past a face is loaded but
+// not used), so that the gradient does not wait on loads issued after the
+// moments.
+struct QNbr {
+    double xp, xm, yp, ym, zp, zm;
+};
+__device__ __forceinline__ QNbr load_q_nbr(const double *__restrict__ Q, const CompactArgs &A, int x, int y, int z,
+                                           int64_t cell)
+{
+    const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
+    QNbr r;
+    r.xp = Q[cell + (x + 1 < A.nx ? 1 : 0)];
+    r.xm = Q[cell - (x > 0 ? 1 : 0)];
+    r.yp = Q[A.N + cell + (y + 1 < A.ny ? sy : 0)];
+    r.ym = Q[A.N + cell - (y > 0 ? sy : 0)];
+    r.zp = Q[2 * A.N + cell + (z + 1 < A.nz ? sz : 0)];
+    r.zm = Q[2 * A.N + cell - (z > 0 ? sz : 0)];
+    return r;
+}
+
 __device__ __forceinline__ bool cell_coords(const CompactArgs &A, int &x, int &y, int &z, int64_t &cell)
 {
     x = int(blockIdx.x * blockDim.x + threadIdx.x);
@@ -685,6 +706,7 @@ __device__ __forceinline__ void collide_body(
     } else if (in) {
         R fv[NQ], gv[NQ];
         pull_cell<BCX>(f, g, A, bc, sh, h16, x, y, z, cell, m, fv, gv);
+        const QNbr qn = load_q_nbr(Q, A, x, y, z, cell);
         MomentsR<R> mo;
         accumulate_moments(fv, gv, mo);
         const MacroR<R> M = macro_from_t<R>(mo.rho, mo.jx, mo.jy, mo.jz, mo.E, P);
@@ -692,13 +714,9 @@ __device__ __forceinline__ void collide_body(
         const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
         const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
         const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
-        const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
-        const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0,
-                                              P.inv_dx[0]);
-        const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
-                                              ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
-        const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
-                                              zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
+        const double dqx = one_sided_gradient(xp, xm, xp ? qn.xp : 0.0, M.Qx, xm ? qn.xm : 0.0, P.inv_dx[0]);
+        const double dqy = one_sided_gradient(yp, ym, yp ? qn.yp : 0.0, M.Qy, ym ? qn.ym : 0.0, P.inv_dx[1]);
+        const double dqz = one_sided_gradient(zp, zm, zp ? qn.zp : 0.0, M.Qz, zm ? qn.zm : 0.0, P.inv_dx[2]);
         if constexpr (std::is_same<R, double>::value) {
             collide_cell_t<R>(fv, gv, mo, M, dqx, dqy, dqz, P, [&](auto qc, R fnew, R gnew) { put(qc, fnew, gnew); });
         } else {
@@ -891,6 +909,7 @@ __device__ __forceinline__ void collide_f32_body(
     }
     float fv[NQ], gv[NQ];
     pull_cell<BCX, float, float, true>(f, g, A, bc, sh, Half16{}, x, y, z, cell, m, fv, gv);
+    const QNbr qn = load_q_nbr(Q, A, x, y, z, cell);
     MomentsR<double> mo;
     F32Sums ds;
     f32_moments(fv, gv, D, mo, ds);
@@ -898,12 +917,9 @@ __device__ __forceinline__ void collide_f32_body(
     const bool xp = x < A.nx - 1 && ((m >> 2) & 1u), xm = x > 0 && ((m >> 1) & 1u);
     const bool yp = y < A.ny - 1 && ((m >> 4) & 1u), ym = y > 0 && ((m >> 3) & 1u);
     const bool zp = z < A.nz - 1 && ((m >> 6) & 1u), zm = z > 0 && ((m >> 5) & 1u);
-    const int64_t sy = A.nx, sz = int64_t(A.nx) * A.ny;
-    const double dqx = one_sided_gradient(xp, xm, xp ? Q[cell + 1] : 0.0, M.Qx, xm ? Q[cell - 1] : 0.0, P.inv_dx[0]);
-    const double dqy = one_sided_gradient(yp, ym, yp ? Q[A.N + cell + sy] : 0.0, M.Qy,
-                                          ym ? Q[A.N + cell - sy] : 0.0, P.inv_dx[1]);
-    const double dqz = one_sided_gradient(zp, zm, zp ? Q[2 * A.N + cell + sz] : 0.0, M.Qz,
-                                          zm ? Q[2 * A.N + cell - sz] : 0.0, P.inv_dx[2]);
+    const double dqx = one_sided_gradient(xp, xm, xp ? qn.xp : 0.0, M.Qx, xm ? qn.xm : 0.0, P.inv_dx[0]);
+    const double dqy = one_sided_gradient(yp, ym, yp ? qn.yp : 0.0, M.Qy, ym ? qn.ym : 0.0, P.inv_dx[1]);
+    const double dqz = one_sided_gradient(zp, zm, zp ? qn.zp : 0.0, M.Qz, zm ? qn.zm : 0.0, P.inv_dx[2]);
     f32_relax(fv, gv, mo, M, ds, dqx, dqy, dqz, D, P, [&](auto qc, float fn, float gn) {
         constexpr int q = decltype(qc)::value;
         store_pair(fo, go, int64_t(q) * A.cs + cell * 2, fn, gn);

@@ -183,7 +183,11 @@ int mrb_collide(mrb_level *lvl, void *stream);
 /* ---- hot path (MRB_LAYOUT_COMPACT) ---------------------------------------
  * One full single-level step (LBM::evolve body, Source/LBM.cpp:416-420 for
  * one level): ghost fill, stream f and g, collide -- fused, with periodic
- * wrap and physical boundaries resolved inside the kernel.  Swaps A/B. */
+ * wrap and physical boundaries resolved inside the kernel.  Swaps A/B.
+ * Steps of one level are ordered by the caller (one stream, or events):
+ * besides the A/B buffers they share the level's Q-correction array and,
+ * for the single-pass sweeps, its ring of published planes and its launch
+ * count. */
 int mrb_stream_collide(mrb_level *lvl, void *stream);
 /* nsteps fused steps. */
 int mrb_advance(mrb_level *lvl, int nsteps, void *stream);

@@ -514,8 +514,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is a design "
-              "note, not a measured path", file=sys.stderr)
+        print("bench.py: the benchmark runs on one GPU only (--gpus 1); sharding the level across GPUs is "
+              "described, not implemented", file=sys.stderr)
         return 2
 
     import torch

@@ -150,26 +150,31 @@ def test_taylor_green_64_matches_oracle(oracle, engine):
 @pytest.mark.timeout(900)
 def test_channel_256_matches_oracle(oracle, engine):
     """The benchmarked workload itself: the 256^3 channel with the r = 24
-    sphere, 3 steps of the default kernel (k_qpass_x256 + k_collide_x256)
-    against the oracle on 64^3 boxes."""
+    sphere, 3 steps of the level's default kernel (k_qpass_x256 +
+    k_collide_x256) and of the single-pass sweep the bench's autotuner
+    selects at this size (k_sweep), each against the oracle on 64^3 boxes."""
     from oracle.pyoracle import build_oracle_level
 
     cfg, n = configs.channel_sphere((256, 256, 256), max_box=64), 3
     O = build_oracle_level(cfg)
-    E = configs.build_level(cfg, layout="compact")
-    assert E.hot_kernel_name() == "k_qpass_x256+k_collide_x256"
-    assert np.array_equal(E.valid("is_fluid"), O.valid("is_fluid"))
     O.evolve(n)
-    E.advance(n)
-    for name in ("f", "g"):
-        a = E.valid(name)
-        b = O.valid(name)
-        err = _rel_err(a, b)
-        exact = bool(np.array_equal(a, b))
-        del a, b
-        assert err <= REL_TOL, (name, err)
-        print(f"channel_256 {name}: max rel {err:.2e}, bit-exact={exact}")
-    E.close()
+    for kernel in (None, "k_sweep"):
+        E = configs.build_level(cfg, layout="compact")
+        if kernel is None:
+            assert E.hot_kernel_name() == "k_qpass_x256+k_collide_x256"
+            assert np.array_equal(E.valid("is_fluid"), O.valid("is_fluid"))
+        else:
+            E.set_hot_kernel(kernel)
+        E.advance(n)
+        for name in ("f", "g"):
+            a = E.valid(name)
+            b = O.valid(name)
+            err = _rel_err(a, b)
+            exact = bool(np.array_equal(a, b))
+            del a, b
+            assert err <= REL_TOL, (E.hot_kernel_name(), name, err)
+            print(f"channel_256 {E.hot_kernel_name()} {name}: max rel {err:.2e}, bit-exact={exact}")
+        E.close()
     O.close()
 
 

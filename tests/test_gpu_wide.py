@@ -178,6 +178,31 @@ def test_channel_256_matches_oracle(oracle, engine):
     O.close()
 
 
+@pytest.mark.timeout(600)
+def test_sweep_long_run_equals_two_pass_256(engine):
+    """1000 steps of the 256^3 channel (all 256 rows resident, neighbours'
+    Q_y through the tagged ring, one block per CU) on the single-pass sweep
+    and on the two-pass step: bit for bit equal at the end, so no ring
+    entry was read stale or overwritten early in 256 x 256 x 1000
+    row-plane exchanges."""
+    cfg, n = configs.channel_sphere((256, 256, 256)), 1000
+    out = {}
+    for kernel in ("k_qpass_x256+k_collide_x256", "k_sweep"):
+        E = configs.build_level(cfg, layout="compact")
+        E.set_hot_kernel(kernel)
+        E.advance(n)
+        out[kernel] = E.conservation()
+        for name in ("f", "g"):
+            out[(kernel, name)] = E.valid(name)
+        E.close()
+    for name in ("f", "g"):
+        a, b = out[("k_sweep", name)], out[("k_qpass_x256+k_collide_x256", name)]
+        assert np.isfinite(b).all()
+        assert np.array_equal(a, b), name
+        del a, b
+    print("sweep 1000 steps: conservation", out["k_sweep"])
+
+
 def _spectral(n=64, kmax=8):
     # tau = 0.55: at 0.52 the thermal model diverges on this field (the
     # precision study's point); T0 = 0.2
